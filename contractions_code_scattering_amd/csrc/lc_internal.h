@@ -1,0 +1,55 @@
+// Internal descriptors shared by the host planner and the HIP kernels.
+//
+// Every diagram is evaluated in spin (x) eigenvector space ("D-space", D = 4*Nv). A D-space
+// matrix is stored as 4x4 spin blocks of Nv x Nv complex entries, block (a,b) at
+// (a*4+b)*Nv*Nv -- the perambulator block layout [s1][s2][v1][v2] of the reference
+// (lib/contract.cpp:140,178). Elementals are single Nv x Nv blocks acting as M (x) 1_spin.
+//
+// Gamma structure never becomes arithmetic: a spin matrix with one non-zero per row and
+// column (gamma_5, gamma_i, gamma_5 gamma_i, ...) multiplying a D-space matrix only renames
+// blocks and multiplies them by an exact phase (+-1, +-i, +-1/2). An LcView carries those
+// renames and phases together with the element operation (conj, transpose of the eigenvector
+// indices, transpose of the spin-block indices), so a kernel reading block (a,b) of a view reads
+//
+//    ph_r[a] * ph_c[b] * f( X[ perm_r[a] ][ perm_c[b] ] )      (spin indices swapped if LC_TS)
+//
+// where f applies LC_CONJ / LC_TV element-wise.
+#pragma once
+#include <stdint.h>
+
+enum { LC_DENSE = 0, LC_KRON = 1 };
+enum { LC_CONJ = 1, LC_TV = 2, LC_TS = 4 };
+// X^dagger of a D-space matrix: conj + eigenvector transpose + spin-block transpose.
+#define LC_DAG (LC_CONJ | LC_TV | LC_TS)
+
+struct LcView {
+  const void* base;         // complex elements, interleaved (re, im)
+  long long s[3];           // batch strides, in complex elements
+  int kind;                 // LC_DENSE or LC_KRON
+  int flags;                // LC_CONJ | LC_TV | LC_TS
+  int perm_r[4], perm_c[4];
+  double ph_r[4][2], ph_c[4][2];
+};
+
+// C = alpha * A * B over a batch (i0 < n0, i1 < n1, i2 < n2). C is a plain D-space matrix.
+struct LcGemmGroup {
+  LcView A, B;
+  void* C;
+  long long cs[3];
+  double alpha[2];
+  int n[3];
+  int tiles;                // output tiles per block side: ceil(nv / TM)
+  int pad;
+  long long wg_begin;       // first workgroup of this group in the launch
+};
+
+// out (+)= alpha * Tr[X * Y] over a batch. out holds complex values of the compute type.
+struct LcTraceGroup {
+  LcView X, Y;
+  void* out;
+  long long os[3];
+  double alpha[2];
+  int n[3];
+  int accumulate;
+  long long wg_begin;
+};

@@ -1,0 +1,105 @@
+// Host runtime: context, Program builder/launcher.
+#include "runtime.h"
+
+#include <cstring>
+
+namespace lc {
+
+Context& ctx() {
+  static Context c;
+  return c;
+}
+
+void require_init() {
+  if (!ctx().on) fail(LC_ERR_NOT_INIT, "contraction library is not initialised (call lc_init)");
+}
+
+Program::Level* Program::open_level(bool is_trace) {
+  if (levels_.empty() || levels_.back().closed || levels_.back().is_trace != is_trace) {
+    Level l;
+    l.is_trace = is_trace;
+    l.closed = false;
+    l.first = is_trace ? traces_.size() : gemms_.size();
+    l.count = 0;
+    l.total_wg = 0;
+    levels_.push_back(l);
+  }
+  return &levels_.back();
+}
+
+void Program::barrier() {
+  if (!levels_.empty()) levels_.back().closed = true;
+}
+
+void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
+                   long long cs2, int n0, int n1, int n2, double ar, double ai) {
+  if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+  if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
+  Level* l = open_level(false);
+  LcGemmGroup g;
+  std::memset(&g, 0, sizeof g);
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.cs[0] = cs0; g.cs[1] = cs1; g.cs[2] = cs2;
+  g.alpha[0] = ar; g.alpha[1] = ai;
+  g.n[0] = n0; g.n[1] = n1; g.n[2] = n2;
+  const int tm = lc_gemm_tile(nv_);
+  g.tiles = (nv_ + tm - 1) / tm;
+  g.wg_begin = l->total_wg;
+  const long long items = (long long)n0 * n1 * n2;
+  l->total_wg += items * 16LL * g.tiles * g.tiles;
+  l->count++;
+  gemms_.push_back(g);
+  const int nterm = (A.kind == LC_KRON || B.kind == LC_KRON) ? 1 : 4;
+  gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
+}
+
+void Program::trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
+                    long long os2, int n0, int n1, int n2, double ar, double ai, bool accumulate) {
+  if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+  Level* l = open_level(true);
+  LcTraceGroup g;
+  std::memset(&g, 0, sizeof g);
+  g.X = X;
+  g.Y = Y;
+  g.out = out;
+  g.os[0] = os0; g.os[1] = os1; g.os[2] = os2;
+  g.alpha[0] = ar; g.alpha[1] = ai;
+  g.n[0] = n0; g.n[1] = n1; g.n[2] = n2;
+  g.accumulate = accumulate ? 1 : 0;
+  g.wg_begin = l->total_wg;
+  const long long items = (long long)n0 * n1 * n2;
+  l->total_wg += items;
+  l->count++;
+  traces_.push_back(g);
+  const int npair = (X.kind == LC_KRON || Y.kind == LC_KRON) ? 4 : 16;
+  trace_flops_ += (double)items * npair * 8.0 * nv_ * nv_;
+}
+
+void Program::run(hipStream_t s) {
+  Context& c = ctx();
+  const size_t gb = gemms_.size() * sizeof(LcGemmGroup);
+  const size_t tb = traces_.size() * sizeof(LcTraceGroup);
+  const size_t toff = (gb + 255) & ~size_t(255);
+  c.groups.ensure(toff + tb + 256);
+  // The descriptor buffer is reused by the next program: wait for the previous one.
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  char* base = static_cast<char*>(c.groups.p);
+  if (gb) hip_check(hipMemcpyAsync(base, gemms_.data(), gb, hipMemcpyHostToDevice, s), "upload gemm groups");
+  if (tb) hip_check(hipMemcpyAsync(base + toff, traces_.data(), tb, hipMemcpyHostToDevice, s), "upload trace groups");
+  // host vectors may be pageable: make sure the copies have landed before they can change
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  const LcGemmGroup* dg = reinterpret_cast<const LcGemmGroup*>(base);
+  const LcTraceGroup* dt = reinterpret_cast<const LcTraceGroup*>(base + toff);
+  for (const Level& l : levels_) {
+    if (l.count == 0) continue;
+    if (l.total_wg > 0x7fffffffLL) fail(LC_ERR_ARG, "launch too large (%lld workgroups)", l.total_wg);
+    hipError_t e = l.is_trace
+                       ? lc_launch_trace(dbl_, dt + l.first, l.count, l.total_wg, nv_, s)
+                       : lc_launch_gemm(dbl_, dg + l.first, l.count, l.total_wg, nv_, s);
+    hip_check(e, l.is_trace ? "trace launch" : "gemm launch");
+  }
+}
+
+}  // namespace lc

@@ -1,0 +1,115 @@
+// Host runtime of the contraction library: device context, persistent buffers, and the
+// "program" builder that turns D-space chains into grouped GEMM / trace launches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "lc_internal.h"
+#include "../../include/lapH_contract.h"
+
+extern "C" hipError_t lc_launch_gemm(int is_double, const LcGemmGroup* d_groups, int ng,
+                                     long long total_wg, int nv, hipStream_t s);
+extern "C" hipError_t lc_launch_trace(int is_double, const LcTraceGroup* d_groups, int ng,
+                                      long long total_wg, int nv, hipStream_t s);
+extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b,
+                                              void* d, hipStream_t s);
+extern "C" int lc_gemm_tile(int nv);
+
+namespace lc {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error{code, buf};
+}
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(LC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) {
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      hip_check(hipFree(p), "hipFree");
+      p = nullptr;
+      cap = 0;
+    }
+    size_t n = bytes < (1u << 20) ? (1u << 20) : bytes;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      fail(LC_ERR_NOMEM, "hipMalloc of %zu bytes failed", n);
+    }
+    cap = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Context {
+  bool on = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  DevBuf groups, scratch, out;
+};
+
+Context& ctx();
+void require_init();
+
+// A sequence of levels; each level is one grouped GEMM launch or one grouped trace launch.
+// Groups inside a level are independent; levels run in order on the library stream.
+class Program {
+ public:
+  explicit Program(int nv, bool dbl) : nv_(nv), dbl_(dbl) {}
+
+  void gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
+            long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0);
+  void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
+             long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
+             bool accumulate = false);
+  // Close the current level.
+  void barrier();
+  // Upload descriptors and enqueue every level on the stream.
+  void run(hipStream_t s);
+
+  double gemm_flops() const { return gemm_flops_; }
+  double trace_flops() const { return trace_flops_; }
+
+ private:
+  struct Level {
+    bool is_trace;
+    bool closed;
+    size_t first;
+    int count;
+    long long total_wg;
+  };
+  int nv_;
+  bool dbl_;
+  std::vector<LcGemmGroup> gemms_;
+  std::vector<LcTraceGroup> traces_;
+  std::vector<Level> levels_;
+  double gemm_flops_ = 0, trace_flops_ = 0;
+  Level* open_level(bool is_trace);
+};
+
+}  // namespace lc

@@ -1,0 +1,133 @@
+"""Torch-facing wrappers of the sink-row launchers of the C-ABI.
+
+Operands are CUDA tensors (complex128 -> LC_DOUBLE, complex64 -> LC_SINGLE) in the reference's
+device layouts; results come back as numpy arrays in the layout the reference launchers write.
+Shapes are checked on the host before any launch.
+"""
+import ctypes
+
+import numpy as np
+
+from ._capi import DOUBLE, SINGLE, check, lib
+
+
+def init(device=0):
+    if not lib().lc_is_init():
+        check(lib().lc_init(device))
+
+
+def finalize():
+    if lib().lc_is_init():
+        check(lib().lc_finalize())
+
+
+def _prec(t):
+    import torch
+    if t.dtype == torch.complex128:
+        return DOUBLE, np.complex128
+    if t.dtype == torch.complex64:
+        return SINGLE, np.complex64
+    raise TypeError("operands must be complex64 or complex128, got %s" % t.dtype)
+
+
+def _p(t, shape, dtype):
+    if not t.is_cuda:
+        raise ValueError("operand must be a CUDA tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError("operand shape %s, expected %s" % (tuple(t.shape), tuple(shape)))
+    if t.dtype != dtype:
+        raise TypeError("mixed precisions")
+    if not t.is_contiguous():
+        raise ValueError("operand must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _out(shape, npdt):
+    o = np.zeros(shape, npdt)
+    return o, ctypes.c_void_p(o.ctypes.data)
+
+
+def _dims(row):
+    if row.dim() != 5 or row.shape[1:3] != (4, 4) or row.shape[3] != row.shape[4]:
+        raise ValueError("row must be [Nt][4][4][Nv][Nv]")
+    return row.shape[0], row.shape[-1]
+
+
+def rho_row(row, mom5, tf):
+    nt, nv = _dims(row)
+    prec, npdt = _prec(row)
+    out, po = _out((nt, 16, 3), npdt)
+    check(lib().lc_rho_row(prec, _p(row, row.shape, row.dtype), _p(mom5, (nt, 5, nv, nv), row.dtype),
+                           tf, nt, nv, po))
+    return out
+
+
+def rhopi_row(row, diag, p1, p2, p12x5, tf, idir):
+    nt, nv = _dims(row)
+    prec, npdt = _prec(row)
+    m = (nt, nv, nv)
+    out, po = _out((nt, 4), npdt)
+    check(lib().lc_rhopi_row(prec, _p(row, row.shape, row.dtype), _p(diag, row.shape, row.dtype),
+                             _p(p1, m, row.dtype), _p(p2, m, row.dtype),
+                             _p(p12x5, (nt, 5, nv, nv), row.dtype), tf, nt, nv, idir, po))
+    return out
+
+
+def pipi_row(iso, row, diag, m1, m2, m3, m4, tf):
+    nt, nv = _dims(row)
+    prec, npdt = _prec(row)
+    m = (nt, nv, nv)
+    out, po = _out((nt, 5), npdt)
+    check(lib().lc_pipi_row(prec, iso, _p(row, row.shape, row.dtype), _p(diag, row.shape, row.dtype),
+                            _p(m1, m, row.dtype), _p(m2, m, row.dtype), _p(m3, m, row.dtype),
+                            _p(m4, m, row.dtype), tf, nt, nv, po))
+    return out
+
+
+def loop_unity(diag, mom):
+    nt, nv = _dims(diag)
+    prec, npdt = _prec(diag)
+    out, po = _out((nt,), npdt)
+    check(lib().lc_loop_unity(prec, _p(diag, diag.shape, diag.dtype), _p(mom, (nt, nv, nv), diag.dtype),
+                              nt, nv, po))
+    return out
+
+
+def sigma_row(row, x2, tf):
+    nt, nv = _dims(row)
+    prec, npdt = _prec(row)
+    out, po = _out((nt, 4), npdt)
+    check(lib().lc_sigma_row(prec, _p(row, row.shape, row.dtype), _p(x2, (nt, 2, nv, nv), row.dtype),
+                             tf, nt, nv, po))
+    return out
+
+
+def sigmapi_fish_hor(diag, p1, p2):
+    nt, nv = _dims(diag)
+    prec, npdt = _prec(diag)
+    m = (nt, nv, nv)
+    out, po = _out((nt,), npdt)
+    check(lib().lc_sigmapi_fish_hor(prec, _p(diag, diag.shape, diag.dtype), _p(p1, m, diag.dtype),
+                                    _p(p2, m, diag.dtype), nt, nv, po))
+    return out
+
+
+def sigmapi_triangle_row(row, diag, p1, p2, x2, tf):
+    nt, nv = _dims(row)
+    prec, npdt = _prec(row)
+    m = (nt, nv, nv)
+    out, po = _out((nt, 2), npdt)
+    check(lib().lc_sigmapi_triangle_row(prec, _p(row, row.shape, row.dtype), _p(diag, row.shape, row.dtype),
+                                        _p(p1, m, row.dtype), _p(p2, m, row.dtype),
+                                        _p(x2, (nt, 2, nv, nv), row.dtype), tf, nt, nv, po))
+    return out
+
+
+def selftest_mfma(a, b, prec=DOUBLE):
+    """A (16x4) * B (4x16) on one matrix-core instruction; returns the 16x16 product."""
+    dt = np.float64 if prec == DOUBLE else np.float32
+    a = np.ascontiguousarray(a, dt)
+    b = np.ascontiguousarray(b, dt)
+    d = np.zeros((16, 16), dt)
+    check(lib().lc_selftest_mfma(prec, a.ctypes.data, b.ctypes.data, d.ctypes.data))
+    return d

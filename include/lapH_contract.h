@@ -1,0 +1,89 @@
+/* C-ABI of the MI355X LapH meson contraction library (libcontract_hip.so).
+ *
+ * Plain pointers and sizes only. Every function returns 0 on success or a negative LC_ERR_*
+ * code; lc_last_error() returns the message of the last failure. "d_" pointers are device
+ * memory (hipMalloc / torch), "h_" pointers host memory. Complex numbers are interleaved
+ * (re, im) of the compute type: prec = LC_SINGLE -> float, LC_DOUBLE -> double, the reference's
+ * PRECISION enum (include/contract.h:33).
+ *
+ * Input layouts are the reference's device layouts (lib/contract.cpp:171-219):
+ *   row   tau(tf, .)          [nt][4][4][nv][nv]   one sink row of the perambulator
+ *   diag  tau(t, t)           [nt][4][4][nv][nv]   (lib/contract.cpp:549-570)
+ *   mom5  Mom_matrices        [nt][5][nv][nv]
+ *   x2    Mom_matrices_x2     [nt][2][nv][nv]
+ *   mom, p1, p2, m1..m4       [nt][nv][nv]
+ * Outputs land in host arrays with the layouts the reference launchers write into h_elem.
+ */
+#ifndef LAPH_CONTRACT_H
+#define LAPH_CONTRACT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { LC_SINGLE = 0, LC_DOUBLE = 1 };
+enum { LC_I0 = 0, LC_I1 = 1 };
+
+enum {
+  LC_OK = 0,
+  LC_ERR_NOT_INIT = -1,   /* library not initialised (lib/contract.cpp:168)            */
+  LC_ERR_ARG = -2,        /* bad size, precision, index or null pointer                 */
+  LC_ERR_HIP = -3,        /* HIP runtime error (message carries hipGetErrorString)      */
+  LC_ERR_ALREADY = -4,    /* initialised twice (lib/contract.cpp:62-63)                 */
+  LC_ERR_NODEV = -5,      /* no GPU or bad device id (lib/contract.cpp:33-49)           */
+  LC_ERR_NOMEM = -6,      /* device allocation failed                                   */
+  LC_ERR_IO = -7          /* file missing, header mismatch or short read (:107-130)     */
+};
+
+/* Library state. Replaces initContract/initDevice/destroyContract (lib/contract.cpp:25-88)
+ * for the kernel layer: selects the device, creates the stream, owns persistent scratch. */
+int lc_init(int device);
+int lc_finalize(void);
+int lc_is_init(void);
+const char* lc_last_error(void);
+int lc_device_count(void);
+
+/* Sink-row launchers. Each replaces one reference run_* function (include/contract.h:50-69)
+ * with device pointers in place of texture objects, and writes the spin-summed,
+ * diagram-weighted per-source values of one sink row tf into host memory. */
+
+/* run_ContractRho (lib/contract_kernels_rho.cu:126-138): h_corr [nt][16][3] complex. */
+int lc_rho_row(int prec, const void* d_row, const void* d_mom5, int tf, int nt, int nv,
+               void* h_corr);
+
+/* run_ContractRhoPi (lib/contract_kernels_rhopi.cu:125-137): h_corr [nt][4] complex. */
+int lc_rhopi_row(int prec, const void* d_row, const void* d_diag, const void* d_p1,
+                 const void* d_p2, const void* d_p12x5, int tf, int nt, int nv, int idir,
+                 void* h_corr);
+
+/* run_ContractPiPi / run_ContractPiPi_I0 (lib/contract_kernels_pipi.cu:507-533):
+ * iso = LC_I1 or LC_I0; h_corr [nt][5] complex = weighted {square, doubleTriangle, star, fish,
+ * doubleTriangleHor (I0 only, else 0)} (lib/contract_kernels_pipi.cu:288-303, 463-478). */
+int lc_pipi_row(int prec, int iso, const void* d_row, const void* d_diag, const void* d_m1,
+                const void* d_m2, const void* d_m3, const void* d_m4, int tf, int nt, int nv,
+                void* h_corr);
+
+/* run_ContractLoopUnity (lib/contract_kernels_loop_unity.cu:95-107): h_loop [nt] complex. */
+int lc_loop_unity(int prec, const void* d_diag, const void* d_mom, int nt, int nv, void* h_loop);
+
+/* run_ContractSigma (lib/contract_kernels_sigma.cu:134-147): h_corr [nt][4] complex. */
+int lc_sigma_row(int prec, const void* d_row, const void* d_x2, int tf, int nt, int nv,
+                 void* h_corr);
+
+/* run_ContractSigmaPi_fish_hor (lib/contract_kernels_sigmapi.cu:190-202): h_corr [nt]. */
+int lc_sigmapi_fish_hor(int prec, const void* d_diag, const void* d_p1, const void* d_p2,
+                        int nt, int nv, void* h_corr);
+
+/* run_ContractSigmaPi_triangle (lib/contract_kernels_sigmapi.cu:204-217): h_corr [nt][2]. */
+int lc_sigmapi_triangle_row(int prec, const void* d_row, const void* d_diag, const void* d_p1,
+                            const void* d_p2, const void* d_x2, int tf, int nt, int nv,
+                            void* h_corr);
+
+/* Diagnostic: one 16x16x4 matrix-core product A(16x4)*B(4x16) row-major -> h_d (16x16),
+ * used by the tests to pin the operand layout the kernels assume. No reference counterpart. */
+int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

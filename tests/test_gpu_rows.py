@@ -1,0 +1,116 @@
+"""GPU parity of the sink-row launchers (C-ABI) against the oracle's row-by-row restatement.
+
+FP64 bar: max |gpu - oracle| <= 1e-12 * max |oracle| per output array, and per element
+|gpu - oracle| <= 1e-12 * max(|oracle_elem|, 1e-3 * max|oracle|).
+FP32 (PRECISION SINGLE) bar: 2e-5 normwise against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from contractions_code_scattering_amd import inputs as I
+from conftest import elemerr, relerr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SIZES = [(4, 3), (8, 4), (25, 3), (33, 2), (64, 2)]
+
+
+def dev(x, dtype=None):
+    import torch
+    x = np.ascontiguousarray(x if dtype is None else x.astype(dtype))
+    return torch.from_numpy(x).to("cuda")
+
+
+def _case(seed, nt, nv):
+    g = I.rng(seed)
+    tau = I.perambulator(g, nt, nv, 0.3)
+    mats = [I.random_elemental(g, nt, nv) for _ in range(4)]
+    mom5 = np.stack([I.hermitian_elemental(g, nt, nv) for _ in range(5)], axis=1)
+    x2 = np.stack([I.random_elemental(g, nt, nv) for _ in range(2)], axis=1)
+    return tau, mats, np.ascontiguousarray(mom5), np.ascontiguousarray(x2)
+
+
+def _ok(got, ref, tol=TOL):
+    assert relerr(got, ref) <= tol, relerr(got, ref)
+    assert elemerr(got, ref) <= tol, elemerr(got, ref)
+
+
+def test_mfma_layout(lc):
+    g = I.rng(0)
+    a = g.standard_normal((16, 4))
+    b = g.standard_normal((4, 16))
+    d = lc.selftest_mfma(a, b)
+    np.testing.assert_allclose(d, a @ b, rtol=1e-14, atol=1e-14)
+    d32 = lc.selftest_mfma(a, b, prec=0)
+    np.testing.assert_allclose(d32, a @ b, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("nv,nt", SIZES + [(100, 2)])
+def test_rho_row(lc, nv, nt):
+    tau, _, mom5, _ = _case(1, nt, nv)
+    for tf in sorted({0, nt - 1}):
+        got = lc.rho_row(dev(tau[tf]), dev(mom5), tf)
+        _ok(got, O.rho_row(tau[tf], mom5, tf))
+
+
+@pytest.mark.parametrize("nv,nt", SIZES)
+@pytest.mark.parametrize("idir", [0, 2])
+def test_rhopi_row(lc, nv, nt, idir):
+    tau, (p1, p2, _, _), mom5, _ = _case(2, nt, nv)
+    diag = I.diag_of(tau)
+    tf = nt - 1
+    got = lc.rhopi_row(dev(tau[tf]), dev(diag), dev(p1), dev(p2), dev(mom5), tf, idir)
+    _ok(got, O.rhopi_row(tau[tf], diag, p1, p2, mom5, tf, idir))
+
+
+@pytest.mark.parametrize("nv,nt", SIZES)
+@pytest.mark.parametrize("iso", [0, 1])
+def test_pipi_row(lc, nv, nt, iso):
+    tau, (m1, m2, m3, m4), _, _ = _case(3, nt, nv)
+    diag = I.diag_of(tau)
+    tf = 1 % nt
+    got = lc.pipi_row(iso, dev(tau[tf]), dev(diag), dev(m1), dev(m2), dev(m3), dev(m4), tf)
+    ref = O.pipi_row(iso, tau[tf], diag, m1, m2, m3, m4, tf)
+    for k in range(5 if iso == 0 else 4):
+        _ok(got[:, k], ref[:, k])
+    if iso == 1:
+        assert np.all(got[:, 4] == 0)
+
+
+@pytest.mark.parametrize("nv,nt", SIZES)
+def test_loop_sigma_sigmapi(lc, nv, nt):
+    tau, (p1, p2, m, _), _, x2 = _case(4, nt, nv)
+    diag = I.diag_of(tau)
+    _ok(lc.loop_unity(dev(diag), dev(m)), O.loop_unity(diag, m))
+    tf = nt // 2
+    _ok(lc.sigma_row(dev(tau[tf]), dev(x2), tf), O.sigma_row(tau[tf], x2, tf))
+    _ok(lc.sigmapi_fish_hor(dev(diag), dev(p1), dev(p2)), O.sigmapi_fish_hor(diag, p1, p2))
+    _ok(lc.sigmapi_triangle_row(dev(tau[tf]), dev(diag), dev(p1), dev(p2), dev(x2), tf),
+        O.sigmapi_triangle_row(tau[tf], diag, p1, p2, x2, tf))
+
+
+def test_single_precision(lc):
+    nv, nt = 25, 3
+    tau, (m1, m2, m3, m4), mom5, x2 = _case(5, nt, nv)
+    diag = I.diag_of(tau)
+    c64 = np.complex64
+    got = lc.rho_row(dev(tau[0], c64), dev(mom5, c64), 0)
+    assert got.dtype == np.complex64
+    assert relerr(got, O.rho_row(tau[0], mom5, 0)) < 2e-5
+    got = lc.pipi_row(0, dev(tau[1], c64), dev(diag, c64), dev(m1, c64), dev(m2, c64), dev(m3, c64),
+                      dev(m4, c64), 1)
+    ref = O.pipi_row(0, tau[1], diag, m1, m2, m3, m4, 1)
+    for k in range(5):
+        assert relerr(got[:, k], ref[:, k]) < 2e-5
+
+
+def test_errors(lc):
+    from contractions_code_scattering_amd import LcError
+    nv, nt = 4, 2
+    tau, _, mom5, _ = _case(6, nt, nv)
+    with pytest.raises(LcError) as e:
+        lc.rho_row(dev(tau[0]), dev(mom5), nt)  # tf out of range
+    assert e.value.code == -2
+    with pytest.raises(ValueError):
+        lc.rho_row(dev(tau[0]), dev(mom5[:, :4]), 0)  # host-side shape check
