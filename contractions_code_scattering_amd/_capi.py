@@ -36,6 +36,13 @@ _SIGS = {
     "lc_sigmapi_fish_hor": [_I, _VP, _VP, _VP, _I, _I, _VP],
     "lc_sigmapi_triangle_row": [_I, _VP, _VP, _VP, _VP, _VP, _I, _I, _I, _VP],
     "lc_selftest_mfma": [_I, _VP, _VP, _VP],
+    "lc_contract_init": [_I, _I, _I], "lc_contract_destroy": [], "lc_set_binary_output": [_I],
+    "lc_set_i_dir": [_I], "lc_read_prop": [ctypes.c_char_p, _I, ctypes.POINTER(_VP)],
+    "lc_calculate_rho": [_I, _I, _VP, _VP, _I, ctypes.c_char_p],
+    "lc_calculate_rhopi": [_I, _I, _I, _VP, _VP, _VP, _VP, _I, ctypes.c_char_p],
+    "lc_calculate_pipi": [_I, _I, _VP, _VP, _VP, _VP, _VP, _I, ctypes.c_char_p, _I],
+    "lc_calculate_sigma": [_I, _I, _VP, _VP, _I, ctypes.c_char_p],
+    "lc_calculate_sigmapi": [_I, _I, _VP, _VP, _VP, _VP, _I, ctypes.c_char_p],
 }
 
 
@@ -49,6 +56,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.lc_last_error.restype = ctypes.c_char_p
         L.lc_last_error.argtypes = []
+        L.lc_free_host.restype = None
+        L.lc_free_host.argtypes = [_VP]
         for name, args in _SIGS.items():
             f = getattr(L, name)
             f.argtypes = args

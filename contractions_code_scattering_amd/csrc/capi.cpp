@@ -35,18 +35,16 @@ using namespace lc;
 
 namespace {
 
-std::string g_err;
-
 template <typename F>
 int guarded(F&& f) {
   try {
     f();
     return LC_OK;
   } catch (const Error& e) {
-    g_err = e.msg;
+    last_error() = e.msg;
     return e.code;
   } catch (const std::exception& e) {
-    g_err = e.what();
+    last_error() = e.what();
     return LC_ERR_ARG;
   }
 }
@@ -339,7 +337,7 @@ void triangle_row(const Shape& S, const void* row, const void* diag, const void*
 // =========================================================================== C-ABI entry points
 extern "C" {
 
-const char* lc_last_error(void) { return g_err.c_str(); }
+const char* lc_last_error(void) { return last_error().c_str(); }
 
 int lc_is_init(void) { return ctx().on ? 1 : 0; }
 

@@ -1,0 +1,193 @@
+// Batched engine: every (sink tf, source t0, momentum p) chain of the rho + pi-pi correlator set
+// over a perambulator resident in HBM, as a few grouped launches per momentum chunk.
+//
+// Inputs (FP64 complex):
+//   tau  [nt][nt][4][4][nv][nv]   tau(t, t0), the reference's host layout (lib/contract.cpp:140)
+//   phi  [np][nt][nv][nv]         Phi(p, t); Phi(-p, t) = Phi(p, t)^dag is read through a view
+// Sources t0_k = src0 + k * src_step, k < nsrc.
+//
+// Per (p, tf, t0), with P1 = P4 = Phi(p), P2 = P3 = Phi(p)^dag (back-to-back pions, the driver
+// convention of tests/compute_pipi_I0.cpp:143-146), rho from the op pair (l,k) = (0,0) of
+// lib/contract_kernels_rho.cu with Gamma = gamma_i:
+//   rho_i  = -Tr[ B^dag (g5 g_i) A (-g_i g5) ]     A = tau Phi(t0),  B = Phi(tf) tau
+//   square = Tr[ U V ],  dT = Tr[ U V' ]            U = Q (g5 tau g5), Q = Phi^dag D(tf) Phi,
+//                                                   V = R tau^dag, R = Phi^dag D(t0) Phi,
+//                                                   V' = R' tau^dag, R' = Phi D(t0) Phi^dag
+//   dTHor  = Tr[ (L W) (Rr tau^dag) ]               L = Phi^dag tau Phi, Rr = Phi tau Phi^dag
+//   star   = Tr[B S1] * Tr[L1 S2], fish = Tr[L1 S1] * Tr[B S2]
+//                                                   S1 = Phi tau^dag, S2 = Phi^dag tau^dag,
+//                                                   L1 = Phi^dag tau
+// Per-time prefixes (Q, R, R') are formed once per (t, p) and shared by every pair.
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "runtime.h"
+#include "spin.h"
+
+extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
+                                                int nsrc, int nt, int src0, int src_step,
+                                                hipStream_t s);
+
+using namespace lc;
+
+namespace {
+
+struct EngineShape {
+  int nt, nv, np, src0, step, nsrc;
+  long long m, blk;
+};
+
+inline LcView dense(const void* p, long long s0, long long s1, long long s2, int flags = 0) {
+  return make_view(p, s0, s1, s2, LC_DENSE, flags);
+}
+inline LcView kron(const void* p, long long s0, long long s1, long long s2, int flags = 0) {
+  return make_view(p, s0, s1, s2, LC_KRON, flags);
+}
+
+const size_t ESZ = 16;  // complex double
+
+inline const char* off(const void* p, long long elems) {
+  return static_cast<const char*>(p) + (size_t)elems * ESZ;
+}
+inline char* off(char* p, long long elems) { return p + (size_t)elems * ESZ; }
+
+// Number of D-space scratch matrices per (tf, t0, p) item and per (t, p) item.
+const int PAIR_BUFS = 12;
+const int TIME_BUFS = 6;
+
+void run_chunk(const EngineShape& E, const void* tau, const void* phi, int p0, int pc, char* scratch,
+               void* raw) {
+  const int nt = E.nt, ns = E.nsrc;
+  const long long m = E.m, b = E.blk;
+  const long long npair = (long long)nt * ns;
+  // pair buffers [pc][nsrc][nt] -> strides {b, nt*b, nsrc*nt*b}
+  char* pb[PAIR_BUFS];
+  for (int i = 0; i < PAIR_BUFS; i++) pb[i] = off(scratch, (long long)i * pc * npair * b);
+  char* tb0 = off(scratch, (long long)PAIR_BUFS * pc * npair * b);
+  char *A = pb[0], *B = pb[1], *L1 = pb[2], *S1 = pb[3], *S2 = pb[4], *L = pb[5], *Rr = pb[6],
+       *U = pb[7], *V = pb[8], *Vp = pb[9], *LW = pb[10], *Zh = pb[11];
+  // per-sink buffers [pc][nt]: strides {b, 0, nt*b}; per-source buffers [pc][nsrc]: {0, b, ns*b}
+  char* Qa = tb0;
+  char* Q = off(tb0, (long long)pc * nt * b);
+  char* T1 = off(tb0, (long long)2 * pc * nt * b);
+  char* T1p = off(T1, (long long)pc * ns * b);
+  char* R = off(T1, (long long)2 * pc * ns * b);
+  char* Rp = off(T1, (long long)3 * pc * ns * b);
+
+  const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
+  const LcView tau_v = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0);
+  const LcView tau_dag = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
+  const LcView W = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
+  const char* phip = off(phi, (long long)p0 * nt * m);
+  const LcView phi_tf = kron(phip, m, 0, nt * m);
+  const LcView phi_tf_dag = kron(phip, m, 0, nt * m, LC_CONJ | LC_TV);
+  const LcView phi_t0 = kron(off(phip, (long long)E.src0 * m), 0, E.step * m, nt * m);
+  const LcView phi_t0_dag = kron(off(phip, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  const LcView D_tf = dense(tau, (nt + 1) * b, 0, 0);
+  const LcView D_t0 = dense(off(tau, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0);
+  auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
+
+  Program pg(E.nv, true);
+  // level 1: elemental x perambulator products and the per-time D Phi products
+  pg.gemm(D_tf, phi_tf, Qa, b, 0, nt * b, nt, 1, pc);
+  pg.gemm(D_t0, phi_t0, T1, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(D_t0, phi_t0_dag, T1p, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(tau_v, phi_t0, A, P0, P1, P2, nt, ns, pc);
+  pg.gemm(phi_tf, tau_v, B, P0, P1, P2, nt, ns, pc);
+  pg.gemm(phi_tf_dag, tau_v, L1, P0, P1, P2, nt, ns, pc);
+  pg.gemm(phi_t0, tau_dag, S1, P0, P1, P2, nt, ns, pc);
+  pg.gemm(phi_t0_dag, tau_dag, S2, P0, P1, P2, nt, ns, pc);
+  pg.barrier();
+  // level 2
+  pg.gemm(phi_tf_dag, dense(Qa, b, 0, nt * b), Q, b, 0, nt * b, nt, 1, pc);
+  pg.gemm(phi_t0_dag, dense(T1, 0, b, ns * b), R, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(phi_t0, dense(T1p, 0, b, ns * b), Rp, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(pair(L1), phi_t0, L, P0, P1, P2, nt, ns, pc);
+  pg.gemm(pair(B), phi_t0_dag, Rr, P0, P1, P2, nt, ns, pc);
+  pg.barrier();
+  // level 3: dense x dense
+  const Spin g5 = gamma(5);
+  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_v, g5), g5), U, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(R, 0, b, ns * b), tau_dag, V, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(Rp, 0, b, ns * b), tau_dag, Vp, P0, P1, P2, nt, ns, pc);
+  pg.gemm(pair(L), W, LW, P0, P1, P2, nt, ns, pc);
+  pg.gemm(pair(Rr), tau_dag, Zh, P0, P1, P2, nt, ns, pc);
+  pg.barrier();
+  // closing traces -> raw [np][nsrc][nt][10]
+  char* rw = off(static_cast<char*>(raw), (long long)p0 * ns * nt * 10);
+  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  for (int i = 0; i < 3; i++) {
+    const Spin Lp = g5 * rhoj(1, i, false);
+    const Spin Kp = rhoj(1, i, true) * g5;
+    pg.trace(pair(B, LC_DAG), right(left(pair(A), Lp), Kp), off(rw, i), O0, O1, O2, nt, ns, pc, -1.0, 0.0);
+  }
+  pg.trace(pair(U), pair(V), off(rw, 3), O0, O1, O2, nt, ns, pc);
+  pg.trace(pair(U), pair(Vp), off(rw, 4), O0, O1, O2, nt, ns, pc);
+  pg.trace(pair(LW), pair(Zh), off(rw, 5), O0, O1, O2, nt, ns, pc);
+  pg.trace(pair(B), pair(S1), off(rw, 6), O0, O1, O2, nt, ns, pc);
+  pg.trace(pair(L1), pair(S2), off(rw, 7), O0, O1, O2, nt, ns, pc);
+  pg.trace(pair(L1), pair(S1), off(rw, 8), O0, O1, O2, nt, ns, pc);
+  pg.trace(pair(B), pair(S2), off(rw, 9), O0, O1, O2, nt, ns, pc);
+  pg.run(ctx().stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Engine entry point; see include/lapH_contract.h.
+int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                       int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                       void* d_corr, void* d_avg) {
+  try {
+    require_init();
+    if (nt < 1 || nv < 1 || nv > 1024 || np < 1) fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d np=%d", nt, nv, np);
+    if (nsrc < 1 || src_step < 1 || src0 < 0 || src0 + (long long)(nsrc - 1) * src_step >= nt)
+      fail(LC_ERR_ARG, "bad source list src0=%d step=%d nsrc=%d (nt=%d)", src0, src_step, nsrc, nt);
+    if (!d_tau || !d_phi || !d_raw || !d_corr || !d_avg) fail(LC_ERR_ARG, "null pointer");
+    EngineShape E{nt, nv, np, src0, src_step, nsrc, (long long)nv * nv, 16LL * nv * nv};
+    const long long npair = (long long)nt * nsrc;
+    const size_t per_p = (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * ESZ;
+    if (max_scratch_bytes <= 0) max_scratch_bytes = 64LL << 30;
+    int pc = (int)(max_scratch_bytes / (long long)per_p);
+    if (pc < 1) fail(LC_ERR_NOMEM, "scratch budget %lld B below one momentum (%zu B)", max_scratch_bytes, per_p);
+    if (pc > np) pc = np;
+    ctx().scratch.ensure(per_p * pc + 4096);
+    char* scratch = static_cast<char*>(ctx().scratch.p);
+    for (int p0 = 0; p0 < np; p0 += pc) {
+      const int n = (np - p0) < pc ? (np - p0) : pc;
+      run_chunk(E, d_tau, d_phi, p0, n, scratch, d_raw);
+    }
+    hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, ctx().stream),
+              "engine epilogue");
+    hip_check(hipStreamSynchronize(ctx().stream), "hipStreamSynchronize");
+    return LC_OK;
+  } catch (const Error& e) {
+    last_error() = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    last_error() = e.what();
+    return LC_ERR_ARG;
+  }
+}
+
+int lc_profile(int enable) {
+  ctx().profile = enable != 0;
+  ctx().times.clear();
+  return LC_OK;
+}
+
+int lc_profile_read(double* out /* [6] */) {
+  if (!out) return LC_ERR_ARG;
+  const KernelTimes& t = ctx().times;
+  out[0] = t.gemm_ms;
+  out[1] = (double)t.gemm_launches;
+  out[2] = t.gemm_flops;
+  out[3] = t.trace_ms;
+  out[4] = (double)t.trace_launches;
+  out[5] = t.trace_flops;
+  return LC_OK;
+}
+
+}  // extern "C"

@@ -389,3 +389,53 @@ extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, cons
                        (const float*)a, (const float*)b, (float*)d);
   return hipGetLastError();
 }
+
+// ------------------------------------------------------------------------------------------
+// Engine epilogue (FP64): per (p, source k, sink tf) combine the raw traces of the rho / pi-pi
+// set into correlators, then average over sources on the device.
+//   raw  [np][nsrc][nt][10] = {rho_0, rho_1, rho_2, square, dT, dTHor, st1, st2, fi1, fi2}
+//   corr [np][nsrc][nt][8]  = {rho_0, rho_1, rho_2, square, dT, dTHor, st1*st2, fi1*fi2}
+//   avg  [np][8][nt]        C(dt) = 1/nsrc sum_k corr[p][k][(t0_k + dt) % nt]
+namespace lc {
+__global__ void lc_engine_combine_kernel(const double2* __restrict__ raw, double2* __restrict__ corr,
+                                         long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2* r = raw + i * 10;
+  double2* c = corr + i * 8;
+#pragma unroll
+  for (int d = 0; d < 6; d++) c[d] = r[d];
+  c[6].x = r[6].x * r[7].x - r[6].y * r[7].y;
+  c[6].y = r[6].x * r[7].y + r[6].y * r[7].x;
+  c[7].x = r[8].x * r[9].x - r[8].y * r[9].y;
+  c[7].y = r[8].x * r[9].y + r[8].y * r[9].x;
+}
+
+__global__ void lc_engine_average_kernel(const double2* __restrict__ corr, double2* __restrict__ avg,
+                                         int np, int nsrc, int nt, int ncorr, int src0, int src_step) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np * ncorr * nt) return;
+  const int dt = i % nt, d = (i / nt) % ncorr, p = i / (nt * ncorr);
+  double sr = 0, si = 0;
+  for (int k = 0; k < nsrc; k++) {
+    const int t0 = src0 + k * src_step;
+    const int tf = (t0 + dt) % nt;
+    const double2 v = corr[(((long long)p * nsrc + k) * nt + tf) * ncorr + d];
+    sr += v.x;
+    si += v.y;
+  }
+  avg[i] = make_double2(sr / nsrc, si / nsrc);
+}
+}  // namespace lc
+
+extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
+                                                int nsrc, int nt, int src0, int src_step,
+                                                hipStream_t s) {
+  const long long n = (long long)np * nsrc * nt;
+  hipLaunchKernelGGL(lc::lc_engine_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     (const double2*)raw, (double2*)corr, n);
+  const int m = np * 8 * nt;
+  hipLaunchKernelGGL(lc::lc_engine_average_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
+                     (const double2*)corr, (double2*)avg, np, nsrc, nt, 8, src0, src_step);
+  return hipGetLastError();
+}

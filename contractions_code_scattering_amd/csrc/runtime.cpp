@@ -10,6 +10,11 @@ Context& ctx() {
   return c;
 }
 
+std::string& last_error() {
+  static std::string e;
+  return e;
+}
+
 void require_init() {
   if (!ctx().on) fail(LC_ERR_NOT_INIT, "contraction library is not initialised (call lc_init)");
 }
@@ -92,13 +97,39 @@ void Program::run(hipStream_t s) {
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   const LcGemmGroup* dg = reinterpret_cast<const LcGemmGroup*>(base);
   const LcTraceGroup* dt = reinterpret_cast<const LcTraceGroup*>(base + toff);
+  const bool prof = c.profile;
+  size_t nev = 0;
+  if (prof) {
+    while (c.ev_pool.size() < 2 * levels_.size()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      c.ev_pool.push_back(e);
+    }
+  }
   for (const Level& l : levels_) {
     if (l.count == 0) continue;
     if (l.total_wg > 0x7fffffffLL) fail(LC_ERR_ARG, "launch too large (%lld workgroups)", l.total_wg);
+    if (prof) hip_check(hipEventRecord(c.ev_pool[nev], s), "hipEventRecord");
     hipError_t e = l.is_trace
                        ? lc_launch_trace(dbl_, dt + l.first, l.count, l.total_wg, nv_, s)
                        : lc_launch_gemm(dbl_, dg + l.first, l.count, l.total_wg, nv_, s);
     hip_check(e, l.is_trace ? "trace launch" : "gemm launch");
+    if (prof) hip_check(hipEventRecord(c.ev_pool[nev + 1], s), "hipEventRecord");
+    nev += 2;
+  }
+  if (prof) {
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    size_t k = 0;
+    for (const Level& l : levels_) {
+      if (l.count == 0) continue;
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, c.ev_pool[k], c.ev_pool[k + 1]), "hipEventElapsedTime");
+      if (l.is_trace) { c.times.trace_ms += ms; c.times.trace_launches++; }
+      else { c.times.gemm_ms += ms; c.times.gemm_launches++; }
+      k += 2;
+    }
+    c.times.gemm_flops += gemm_flops_;
+    c.times.trace_flops += trace_flops_;
   }
 }
 

@@ -66,15 +66,28 @@ struct DevBuf {
   }
 };
 
+// Per-launch timing with HIP events on the library stream (enabled by lc_profile(1)).
+struct KernelTimes {
+  double gemm_ms = 0, trace_ms = 0, other_ms = 0;
+  long long gemm_launches = 0, trace_launches = 0;
+  double gemm_flops = 0, trace_flops = 0;
+  void clear() { *this = KernelTimes(); }
+};
+
 struct Context {
   bool on = false;
   int device = -1;
   hipStream_t stream = nullptr;
   DevBuf groups, scratch, out;
+  bool profile = false;
+  KernelTimes times;
+  std::vector<hipEvent_t> ev_pool;
 };
 
 Context& ctx();
 void require_init();
+// Message of the last failed C-ABI call (shared by every entry point).
+std::string& last_error();
 
 // A sequence of levels; each level is one grouped GEMM launch or one grouped trace launch.
 // Groups inside a level are independent; levels run in order on the library stream.

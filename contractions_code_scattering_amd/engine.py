@@ -1,0 +1,58 @@
+"""Torch-facing wrapper of the batched engine (lc_engine_rho_pipi).
+
+tau: CUDA complex128 [nt][nt][4][4][nv][nv]; phi: CUDA complex128 [np][nt][nv][nv].
+Returns (raw, corr, avg) CUDA tensors:
+  raw  [np][nsrc][nt][10]  {rho_0..2, square, dT, dTHor, star1, star2, fish1, fish2} traces
+  corr [np][nsrc][nt][8]   {rho_0..2, square, dT, dTHor, star, fish}
+  avg  [np][8][nt]         source average C(dt)
+"""
+import ctypes
+
+from ._capi import check, lib, register
+
+CORR_NAMES = ["rho_x", "rho_y", "rho_z", "square", "doubleTriangle", "doubleTriangleHor", "star", "fish"]
+N_CORR = len(CORR_NAMES)
+
+_VP, _I, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
+register("lc_engine_rho_pipi", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
+register("lc_profile", [_I])
+register("lc_profile_read", [_VP])
+
+
+def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None):
+    import torch
+    if tau.dtype != torch.complex128 or phi.dtype != torch.complex128:
+        raise TypeError("engine operands are complex128")
+    if not (tau.is_cuda and phi.is_cuda and tau.is_contiguous() and phi.is_contiguous()):
+        raise ValueError("engine operands must be contiguous CUDA tensors")
+    nt, nv = tau.shape[0], tau.shape[-1]
+    if tuple(tau.shape) != (nt, nt, 4, 4, nv, nv):
+        raise ValueError("tau must be [nt][nt][4][4][nv][nv]")
+    np_ = phi.shape[0]
+    if tuple(phi.shape) != (np_, nt, nv, nv):
+        raise ValueError("phi must be [np][nt][nv][nv]")
+    if nsrc is None:
+        nsrc = (nt - src0 + src_step - 1) // src_step
+    if out is None:
+        dev = tau.device
+        raw = torch.empty((np_, nsrc, nt, 10), dtype=torch.complex128, device=dev)
+        corr = torch.empty((np_, nsrc, nt, N_CORR), dtype=torch.complex128, device=dev)
+        avg = torch.empty((np_, N_CORR, nt), dtype=torch.complex128, device=dev)
+    else:
+        raw, corr, avg = out
+    torch.cuda.synchronize()
+    check(lib().lc_engine_rho_pipi(tau.data_ptr(), phi.data_ptr(), nt, nv, np_, src0, src_step, nsrc,
+                                   max_scratch_bytes, raw.data_ptr(), corr.data_ptr(), avg.data_ptr()))
+    return raw, corr, avg
+
+
+def profile(enable=True):
+    check(lib().lc_profile(1 if enable else 0))
+
+
+def profile_read():
+    import numpy as np
+    buf = np.zeros(6)
+    check(lib().lc_profile_read(buf.ctypes.data))
+    return dict(gemm_ms=buf[0], gemm_launches=int(buf[1]), gemm_flops=buf[2],
+                trace_ms=buf[3], trace_launches=int(buf[4]), trace_flops=buf[5])

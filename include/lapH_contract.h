@@ -79,6 +79,46 @@ int lc_sigmapi_triangle_row(int prec, const void* d_row, const void* d_diag, con
                             const void* d_p2, const void* d_x2, int tf, int nt, int nv,
                             void* h_corr);
 
+/* Batched engine (no reference counterpart; the batching form that replaces the reference's
+ * per-sink-row loops, lib/contract.cpp:602-605 and :775-779, for a resident perambulator):
+ * the rho (op pair (0,0), Gamma = gamma_i) + pi-pi (back-to-back Phi(p), Phi(p)^dag) set for
+ * every (sink tf, source t0 = src0 + k*src_step, momentum p), FP64.
+ *   d_tau [nt][nt][4][4][nv][nv], d_phi [np][nt][nv][nv]
+ *   d_raw  [np][nsrc][nt][10]  {rho_0..2, square, dT, dTHor, star1, star2, fish1, fish2} traces
+ *   d_corr [np][nsrc][nt][8]   {rho_0..2, square, dT, dTHor, star, fish} correlators
+ *   d_avg  [np][8][nt]         source-averaged C(dt), FP64 on the device
+ * Unweighted diagrams; max_scratch_bytes bounds the intermediate storage (<= 0: 64 GiB). */
+int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                       int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                       void* d_corr, void* d_avg);
+
+/* Kernel timing with HIP events on the library stream: lc_profile(1) clears and enables,
+ * lc_profile_read fills {gemm_ms, gemm_launches, gemm_flops, trace_ms, trace_launches,
+ * trace_flops} accumulated since. */
+int lc_profile(int enable);
+int lc_profile_read(double* out6);
+
+/* Whole-call drop-ins for FFI callers: the reference's initContract / destroyContract /
+ * readPropH / calculate* flows (lib/contract.cpp:58-160, lib/interface.cpp:393-426) with error
+ * codes in place of exit(). prop is the host buffer of readPropH ([nt][nt][4][4][nev][nev] of
+ * the precision's type), elementals are FP64 driver arrays [type][nt][nev][nev]; the text output
+ * is appended to outfile in the reference's format. lc_set_binary_output(1) also appends the
+ * printed values as raw doubles to <outfile>.bin. */
+int lc_contract_init(int nev, int nt, int dev);
+int lc_contract_destroy(void);
+int lc_set_binary_output(int on);
+int lc_set_i_dir(int idir);  /* the process-global direction the rho-pi text prints */
+int lc_read_prop(const char* path, int prec, void** out);
+void lc_free_host(void* p);
+int lc_calculate_rho(int nev, int nt, void* prop, double* mom, int prec, const char* outfile);
+int lc_calculate_rhopi(int idir, int nev, int nt, void* prop, double* p1, double* p2, double* p12,
+                       int prec, const char* outfile);
+int lc_calculate_pipi(int nev, int nt, void* prop, double* p1, double* p2, double* p3, double* p4,
+                      int prec, const char* outfile, int iso);
+int lc_calculate_sigma(int nev, int nt, void* prop, double* mom, int prec, const char* outfile);
+int lc_calculate_sigmapi(int nev, int nt, void* prop, double* p1, double* p2, double* p12, int prec,
+                         const char* outfile);
+
 /* Diagnostic: one 16x16x4 matrix-core product A(16x4)*B(4x16) row-major -> h_d (16x16),
  * used by the tests to pin the operand layout the kernels assume. No reference counterpart. */
 int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d);

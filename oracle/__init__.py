@@ -68,6 +68,17 @@ def rho_row(row, mom5, tf):
     return out
 
 
+def rho_row_range(row, mom5, tf, r0, r1):
+    """rho_row restricted to spin-table rows [r0, r1) (rows 0..47 = op pair (l,k) = (0,0))."""
+    row, mom5 = _cc(row), _cc(mom5)
+    nt, nv = row.shape[0], row.shape[-1]
+    out = np.zeros((nt, 16, 3), np.complex128)
+    T = TABLES
+    lib().orc_rho_row_range(_p(row), _p(mom5), tf, nt, nv, _p(T["spinIndices_rho"]),
+                            _p(T["lp_rp_rho"]), _p(T["coef_rho"]), r0, r1, _p(out))
+    return out
+
+
 def rhopi_row(row, diag, p1, p2, p12x5, tf, idir):
     row, diag, p1, p2, p12x5 = map(_cc, (row, diag, p1, p2, p12x5))
     nt, nv = row.shape[0], row.shape[-1]

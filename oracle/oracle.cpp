@@ -80,16 +80,28 @@ extern "C" {
 
 // lib/calculate_rho_core.h: tmp1 = P[ti]_{s3s4} M_lp(ti); tmp2 = M_rp(tf) P[ti]_{s1s2};
 // out[ti][bx] = coef * sum conj(tmp2) o tmp1. Host: corr[ti][ico][igi] = sum_is out.
+void orc_rho_row_range(const double* row, const double* mom5, int tf, int Nt, int Nv,
+                       const short* idx, const short* lprp, const float* coef, int r0, int r1,
+                       double* corr);
+
 void orc_rho_row(const double* row, const double* mom5, int tf, int Nt, int Nv,
                  const short* idx, const short* lprp, const float* coef, double* corr) {
+  orc_rho_row_range(row, mom5, tf, Nt, Nv, idx, lprp, coef, 0, 768, corr);
+}
+
+// Same, restricted to table rows [r0, r1) (other rows contribute zero): lets the CPU baseline
+// time exactly the op pairs a workload asks for.
+void orc_rho_row_range(const double* row, const double* mom5, int tf, int Nt, int Nv,
+                       const short* idx, const short* lprp, const float* coef, int r0, int r1,
+                       double* corr) {
   const int nrow = 768;
-  std::vector<cd> part((size_t)Nt * nrow);
+  std::vector<cd> part((size_t)Nt * nrow, cd(0.0));
 #pragma omp parallel
   {
     Scratch s(Nv);
 #pragma omp for collapse(2) schedule(dynamic, 4)
     for (int ti = 0; ti < Nt; ti++)
-      for (int bx = 0; bx < nrow; bx++) {
+      for (int bx = r0; bx < r1; bx++) {
         const int s1 = idx[bx * 4 + 0], s2 = idx[bx * 4 + 1], s3 = idx[bx * 4 + 2], s4 = idx[bx * 4 + 3];
         const int lp = lprp[bx * 2 + 0], rp = lprp[bx * 2 + 1];
         mm(blk(row, ti * 16 + s3 * 4 + s4, Nv), blk(mom5, ti * 5 + lp, Nv), s.a.data(), Nv);

@@ -13,9 +13,9 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU case")
 
 
-@pytest.fixture(scope="session")
+@pytest.fixture
 def lc():
-    """Initialise the HIP library once per session on device 0."""
+    """The HIP library, initialised on device 0 (re-initialised if a test finalised it)."""
     from contractions_code_scattering_amd import rows
     rows.init(0)
     yield rows
