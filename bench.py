@@ -26,6 +26,10 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def workload(nv=64, nt=48, nmom=19, seed=3, mass=0.3):
     import numpy as np
     from contractions_code_scattering_amd import inputs as I
@@ -82,27 +86,31 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
-    import torch
+    import numpy as np
     from contractions_code_scattering_amd import engine, rows
+    from contractions_code_scattering_amd.devmem import DeviceArray, synchronize, to_device
 
     nv, nt, nmom = 64, 48, args.nmom
     tau_h, phi_h = workload(nv, nt, nmom)
     rows.init(0)
-    tau = torch.from_numpy(tau_h).to("cuda")
-    phi = torch.from_numpy(phi_h).to("cuda")
+    tau = to_device(tau_h)
+    phi = to_device(phi_h)
     nsrc = nt
-    out = (torch.empty((nmom, nsrc, nt, 10), dtype=torch.complex128, device="cuda"),
-           torch.empty((nmom, nsrc, nt, 8), dtype=torch.complex128, device="cuda"),
-           torch.empty((nmom, 8, nt), dtype=torch.complex128, device="cuda"))
-    for _ in range(args.warmup):
+    out = (DeviceArray((nmom, nsrc, nt, 10), np.complex128),
+           DeviceArray((nmom, nsrc, nt, 8), np.complex128),
+           DeviceArray((nmom, 8, nt), np.complex128))
+    log("inputs resident: tau %.2f GB, %d momenta" % (tau.nbytes / 1e9, nmom))
+    for i in range(args.warmup):
         engine.rho_pipi(tau, phi, out=out)
-    torch.cuda.synchronize()
+        log("warmup step %d done" % i)
+    synchronize()
     engine.profile(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         engine.rho_pipi(tau, phi, out=out)
-    torch.cuda.synchronize()
+    synchronize()
     elapsed = time.perf_counter() - t0
+    log("timed %d steps: %.3f s" % (args.steps, elapsed))
     prof = engine.profile_read()
     engine.profile(False)
 

@@ -465,19 +465,63 @@ int lc_sigmapi_triangle_row(int prec, const void* d_row, const void* d_diag, con
   });
 }
 
+int lc_dev_alloc(long long bytes, void** out) {
+  return guarded([&] {
+    require_init();
+    if (bytes < 0 || !out) fail(LC_ERR_ARG, "bad allocation request");
+    *out = nullptr;
+    if (hipMalloc(out, bytes > 0 ? (size_t)bytes : 1) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      fail(LC_ERR_NOMEM, "hipMalloc of %lld bytes failed", bytes);
+    }
+  });
+}
+
+int lc_dev_free(void* p) {
+  return guarded([&] {
+    if (p) hip_check(hipFree(p), "hipFree");
+  });
+}
+
+int lc_memcpy_h2d(void* d, const void* h, long long bytes) {
+  return guarded([&] {
+    require_init();
+    if (bytes < 0 || (bytes > 0 && (!d || !h))) fail(LC_ERR_ARG, "bad copy");
+    hip_check(hipMemcpyAsync(d, h, (size_t)bytes, hipMemcpyHostToDevice, ctx().stream), "copy h2d");
+    hip_check(hipStreamSynchronize(ctx().stream), "sync");
+  });
+}
+
+int lc_memcpy_d2h(void* h, const void* d, long long bytes) {
+  return guarded([&] {
+    require_init();
+    if (bytes < 0 || (bytes > 0 && (!d || !h))) fail(LC_ERR_ARG, "bad copy");
+    hip_check(hipMemcpyAsync(h, d, (size_t)bytes, hipMemcpyDeviceToHost, ctx().stream), "copy d2h");
+    hip_check(hipStreamSynchronize(ctx().stream), "sync");
+  });
+}
+
+int lc_synchronize(void) {
+  return guarded([&] {
+    require_init();
+    hip_check(hipStreamSynchronize(ctx().stream), "sync");
+  });
+}
+
 int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d) {
   return guarded([&] {
     require_init();
     if (prec != LC_SINGLE && prec != LC_DOUBLE) fail(LC_ERR_ARG, "bad precision");
     check_ptr(h_a, "h_a"); check_ptr(h_b, "h_b"); check_ptr(h_d, "h_d");
     const size_t es = prec == LC_DOUBLE ? 8 : 4;
-    ctx().out.ensure(4096);
+    ctx().out.ensure(8192);
     char* d = static_cast<char*>(ctx().out.p);
     hipStream_t s = ctx().stream;
     hip_check(hipMemcpyAsync(d, h_a, 64 * es, hipMemcpyHostToDevice, s), "copy a");
     hip_check(hipMemcpyAsync(d + 1024, h_b, 64 * es, hipMemcpyHostToDevice, s), "copy b");
     hip_check(lc_launch_mfma_selftest(prec == LC_DOUBLE, d, d + 1024, d + 2048, s), "selftest launch");
-    hip_check(hipMemcpyAsync(h_d, d + 2048, 256 * es, hipMemcpyDeviceToHost, s), "copy d");
+    hip_check(hipMemcpyAsync(h_d, d + 2048, 512 * es, hipMemcpyDeviceToHost, s), "copy d");
     hip_check(hipStreamSynchronize(s), "sync");
   });
 }

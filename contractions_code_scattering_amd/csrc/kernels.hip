@@ -33,6 +33,13 @@ __device__ __forceinline__ fx4 mfma16(float a, float b, fx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Row of the 16x16 result tile held in accumulator element q of lane l (the column is l % 16),
+// as measured on gfx950 with lc_mfma_selftest_kernel (raw accumulator dump):
+//   v_mfma_f64_16x16x4_f64: row = l/16 + 4q       v_mfma_f32_16x16x4_f32: row = 4(l/16) + q
+template <typename T> __device__ __forceinline__ int acc_row(int l, int q);
+template <> __device__ __forceinline__ int acc_row<double>(int l, int q) { return (l >> 4) + 4 * q; }
+template <> __device__ __forceinline__ int acc_row<float>(int l, int q) { return ((l >> 4) << 2) + q; }
+
 // Raw block of a view for spin block (a,b): pointer, element flags, exact phase.
 template <typename T> struct BlockRef {
   const Cplx<T>* p;
@@ -267,7 +274,7 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
     for (int n = 0; n < MI; n++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int row = row0 + wm * WM + m * 16 + (lane >> 4) * 4 + q;
+        const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
         const int col = col0 + wn * WM + n * 16 + (lane & 15);
         if (row < nv && col < nv) {
           const T xr = cre[m][n][q], xi = cim[m][n][q];
@@ -338,7 +345,8 @@ __global__ void __launch_bounds__(256) lc_trace_kernel(const LcTraceGroup* __res
 
 // ------------------------------------------------------------------------------------------
 // Self-test of the 16x16x4 MFMA operand/result layout: D = A(16x4) * B(4x16), A[i][k] = a[i*4+k],
-// B[k][j] = b[k*16+j], D written row-major to d (16x16). One wave.
+// B[k][j] = b[k*16+j]. d[0..255]: D row-major through acc_row<T>(); d[256..511]: the raw
+// accumulators, d[256 + l*4 + q] = element q of lane l. One wave.
 template <typename T>
 __global__ void lc_mfma_selftest_kernel(const T* a, const T* b, T* d) {
   typedef typename Acc4<T>::type acc_t;
@@ -347,7 +355,10 @@ __global__ void lc_mfma_selftest_kernel(const T* a, const T* b, T* d) {
   const T bv = b[(l >> 4) * 16 + (l & 15)];
   acc_t c = acc_t{0, 0, 0, 0};
   c = mfma16(av, bv, c);
-  for (int q = 0; q < 4; q++) d[((l >> 4) * 4 + q) * 16 + (l & 15)] = c[q];
+  for (int q = 0; q < 4; q++) {
+    d[acc_row<T>(l, q) * 16 + (l & 15)] = c[q];
+    d[256 + l * 4 + q] = c[q];
+  }
 }
 
 }  // namespace lc

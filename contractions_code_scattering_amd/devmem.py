@@ -1,0 +1,65 @@
+"""Device buffers on the library's own HIP runtime (lc_dev_alloc / lc_memcpy_*).
+
+A DeviceArray owns one allocation and remembers shape and dtype; it is created from a numpy
+array (copied to HBM) or empty, and copied back with to_numpy().
+"""
+import ctypes
+
+import numpy as np
+
+from ._capi import check, lib, register
+
+_VP, _LL = ctypes.c_void_p, ctypes.c_longlong
+register("lc_dev_alloc", [_LL, ctypes.POINTER(_VP)])
+register("lc_dev_free", [_VP])
+register("lc_memcpy_h2d", [_VP, _VP, _LL])
+register("lc_memcpy_d2h", [_VP, _VP, _LL])
+register("lc_synchronize", [])
+
+
+class DeviceArray:
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = _VP()
+        check(lib().lc_dev_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        self._lib = lib()
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None):
+        a = np.ascontiguousarray(a if dtype is None else a.astype(dtype))
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            check(lib().lc_memcpy_h2d(d.ptr, a.ctypes.data, d.nbytes))
+        return d
+
+    def to_numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().lc_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._lib.lc_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def data_ptr(self):
+        return self.ptr
+
+
+def to_device(a, dtype=None):
+    return DeviceArray.from_numpy(a, dtype)
+
+
+def synchronize():
+    check(lib().lc_synchronize())

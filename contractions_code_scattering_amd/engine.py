@@ -1,7 +1,7 @@
 """Torch-facing wrapper of the batched engine (lc_engine_rho_pipi).
 
-tau: CUDA complex128 [nt][nt][4][4][nv][nv]; phi: CUDA complex128 [np][nt][nv][nv].
-Returns (raw, corr, avg) CUDA tensors:
+tau: DeviceArray complex128 [nt][nt][4][4][nv][nv]; phi: DeviceArray complex128 [np][nt][nv][nv].
+Returns (raw, corr, avg) DeviceArrays:
   raw  [np][nsrc][nt][10]  {rho_0..2, square, dT, dTHor, star1, star2, fish1, fish2} traces
   corr [np][nsrc][nt][8]   {rho_0..2, square, dT, dTHor, star, fish}
   avg  [np][8][nt]         source average C(dt)
@@ -20,11 +20,12 @@ register("lc_profile_read", [_VP])
 
 
 def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None):
-    import torch
-    if tau.dtype != torch.complex128 or phi.dtype != torch.complex128:
+    import numpy as np
+    from .devmem import DeviceArray
+    if not (isinstance(tau, DeviceArray) and isinstance(phi, DeviceArray)):
+        raise TypeError("engine operands must be DeviceArrays")
+    if tau.dtype != np.complex128 or phi.dtype != np.complex128:
         raise TypeError("engine operands are complex128")
-    if not (tau.is_cuda and phi.is_cuda and tau.is_contiguous() and phi.is_contiguous()):
-        raise ValueError("engine operands must be contiguous CUDA tensors")
     nt, nv = tau.shape[0], tau.shape[-1]
     if tuple(tau.shape) != (nt, nt, 4, 4, nv, nv):
         raise ValueError("tau must be [nt][nt][4][4][nv][nv]")
@@ -34,15 +35,16 @@ def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=N
     if nsrc is None:
         nsrc = (nt - src0 + src_step - 1) // src_step
     if out is None:
-        dev = tau.device
-        raw = torch.empty((np_, nsrc, nt, 10), dtype=torch.complex128, device=dev)
-        corr = torch.empty((np_, nsrc, nt, N_CORR), dtype=torch.complex128, device=dev)
-        avg = torch.empty((np_, N_CORR, nt), dtype=torch.complex128, device=dev)
+        raw = DeviceArray((np_, nsrc, nt, 10), np.complex128)
+        corr = DeviceArray((np_, nsrc, nt, N_CORR), np.complex128)
+        avg = DeviceArray((np_, N_CORR, nt), np.complex128)
     else:
         raw, corr, avg = out
-    torch.cuda.synchronize()
-    check(lib().lc_engine_rho_pipi(tau.data_ptr(), phi.data_ptr(), nt, nv, np_, src0, src_step, nsrc,
-                                   max_scratch_bytes, raw.data_ptr(), corr.data_ptr(), avg.data_ptr()))
+        if (raw.shape != (np_, nsrc, nt, 10) or corr.shape != (np_, nsrc, nt, N_CORR)
+                or avg.shape != (np_, N_CORR, nt)):
+            raise ValueError("output buffers have the wrong shape")
+    check(lib().lc_engine_rho_pipi(tau.ptr, phi.ptr, nt, nv, np_, src0, src_step, nsrc,
+                                   max_scratch_bytes, raw.ptr, corr.ptr, avg.ptr))
     return raw, corr, avg
 
 

@@ -1,6 +1,6 @@
 """Torch-facing wrappers of the sink-row launchers of the C-ABI.
 
-Operands are CUDA tensors (complex128 -> LC_DOUBLE, complex64 -> LC_SINGLE) in the reference's
+Operands are DeviceArrays (complex128 -> LC_DOUBLE, complex64 -> LC_SINGLE) in the reference's
 device layouts; results come back as numpy arrays in the layout the reference launchers write.
 Shapes are checked on the host before any launch.
 """
@@ -22,24 +22,22 @@ def finalize():
 
 
 def _prec(t):
-    import torch
-    if t.dtype == torch.complex128:
+    if t.dtype == np.complex128:
         return DOUBLE, np.complex128
-    if t.dtype == torch.complex64:
+    if t.dtype == np.complex64:
         return SINGLE, np.complex64
     raise TypeError("operands must be complex64 or complex128, got %s" % t.dtype)
 
 
 def _p(t, shape, dtype):
-    if not t.is_cuda:
-        raise ValueError("operand must be a CUDA tensor")
+    from .devmem import DeviceArray
+    if not isinstance(t, DeviceArray):
+        raise TypeError("operand must be a DeviceArray")
     if tuple(t.shape) != tuple(shape):
         raise ValueError("operand shape %s, expected %s" % (tuple(t.shape), tuple(shape)))
     if t.dtype != dtype:
         raise TypeError("mixed precisions")
-    if not t.is_contiguous():
-        raise ValueError("operand must be contiguous")
-    return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.ptr)
 
 
 def _out(shape, npdt):
@@ -48,7 +46,7 @@ def _out(shape, npdt):
 
 
 def _dims(row):
-    if row.dim() != 5 or row.shape[1:3] != (4, 4) or row.shape[3] != row.shape[4]:
+    if len(row.shape) != 5 or row.shape[1:3] != (4, 4) or row.shape[3] != row.shape[4]:
         raise ValueError("row must be [Nt][4][4][Nv][Nv]")
     return row.shape[0], row.shape[-1]
 
@@ -124,10 +122,11 @@ def sigmapi_triangle_row(row, diag, p1, p2, x2, tf):
 
 
 def selftest_mfma(a, b, prec=DOUBLE):
-    """A (16x4) * B (4x16) on one matrix-core instruction; returns the 16x16 product."""
+    """A (16x4) * B (4x16) on one matrix-core instruction. Returns (D as the kernels map the
+    accumulators, raw accumulators [64 lanes][4])."""
     dt = np.float64 if prec == DOUBLE else np.float32
     a = np.ascontiguousarray(a, dt)
     b = np.ascontiguousarray(b, dt)
-    d = np.zeros((16, 16), dt)
+    d = np.zeros(512, dt)
     check(lib().lc_selftest_mfma(prec, a.ctypes.data, b.ctypes.data, d.ctypes.data))
-    return d
+    return d[:256].reshape(16, 16), d[256:].reshape(64, 4)

@@ -119,8 +119,17 @@ int lc_calculate_sigma(int nev, int nt, void* prop, double* mom, int prec, const
 int lc_calculate_sigmapi(int nev, int nt, void* prop, double* p1, double* p2, double* p12, int prec,
                          const char* outfile);
 
-/* Diagnostic: one 16x16x4 matrix-core product A(16x4)*B(4x16) row-major -> h_d (16x16),
- * used by the tests to pin the operand layout the kernels assume. No reference counterpart. */
+/* Device memory on the library's own HIP runtime and stream (for callers without a GPU
+ * framework; no reference counterpart: the reference allocated inside each Base object). */
+int lc_dev_alloc(long long bytes, void** out);
+int lc_dev_free(void* p);
+int lc_memcpy_h2d(void* d_dst, const void* h_src, long long bytes);
+int lc_memcpy_d2h(void* h_dst, const void* d_src, long long bytes);
+int lc_synchronize(void);
+
+/* Diagnostic: one 16x16x4 matrix-core product A(16x4)*B(4x16). h_d[0..255] receives D
+ * row-major as the kernels map accumulators, h_d[256..511] the raw accumulator element q of
+ * lane l at 256 + 4l + q. Used by the tests to pin the layout. No reference counterpart. */
 int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d);
 
 #ifdef __cplusplus

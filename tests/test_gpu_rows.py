@@ -17,9 +17,8 @@ SIZES = [(4, 3), (8, 4), (25, 3), (33, 2), (64, 2)]
 
 
 def dev(x, dtype=None):
-    import torch
-    x = np.ascontiguousarray(x if dtype is None else x.astype(dtype))
-    return torch.from_numpy(x).to("cuda")
+    from contractions_code_scattering_amd.devmem import DeviceArray
+    return DeviceArray.from_numpy(x, dtype)
 
 
 def _case(seed, nt, nv):
@@ -36,14 +35,28 @@ def _ok(got, ref, tol=TOL):
     assert elemerr(got, ref) <= tol, elemerr(got, ref)
 
 
-def test_mfma_layout(lc):
+def _acc_map(raw, ref):
+    """(lane, q) -> (row, col) of the reference product each raw accumulator equals."""
+    m = {}
+    for l in range(64):
+        for q in range(4):
+            i, j = np.unravel_index(np.argmin(np.abs(ref - raw[l, q])), ref.shape)
+            m[(l, q)] = (int(i), int(j))
+    return m
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_mfma_layout(lc, prec):
     g = I.rng(0)
     a = g.standard_normal((16, 4))
     b = g.standard_normal((4, 16))
-    d = lc.selftest_mfma(a, b)
-    np.testing.assert_allclose(d, a @ b, rtol=1e-14, atol=1e-14)
-    d32 = lc.selftest_mfma(a, b, prec=0)
-    np.testing.assert_allclose(d32, a @ b, rtol=1e-5, atol=1e-5)
+    ref = a @ b
+    d, raw = lc.selftest_mfma(a, b, prec=prec)
+    m = _acc_map(raw, ref)
+    print("accumulator map (lane, q) -> (row, col):",
+          sorted(set((l >> 4, q, i - 0, j - (l & 15)) for (l, q), (i, j) in m.items()))[:16])
+    tol = 1e-14 if prec == 1 else 1e-5
+    np.testing.assert_allclose(d, ref, rtol=tol, atol=tol)
 
 
 @pytest.mark.parametrize("nv,nt", SIZES + [(100, 2)])
