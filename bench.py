@@ -40,7 +40,8 @@ def workload(nv=64, nt=48, nmom=19, seed=3, mass=0.3):
 
 
 def cpu_baseline(tau, phi, rows):
-    """Oracle on `rows` sink rows of momentum 0: rho op pair (0,0) + the I=0 pi-pi set."""
+    """Oracle on `rows` sink rows of momentum 0: rho op pair (0,0) + the I=0 pi-pi set.
+    Returns the baseline record and the oracle's correlators [rows][nt][8] for a parity check."""
     import numpy as np
     import oracle as O
     from contractions_code_scattering_amd import inputs as I
@@ -49,15 +50,23 @@ def cpu_baseline(tau, phi, rows):
     Pd = I.dagger(P)
     mom5 = np.ascontiguousarray(np.stack([P] * 5, axis=1))
     diag = O.diag_of(tau)
+    ref = np.zeros((rows, nt, 8), complex)
     t = time.perf_counter()
     for tf in range(rows):
-        O.rho_row_range(tau[tf], mom5, tf, 0, 48)
-        O.pipi_row(0, tau[tf], diag, P, Pd, Pd, P, tf)
+        r = O.rho_row_range(tau[tf], mom5, tf, 0, 48)
+        q = O.pipi_row(0, tau[tf], diag, P, Pd, Pd, P, tf)
+        ref[tf, :, 0:3] = r[:, 0, :]
+        ref[tf, :, 3] = -3.0 * q[:, 0]
+        ref[tf, :, 4] = -3.0 * q[:, 1]
+        ref[tf, :, 5] = -0.6 * q[:, 4]
+        ref[tf, :, 6] = q[:, 2]
+        ref[tf, :, 7] = q[:, 3]
     dt = time.perf_counter() - t
     ncorr = rows * nt * 8
-    return dict(value=ncorr / dt, unit="correlators/s", cores=O.num_threads(), kind="port",
-                sample="%d sink rows x %d sources x 8 correlators of momentum 0 (rho op pair (0,0) "
-                       "+ pi-pi I=0 set) at Nv=64, Lt=48, FP64, %.1f s" % (rows, nt, dt))
+    rec = dict(value=ncorr / dt, unit="correlators/s", cores=O.num_threads(), kind="port",
+               sample="%d sink rows x %d sources x 8 correlators of momentum 0 (rho op pair (0,0) "
+                      "+ pi-pi I=0 set) at Nv=64, Lt=48, FP64, %.1f s" % (rows, nt, dt))
+    return rec, ref
 
 
 def traffic_from_profiles():
@@ -79,6 +88,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=2, help="sink rows for the CPU baseline (0: skip)")
     ap.add_argument("--nmom", type=int, default=19)
+    ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both",
+                    help="profiling aid: run only one form (the JSON line needs both)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="profiling aid: one device-to-device copy of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -96,46 +109,87 @@ def main():
     tau = to_device(tau_h)
     phi = to_device(phi_h)
     nsrc = nt
-    out = (DeviceArray((nmom, nsrc, nt, 10), np.complex128),
-           DeviceArray((nmom, nsrc, nt, 8), np.complex128),
-           DeviceArray((nmom, 8, nt), np.complex128))
-    log("inputs resident: tau %.2f GB, %d momenta" % (tau.nbytes / 1e9, nmom))
-    for i in range(args.warmup):
-        engine.rho_pipi(tau, phi, out=out)
-        log("warmup step %d done" % i)
-    synchronize()
-    engine.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        engine.rho_pipi(tau, phi, out=out)
-    synchronize()
-    elapsed = time.perf_counter() - t0
-    log("timed %d steps: %.3f s" % (args.steps, elapsed))
-    prof = engine.profile_read()
-    engine.profile(False)
-
     ncorr = nmom * nsrc * nt * 8
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = ncorr * args.steps / elapsed
-    gemm_avg_ms = prof["gemm_ms"] / max(prof["gemm_launches"], 1)
-    flops_per_launch = prof["gemm_flops"] / max(prof["gemm_launches"], 1)
-    achieved = prof["gemm_flops"] / (prof["gemm_ms"] * 1e-3) / 1e12 if prof["gemm_ms"] > 0 else 0.0
-    roof = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_TFLOPS["f64"], unit="TFLOP/s",
-                frac=round(achieved / PEAK_TFLOPS["f64"], 4), traffic=traffic_from_profiles(),
-                kernel="lc_gemm_kernel<double,64>", flops_per_launch=flops_per_launch,
-                avg_launch_ms=round(gemm_avg_ms, 4))
-    line = dict(metric=METRIC, value=round(value, 1), unit="correlators/s", n_gpus=1,
-                steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_per_step, 3),
-                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+    log("inputs resident: tau %.2f GB, %d momenta" % (tau.nbytes / 1e9, nmom))
+
+    def run(mixed):
+        out = (DeviceArray((nmom, nsrc, nt, 10), np.complex128),
+               DeviceArray((nmom, nsrc, nt, 8), np.complex128),
+               DeviceArray((nmom, 8, nt), np.complex128))
+        for i in range(args.warmup):
+            engine.rho_pipi(tau, phi, out=out, mixed=mixed)
+        synchronize()
+        engine.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            engine.rho_pipi(tau, phi, out=out, mixed=mixed)
+        synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = engine.profile_read()
+        engine.profile(False)
+        log("%s: %d steps in %.3f s" % ("mixed" if mixed else "fp64", args.steps, elapsed))
+        return dict(elapsed=elapsed, prof=prof, corr=out[1].to_numpy(), avg=out[2].to_numpy())
+
+    if args.calib_copy_mib > 0:
+        from contractions_code_scattering_amd.devmem import copy
+        n = args.calib_copy_mib << 20
+        src, dst = DeviceArray((n,), np.uint8), DeviceArray((n,), np.uint8)
+        copy(dst, src)
+        log("calibration copy of %d MiB done" % args.calib_copy_mib)
+    if args.mode != "both":
+        r = run(args.mode == "mixed")
+        log("single-mode run: %.1f correlators/s" % (ncorr * args.steps / r["elapsed"]))
+        return
+    r64 = run(False)
+    rmx = run(True)
+
+    def stats(r, peak):
+        p = r["prof"]
+        ms = r["elapsed"] * 1e3 / args.steps
+        ach = p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12 if p["gemm_ms"] > 0 else 0.0
+        return dict(value=ncorr * args.steps / r["elapsed"], ms_per_step=ms, achieved=ach,
+                    frac=ach / peak, avg_launch_ms=p["gemm_ms"] / max(p["gemm_launches"], 1),
+                    flops_per_launch=p["gemm_flops"] / max(p["gemm_launches"], 1),
+                    effective_tflops=p["gemm_flops"] / args.steps / (ms * 1e-3) / 1e12)
+
+    # BF16x3 split: 6 piece products x 4 real products per complex multiply-add = 48 BF16 MFMA
+    # flops per 8 algorithmic complex flops, so the algorithmic ceiling is the BF16 peak / 6.
+    peak_mx = PEAK_TFLOPS["bf16"] / 6.0
+    s64 = stats(r64, PEAK_TFLOPS["f64"])
+    smx = stats(rmx, peak_mx)
+    errs = engine.rel_errors(rmx["avg"], r64["avg"])
+    max_rel = max(e["max_rel"] for e in errs.values())
+    max_norm = max(e["max_normwise"] for e in errs.values())
+    roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",
+                frac=round(smx["frac"], 4), traffic=traffic_from_profiles(),
+                kernel="lc_gemm_mx_kernel (BF16x3 on v_mfma_f32_16x16x32_bf16)",
+                flops_per_launch=smx["flops_per_launch"], avg_launch_ms=round(smx["avg_launch_ms"], 4),
+                frac_of_peaks={k: round(smx["achieved"] / v, 4) for k, v in PEAK_TFLOPS.items()},
+                bf16_mfma_utilization=round(smx["achieved"] * 6.0 / PEAK_TFLOPS["bf16"], 4))
+    line = dict(metric=METRIC, value=round(smx["value"], 1), unit="correlators/s", n_gpus=1,
+                steps=args.steps, warmup=args.warmup, ms_per_step=round(smx["ms_per_step"], 3),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="bf16x3",
                 data="synthetic",
                 config=dict(workload="rho(3 pol)+pipi(5 diagrams) Nv=64 D=256 Lt=48 all 48 sources "
-                                     "%d momenta, plain FP64" % nmom,
-                            nv=nv, D=4 * nv, lt=nt, sources=nsrc, momenta=nmom, mode="plain_fp64",
+                                     "%d momenta" % nmom,
+                            nv=nv, D=4 * nv, lt=nt, sources=nsrc, momenta=nmom,
+                            mode="mixed: per-slice rescaled FP32 storage, BF16x3 split MFMA, FP64 traces",
                             correlators_per_step=ncorr),
                 roofline=roof,
-                effective_tflops=round(prof["gemm_flops"] / args.steps / (ms_per_step * 1e-3) / 1e12, 3))
+                mixed_vs_fp64=round(smx["value"] / s64["value"], 3),
+                max_rel_err=max_rel, max_normwise_err=max_norm, errors_per_diagram=errs,
+                plain_fp64=dict(value=round(s64["value"], 1), ms_per_step=round(s64["ms_per_step"], 3),
+                                roofline=dict(bound="mfma", achieved=round(s64["achieved"], 3),
+                                              peak=PEAK_TFLOPS["f64"], unit="TFLOP/s",
+                                              frac=round(s64["frac"], 4),
+                                              kernel="lc_gemm_kernel<double,64>",
+                                              avg_launch_ms=round(s64["avg_launch_ms"], 4))),
+                effective_tflops=round(smx["effective_tflops"], 3))
     if rank == 0 and args.cpu_rows > 0:
-        line["cpu_baseline"] = cpu_baseline(tau_h, phi_h, args.cpu_rows)
+        rec, ref = cpu_baseline(tau_h, phi_h, args.cpu_rows)
+        line["cpu_baseline"] = rec
+        got = r64["corr"][0][:, :args.cpu_rows, :].transpose(1, 0, 2)  # [tf][t0][8]
+        line["fp64_vs_oracle_max_rel"] = float(np.abs(got - ref).max() / np.abs(ref).max())
     if rank == 0:
         print(json.dumps(line), flush=True)
 

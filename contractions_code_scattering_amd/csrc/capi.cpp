@@ -502,10 +502,35 @@ int lc_memcpy_d2h(void* h, const void* d, long long bytes) {
   });
 }
 
+int lc_memcpy_d2d(void* d_dst, const void* d_src, long long bytes) {
+  return guarded([&] {
+    require_init();
+    if (bytes < 0 || (bytes > 0 && (!d_dst || !d_src))) fail(LC_ERR_ARG, "bad copy");
+    hip_check(hipMemcpyAsync(d_dst, d_src, (size_t)bytes, hipMemcpyDeviceToDevice, ctx().stream), "copy d2d");
+    hip_check(hipStreamSynchronize(ctx().stream), "sync");
+  });
+}
+
 int lc_synchronize(void) {
   return guarded([&] {
     require_init();
     hip_check(hipStreamSynchronize(ctx().stream), "sync");
+  });
+}
+
+int lc_selftest_mfma_bf16(const float* h_a, const float* h_b, float* h_d) {
+  return guarded([&] {
+    require_init();
+    check_ptr(h_a, "h_a"); check_ptr(h_b, "h_b"); check_ptr(h_d, "h_d");
+    ctx().out.ensure(16384);
+    char* d = static_cast<char*>(ctx().out.p);
+    hipStream_t s = ctx().stream;
+    hip_check(hipMemcpyAsync(d, h_a, 512 * 4, hipMemcpyHostToDevice, s), "copy a");
+    hip_check(hipMemcpyAsync(d + 4096, h_b, 512 * 4, hipMemcpyHostToDevice, s), "copy b");
+    hip_check(lc_launch_mfma_bf16_selftest((const float*)d, (const float*)(d + 4096), (float*)(d + 8192), s),
+              "selftest launch");
+    hip_check(hipMemcpyAsync(h_d, d + 8192, 512 * 4, hipMemcpyDeviceToHost, s), "copy d");
+    hip_check(hipStreamSynchronize(s), "sync");
   });
 }
 

@@ -21,12 +21,14 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "runtime.h"
 #include "spin.h"
 
 extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
                                                 int nsrc, int nt, int src0, int src_step,
+                                                const double* s_tau, const double* s_phi,
                                                 hipStream_t s);
 
 using namespace lc;
@@ -45,19 +47,31 @@ inline LcView kron(const void* p, long long s0, long long s1, long long s2, int 
   return make_view(p, s0, s1, s2, LC_KRON, flags);
 }
 
-const size_t ESZ = 16;  // complex double
-
-inline const char* off(const void* p, long long elems) {
-  return static_cast<const char*>(p) + (size_t)elems * ESZ;
+// Byte offsets of complex arrays: esz = 16 (complex double) or 8 (complex float).
+inline const char* off(const void* p, long long elems, size_t esz) {
+  return static_cast<const char*>(p) + (size_t)elems * esz;
 }
-inline char* off(char* p, long long elems) { return p + (size_t)elems * ESZ; }
+inline char* off(char* p, long long elems, size_t esz) { return p + (size_t)elems * esz; }
 
 // Number of D-space scratch matrices per (tf, t0, p) item and per (t, p) item.
-const int PAIR_BUFS = 12;
-const int TIME_BUFS = 6;
+const int PAIR_BUFS = 7;
 
-void run_chunk(const EngineShape& E, const void* tau, const void* phi, int p0, int pc, char* scratch,
-               void* raw) {
+// Levels (each one grouped launch; closing traces fused into the GEMM that forms their left
+// factor, so A, S1, S2, U and LW are never stored):
+//   1  B = Phi(tf) tau, L1 = Phi(tf)^dag tau;  Qa = D(tf) Phi, T1 = D(t0) Phi, T1' = D(t0) Phi^dag
+//   2  A = tau Phi(t0)          -> rho_i = -Tr[A (K'_i B^dag L'_i)]
+//      S1 = Phi(t0) tau^dag     -> star1 = Tr[S1 B], fish1 = Tr[S1 L1]
+//      S2 = Phi(t0)^dag tau^dag -> star2 = Tr[S2 L1], fish2 = Tr[S2 B]
+//      L = L1 Phi(t0), Rr = B Phi(t0)^dag;  Q = Phi^dag Qa, R = Phi^dag T1, R' = Phi T1'
+//   3  V = R tau^dag, V' = R' tau^dag, Zh = Rr tau^dag
+//   4  U = Q (g5 tau g5)        -> square = Tr[U V], dT = Tr[U V']
+//      LW = L W                 -> dTHor = Tr[LW Zh]
+// tau / phi / scratch hold complex double (mode F64) or complex float (mode MX); raw is always
+// complex double.
+void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const void* phi, int p0,
+               int pc, char* scratch, void* raw) {
+  const size_t es = mode == Program::F64 ? 16 : 8;
+  auto off = [es](auto p, long long elems) { return ::off(p, elems, es); };
   const int nt = E.nt, ns = E.nsrc;
   const long long m = E.m, b = E.blk;
   const long long npair = (long long)nt * ns;
@@ -65,8 +79,7 @@ void run_chunk(const EngineShape& E, const void* tau, const void* phi, int p0, i
   char* pb[PAIR_BUFS];
   for (int i = 0; i < PAIR_BUFS; i++) pb[i] = off(scratch, (long long)i * pc * npair * b);
   char* tb0 = off(scratch, (long long)PAIR_BUFS * pc * npair * b);
-  char *A = pb[0], *B = pb[1], *L1 = pb[2], *S1 = pb[3], *S2 = pb[4], *L = pb[5], *Rr = pb[6],
-       *U = pb[7], *V = pb[8], *Vp = pb[9], *LW = pb[10], *Zh = pb[11];
+  char *B = pb[0], *L1 = pb[1], *L = pb[2], *Rr = pb[3], *V = pb[4], *Vp = pb[5], *Zh = pb[6];
   // per-sink buffers [pc][nt]: strides {b, 0, nt*b}; per-source buffers [pc][nsrc]: {0, b, ns*b}
   char* Qa = tb0;
   char* Q = off(tb0, (long long)pc * nt * b);
@@ -87,48 +100,49 @@ void run_chunk(const EngineShape& E, const void* tau, const void* phi, int p0, i
   const LcView D_tf = dense(tau, (nt + 1) * b, 0, 0);
   const LcView D_t0 = dense(off(tau, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0);
   auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
+  // raw [np][nsrc][nt][10]
+  char* rw = ::off(static_cast<char*>(raw), (long long)p0 * ns * nt * 10, 16);
+  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  auto slot = [&](int k, const LcView& Y, double ar = 1.0) {
+    FusedTrace f = fused(Y, ::off(rw, k, 16), O0, O1, O2, ar);
+    f.out_double = 1;
+    return f;
+  };
 
-  Program pg(E.nv, true);
-  // level 1: elemental x perambulator products and the per-time D Phi products
+  Program pg(E.nv, mode);
+  // level 1
+  pg.gemm(phi_tf, tau_v, B, P0, P1, P2, nt, ns, pc);
+  pg.gemm(phi_tf_dag, tau_v, L1, P0, P1, P2, nt, ns, pc);
   pg.gemm(D_tf, phi_tf, Qa, b, 0, nt * b, nt, 1, pc);
   pg.gemm(D_t0, phi_t0, T1, 0, b, ns * b, 1, ns, pc);
   pg.gemm(D_t0, phi_t0_dag, T1p, 0, b, ns * b, 1, ns, pc);
-  pg.gemm(tau_v, phi_t0, A, P0, P1, P2, nt, ns, pc);
-  pg.gemm(phi_tf, tau_v, B, P0, P1, P2, nt, ns, pc);
-  pg.gemm(phi_tf_dag, tau_v, L1, P0, P1, P2, nt, ns, pc);
-  pg.gemm(phi_t0, tau_dag, S1, P0, P1, P2, nt, ns, pc);
-  pg.gemm(phi_t0_dag, tau_dag, S2, P0, P1, P2, nt, ns, pc);
   pg.barrier();
   // level 2
-  pg.gemm(phi_tf_dag, dense(Qa, b, 0, nt * b), Q, b, 0, nt * b, nt, 1, pc);
-  pg.gemm(phi_t0_dag, dense(T1, 0, b, ns * b), R, 0, b, ns * b, 1, ns, pc);
-  pg.gemm(phi_t0, dense(T1p, 0, b, ns * b), Rp, 0, b, ns * b, 1, ns, pc);
-  pg.gemm(pair(L1), phi_t0, L, P0, P1, P2, nt, ns, pc);
-  pg.gemm(pair(B), phi_t0_dag, Rr, P0, P1, P2, nt, ns, pc);
-  pg.barrier();
-  // level 3: dense x dense
   const Spin g5 = gamma(5);
-  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_v, g5), g5), U, P0, P1, P2, nt, ns, pc);
-  pg.gemm(dense(R, 0, b, ns * b), tau_dag, V, P0, P1, P2, nt, ns, pc);
-  pg.gemm(dense(Rp, 0, b, ns * b), tau_dag, Vp, P0, P1, P2, nt, ns, pc);
-  pg.gemm(pair(L), W, LW, P0, P1, P2, nt, ns, pc);
-  pg.gemm(pair(Rr), tau_dag, Zh, P0, P1, P2, nt, ns, pc);
-  pg.barrier();
-  // closing traces -> raw [np][nsrc][nt][10]
-  char* rw = off(static_cast<char*>(raw), (long long)p0 * ns * nt * 10);
-  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  std::vector<FusedTrace> rho;
   for (int i = 0; i < 3; i++) {
     const Spin Lp = g5 * rhoj(1, i, false);
     const Spin Kp = rhoj(1, i, true) * g5;
-    pg.trace(pair(B, LC_DAG), right(left(pair(A), Lp), Kp), off(rw, i), O0, O1, O2, nt, ns, pc, -1.0, 0.0);
+    rho.push_back(slot(i, right(left(pair(B, LC_DAG), Kp), Lp), -1.0));
   }
-  pg.trace(pair(U), pair(V), off(rw, 3), O0, O1, O2, nt, ns, pc);
-  pg.trace(pair(U), pair(Vp), off(rw, 4), O0, O1, O2, nt, ns, pc);
-  pg.trace(pair(LW), pair(Zh), off(rw, 5), O0, O1, O2, nt, ns, pc);
-  pg.trace(pair(B), pair(S1), off(rw, 6), O0, O1, O2, nt, ns, pc);
-  pg.trace(pair(L1), pair(S2), off(rw, 7), O0, O1, O2, nt, ns, pc);
-  pg.trace(pair(L1), pair(S1), off(rw, 8), O0, O1, O2, nt, ns, pc);
-  pg.trace(pair(B), pair(S2), off(rw, 9), O0, O1, O2, nt, ns, pc);
+  pg.gemm(tau_v, phi_t0, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, rho);
+  pg.gemm(phi_t0, tau_dag, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(6, pair(B)), slot(8, pair(L1))});
+  pg.gemm(phi_t0_dag, tau_dag, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(7, pair(L1)), slot(9, pair(B))});
+  pg.gemm(pair(L1), phi_t0, L, P0, P1, P2, nt, ns, pc);
+  pg.gemm(pair(B), phi_t0_dag, Rr, P0, P1, P2, nt, ns, pc);
+  pg.gemm(phi_tf_dag, dense(Qa, b, 0, nt * b), Q, b, 0, nt * b, nt, 1, pc);
+  pg.gemm(phi_t0_dag, dense(T1, 0, b, ns * b), R, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(phi_t0, dense(T1p, 0, b, ns * b), Rp, 0, b, ns * b, 1, ns, pc);
+  pg.barrier();
+  // level 3
+  pg.gemm(dense(R, 0, b, ns * b), tau_dag, V, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(Rp, 0, b, ns * b), tau_dag, Vp, P0, P1, P2, nt, ns, pc);
+  pg.gemm(pair(Rr), tau_dag, Zh, P0, P1, P2, nt, ns, pc);
+  pg.barrier();
+  // level 4
+  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_v, g5), g5), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+          {slot(3, pair(V)), slot(4, pair(Vp))});
+  pg.gemm(pair(L), W, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
   pg.run(ctx().stream);
 }
 
@@ -136,9 +150,9 @@ void run_chunk(const EngineShape& E, const void* tau, const void* phi, int p0, i
 
 extern "C" {
 
-// Engine entry point; see include/lapH_contract.h.
-int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
-                       int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+// Engine entry points; see include/lapH_contract.h.
+static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt, int nv, int np,
+                       int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                        void* d_corr, void* d_avg) {
   try {
     require_init();
@@ -147,21 +161,45 @@ int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int
       fail(LC_ERR_ARG, "bad source list src0=%d step=%d nsrc=%d (nt=%d)", src0, src_step, nsrc, nt);
     if (!d_tau || !d_phi || !d_raw || !d_corr || !d_avg) fail(LC_ERR_ARG, "null pointer");
     EngineShape E{nt, nv, np, src0, src_step, nsrc, (long long)nv * nv, 16LL * nv * nv};
+    const size_t es = mixed ? 8 : 16;
     const long long npair = (long long)nt * nsrc;
-    const size_t per_p = (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * ESZ;
+    // mixed path: per-slice rescaled FP32 copies of tau and Phi + FP64 scale factors, ahead of
+    // the scratch area
+    const size_t tau_bytes = mixed ? (size_t)nt * nt * E.blk * 8 : 0;
+    const size_t phi_bytes = mixed ? (size_t)np * nt * E.m * 8 : 0;
+    const size_t scale_bytes = mixed ? (size_t)(nt * nt + np * nt) * 8 : 0;
+    const size_t head = ((tau_bytes + 255) & ~size_t(255)) + ((phi_bytes + 255) & ~size_t(255)) +
+                        ((scale_bytes + 255) & ~size_t(255));
+    const size_t per_p = (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * es;
     if (max_scratch_bytes <= 0) max_scratch_bytes = 64LL << 30;
-    int pc = (int)(max_scratch_bytes / (long long)per_p);
-    if (pc < 1) fail(LC_ERR_NOMEM, "scratch budget %lld B below one momentum (%zu B)", max_scratch_bytes, per_p);
+    long long avail = max_scratch_bytes - (long long)head;
+    int pc = avail > 0 ? (int)(avail / (long long)per_p) : 0;
+    if (pc < 1) fail(LC_ERR_NOMEM, "scratch budget %lld B below one momentum (%zu B)", max_scratch_bytes, per_p + head);
     if (pc > np) pc = np;
-    ctx().scratch.ensure(per_p * pc + 4096);
-    char* scratch = static_cast<char*>(ctx().scratch.p);
+    ctx().scratch.ensure(head + per_p * pc + 4096);
+    char* base = static_cast<char*>(ctx().scratch.p);
+    hipStream_t s = ctx().stream;
+    const void* tau = d_tau;
+    const void* phi = d_phi;
+    double* scales = nullptr;
+    if (mixed) {
+      char* t32 = base;
+      char* p32 = t32 + ((tau_bytes + 255) & ~size_t(255));
+      scales = reinterpret_cast<double*>(p32 + ((phi_bytes + 255) & ~size_t(255)));
+      hip_check(lc_launch_rescale(d_tau, t32, E.blk, (long long)nt * nt, scales, s), "rescale tau");
+      hip_check(lc_launch_rescale(d_phi, p32, E.m, (long long)np * nt, scales + (size_t)nt * nt, s), "rescale phi");
+      tau = t32;
+      phi = p32;
+    }
+    char* scratch = base + head;
     for (int p0 = 0; p0 < np; p0 += pc) {
       const int n = (np - p0) < pc ? (np - p0) : pc;
-      run_chunk(E, d_tau, d_phi, p0, n, scratch, d_raw);
+      run_chunk(E, mixed ? Program::MX : Program::F64, tau, phi, p0, n, scratch, d_raw);
     }
-    hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, ctx().stream),
+    hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales,
+                                        scales ? scales + (size_t)nt * nt : nullptr, s),
               "engine epilogue");
-    hip_check(hipStreamSynchronize(ctx().stream), "hipStreamSynchronize");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     return LC_OK;
   } catch (const Error& e) {
     last_error() = e.msg;
@@ -170,6 +208,20 @@ int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int
     last_error() = e.what();
     return LC_ERR_ARG;
   }
+}
+
+int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                       int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                       void* d_corr, void* d_avg) {
+  return engine_impl(false, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw,
+                     d_corr, d_avg);
+}
+
+int lc_engine_rho_pipi_mixed(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                             int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                             void* d_corr, void* d_avg) {
+  return engine_impl(true, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw,
+                     d_corr, d_avg);
 }
 
 int lc_profile(int enable) {

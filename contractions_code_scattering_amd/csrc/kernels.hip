@@ -1,108 +1,26 @@
-// HIP kernels of the contraction engine (gfx950 / CDNA4).
+// HIP kernels of the contraction engine (gfx950 / CDNA4), FP64 and FP32.
 //
 // lc_gemm_kernel   grouped block-complex GEMM  C = alpha * A * B  on D-space views, on the
-//                  matrix cores (v_mfma_f64_16x16x4_f64 for FP64, v_mfma_f32_16x16x4_f32 for FP32),
-//                  complex product as four real MFMAs into separate real/imaginary accumulators.
-// lc_trace_kernel  grouped closing trace  out = alpha * Tr[X * Y] = alpha * sum_ij X_ij Y_ji,
-//                  so the final product of a chain is never formed; wavefront shuffle + LDS
-//                  reduction, fixed summation order (deterministic).
+//                  matrix cores (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32), complex
+//                  product as four real MFMAs into separate real and imaginary accumulators;
+//                  optionally fused closing traces Tr[C * Y_t] so C need not be stored.
+// lc_trace_kernel  grouped stand-alone closing trace  out = alpha * Tr[X * Y] = sum_ij X_ij Y_ji.
+// lc_reduce_kernel fixed-order sum of the per-tile trace pieces (deterministic results).
 //
-// One workgroup = 256 threads = 4 wave64. A GEMM workgroup owns one TM x TM output tile of one
-// spin block (a,c) of one batch item, loops over the spin index b of the product (4 terms for
-// dense x dense, 1 when one operand is an elemental M (x) 1) and over K in chunks of KT,
-// staging both operand tiles through LDS with the view's phase applied, and prefetching the
-// next chunk into registers while the matrix cores work on the current one.
-#include <hip/hip_runtime.h>
-
-#include "lc_internal.h"
+// A GEMM workgroup (256 threads = 4 wave64 in a 2x2 arrangement) owns one TM x TM tile of one
+// spin block (a,c) of one batch item. It loops over the spin index b of the product (4 terms
+// for dense x dense, 1 when an operand is an elemental M (x) 1) and over K in chunks of KT,
+// staging both operand tiles through double-buffered LDS with the view's phase applied; the
+// next chunk is fetched into registers while the matrix cores work on the current one, so one
+// barrier per chunk suffices. Workgroups are renumbered so that every tile of a product runs
+// on the same XCD (shared L2).
+#include "lc_device.h"
 
 namespace lc {
 
-typedef double dx4 __attribute__((ext_vector_type(4)));
-typedef float fx4 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct Cplx { T x, y; };
-template <typename T> struct Acc4;
-template <> struct Acc4<double> { typedef dx4 type; };
-template <> struct Acc4<float> { typedef fx4 type; };
-
-__device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ fx4 mfma16(float a, float b, fx4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-
-// Row of the 16x16 result tile held in accumulator element q of lane l (the column is l % 16),
-// as measured on gfx950 with lc_mfma_selftest_kernel (raw accumulator dump):
-//   v_mfma_f64_16x16x4_f64: row = l/16 + 4q       v_mfma_f32_16x16x4_f32: row = 4(l/16) + q
-template <typename T> __device__ __forceinline__ int acc_row(int l, int q);
-template <> __device__ __forceinline__ int acc_row<double>(int l, int q) { return (l >> 4) + 4 * q; }
-template <> __device__ __forceinline__ int acc_row<float>(int l, int q) { return ((l >> 4) << 2) + q; }
-
-// Raw block of a view for spin block (a,b): pointer, element flags, exact phase.
-template <typename T> struct BlockRef {
-  const Cplx<T>* p;
-  int flags;
-  T sr, si;
-};
-
-template <typename T>
-__device__ __forceinline__ BlockRef<T> view_block(const LcView& v, long long off, int a, int b,
-                                                  long long nv2) {
-  BlockRef<T> r;
-  const Cplx<T>* base = reinterpret_cast<const Cplx<T>*>(v.base) + off;
-  r.flags = v.flags;
-  if (v.kind == LC_KRON) {
-    r.p = base;
-    r.sr = T(1);
-    r.si = T(0);
-    return r;
-  }
-  const int ra = v.perm_r[a], cb = v.perm_c[b];
-  const int blk = (v.flags & LC_TS) ? (cb * 4 + ra) : (ra * 4 + cb);
-  r.p = base + (long long)blk * nv2;
-  const double pr = v.ph_r[a][0] * v.ph_c[b][0] - v.ph_r[a][1] * v.ph_c[b][1];
-  const double pi = v.ph_r[a][0] * v.ph_c[b][1] + v.ph_r[a][1] * v.ph_c[b][0];
-  r.sr = T(pr);
-  r.si = T(pi);
-  return r;
-}
-
-// Element (i,j) of f(X) for a raw block.
-template <typename T>
-__device__ __forceinline__ Cplx<T> elem(const BlockRef<T>& r, int i, int j, int nv) {
-  const long long idx = (r.flags & LC_TV) ? (long long)j * nv + i : (long long)i * nv + j;
-  Cplx<T> z = r.p[idx];
-  if (r.flags & LC_CONJ) z.y = -z.y;
-  return z;
-}
-
-__device__ __forceinline__ void decode_item(long long item, const int* n, long long* c) {
-  c[0] = item % n[0];
-  const long long q = item / n[0];
-  c[1] = q % n[1];
-  c[2] = q / n[1];
-}
-
-__device__ __forceinline__ long long voff(const long long* s, const long long* c) {
-  return s[0] * c[0] + s[1] * c[1] + s[2] * c[2];
-}
-
-template <typename G>
-__device__ __forceinline__ int find_group(const G* groups, int ng, long long wg) {
-  int lo = 0, hi = ng - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (groups[mid].wg_begin <= wg) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-// ------------------------------------------------------------------------------------------
 template <typename T, int TM>
 __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restrict__ groups,
-                                                      int ng, int nv) {
+                                                      int ng, int nv, long long total) {
   constexpr int KT = 16;
   constexpr int E = TM * KT / 256;  // complex elements staged per thread per operand
   constexpr int WM = TM / 2;        // per-wave output tile (2 x 2 waves)
@@ -110,10 +28,12 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
   constexpr int PAD = 2;
   typedef typename Acc4<T>::type acc_t;
 
-  __shared__ T As_re[KT][TM + PAD], As_im[KT][TM + PAD];
-  __shared__ T Bs_re[KT][TM + PAD], Bs_im[KT][TM + PAD];
+  __shared__ T As_re[2][KT][TM + PAD], As_im[2][KT][TM + PAD];
+  __shared__ T Bs_re[2][KT][TM + PAD], Bs_im[2][KT][TM + PAD];
+  __shared__ double red[8];
 
-  const long long wg = blockIdx.x;
+  const long long wg = xcd_order(total);
+  if (wg < 0) return;
   const int gi = find_group(groups, ng, wg);
   const LcGemmGroup& g = groups[gi];
   const int tiles = g.tiles;
@@ -153,7 +73,7 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
   BlockRef<T> A, B;
   int cur_term = -1;
 
-  // Stage chunk `ch` of both operands into registers (phase applied to A).
+  // Fetch chunk `ch` of both operands into registers (phase of the term applied to A).
   auto load = [&](int ch) {
     const int term = ch / nk;
     const int k0 = (ch - term * nk) * KT;
@@ -164,12 +84,11 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
       else { ab = a; bb = term; ba = term; bc = c; }
       A = view_block<T>(g.A, offA, ab, bb, nv2);
       B = view_block<T>(g.B, offB, ba, bc, nv2);
-      // fold B's phase into A's scalar
       const T sr = A.sr * B.sr - A.si * B.si, si = A.sr * B.si + A.si * B.sr;
-      A.sr = sr; A.si = si;
+      A.sr = sr;
+      A.si = si;
       cur_term = term;
     }
-    // A tile: rows row0.., k k0..
     if (A.flags & LC_TV) {
       const int k = tid / (TM / E), i0 = (tid % (TM / E)) * E;
 #pragma unroll
@@ -191,7 +110,6 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
         ra[e].y = A.sr * z.y + A.si * z.x;
       }
     }
-    // B tile: k k0.., cols col0..
     if (B.flags & LC_TV) {
       const int j = tid / (KT / E), kq = (tid % (KT / E)) * E;
 #pragma unroll
@@ -213,32 +131,32 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
     }
   };
 
-  auto store = [&]() {
+  auto store = [&](int buf) {
     if (A.flags & LC_TV) {
       const int k = tid / (TM / E), i0 = (tid % (TM / E)) * E;
 #pragma unroll
-      for (int e = 0; e < E; e++) { As_re[k][i0 + e] = ra[e].x; As_im[k][i0 + e] = ra[e].y; }
+      for (int e = 0; e < E; e++) { As_re[buf][k][i0 + e] = ra[e].x; As_im[buf][k][i0 + e] = ra[e].y; }
     } else {
       const int i = tid / (KT / E), kq = (tid % (KT / E)) * E;
 #pragma unroll
-      for (int e = 0; e < E; e++) { As_re[kq + e][i] = ra[e].x; As_im[kq + e][i] = ra[e].y; }
+      for (int e = 0; e < E; e++) { As_re[buf][kq + e][i] = ra[e].x; As_im[buf][kq + e][i] = ra[e].y; }
     }
     if (B.flags & LC_TV) {
       const int j = tid / (KT / E), kq = (tid % (KT / E)) * E;
 #pragma unroll
-      for (int e = 0; e < E; e++) { Bs_re[kq + e][j] = rb[e].x; Bs_im[kq + e][j] = rb[e].y; }
+      for (int e = 0; e < E; e++) { Bs_re[buf][kq + e][j] = rb[e].x; Bs_im[buf][kq + e][j] = rb[e].y; }
     } else {
       const int k = tid / (TM / E), j0 = (tid % (TM / E)) * E;
 #pragma unroll
-      for (int e = 0; e < E; e++) { Bs_re[k][j0 + e] = rb[e].x; Bs_im[k][j0 + e] = rb[e].y; }
+      for (int e = 0; e < E; e++) { Bs_re[buf][k][j0 + e] = rb[e].x; Bs_im[buf][k][j0 + e] = rb[e].y; }
     }
   };
 
   load(0);
+  store(0);
+  __syncthreads();
   for (int ch = 0; ch < nchunk; ch++) {
-    __syncthreads();
-    store();
-    __syncthreads();
+    const int buf = ch & 1;
     if (ch + 1 < nchunk) load(ch + 1);
 #pragma unroll
     for (int ks = 0; ks < KT / 4; ks++) {
@@ -247,11 +165,11 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
 #pragma unroll
       for (int m = 0; m < MI; m++) {
         const int rr = wm * WM + m * 16 + (lane & 15);
-        are[m] = As_re[kr][rr];
-        aim[m] = As_im[kr][rr];
+        are[m] = As_re[buf][kr][rr];
+        aim[m] = As_im[buf][kr][rr];
         const int cc = wn * WM + m * 16 + (lane & 15);
-        bre[m] = Bs_re[kr][cc];
-        bim[m] = Bs_im[kr][cc];
+        bre[m] = Bs_re[buf][kr][cc];
+        bim[m] = Bs_im[buf][kr][cc];
       }
 #pragma unroll
       for (int m = 0; m < MI; m++)
@@ -263,37 +181,82 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
           cim[m][n] = mfma16(aim[m], bre[n], cim[m][n]);
         }
     }
+    // buffer buf^1 was last read in iteration ch-1, which every wave finished before the
+    // barrier that closed it: it can be refilled now.
+    if (ch + 1 < nchunk) store(buf ^ 1);
+    __syncthreads();
   }
 
-  // Epilogue: scale by alpha, store block (a,c).
+  // Epilogue: scale by alpha, optionally store block (a,c), fused trace pieces.
   const T alr = T(g.alpha[0]), ali = T(g.alpha[1]);
-  Cplx<T>* C = reinterpret_cast<Cplx<T>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
+  Cplx<T>* C = g.C ? reinterpret_cast<Cplx<T>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2
+                   : nullptr;
+  T zr[MI][MI][4], zi[MI][MI][4];
 #pragma unroll
   for (int m = 0; m < MI; m++)
 #pragma unroll
     for (int n = 0; n < MI; n++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
-        const int col = col0 + wn * WM + n * 16 + (lane & 15);
-        if (row < nv && col < nv) {
-          const T xr = cre[m][n][q], xi = cim[m][n][q];
-          Cplx<T> z;
-          z.x = alr * xr - ali * xi;
-          z.y = alr * xi + ali * xr;
-          C[(long long)row * nv + col] = z;
-        }
+        const T xr = cre[m][n][q], xi = cim[m][n][q];
+        zr[m][n][q] = alr * xr - ali * xi;
+        zi[m][n][q] = alr * xi + ali * xr;
       }
+  if (C) {
+#pragma unroll
+    for (int m = 0; m < MI; m++)
+#pragma unroll
+      for (int n = 0; n < MI; n++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
+          const int col = col0 + wn * WM + n * 16 + (lane & 15);
+          if (row < nv && col < nv) {
+            Cplx<T> z;
+            z.x = zr[m][n][q];
+            z.y = zi[m][n][q];
+            C[(long long)row * nv + col] = z;
+          }
+        }
+  }
+  for (int t = 0; t < g.ntr; t++) {
+    const BlockRef<T> Y = view_block<T>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
+    double sr = 0, si = 0;
+    if (!Y.zero) {
+#pragma unroll
+      for (int m = 0; m < MI; m++)
+#pragma unroll
+        for (int n = 0; n < MI; n++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
+            const int col = col0 + wn * WM + n * 16 + (lane & 15);
+            if (row < nv && col < nv) {
+              const Cplx<T> y = elem(Y, col, row, nv);
+              const double xr = zr[m][n][q], xi = zi[m][n][q];
+              sr += xr * (double)y.x - xi * (double)y.y;
+              si += xr * (double)y.y + xi * (double)y.x;
+            }
+          }
+    }
+    block_sum2(sr, si, red);
+    if (tid == 0) {
+      // Y block phase, then the trace coefficient
+      const double yr = Y.sr, yi = Y.si;
+      const double pr = sr * yr - si * yi, pi = sr * yi + si * yr;
+      const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
+      double* out = g.part[t] + 2 * (item * per_item + r);
+      out[0] = ar * pr - ai * pi;
+      out[1] = ar * pi + ai * pr;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T shfl_xor_t(T v, int m) { return __shfl_xor(v, m, 64); }
-
-template <typename T>
 __global__ void __launch_bounds__(256) lc_trace_kernel(const LcTraceGroup* __restrict__ groups,
                                                        int ng, int nv) {
-  __shared__ T red[2][4];
+  __shared__ double red[8];
   const long long wg = blockIdx.x;
   const int gi = find_group(groups, ng, wg);
   const LcTraceGroup& g = groups[gi];
@@ -305,16 +268,17 @@ __global__ void __launch_bounds__(256) lc_trace_kernel(const LcTraceGroup* __res
   const bool xk = g.X.kind == LC_KRON, yk = g.Y.kind == LC_KRON;
   const int npair = (xk || yk) ? 4 : 16;
 
-  T accr = 0, acci = 0;
+  double accr = 0, acci = 0;
   for (int pr = 0; pr < npair; pr++) {
     int a, b;
     if (npair == 4) { a = pr; b = pr; } else { a = pr >> 2; b = pr & 3; }
     const BlockRef<T> X = view_block<T>(g.X, offX, a, b, nv2);
     const BlockRef<T> Y = view_block<T>(g.Y, offY, b, a, nv2);
-    const T sr = X.sr * Y.sr - X.si * Y.si, si = X.sr * Y.si + X.si * Y.sr;
+    const double sr = (double)X.sr * Y.sr - (double)X.si * Y.si;
+    const double si = (double)X.sr * Y.si + (double)X.si * Y.sr;
     T pr_ = 0, pi_ = 0;
-    for (long long e = threadIdx.x; e < nv2; e += 256) {
-      const int i = (int)(e / nv), j = (int)(e - (long long)i * nv);
+    for (int e = threadIdx.x; e < (int)nv2; e += 256) {
+      const int i = e / nv, j = e - i * nv;
       const Cplx<T> x = elem(X, i, j, nv);
       const Cplx<T> y = elem(Y, j, i, nv);
       pr_ += x.x * y.x - x.y * y.y;
@@ -323,23 +287,45 @@ __global__ void __launch_bounds__(256) lc_trace_kernel(const LcTraceGroup* __res
     accr += sr * pr_ - si * pi_;
     acci += sr * pi_ + si * pr_;
   }
-  for (int m = 32; m >= 1; m >>= 1) {
-    accr += shfl_xor_t(accr, m);
-    acci += shfl_xor_t(acci, m);
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) { red[0][wave] = accr; red[1][wave] = acci; }
-  __syncthreads();
+  block_sum2(accr, acci, red);
   if (threadIdx.x == 0) {
-    const T sr = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    const T si = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-    const T alr = T(g.alpha[0]), ali = T(g.alpha[1]);
+    const double alr = g.alpha[0], ali = g.alpha[1];
     Cplx<T>* out = reinterpret_cast<Cplx<T>*>(g.out) + voff(g.os, co);
-    Cplx<T> z;
-    z.x = alr * sr - ali * si;
-    z.y = alr * si + ali * sr;
-    if (g.accumulate) { z.x += out->x; z.y += out->y; }
-    *out = z;
+    double zx = alr * accr - ali * acci, zy = alr * acci + ali * accr;
+    if (g.accumulate) { zx += out->x; zy += out->y; }
+    out->x = T(zx);
+    out->y = T(zy);
+  }
+}
+
+// One thread per item: out[item] (+)= sum_k part[item][k], in a fixed order.
+__global__ void lc_reduce_kernel(const LcReduceJob* __restrict__ jobs, int njobs,
+                                 const long long* __restrict__ begins) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 0;
+  while (j + 1 < njobs && begins[j + 1] <= gid) j++;
+  const LcReduceJob& J = jobs[j];
+  const long long item = gid - begins[j];
+  if (item >= (long long)J.n[0] * J.n[1] * J.n[2]) return;
+  long long co[3];
+  decode_item(item, J.n, co);
+  const double* p = J.part + 2 * item * J.nk;
+  double sr = 0, si = 0;
+  for (int k = 0; k < J.nk; k++) {
+    sr += p[2 * k];
+    si += p[2 * k + 1];
+  }
+  const long long o = voff(J.os, co);
+  if (J.out_double) {
+    double* out = reinterpret_cast<double*>(J.out) + 2 * o;
+    if (J.accumulate) { sr += out[0]; si += out[1]; }
+    out[0] = sr;
+    out[1] = si;
+  } else {
+    float* out = reinterpret_cast<float*>(J.out) + 2 * o;
+    if (J.accumulate) { sr += out[0]; si += out[1]; }
+    out[0] = (float)sr;
+    out[1] = (float)si;
   }
 }
 
@@ -361,65 +347,52 @@ __global__ void lc_mfma_selftest_kernel(const T* a, const T* b, T* d) {
   }
 }
 
-}  // namespace lc
-
-// ------------------------------------------------------------------------------------------
-// Host launchers (called from the planner in capi.cpp).
-extern "C" hipError_t lc_launch_gemm(int is_double, const LcGemmGroup* d_groups, int ng,
-                                     long long total_wg, int nv, hipStream_t s) {
-  if (ng <= 0 || total_wg <= 0) return hipSuccess;
-  const dim3 grid((unsigned)total_wg), block(256);
-  if (is_double) {
-    if (nv <= 32) hipLaunchKernelGGL((lc::lc_gemm_kernel<double, 32>), grid, block, 0, s, d_groups, ng, nv);
-    else hipLaunchKernelGGL((lc::lc_gemm_kernel<double, 64>), grid, block, 0, s, d_groups, ng, nv);
-  } else {
-    if (nv <= 32) hipLaunchKernelGGL((lc::lc_gemm_kernel<float, 32>), grid, block, 0, s, d_groups, ng, nv);
-    else hipLaunchKernelGGL((lc::lc_gemm_kernel<float, 64>), grid, block, 0, s, d_groups, ng, nv);
-  }
-  return hipGetLastError();
-}
-
-extern "C" int lc_gemm_tile(int nv) { return nv <= 32 ? 32 : 64; }
-
-extern "C" hipError_t lc_launch_trace(int is_double, const LcTraceGroup* d_groups, int ng,
-                                      long long total_wg, int nv, hipStream_t s) {
-  if (ng <= 0 || total_wg <= 0) return hipSuccess;
-  const dim3 grid((unsigned)total_wg), block(256);
-  if (is_double) hipLaunchKernelGGL((lc::lc_trace_kernel<double>), grid, block, 0, s, d_groups, ng, nv);
-  else hipLaunchKernelGGL((lc::lc_trace_kernel<float>), grid, block, 0, s, d_groups, ng, nv);
-  return hipGetLastError();
-}
-
-extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b, void* d,
-                                              hipStream_t s) {
-  if (is_double)
-    hipLaunchKernelGGL((lc::lc_mfma_selftest_kernel<double>), dim3(1), dim3(64), 0, s,
-                       (const double*)a, (const double*)b, (double*)d);
-  else
-    hipLaunchKernelGGL((lc::lc_mfma_selftest_kernel<float>), dim3(1), dim3(64), 0, s,
-                       (const float*)a, (const float*)b, (float*)d);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------------------------------------------
 // Engine epilogue (FP64): per (p, source k, sink tf) combine the raw traces of the rho / pi-pi
 // set into correlators, then average over sources on the device.
 //   raw  [np][nsrc][nt][10] = {rho_0, rho_1, rho_2, square, dT, dTHor, st1, st2, fi1, fi2}
 //   corr [np][nsrc][nt][8]  = {rho_0, rho_1, rho_2, square, dT, dTHor, st1*st2, fi1*fi2}
 //   avg  [np][8][nt]        C(dt) = 1/nsrc sum_k corr[p][k][(t0_k + dt) % nt]
-namespace lc {
+// With per-slice rescaled inputs (mixed path) every trace is multiplied back by the FP64 scale
+// factors of the matrices it contains: s(tf,t0) of tau, s(t,t) of D(t) = tau(t,t), s(p,t) of
+// Phi. rho, star and fish traces hold tau^2 Phi(tf) Phi(t0); square and doubleTriangle
+// tau^2 D(tf) D(t0) Phi(tf)^2 Phi(t0)^2; doubleTriangleHor tau^4 Phi(tf)^2 Phi(t0)^2.
 __global__ void lc_engine_combine_kernel(const double2* __restrict__ raw, double2* __restrict__ corr,
-                                         long long n) {
+                                         long long n, int nsrc, int nt, int src0, int src_step,
+                                         const double* __restrict__ s_tau,
+                                         const double* __restrict__ s_phi) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double2* r = raw + i * 10;
   double2* c = corr + i * 8;
+  double f[10];
 #pragma unroll
-  for (int d = 0; d < 6; d++) c[d] = r[d];
-  c[6].x = r[6].x * r[7].x - r[6].y * r[7].y;
-  c[6].y = r[6].x * r[7].y + r[6].y * r[7].x;
-  c[7].x = r[8].x * r[9].x - r[8].y * r[9].y;
-  c[7].y = r[8].x * r[9].y + r[8].y * r[9].x;
+  for (int d = 0; d < 10; d++) f[d] = 1.0;
+  if (s_tau) {
+    const int tf = (int)(i % nt);
+    const int k = (int)((i / nt) % nsrc);
+    const long long p = i / ((long long)nt * nsrc);
+    const int t0 = src0 + k * src_step;
+    const double st = s_tau[(long long)tf * nt + t0];
+    const double dtf = s_tau[(long long)tf * nt + tf], dt0 = s_tau[(long long)t0 * nt + t0];
+    const double pf = s_phi[p * nt + tf], p0 = s_phi[p * nt + t0];
+    const double f1 = st * st * pf * p0;
+    const double f2 = st * st * dtf * dt0 * pf * pf * p0 * p0;
+    const double f3 = st * st * st * st * pf * pf * p0 * p0;
+    f[0] = f[1] = f[2] = f1;
+    f[3] = f[4] = f2;
+    f[5] = f3;
+    f[6] = f[7] = f[8] = f[9] = f1;
+  }
+  double2 v[10];
+#pragma unroll
+  for (int d = 0; d < 10; d++) v[d] = make_double2(r[d].x * f[d], r[d].y * f[d]);
+#pragma unroll
+  for (int d = 0; d < 6; d++) c[d] = v[d];
+  c[6].x = v[6].x * v[7].x - v[6].y * v[7].y;
+  c[6].y = v[6].x * v[7].y + v[6].y * v[7].x;
+  c[7].x = v[8].x * v[9].x - v[8].y * v[9].y;
+  c[7].y = v[8].x * v[9].y + v[8].y * v[9].x;
 }
 
 __global__ void lc_engine_average_kernel(const double2* __restrict__ corr, double2* __restrict__ avg,
@@ -437,14 +410,63 @@ __global__ void lc_engine_average_kernel(const double2* __restrict__ corr, doubl
   }
   avg[i] = make_double2(sr / nsrc, si / nsrc);
 }
+
 }  // namespace lc
+
+// ------------------------------------------------------------------------------------------
+// Host launchers (called from the planner).
+extern "C" int lc_gemm_tile(int nv) { return nv <= 32 ? 32 : 64; }
+
+extern "C" hipError_t lc_launch_gemm(int is_double, const LcGemmGroup* d_groups, int ng,
+                                     long long total_wg, int nv, hipStream_t s) {
+  if (ng <= 0 || total_wg <= 0) return hipSuccess;
+  const long long per = (total_wg + 7) / 8;
+  const dim3 grid((unsigned)(8 * per)), block(256);
+  if (is_double) {
+    if (nv <= 32) hipLaunchKernelGGL((lc::lc_gemm_kernel<double, 32>), grid, block, 0, s, d_groups, ng, nv, total_wg);
+    else hipLaunchKernelGGL((lc::lc_gemm_kernel<double, 64>), grid, block, 0, s, d_groups, ng, nv, total_wg);
+  } else {
+    if (nv <= 32) hipLaunchKernelGGL((lc::lc_gemm_kernel<float, 32>), grid, block, 0, s, d_groups, ng, nv, total_wg);
+    else hipLaunchKernelGGL((lc::lc_gemm_kernel<float, 64>), grid, block, 0, s, d_groups, ng, nv, total_wg);
+  }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_trace(int is_double, const LcTraceGroup* d_groups, int ng,
+                                      long long total_wg, int nv, hipStream_t s) {
+  if (ng <= 0 || total_wg <= 0) return hipSuccess;
+  const dim3 grid((unsigned)total_wg), block(256);
+  if (is_double) hipLaunchKernelGGL((lc::lc_trace_kernel<double>), grid, block, 0, s, d_groups, ng, nv);
+  else hipLaunchKernelGGL((lc::lc_trace_kernel<float>), grid, block, 0, s, d_groups, ng, nv);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_reduce(const LcReduceJob* d_jobs, const long long* d_begins, int njobs,
+                                       long long total, hipStream_t s) {
+  if (njobs <= 0 || total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(lc::lc_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     d_jobs, njobs, d_begins);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b, void* d,
+                                              hipStream_t s) {
+  if (is_double)
+    hipLaunchKernelGGL((lc::lc_mfma_selftest_kernel<double>), dim3(1), dim3(64), 0, s,
+                       (const double*)a, (const double*)b, (double*)d);
+  else
+    hipLaunchKernelGGL((lc::lc_mfma_selftest_kernel<float>), dim3(1), dim3(64), 0, s,
+                       (const float*)a, (const float*)b, (float*)d);
+  return hipGetLastError();
+}
 
 extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
                                                 int nsrc, int nt, int src0, int src_step,
+                                                const double* s_tau, const double* s_phi,
                                                 hipStream_t s) {
   const long long n = (long long)np * nsrc * nt;
   hipLaunchKernelGGL(lc::lc_engine_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     (const double2*)raw, (double2*)corr, n);
+                     (const double2*)raw, (double2*)corr, n, nsrc, nt, src0, src_step, s_tau, s_phi);
   const int m = np * 8 * nt;
   hipLaunchKernelGGL(lc::lc_engine_average_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
                      (const double2*)corr, (double2*)avg, np, nsrc, nt, 8, src0, src_step);

@@ -31,7 +31,12 @@ struct LcView {
   double ph_r[4][2], ph_c[4][2];
 };
 
-// C = alpha * A * B over a batch (i0 < n0, i1 < n1, i2 < n2). C is a plain D-space matrix.
+#define LC_MAX_FUSED_TRACES 3
+
+// C = alpha * A * B over a batch (i0 < n0, i1 < n1, i2 < n2). C is a plain D-space matrix
+// (not stored when C == nullptr). Optionally fused closing traces: for t < ntr,
+//   part_t[item][tile] = tr_alpha_t * sum over the tile of C_ij * Y_t(c,a)_ji
+// i.e. the per-tile pieces of tr_alpha_t * Tr[C * Y_t]; a fixed-order reduction sums the tiles.
 struct LcGemmGroup {
   LcView A, B;
   void* C;
@@ -39,8 +44,23 @@ struct LcGemmGroup {
   double alpha[2];
   int n[3];
   int tiles;                // output tiles per block side: ceil(nv / TM)
-  int pad;
+  int ntr;
   long long wg_begin;       // first workgroup of this group in the launch
+  LcView Y[LC_MAX_FUSED_TRACES];
+  double tr_alpha[LC_MAX_FUSED_TRACES][2];
+  double* part[LC_MAX_FUSED_TRACES];  // complex double [items][16*tiles*tiles]
+};
+
+// out[item] (+)= sum_{k < nk} part[item*nk + k] (fixed order); out addressed through strides.
+struct LcReduceJob {
+  const double* part;
+  void* out;                // complex of the output type
+  long long os[3];
+  int n[3];
+  int nk;
+  int accumulate;
+  int out_double;           // 1: out is complex double, 0: complex float
+  int pad;
 };
 
 // out (+)= alpha * Tr[X * Y] over a batch. out holds complex values of the compute type.

@@ -37,9 +37,12 @@ void Program::barrier() {
 }
 
 void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
-                   long long cs2, int n0, int n1, int n2, double ar, double ai) {
+                   long long cs2, int n0, int n1, int n2, double ar, double ai,
+                   const std::vector<FusedTrace>& traces) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
+  if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
+  if (!C && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
   Level* l = open_level(false);
   LcGemmGroup g;
   std::memset(&g, 0, sizeof g);
@@ -49,12 +52,36 @@ void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, lon
   g.cs[0] = cs0; g.cs[1] = cs1; g.cs[2] = cs2;
   g.alpha[0] = ar; g.alpha[1] = ai;
   g.n[0] = n0; g.n[1] = n1; g.n[2] = n2;
-  const int tm = lc_gemm_tile(nv_);
+  const int tm = mode_ == MX ? 64 : lc_gemm_tile(nv_);
   g.tiles = (nv_ + tm - 1) / tm;
   g.wg_begin = l->total_wg;
   const long long items = (long long)n0 * n1 * n2;
-  l->total_wg += items * 16LL * g.tiles * g.tiles;
+  // workgroups (and trace pieces) per item: one per output tile, or one per `bpw` tiles in MX mode
+  const long long per_item = 16LL * g.tiles * g.tiles / (mode_ == MX ? lc_gemm_mx_bpw(nv_) : 1);
+  l->total_wg += items * per_item;
   l->count++;
+  g.ntr = (int)traces.size();
+  for (int t = 0; t < g.ntr; t++) {
+    const FusedTrace& f = traces[t];
+    g.Y[t] = f.Y;
+    g.tr_alpha[t][0] = f.ar;
+    g.tr_alpha[t][1] = f.ai;
+    g.part[t] = nullptr;
+    const size_t bytes = (size_t)items * per_item * 16;
+    patches_.push_back(Patch{gemms_.size(), t, partial_bytes_});
+    LcReduceJob j;
+    std::memset(&j, 0, sizeof j);
+    j.out = f.out;
+    for (int d = 0; d < 3; d++) j.os[d] = f.os[d];
+    j.n[0] = n0; j.n[1] = n1; j.n[2] = n2;
+    j.nk = (int)per_item;
+    j.accumulate = f.accumulate ? 1 : 0;
+    j.out_double = f.out_double < 0 ? (dbl_ ? 1 : 0) : f.out_double;
+    l->jobs.push_back((int)jobs_.size());
+    jobs_.push_back(j);
+    job_part_off_.push_back(partial_bytes_);
+    partial_bytes_ += (bytes + 255) & ~size_t(255);
+  }
   gemms_.push_back(g);
   const int nterm = (A.kind == LC_KRON || B.kind == LC_KRON) ? 1 : 4;
   gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
@@ -84,15 +111,41 @@ void Program::trace(const LcView& X, const LcView& Y, void* out, long long os0, 
 
 void Program::run(hipStream_t s) {
   Context& c = ctx();
+  // The descriptor and partial buffers are reused by the next program: wait for the previous one.
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (partial_bytes_) c.partials.ensure(partial_bytes_);
+  char* pbase = static_cast<char*>(c.partials.p);
+  for (const Patch& p : patches_) gemms_[p.group].part[p.slot] = reinterpret_cast<double*>(pbase + p.part_off);
+  for (size_t j = 0; j < jobs_.size(); j++) jobs_[j].part = reinterpret_cast<const double*>(pbase + job_part_off_[j]);
+  // reduce jobs of each level, contiguous, with their thread offsets
+  std::vector<LcReduceJob> jobs;
+  std::vector<long long> begins;
+  std::vector<std::pair<size_t, long long>> level_jobs;  // (first job, total threads) per level
+  for (Level& l : levels_) {
+    long long tot = 0;
+    const size_t first = jobs.size();
+    for (int j : l.jobs) {
+      begins.push_back(tot);
+      jobs.push_back(jobs_[j]);
+      tot += (long long)jobs_[j].n[0] * jobs_[j].n[1] * jobs_[j].n[2];
+    }
+    level_jobs.push_back({first, tot});
+  }
   const size_t gb = gemms_.size() * sizeof(LcGemmGroup);
   const size_t tb = traces_.size() * sizeof(LcTraceGroup);
+  const size_t jb = jobs.size() * sizeof(LcReduceJob);
+  const size_t bb = begins.size() * sizeof(long long);
   const size_t toff = (gb + 255) & ~size_t(255);
-  c.groups.ensure(toff + tb + 256);
-  // The descriptor buffer is reused by the next program: wait for the previous one.
-  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  const size_t joff = (toff + tb + 255) & ~size_t(255);
+  const size_t boff = (joff + jb + 255) & ~size_t(255);
+  c.groups.ensure(boff + bb + 256);
   char* base = static_cast<char*>(c.groups.p);
   if (gb) hip_check(hipMemcpyAsync(base, gemms_.data(), gb, hipMemcpyHostToDevice, s), "upload gemm groups");
   if (tb) hip_check(hipMemcpyAsync(base + toff, traces_.data(), tb, hipMemcpyHostToDevice, s), "upload trace groups");
+  if (jb) hip_check(hipMemcpyAsync(base + joff, jobs.data(), jb, hipMemcpyHostToDevice, s), "upload reduce jobs");
+  if (bb) hip_check(hipMemcpyAsync(base + boff, begins.data(), bb, hipMemcpyHostToDevice, s), "upload reduce offsets");
+  const LcReduceJob* dj = reinterpret_cast<const LcReduceJob*>(base + joff);
+  const long long* db = reinterpret_cast<const long long*>(base + boff);
   // host vectors may be pageable: make sure the copies have landed before they can change
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   const LcGemmGroup* dg = reinterpret_cast<const LcGemmGroup*>(base);
@@ -106,16 +159,22 @@ void Program::run(hipStream_t s) {
       c.ev_pool.push_back(e);
     }
   }
-  for (const Level& l : levels_) {
+  for (size_t li = 0; li < levels_.size(); li++) {
+    const Level& l = levels_[li];
     if (l.count == 0) continue;
-    if (l.total_wg > 0x7fffffffLL) fail(LC_ERR_ARG, "launch too large (%lld workgroups)", l.total_wg);
+    if (l.total_wg > 0x7ffffff0LL) fail(LC_ERR_ARG, "launch too large (%lld workgroups)", l.total_wg);
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev], s), "hipEventRecord");
     hipError_t e = l.is_trace
                        ? lc_launch_trace(dbl_, dt + l.first, l.count, l.total_wg, nv_, s)
-                       : lc_launch_gemm(dbl_, dg + l.first, l.count, l.total_wg, nv_, s);
+                       : (mode_ == MX ? lc_launch_gemm_mx(dg + l.first, l.count, l.total_wg, nv_, s)
+                                      : lc_launch_gemm(dbl_, dg + l.first, l.count, l.total_wg, nv_, s));
     hip_check(e, l.is_trace ? "trace launch" : "gemm launch");
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev + 1], s), "hipEventRecord");
     nev += 2;
+    if (!l.jobs.empty())
+      hip_check(lc_launch_reduce(dj + level_jobs[li].first, db + level_jobs[li].first, (int)l.jobs.size(),
+                                 level_jobs[li].second, s),
+                "reduce launch");
   }
   if (prof) {
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");

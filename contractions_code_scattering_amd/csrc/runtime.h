@@ -15,6 +15,15 @@ extern "C" hipError_t lc_launch_gemm(int is_double, const LcGemmGroup* d_groups,
                                      long long total_wg, int nv, hipStream_t s);
 extern "C" hipError_t lc_launch_trace(int is_double, const LcTraceGroup* d_groups, int ng,
                                       long long total_wg, int nv, hipStream_t s);
+extern "C" hipError_t lc_launch_reduce(const LcReduceJob* d_jobs, const long long* d_begins,
+                                       int njobs, long long total, hipStream_t s);
+extern "C" int lc_gemm_mx_bpw(int nv);
+extern "C" hipError_t lc_launch_gemm_mx(const LcGemmGroup* d_groups, int ng, long long total_wg,
+                                        int nv, hipStream_t s);
+extern "C" hipError_t lc_launch_mfma_bf16_selftest(const float* a, const float* b, float* d,
+                                                   hipStream_t s);
+extern "C" hipError_t lc_launch_rescale(const void* x, void* y, long long len, long long nslice,
+                                        double* scale, hipStream_t s);
 extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b,
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
@@ -78,7 +87,7 @@ struct Context {
   bool on = false;
   int device = -1;
   hipStream_t stream = nullptr;
-  DevBuf groups, scratch, out;
+  DevBuf groups, scratch, out, partials;
   bool profile = false;
   KernelTimes times;
   std::vector<hipEvent_t> ev_pool;
@@ -89,14 +98,42 @@ void require_init();
 // Message of the last failed C-ABI call (shared by every entry point).
 std::string& last_error();
 
-// A sequence of levels; each level is one grouped GEMM launch or one grouped trace launch.
+// A closing trace fused into a GEMM: out (+)= alpha * Tr[C * Y] for the product C being formed.
+struct FusedTrace {
+  LcView Y;
+  void* out;
+  long long os[3];
+  double ar = 1.0, ai = 0.0;
+  bool accumulate = false;
+  int out_double = -1;  // -1: the program's compute type
+};
+
+inline FusedTrace fused(const LcView& Y, void* out, long long os0, long long os1, long long os2,
+                        double ar = 1.0, double ai = 0.0) {
+  FusedTrace f;
+  f.Y = Y;
+  f.out = out;
+  f.os[0] = os0; f.os[1] = os1; f.os[2] = os2;
+  f.ar = ar;
+  f.ai = ai;
+  return f;
+}
+
+// A sequence of levels; each level is one grouped GEMM launch or one grouped trace launch
+// (GEMM levels with fused traces are followed by their fixed-order reduction).
 // Groups inside a level are independent; levels run in order on the library stream.
 class Program {
  public:
-  explicit Program(int nv, bool dbl) : nv_(nv), dbl_(dbl) {}
+  // F64 / F32: FP64 or FP32 matrix cores on FP64 / FP32 data. MX: FP32 data, GEMMs on the BF16
+  // matrix cores with the exact three-way split (gemm_mx.hip); traces accumulate in FP64.
+  enum Mode { F64, F32, MX };
+  explicit Program(int nv, bool dbl) : nv_(nv), dbl_(dbl), mode_(dbl ? F64 : F32) {}
+  Program(int nv, Mode m) : nv_(nv), dbl_(m == F64), mode_(m) {}
 
+  // C = alpha * A * B over the batch; C may be nullptr when only fused traces are wanted.
   void gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
-            long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0);
+            long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
+            const std::vector<FusedTrace>& traces = {});
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -115,11 +152,22 @@ class Program {
     size_t first;
     int count;
     long long total_wg;
+    std::vector<int> jobs;  // reduce jobs run after this level
+  };
+  struct Patch {
+    size_t group;
+    int slot;
+    size_t part_off;  // bytes into the partials buffer
   };
   int nv_;
   bool dbl_;
+  Mode mode_;
   std::vector<LcGemmGroup> gemms_;
   std::vector<LcTraceGroup> traces_;
+  std::vector<LcReduceJob> jobs_;
+  std::vector<size_t> job_part_off_;
+  std::vector<Patch> patches_;
+  size_t partial_bytes_ = 0;
   std::vector<Level> levels_;
   double gemm_flops_ = 0, trace_flops_ = 0;
   Level* open_level(bool is_trace);

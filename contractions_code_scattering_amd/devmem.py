@@ -15,6 +15,7 @@ register("lc_dev_free", [_VP])
 register("lc_memcpy_h2d", [_VP, _VP, _LL])
 register("lc_memcpy_d2h", [_VP, _VP, _LL])
 register("lc_synchronize", [])
+register("lc_memcpy_d2d", [_VP, _VP, _LL])
 
 
 class DeviceArray:
@@ -63,3 +64,10 @@ def to_device(a, dtype=None):
 
 def synchronize():
     check(lib().lc_synchronize())
+
+
+def copy(dst, src):
+    """Device-to-device copy of src into dst (same byte size)."""
+    if dst.nbytes != src.nbytes:
+        raise ValueError("size mismatch")
+    check(lib().lc_memcpy_d2d(dst.ptr, src.ptr, src.nbytes))

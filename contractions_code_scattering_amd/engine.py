@@ -15,11 +15,15 @@ N_CORR = len(CORR_NAMES)
 
 _VP, _I, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
 register("lc_engine_rho_pipi", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
+register("lc_engine_rho_pipi_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
+register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
 register("lc_profile", [_I])
 register("lc_profile_read", [_VP])
 
 
-def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None):
+def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None, mixed=False):
+    """mixed=False: plain FP64 on the FP64 matrix cores; mixed=True: per-slice rescaled FP32
+    storage, BF16x3 split products on the BF16 matrix cores, FP64 traces and sums."""
     import numpy as np
     from .devmem import DeviceArray
     if not (isinstance(tau, DeviceArray) and isinstance(phi, DeviceArray)):
@@ -43,8 +47,9 @@ def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=N
         if (raw.shape != (np_, nsrc, nt, 10) or corr.shape != (np_, nsrc, nt, N_CORR)
                 or avg.shape != (np_, N_CORR, nt)):
             raise ValueError("output buffers have the wrong shape")
-    check(lib().lc_engine_rho_pipi(tau.ptr, phi.ptr, nt, nv, np_, src0, src_step, nsrc,
-                                   max_scratch_bytes, raw.ptr, corr.ptr, avg.ptr))
+    fn = lib().lc_engine_rho_pipi_mixed if mixed else lib().lc_engine_rho_pipi
+    check(fn(tau.ptr, phi.ptr, nt, nv, np_, src0, src_step, nsrc, max_scratch_bytes,
+             raw.ptr, corr.ptr, avg.ptr))
     return raw, corr, avg
 
 
@@ -58,3 +63,24 @@ def profile_read():
     check(lib().lc_profile_read(buf.ctypes.data))
     return dict(gemm_ms=buf[0], gemm_launches=int(buf[1]), gemm_flops=buf[2],
                 trace_ms=buf[3], trace_launches=int(buf[4]), trace_flops=buf[5])
+
+
+def selftest_mfma_bf16(a, b):
+    """One v_mfma_f32_16x16x32_bf16 on A (16x32), B (32x16) exact in BF16: (D, raw [64][4])."""
+    import numpy as np
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    d = np.zeros(512, np.float32)
+    check(lib().lc_selftest_mfma_bf16(a.ctypes.data, b.ctypes.data, d.ctypes.data))
+    return d[:256].reshape(16, 16), d[256:].reshape(64, 4)
+
+
+def rel_errors(c_test, c_ref):
+    """Per diagram: max over (momentum, t) of |dC|/|C| and of |dC|/max_t|C| (normwise)."""
+    import numpy as np
+    d = np.abs(c_test - c_ref)          # [np][ncorr][nt]
+    a = np.abs(c_ref)
+    rel = d / np.where(a > 0, a, np.inf)
+    norm = d / a.max(axis=2, keepdims=True)
+    return {name: dict(max_rel=float(rel[:, k].max()), max_normwise=float(norm[:, k].max()))
+            for k, name in enumerate(CORR_NAMES)}

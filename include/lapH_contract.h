@@ -92,6 +92,19 @@ int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int
                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                        void* d_corr, void* d_avg);
 
+/* Mixed-precision form of lc_engine_rho_pipi, same arguments and outputs: tau and Phi are
+ * rescaled per time slice to unit Frobenius norm (scales kept in FP64) and stored as FP32;
+ * every product runs on the BF16 matrix cores with an exact three-way BF16 split of each FP32
+ * operand (six cross products, FP32 accumulation); closing traces, scale factors and source
+ * sums are FP64. */
+int lc_engine_rho_pipi_mixed(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                             int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                             void* d_corr, void* d_avg);
+
+/* Diagnostic: one v_mfma_f32_16x16x32_bf16 on A(16x32), B(32x16) given as FP32 values exact in
+ * BF16; h_d as for lc_selftest_mfma (D row-major, then raw accumulators). */
+int lc_selftest_mfma_bf16(const float* h_a, const float* h_b, float* h_d);
+
 /* Kernel timing with HIP events on the library stream: lc_profile(1) clears and enables,
  * lc_profile_read fills {gemm_ms, gemm_launches, gemm_flops, trace_ms, trace_launches,
  * trace_flops} accumulated since. */
@@ -125,6 +138,7 @@ int lc_dev_alloc(long long bytes, void** out);
 int lc_dev_free(void* p);
 int lc_memcpy_h2d(void* d_dst, const void* h_src, long long bytes);
 int lc_memcpy_d2h(void* h_dst, const void* d_src, long long bytes);
+int lc_memcpy_d2d(void* d_dst, const void* d_src, long long bytes);
 int lc_synchronize(void);
 
 /* Diagnostic: one 16x16x4 matrix-core product A(16x4)*B(4x16). h_d[0..255] receives D

@@ -72,6 +72,41 @@ def test_engine_chunking_is_invisible(lc):
     tau, phi = _inputs(12, nt, nv, np_)
     t, f = to_device(tau), to_device(phi)
     _, c1, _ = engine.rho_pipi(t, f)
-    per_p = (12 * nt * nt + 2 * nt + 4 * nt) * 16 * nv * nv * 16
+    per_p = (7 * nt * nt + 2 * nt + 4 * nt) * 16 * nv * nv * 16
     _, c2, _ = engine.rho_pipi(t, f, max_scratch_bytes=per_p)
     assert np.array_equal(c1.to_numpy(), c2.to_numpy())
+
+
+def _bf16_exact(g, shape):
+    import numpy as np
+    x = g.standard_normal(shape).astype(np.float32)
+    u = x.view(np.uint32) & np.uint32(0xFFFF0000)
+    return u.view(np.float32)
+
+
+def test_mfma_bf16_layout(lc):
+    from contractions_code_scattering_amd import engine
+    g = I.rng(21)
+    a = _bf16_exact(g, (16, 32))
+    b = _bf16_exact(g, (32, 16))
+    ref = a.astype(np.float64) @ b.astype(np.float64)
+    d, raw = engine.selftest_mfma_bf16(a, b)
+    print("raw accumulator rows for lanes 0,16,32,48:",
+          [[int(np.argmin(np.abs(ref[:, l & 15] - raw[l, q]))) for q in range(4)] for l in (0, 16, 32, 48)])
+    np.testing.assert_allclose(d, ref, rtol=1e-6, atol=1e-5)
+
+
+@pytest.mark.parametrize("nt,nv,np_", [(4, 16, 2), (3, 64, 1), (4, 33, 1)])
+def test_engine_mixed_vs_fp64(lc, nt, nv, np_):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    tau, phi = _inputs(13, nt, nv, np_)
+    t, f = to_device(tau), to_device(phi)
+    _, c64, a64 = engine.rho_pipi(t, f)
+    _, cmx, amx = engine.rho_pipi(t, f, mixed=True)
+    c64, cmx = c64.to_numpy(), cmx.to_numpy()
+    errs = engine.rel_errors(amx.to_numpy(), a64.to_numpy())
+    print("mixed vs fp64:", errs)
+    for d in range(8):
+        assert relerr(cmx[..., d], c64[..., d]) < 2e-6, (d, relerr(cmx[..., d], c64[..., d]))
+    assert max(e["max_normwise"] for e in errs.values()) < 2e-6
