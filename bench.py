@@ -69,16 +69,16 @@ def cpu_baseline(tau, phi, rows):
     return rec, ref
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "gemm_f64_traffic.json")
-    if os.path.exists(path):
-        try:
-            with open(path) as f:
-                return json.load(f).get("bytes_per_launch")
-        except (OSError, ValueError):
-            return None
-    return None
+def traffic_from_profiles(kernel, nmom):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a 1 GiB copy); None when no summary exists for this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_rho_pipi_D256_Lt48_%dmom.json" % nmom)
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+        return None if k is None else round(k["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def main():
@@ -161,7 +161,7 @@ def main():
     max_rel = max(e["max_rel"] for e in errs.values())
     max_norm = max(e["max_normwise"] for e in errs.values())
     roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",
-                frac=round(smx["frac"], 4), traffic=traffic_from_profiles(),
+                frac=round(smx["frac"], 4), traffic=traffic_from_profiles("lc::lc_gemm_mx_kernel", nmom),
                 kernel="lc_gemm_mx_kernel (BF16x3 on v_mfma_f32_16x16x32_bf16)",
                 flops_per_launch=smx["flops_per_launch"], avg_launch_ms=round(smx["avg_launch_ms"], 4),
                 frac_of_peaks={k: round(smx["achieved"] / v, 4) for k, v in PEAK_TFLOPS.items()},
@@ -182,6 +182,7 @@ def main():
                                 roofline=dict(bound="mfma", achieved=round(s64["achieved"], 3),
                                               peak=PEAK_TFLOPS["f64"], unit="TFLOP/s",
                                               frac=round(s64["frac"], 4),
+                                              traffic=traffic_from_profiles("lc::lc_gemm_kernel<double, 64>", nmom),
                                               kernel="lc_gemm_kernel<double,64>",
                                               avg_launch_ms=round(s64["avg_launch_ms"], 4))),
                 effective_tflops=round(smx["effective_tflops"], 3))
