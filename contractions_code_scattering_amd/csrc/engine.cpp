@@ -146,6 +146,105 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   pg.run(ctx().stream);
 }
 
+// Per-momentum-chunk scratch of the mixed form, in bytes per (pc = 1): FP32 pair buffers B, L1,
+// V, V', Zh (8 B per complex), split pair buffers B, L1, L, Rr (12 B), split per-sink Qa, Q and
+// per-source T1, T1', R, R' buffers (12 B).
+size_t mx_bytes_per_momentum(const EngineShape& E) {
+  const long long npair = (long long)E.nt * E.nsrc;
+  return (size_t)E.blk * (size_t)(npair * (5 * 8 + 4 * 12) + (2LL * E.nt + 4LL * E.nsrc) * 12);
+}
+
+// The same chain as run_chunk in the mixed form (Program::MX). tau and Phi are held split, twice:
+// row-major planes (_r) and with every block transposed (_t); a GEMM operand reads whichever of
+// the two has its K dimension contiguous (A: rows, B: columns), i.e. a view of the transposed
+// copy carries LC_TV toggled. Intermediates are written by their producer directly in the form
+// their consumer reads: split row-major when used as A, split transposed when used as B, FP32
+// when used as a closing-trace factor.
+void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, const char* phi_r,
+                  const char* phi_t, int p0, int pc, char* scratch, void* raw) {
+  const int nt = E.nt, ns = E.nsrc;
+  const long long m = E.m, b = E.blk;
+  const long long npair = (long long)nt * ns;
+  auto f32 = [](auto p, long long e) { return ::off(p, e, 8); };
+  auto spl = [](auto p, long long e) { return ::off(p, e, 12); };
+  char* cur = scratch;
+  auto take = [&](long long elems, size_t es) {
+    char* p = cur;
+    cur += ((size_t)elems * es + 255) & ~size_t(255);
+    return p;
+  };
+  char *B32 = take(pc * npair * b, 8), *L1_32 = take(pc * npair * b, 8), *V = take(pc * npair * b, 8),
+       *Vp = take(pc * npair * b, 8), *Zh = take(pc * npair * b, 8);
+  char *Bs = take(pc * npair * b, 12), *L1s = take(pc * npair * b, 12), *Ls = take(pc * npair * b, 12),
+       *Rrs = take(pc * npair * b, 12);
+  char *Qa = take((long long)pc * nt * b, 12), *Q = take((long long)pc * nt * b, 12);
+  char *T1 = take((long long)pc * ns * b, 12), *T1p = take((long long)pc * ns * b, 12),
+       *R = take((long long)pc * ns * b, 12), *Rp = take((long long)pc * ns * b, 12);
+
+  const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
+  // tau(tf, t0_k): A role from the row-major planes, B role from the transposed planes
+  const LcView tau_A = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0);
+  const LcView tau_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TV);
+  const LcView tau_dag_B = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
+  const LcView W_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
+  const char* pr = spl(phi_r, (long long)p0 * nt * m);
+  const char* pt = spl(phi_t, (long long)p0 * nt * m);
+  const LcView phi_tf_A = kron(pr, m, 0, nt * m);
+  const LcView phi_tf_B = kron(pt, m, 0, nt * m, LC_TV);
+  const LcView phi_tf_dag_A = kron(pt, m, 0, nt * m, LC_CONJ);
+  const LcView phi_t0_A = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m);
+  const LcView phi_t0_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
+  const LcView phi_t0_dag_A = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ);
+  const LcView phi_t0_dag_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  const LcView D_tf = dense(tau_r, (nt + 1) * b, 0, 0);
+  const LcView D_t0 = dense(spl(tau_r, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0);
+  auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
+  char* rw = ::off(static_cast<char*>(raw), (long long)p0 * ns * nt * 10, 16);
+  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  auto slot = [&](int k, const LcView& Y, double ar = 1.0) {
+    FusedTrace f = fused(Y, ::off(rw, k, 16), O0, O1, O2, ar);
+    f.out_double = 1;
+    return f;
+  };
+  (void)f32;
+
+  Program pg(E.nv, Program::MX);
+  // level 1
+  pg.gemm(phi_tf_A, tau_B, B32, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Bs);
+  pg.gemm(phi_tf_dag_A, tau_B, L1_32, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, L1s);
+  pg.gemm(D_tf, phi_tf_B, nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Qa, true);
+  pg.gemm(D_t0, phi_t0_B, nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, T1, true);
+  pg.gemm(D_t0, phi_t0_dag_B, nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, T1p, true);
+  pg.barrier();
+  // level 2
+  const Spin g5 = gamma(5);
+  std::vector<FusedTrace> rho;
+  for (int i = 0; i < 3; i++) {
+    const Spin Lp = g5 * rhoj(1, i, false);
+    const Spin Kp = rhoj(1, i, true) * g5;
+    rho.push_back(slot(i, right(left(pair(B32, LC_DAG), Kp), Lp), -1.0));
+  }
+  pg.gemm(tau_A, phi_t0_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, rho);
+  pg.gemm(phi_t0_A, tau_dag_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(6, pair(B32)), slot(8, pair(L1_32))});
+  pg.gemm(phi_t0_dag_A, tau_dag_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(7, pair(L1_32)), slot(9, pair(B32))});
+  pg.gemm(pair(L1s), phi_t0_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Ls);
+  pg.gemm(pair(Bs), phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Rrs);
+  pg.gemm(phi_tf_dag_A, dense(Qa, b, 0, nt * b, LC_TV), nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Q);
+  pg.gemm(phi_t0_dag_A, dense(T1, 0, b, ns * b, LC_TV), nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, R);
+  pg.gemm(phi_t0_A, dense(T1p, 0, b, ns * b, LC_TV), nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, Rp);
+  pg.barrier();
+  // level 3
+  pg.gemm(dense(R, 0, b, ns * b), tau_dag_B, V, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(Rp, 0, b, ns * b), tau_dag_B, Vp, P0, P1, P2, nt, ns, pc);
+  pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  pg.barrier();
+  // level 4
+  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+          {slot(3, pair(V)), slot(4, pair(Vp))});
+  pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
+  pg.run(ctx().stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -161,16 +260,15 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
       fail(LC_ERR_ARG, "bad source list src0=%d step=%d nsrc=%d (nt=%d)", src0, src_step, nsrc, nt);
     if (!d_tau || !d_phi || !d_raw || !d_corr || !d_avg) fail(LC_ERR_ARG, "null pointer");
     EngineShape E{nt, nv, np, src0, src_step, nsrc, (long long)nv * nv, 16LL * nv * nv};
-    const size_t es = mixed ? 8 : 16;
     const long long npair = (long long)nt * nsrc;
-    // mixed path: per-slice rescaled FP32 copies of tau and Phi + FP64 scale factors, ahead of
-    // the scratch area
-    const size_t tau_bytes = mixed ? (size_t)nt * nt * E.blk * 8 : 0;
-    const size_t phi_bytes = mixed ? (size_t)np * nt * E.m * 8 : 0;
-    const size_t scale_bytes = mixed ? (size_t)(nt * nt + np * nt) * 8 : 0;
-    const size_t head = ((tau_bytes + 255) & ~size_t(255)) + ((phi_bytes + 255) & ~size_t(255)) +
-                        ((scale_bytes + 255) & ~size_t(255));
-    const size_t per_p = (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * es;
+    // mixed form: tau and Phi split twice (row-major and transposed planes, 12 B per complex
+    // each) + FP64 per-slice scale factors, ahead of the scratch area
+    const size_t tau_split = mixed ? (((size_t)nt * nt * E.blk * 12 + 255) & ~size_t(255)) : 0;
+    const size_t phi_split = mixed ? (((size_t)np * nt * E.m * 12 + 255) & ~size_t(255)) : 0;
+    const size_t scale_bytes = mixed ? (((size_t)(nt * nt + np * nt) * 8 + 255) & ~size_t(255)) : 0;
+    const size_t head = 2 * tau_split + 2 * phi_split + scale_bytes;
+    const size_t per_p = mixed ? mx_bytes_per_momentum(E) + 20 * 256
+                               : (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * 16;
     if (max_scratch_bytes <= 0) max_scratch_bytes = 64LL << 30;
     long long avail = max_scratch_bytes - (long long)head;
     int pc = avail > 0 ? (int)(avail / (long long)per_p) : 0;
@@ -179,22 +277,26 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     ctx().scratch.ensure(head + per_p * pc + 4096);
     char* base = static_cast<char*>(ctx().scratch.p);
     hipStream_t s = ctx().stream;
-    const void* tau = d_tau;
-    const void* phi = d_phi;
     double* scales = nullptr;
-    if (mixed) {
-      char* t32 = base;
-      char* p32 = t32 + ((tau_bytes + 255) & ~size_t(255));
-      scales = reinterpret_cast<double*>(p32 + ((phi_bytes + 255) & ~size_t(255)));
-      hip_check(lc_launch_rescale(d_tau, t32, E.blk, (long long)nt * nt, scales, s), "rescale tau");
-      hip_check(lc_launch_rescale(d_phi, p32, E.m, (long long)np * nt, scales + (size_t)nt * nt, s), "rescale phi");
-      tau = t32;
-      phi = p32;
-    }
     char* scratch = base + head;
-    for (int p0 = 0; p0 < np; p0 += pc) {
-      const int n = (np - p0) < pc ? (np - p0) : pc;
-      run_chunk(E, mixed ? Program::MX : Program::F64, tau, phi, p0, n, scratch, d_raw);
+    if (mixed) {
+      char* tau_r = base;
+      char* tau_t = tau_r + tau_split;
+      char* phi_r = tau_t + tau_split;
+      char* phi_t = phi_r + phi_split;
+      scales = reinterpret_cast<double*>(phi_t + phi_split);
+      hip_check(lc_launch_split_rescale(d_tau, tau_r, tau_t, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
+      hip_check(lc_launch_split_rescale(d_phi, phi_r, phi_t, E.m, nv, (long long)np * nt, scales + (size_t)nt * nt, s),
+                "split phi");
+      for (int p0 = 0; p0 < np; p0 += pc) {
+        const int n = (np - p0) < pc ? (np - p0) : pc;
+        run_chunk_mx(E, tau_r, tau_t, phi_r, phi_t, p0, n, scratch, d_raw);
+      }
+    } else {
+      for (int p0 = 0; p0 < np; p0 += pc) {
+        const int n = (np - p0) < pc ? (np - p0) : pc;
+        run_chunk(E, Program::F64, d_tau, d_phi, p0, n, scratch, d_raw);
+      }
     }
     hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales,
                                         scales ? scales + (size_t)nt * nt : nullptr, s),

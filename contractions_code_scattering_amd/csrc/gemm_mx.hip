@@ -1,24 +1,30 @@
 // Mixed-precision block-complex GEMM on the BF16 matrix cores (gfx950).
 //
-// Operands are stored as complex FP32. Every real value x entering a product is split exactly
-// into three BF16 pieces (x = x0 + x1 + x2, each piece the round-to-nearest BF16 of what the
-// previous ones leave; an FP32 significand has 24 bits = 3 x 8, so the split loses nothing).
-// A real product a*b is formed from the six piece products of weight >= 2^-16:
+// Every real value x of an operand is held as three BF16 pieces, x = x0 + x1 + x2 exactly
+// (each piece the round-to-nearest BF16 of what the previous ones leave; an FP32 significand
+// has 24 bits = 3 x 8, so nothing is lost). A real product a*b is formed from the six piece
+// products of weight >= 2^-16:
 //   a0b0 + [a0b1 + a1b0 + a0b2 + a1b1 + a2b0]    (dropped terms are below 2^-24 |a||b|)
 // on v_mfma_f32_16x16x32_bf16 with FP32 accumulation, and the complex product from four real
-// products (re = ArBr - AiBi, im = ArBi + AiBr): 24 MFMAs per 16x16x32 complex tile. The leading
+// products (re = ArBr - AiBi, im = ArBi + AiBr): 24 MFMAs per 16x16x32 complex step. The leading
 // a0b0 terms and the bracketed correction terms accumulate in separate registers, so the many
-// small corrections never round the leading sum. Output is complex FP32; fused closing traces
-// accumulate in FP64 with the same descriptors and per-tile partials as the FP64 kernel.
+// small corrections never round the leading sum.
+//
+// Operands arrive already split ("split storage", lc_internal.h): the epilogue of the GEMM that
+// produces a matrix writes its pieces once, and the inputs are split once by
+// lc_split_convert_kernel. The main loop therefore only moves 16-byte BF16 vectors from global
+// memory to LDS and feeds the matrix cores; view phases (powers of i) and conjugation are sign
+// flips and a re/im plane exchange applied on the way into LDS. Output is FP32 complex and/or
+// split pieces; fused closing traces accumulate in FP64 with the same descriptors and per-tile
+// partials as the FP64 kernel.
 //
 // Workgroup: 256 threads = 4 wave64 (2 x 2), 64 x 64 output tile of one spin block, K step 32.
-// Each element is split once, when staged: a thread fetches 8 consecutive-K values of one row
-// of A and of one column of B (the mapping follows the operand's memory order, so the global
-// reads are coalesced or vectorized), splits them with packed two-at-a-time conversions while
-// the matrix cores work on the previous step, and writes 16-byte BF16 vectors into six LDS
-// planes [piece][re/im]. A workgroup processes `bpw` consecutive spin blocks of one product
-// (bpw = 4 for Nv <= 64): prologue, elemental tile and trace reductions are amortized over four
-// blocks, and the fetch pipeline runs across block boundaries.
+// The MFMA is issued with the operands exchanged (B tile as the first operand), which yields the
+// transposed product: lane l then holds four consecutive columns 4(l/16)..+3 of row l%16 of C,
+// so the epilogue writes 8-byte piece vectors and 32-byte FP32 vectors. A workgroup processes
+// `bpw` consecutive spin blocks of one product (bpw = 4 for Nv <= 64): prologue, elemental tile
+// and trace reductions are amortized over four blocks, and the fetch pipeline runs across block
+// boundaries.
 #include "lc_device.h"
 
 namespace lc {
@@ -28,6 +34,7 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float fx4v __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ fx4v mfma_bf16(const bf16x8& a, const bf16x8& b, fx4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -37,7 +44,7 @@ __device__ __forceinline__ fx4v mfma_bf16(const bf16x8& a, const bf16x8& b, fx4v
 // (measured with lc_mfma_bf16_selftest_kernel).
 __device__ __forceinline__ int acc_row_bf16(int l, int q) { return ((l >> 4) << 2) + q; }
 
-// Two FP32 -> packed round-to-nearest BF16 pair (one v_cvt_pk_bf16_f32).
+// Two FP32 -> packed round-to-nearest BF16 pair (one v_cvt_pk_bf16_f32), a in the low half.
 __device__ __forceinline__ unsigned cvt2(float a, float b) {
   const bf16x2 v = __builtin_convertvector((f2v){a, b}, bf16x2);
   return __builtin_bit_cast(unsigned, v);
@@ -45,7 +52,7 @@ __device__ __forceinline__ unsigned cvt2(float a, float b) {
 __device__ __forceinline__ float lo_f(unsigned u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float hi_f(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
 
-// Exact three-way split of a pair: p0,p1,p2 hold the pieces of (a, b), a in the low half.
+// Exact three-way split of a pair: p0, p1, p2 hold the pieces of (a, b), a in the low half.
 __device__ __forceinline__ void split2(float a, float b, unsigned& p0, unsigned& p1, unsigned& p2) {
   p0 = cvt2(a, b);
   const float a1 = a - lo_f(p0), b1 = b - hi_f(p0);
@@ -57,64 +64,64 @@ __device__ __forceinline__ void split2(float a, float b, unsigned& p0, unsigned&
 constexpr int MX_TM = 64;
 constexpr int MX_KT = 32;
 constexpr int MX_LD = MX_KT + 8;  // padded LDS row length, in BF16
+constexpr unsigned SIGN2 = 0x80008000u;
 
-// 8 consecutive-K complex values of one line of an operand: global -> registers.
-struct Fetch {
-  float re[8], im[8];
+// Block (a,b) of a split view: base of its six planes, element flags, exact phase.
+struct SplitRef {
+  const unsigned short* p;
+  int flags;
+  float sr, si;
 };
 
-template <bool IS_A>
-__device__ __forceinline__ void fetch8(const BlockRef<float>& R, int line, int k, int nv, bool full,
-                                       Fetch& f) {
-  // A element (line, k) sits at line*nv + k unless LC_TV; B element (k, line) at k*nv + line
-  // unless LC_TV. "contig": the 8 values are consecutive in memory.
-  const bool tv = (R.flags & LC_TV) != 0;
-  const bool contig = IS_A ? !tv : tv;
-  const float2* p = reinterpret_cast<const float2*>(R.p);
-  if (full) {
-    const float2* c8 = p + (long long)line * nv + k;
-    if (contig && (reinterpret_cast<uintptr_t>(c8) & 15) == 0) {
-      const float4* q = reinterpret_cast<const float4*>(c8);
+__device__ __forceinline__ SplitRef split_block(const LcView& v, long long off, int a, int b,
+                                                long long nv2) {
+  SplitRef r;
+  long long e = off;
+  r.flags = v.flags;
+  r.sr = 1.f;
+  r.si = 0.f;
+  if (v.kind != LC_KRON) {
+    const int ra = v.perm_r[a], cb = v.perm_c[b];
+    const int blk = (v.flags & LC_TS) ? (cb * 4 + ra) : (ra * 4 + cb);
+    e += (long long)blk * nv2;
+    r.sr = (float)(v.ph_r[a][0] * v.ph_c[b][0] - v.ph_r[a][1] * v.ph_c[b][1]);
+    r.si = (float)(v.ph_r[a][0] * v.ph_c[b][1] + v.ph_r[a][1] * v.ph_c[b][0]);
+  }
+  r.p = reinterpret_cast<const unsigned short*>(v.base) + 6 * e;
+  return r;
+}
+
+// Six planes of 8 consecutive-K values of one line: element (line, k + e) of a plane sits at
+// line * nv + k + e (A rows / B columns are contiguous along K, enforced by the planner).
+__device__ __forceinline__ void fetch_split(const unsigned short* p, int line, int k, int nv, long long nv2,
+                                            bool vec, u32x4 (&v)[6]) {
+  if (vec) {
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const float4 v = q[e];
-        f.re[2 * e] = v.x; f.im[2 * e] = v.y; f.re[2 * e + 1] = v.z; f.im[2 * e + 1] = v.w;
-      }
-    } else if (contig) {
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const float2 v = c8[e];
-        f.re[e] = v.x; f.im[e] = v.y;
-      }
-    } else {
-      const float2* q = p + (long long)k * nv + line;
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const float2 v = q[(long long)e * nv];
-        f.re[e] = v.x; f.im[e] = v.y;
-      }
-    }
+    for (int q = 0; q < 6; q++)
+      v[q] = gload<u32x4>(p + q * nv2 + (long long)line * nv + k);
   } else {
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      float2 v = make_float2(0.f, 0.f);
-      if (line < nv && k + e < nv)
-        v = contig ? p[(long long)line * nv + k + e] : p[(long long)(k + e) * nv + line];
-      f.re[e] = v.x; f.im[e] = v.y;
-    }
-  }
-  if (R.flags & LC_CONJ) {
+    for (int q = 0; q < 6; q++) {
+      unsigned w[4];
 #pragma unroll
-    for (int e = 0; e < 8; e++) f.im[e] = -f.im[e];
+      for (int h = 0; h < 4; h++) {
+        unsigned lo = 0, hi = 0;
+        if (line < nv && k + 2 * h < nv) lo = gload<unsigned short>(p + q * nv2 + (long long)line * nv + k + 2 * h);
+        if (line < nv && k + 2 * h + 1 < nv)
+          hi = gload<unsigned short>(p + q * nv2 + (long long)line * nv + k + 2 * h + 1);
+        w[h] = lo | (hi << 16);
+      }
+      v[q] = u32x4{w[0], w[1], w[2], w[3]};
+    }
   }
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total, int bpw) {
+lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total) {
   typedef float T;
-  // planes [piece][re/im][line][k]
-  __shared__ __attribute__((aligned(16))) __bf16 As[3][2][MX_TM][MX_LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][2][MX_TM][MX_LD];
+  // planes [piece * 2 + re/im][line][k]
+  __shared__ __attribute__((aligned(16))) __bf16 As[6][MX_TM][MX_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[6][MX_TM][MX_LD];
   __shared__ double red[8];
 
   const long long wg = xcd_order(total);
@@ -123,6 +130,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const LcGemmGroup& g = groups[gi];
   const int tiles = g.tiles;
   const int tt = tiles * tiles;
+  const int bpw = g.bpw;
   const long long local = wg - g.wg_begin;
   const long long per_item = 16LL * tt;            // output tiles per item
   const long long wg_per_item = per_item / bpw;    // workgroups per item
@@ -134,6 +142,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const long long nv2 = (long long)nv * nv;
   const long long offA = voff(g.A.s, co), offB = voff(g.B.s, co);
   const bool akron = g.A.kind == LC_KRON, bkron = g.B.kind == LC_KRON;
+  const bool vec_ok = (nv & 7) == 0;  // 16-byte plane vectors stay aligned
 
   const int nterm = (akron || bkron) ? 1 : 4;
   const int nk = (nv + MX_KT - 1) / MX_KT;
@@ -143,6 +152,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
+  const int s_line = tid >> 2, s_kg = tid & 3;  // staging: line and 8-wide K group of this thread
 
   fx4v bre[2][2], bim[2][2], sre[2][2], sim[2][2];  // leading (a0b0) and correction sums
   auto zero_acc = [&]() {
@@ -157,9 +167,9 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   zero_acc();
 
   // staging state of the chunk in flight
-  Fetch fa, fb;
-  unsigned pa[3][2][4], pb[3][2][4];  // split pieces [piece][re/im][pair]
-  int a_line = 0, a_kg = 0, b_line = 0, b_kg = 0;
+  u32x4 va[6], vb[6];
+  int a_swap = 0;
+  unsigned a_mr = 0, a_mi = 0, b_mi = 0;
 
   auto tile_of = [&](int tl, int& a, int& c, int& tm, int& tn) {
     const int rr = r0 + tl;
@@ -171,7 +181,10 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     c = blk & 3;
   };
 
-  // Fetch chunk ch of both operands into registers; the view phase goes with A.
+  // Fetch chunk ch of both operands into registers and work out how the combined view phase
+  // (carried by A) and the conjugations apply: with A' = conj?(A) the stored plane pair (R, I),
+  //   phase  1: (R, I)      -1: (-R, -I)      i: (-I, R)      -i: (I, -R)
+  // i.e. R goes to plane slot a_swap with sign mask a_mr, I to slot 1 - a_swap with a_mi.
   auto fetch = [&](int ch) {
     const int tl = ch / cpb;
     const int rem = ch - tl * cpb;
@@ -183,59 +196,60 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     if (akron) { ab = a; bb = a; ba = a; bc = c; }
     else if (bkron) { ab = a; bb = c; ba = c; bc = c; }
     else { ab = a; bb = term; ba = term; bc = c; }
-    BlockRef<T> A = view_block<T>(g.A, offA, ab, bb, nv2);
-    const BlockRef<T> B = view_block<T>(g.B, offB, ba, bc, nv2);
-    const T sr = A.sr * B.sr - A.si * B.si, si = A.sr * B.si + A.si * B.sr;
-    // thread -> (line, k-group) following the memory order of each operand
-    if (A.flags & LC_TV) { a_line = tid & 63; a_kg = tid >> 6; } else { a_line = tid >> 2; a_kg = tid & 3; }
-    if (B.flags & LC_TV) { b_line = tid >> 2; b_kg = tid & 3; } else { b_line = tid & 63; b_kg = tid >> 6; }
-    const bool fullA = (tm + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
-    const bool fullB = (tn + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
-    fetch8<true>(A, tm * MX_TM + a_line, k0 + a_kg * 8, nv, fullA, fa);
-    fetch8<false>(B, tn * MX_TM + b_line, k0 + b_kg * 8, nv, fullB, fb);
-    if (!(sr == 1.f && si == 0.f)) {
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const float x = fa.re[e], y = fa.im[e];
-        fa.re[e] = sr * x - si * y;
-        fa.im[e] = sr * y + si * x;
-      }
+    const SplitRef A = split_block(g.A, offA, ab, bb, nv2);
+    const SplitRef B = split_block(g.B, offB, ba, bc, nv2);
+    const float sr = A.sr * B.sr - A.si * B.si, si = A.sr * B.si + A.si * B.sr;
+    const unsigned ca = (A.flags & LC_CONJ) ? SIGN2 : 0u;
+    if (si == 0.f) {
+      a_swap = 0;
+      a_mr = sr < 0.f ? SIGN2 : 0u;
+      a_mi = a_mr ^ ca;
+    } else {
+      a_swap = 1;
+      // phase i: slot re <- -I', slot im <- R;  phase -i: slot re <- I', slot im <- -R
+      a_mr = si > 0.f ? 0u : SIGN2;         // sign applied to R (lands in the im slot)
+      a_mi = (si > 0.f ? SIGN2 : 0u) ^ ca;  // sign applied to I (lands in the re slot)
     }
-  };
-
-  auto split = [&]() {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      split2(fa.re[2 * q], fa.re[2 * q + 1], pa[0][0][q], pa[1][0][q], pa[2][0][q]);
-      split2(fa.im[2 * q], fa.im[2 * q + 1], pa[0][1][q], pa[1][1][q], pa[2][1][q]);
-      split2(fb.re[2 * q], fb.re[2 * q + 1], pb[0][0][q], pb[1][0][q], pb[2][0][q]);
-      split2(fb.im[2 * q], fb.im[2 * q + 1], pb[0][1][q], pb[1][1][q], pb[2][1][q]);
-    }
+    b_mi = (B.flags & LC_CONJ) ? SIGN2 : 0u;
+    const int k = k0 + s_kg * 8;
+    const bool fullA = vec_ok && (reinterpret_cast<uintptr_t>(A.p) & 15) == 0 && (tm + 1) * MX_TM <= nv &&
+                       k0 + MX_KT <= nv;
+    const bool fullB = vec_ok && (reinterpret_cast<uintptr_t>(B.p) & 15) == 0 && (tn + 1) * MX_TM <= nv &&
+                       k0 + MX_KT <= nv;
+    fetch_split(A.p, tm * MX_TM + s_line, k, nv, nv2, fullA, va);
+    fetch_split(B.p, tn * MX_TM + s_line, k, nv, nv2, fullB, vb);
   };
 
   auto store = [&]() {
 #pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-      for (int ri = 0; ri < 2; ri++) {
-        *reinterpret_cast<u32x4*>(&As[p][ri][a_line][a_kg * 8]) =
-            u32x4{pa[p][ri][0], pa[p][ri][1], pa[p][ri][2], pa[p][ri][3]};
-        *reinterpret_cast<u32x4*>(&Bs[p][ri][b_line][b_kg * 8]) =
-            u32x4{pb[p][ri][0], pb[p][ri][1], pb[p][ri][2], pb[p][ri][3]};
-      }
+    for (int p = 0; p < 3; p++) {
+      *reinterpret_cast<u32x4*>(&As[2 * p + a_swap][s_line][s_kg * 8]) = va[2 * p] ^ a_mr;
+      *reinterpret_cast<u32x4*>(&As[2 * p + 1 - a_swap][s_line][s_kg * 8]) = va[2 * p + 1] ^ a_mi;
+      *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_kg * 8]) = vb[2 * p];
+      *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_kg * 8]) = vb[2 * p + 1] ^ b_mi;
+    }
   };
 
-  // per-tile epilogue: store and fused trace accumulation (FP64, carried across tiles)
+  // per-tile epilogue: stores and fused trace accumulation (FP64, carried across tiles)
   double tsr[LC_MAX_FUSED_TRACES], tsi[LC_MAX_FUSED_TRACES];
 #pragma unroll
   for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) { tsr[t] = 0; tsi[t] = 0; }
   const T alr = T(g.alpha[0]), ali = T(g.alpha[1]);
+  const bool vec_c = (nv & 3) == 0;
 
+  // Per-tile epilogue: stores, and the fused trace pieces sum_ij C_ij Y_ji in FP64. Every load
+  // of the trace operands (all traces, all 16 values of this lane) is issued before the first
+  // one is used, so the tile pays one memory round trip instead of one per value.
   auto epilogue = [&](int tl) {
+    // keep the scheduler from hoisting the epilogue's loads into the matrix-core section
+    __builtin_amdgcn_sched_barrier(0);
     int a, c, tm, tn;
     tile_of(tl, a, c, tm, tn);
-    const int row0 = tm * MX_TM, col0 = tn * MX_TM;
-    T zr[2][2][4], zi[2][2][4];
+    const long long cofs = voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
+    const int rbase = tm * MX_TM + wm * 32 + (lane & 15);
+    const int cbase = tn * MX_TM + wn * 32 + 4 * (lane >> 4);
+    // the tile's values (alpha applied); the accumulators are free from here on
+    float zr_[2][2][4], zi_[2][2][4];
 #pragma unroll
     for (int m = 0; m < 2; m++)
 #pragma unroll
@@ -243,30 +257,94 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const T xr = bre[m][n][q] + sre[m][n][q], xi = bim[m][n][q] + sim[m][n][q];
-          zr[m][n][q] = alr * xr - ali * xi;
-          zi[m][n][q] = alr * xi + ali * xr;
+          zr_[m][n][q] = alr * xr - ali * xi;
+          zi_[m][n][q] = alr * xi + ali * xr;
         }
-    if (g.C) {
-      Cplx<T>* C = reinterpret_cast<Cplx<T>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
+    zero_acc();
+    // trace operand of trace t: this lane's 16 values of f(Y), element (col, row), loaded with no
+    // use in between; block phase and conjugation kept aside
+    struct YTile {
+      f2v v[2][2][4];
+      double pr, pi;
+      float cj;
+    };
+    auto load_y = [&](int t, YTile& y) {
+      y.pr = 0; y.pi = 0; y.cj = 1.f;
+      BlockRef<T> Y;
+      Y.zero = true;
+      if (t < g.ntr) Y = view_block<T>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
+      if (!Y.zero) { y.pr = Y.sr; y.pi = Y.si; y.cj = (Y.flags & LC_CONJ) ? -1.f : 1.f; }
+      const bool tv = (Y.flags & LC_TV) != 0;
 #pragma unroll
       for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int n = 0; n < 2; n++)
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            const int row = row0 + wm * 32 + m * 16 + acc_row_bf16(lane, q);
-            const int col = col0 + wn * 32 + n * 16 + (lane & 15);
-            if (row < nv && col < nv) {
-              Cplx<T> z;
-              z.x = zr[m][n][q];
-              z.y = zi[m][n][q];
-              C[(long long)row * nv + col] = z;
+            const int row = rbase + m * 16, col = cbase + n * 16 + q;
+            const long long idx = tv ? (long long)row * nv + col : (long long)col * nv + row;
+            y.v[m][n][q] = (!Y.zero && row < nv && col < nv) ? gload<f2v>(Y.p + idx) : f2v{0.f, 0.f};
+          }
+    };
+    YTile ycur, ynext;
+    if (g.ntr > 0) load_y(0, ycur);
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int n = 0; n < 2; n++) {
+        // this lane's values: row `row`, columns col..col+3
+        const int row = rbase + m * 16;
+        const int col = cbase + n * 16;
+        const float* zr = zr_[m][n];
+        const float* zi = zi_[m][n];
+        const bool rin = row < nv;
+        const bool full4 = rin && col + 3 < nv;
+        if (g.C) {
+          float* C = reinterpret_cast<float*>(g.C) + 2 * (cofs + (long long)row * nv + col);
+          if (full4 && vec_c && (cofs & 1) == 0) {
+            gstore(C, fx4v{zr[0], zi[0], zr[1], zi[1]});
+            gstore(C + 4, fx4v{zr[2], zi[2], zr[3], zi[3]});
+          } else if (rin) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (col + q < nv) gstore(C + 2 * q, f2v{zr[q], zi[q]});
+          }
+        }
+        if (g.Cs) {
+          unsigned r01[3], r23[3], i01[3], i23[3];
+          split2(zr[0], zr[1], r01[0], r01[1], r01[2]);
+          split2(zr[2], zr[3], r23[0], r23[1], r23[2]);
+          split2(zi[0], zi[1], i01[0], i01[1], i01[2]);
+          split2(zi[2], zi[3], i23[0], i23[1], i23[2]);
+          unsigned short* S = reinterpret_cast<unsigned short*>(g.Cs) + 6 * cofs;
+          if (!g.cs_trans && full4 && vec_c && (cofs & 1) == 0) {
+            const long long e = (long long)row * nv + col;
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+              gstore(S + (2 * p) * nv2 + e, u32x2{r01[p], r23[p]});
+              gstore(S + (2 * p + 1) * nv2 + e, u32x2{i01[p], i23[p]});
+            }
+          } else if (rin) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              if (col + q >= nv) continue;
+              const long long e = g.cs_trans ? (long long)(col + q) * nv + row : (long long)row * nv + col + q;
+              const int sh = (q & 1) * 16;
+#pragma unroll
+              for (int p = 0; p < 3; p++) {
+                const unsigned wr = q < 2 ? r01[p] : r23[p], wi = q < 2 ? i01[p] : i23[p];
+                gstore(S + (2 * p) * nv2 + e, (unsigned short)(wr >> sh));
+                gstore(S + (2 * p + 1) * nv2 + e, (unsigned short)(wi >> sh));
+              }
             }
           }
-    }
-    for (int t = 0; t < g.ntr; t++) {
-      const BlockRef<T> Y = view_block<T>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
-      if (Y.zero) continue;
+        }
+      }
+    // traces, the loads of trace t+1 in flight while trace t is summed
+#pragma unroll
+    for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
+      if (t >= g.ntr) break;
+      if (t + 1 < g.ntr) load_y(t + 1, ynext);
       double sr = 0, si = 0;
 #pragma unroll
       for (int m = 0; m < 2; m++)
@@ -274,24 +352,19 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         for (int n = 0; n < 2; n++)
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            const int row = row0 + wm * 32 + m * 16 + acc_row_bf16(lane, q);
-            const int col = col0 + wn * 32 + n * 16 + (lane & 15);
-            if (row < nv && col < nv) {
-              const Cplx<T> y = elem(Y, col, row, nv);
-              const double xr = zr[m][n][q], xi = zi[m][n][q];
-              sr += xr * (double)y.x - xi * (double)y.y;
-              si += xr * (double)y.y + xi * (double)y.x;
-            }
+            const double yr = ycur.v[m][n][q][0], yi = ycur.cj * ycur.v[m][n][q][1];
+            const double xr = zr_[m][n][q], xi = zi_[m][n][q];
+            sr += xr * yr - xi * yi;
+            si += xr * yi + xi * yr;
           }
-      const double yr = Y.sr, yi = Y.si;  // the Y block phase is uniform over the tile
-      tsr[t] += sr * yr - si * yi;
-      tsi[t] += sr * yi + si * yr;
+      // the Y block phase is uniform over the tile
+      tsr[t] += sr * ycur.pr - si * ycur.pi;
+      tsi[t] += sr * ycur.pi + si * ycur.pr;
+      ycur = ynext;
     }
-    zero_acc();
   };
 
   fetch(0);
-  split();
   store();
   __syncthreads();
   const int kq = (lane >> 4) * 8;
@@ -305,7 +378,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++)
-          bf[n][p][ri] = *reinterpret_cast<const bf16x8*>(&Bs[p][ri][wn * 32 + n * 16 + (lane & 15)][kq]);
+          bf[n][p][ri] = *reinterpret_cast<const bf16x8*>(&Bs[2 * p + ri][wn * 32 + n * 16 + (lane & 15)][kq]);
 #pragma unroll
     for (int m = 0; m < 2; m++) {
       bf16x8 af[3][2], nai[3];
@@ -313,30 +386,30 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++)
-          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[p][ri][wm * 32 + m * 16 + (lane & 15)][kq]);
+          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[2 * p + ri][wm * 32 + m * 16 + (lane & 15)][kq]);
 #pragma unroll
       for (int p = 0; p < 3; p++)
-        nai[p] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, af[p][1]) ^ 0x80008000u);
+        nai[p] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, af[p][1]) ^ SIGN2);
 #pragma unroll
       for (int n = 0; n < 2; n++) {
-        // correction terms (weights 2^-8 and 2^-16), then the leading term
+        // correction terms (weights 2^-8 and 2^-16), then the leading term; B tile first, so
+        // the accumulators hold the transposed product
         const int PI[5] = {2, 1, 0, 1, 0};
         const int PJ[5] = {0, 1, 2, 0, 1};
 #pragma unroll
         for (int s = 0; s < 5; s++) {
           const int i = PI[s], j = PJ[s];
-          sre[m][n] = mfma_bf16(af[i][0], bf[n][j][0], sre[m][n]);
-          sre[m][n] = mfma_bf16(nai[i], bf[n][j][1], sre[m][n]);
-          sim[m][n] = mfma_bf16(af[i][0], bf[n][j][1], sim[m][n]);
-          sim[m][n] = mfma_bf16(af[i][1], bf[n][j][0], sim[m][n]);
+          sre[m][n] = mfma_bf16(bf[n][j][0], af[i][0], sre[m][n]);
+          sre[m][n] = mfma_bf16(bf[n][j][1], nai[i], sre[m][n]);
+          sim[m][n] = mfma_bf16(bf[n][j][1], af[i][0], sim[m][n]);
+          sim[m][n] = mfma_bf16(bf[n][j][0], af[i][1], sim[m][n]);
         }
-        bre[m][n] = mfma_bf16(af[0][0], bf[n][0][0], bre[m][n]);
-        bre[m][n] = mfma_bf16(nai[0], bf[n][0][1], bre[m][n]);
-        bim[m][n] = mfma_bf16(af[0][0], bf[n][0][1], bim[m][n]);
-        bim[m][n] = mfma_bf16(af[0][1], bf[n][0][0], bim[m][n]);
+        bre[m][n] = mfma_bf16(bf[n][0][0], af[0][0], bre[m][n]);
+        bre[m][n] = mfma_bf16(bf[n][0][1], nai[0], bre[m][n]);
+        bim[m][n] = mfma_bf16(bf[n][0][1], af[0][0], bim[m][n]);
+        bim[m][n] = mfma_bf16(bf[n][0][0], af[0][1], bim[m][n]);
       }
     }
-    if (more) split();  // VALU work that can overlap the matrix-core work issued above
     if ((ch + 1) % cpb == 0) epilogue(ch / cpb);
     __syncthreads();
     if (more) {
@@ -345,14 +418,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     }
   }
 
-  for (int t = 0; t < g.ntr; t++) {
+#pragma unroll
+  for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
+    if (t >= g.ntr) break;
     double sr = tsr[t], si = tsi[t];
     block_sum2(sr, si, red);
     if (tid == 0) {
       const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
       double* out = g.part[t] + 2 * (item * wg_per_item + r0 / bpw);
-      out[0] = ar * sr - ai * si;
-      out[1] = ar * si + ai * sr;
+      typedef double d2v __attribute__((ext_vector_type(2)));
+      gstore(out, d2v{ar * sr - ai * si, ar * si + ai * sr});
     }
   }
 }
@@ -376,8 +451,10 @@ __global__ void lc_mfma_bf16_selftest_kernel(const float* a, const float* b, flo
 }
 
 // ------------------------------------------------------------------------------------------
-// Per-slice rescaling for the mixed path: y[i] = x[i] / ||x_i||_F, in FP32, with the scale
-// ||x_i||_F of every slice i (nslice contiguous slices of len complex values) kept in FP64.
+// Inputs of the mixed path: per-slice Frobenius scale ||x_i||_F (FP64), then x_i / ||x_i||_F
+// rounded to FP32 and split into the six BF16 planes, written twice: row-major planes (y_rows)
+// and with every Nv x Nv block transposed (y_trans), so that each operand role finds its K
+// dimension contiguous. A slice is `len` complex values made of len / Nv^2 blocks.
 __global__ void __launch_bounds__(256) lc_slice_norm_kernel(const double2* __restrict__ x,
                                                             long long len, long long stride,
                                                             double* __restrict__ scale) {
@@ -392,27 +469,40 @@ __global__ void __launch_bounds__(256) lc_slice_norm_kernel(const double2* __res
   if (threadIdx.x == 0) scale[blockIdx.x] = s > 0 ? sqrt(s) : 1.0;
 }
 
-__global__ void lc_slice_convert_kernel(const double2* __restrict__ x, float2* __restrict__ y,
-                                        long long len, const double* __restrict__ scale,
-                                        long long nslice) {
+__global__ void lc_split_convert_kernel(const double2* __restrict__ x, unsigned short* __restrict__ yr,
+                                        unsigned short* __restrict__ yt, long long len, int nv,
+                                        const double* __restrict__ scale, long long nslice) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len * nslice) return;
+  const long long nv2 = (long long)nv * nv;
+  const long long bi = i / nv2;  // global block index
+  const long long e = i - bi * nv2;
+  const int row = (int)(e / nv), col = (int)(e - (long long)(e / nv) * nv);
   const double inv = 1.0 / scale[i / len];
   const double2 v = x[i];
-  y[i] = make_float2((float)(v.x * inv), (float)(v.y * inv));
+  unsigned p0, p1, p2;
+  split2((float)(v.x * inv), (float)(v.y * inv), p0, p1, p2);
+  const unsigned pc[3] = {p0, p1, p2};
+  unsigned short* br = yr + 6 * bi * nv2;
+  unsigned short* bt = yt + 6 * bi * nv2;
+  const long long et = (long long)col * nv + row;
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    br[(2 * p) * nv2 + e] = (unsigned short)(pc[p] & 0xffff);
+    br[(2 * p + 1) * nv2 + e] = (unsigned short)(pc[p] >> 16);
+    bt[(2 * p) * nv2 + et] = (unsigned short)(pc[p] & 0xffff);
+    bt[(2 * p + 1) * nv2 + et] = (unsigned short)(pc[p] >> 16);
+  }
 }
 
 }  // namespace lc
 
-// Spin blocks per workgroup: 4 when one 64 x 64 tile covers a block.
-extern "C" int lc_gemm_mx_bpw(int nv) { return nv <= 64 ? 4 : 1; }
-
-extern "C" hipError_t lc_launch_gemm_mx(const LcGemmGroup* d_groups, int ng, long long total_wg,
-                                        int nv, hipStream_t s) {
+extern "C" hipError_t lc_launch_gemm_mx(const LcGemmGroup* d_groups, int ng, long long total_wg, int nv,
+                                        hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
   const long long per = (total_wg + 7) / 8;
   hipLaunchKernelGGL(lc::lc_gemm_mx_kernel, dim3((unsigned)(8 * per)), dim3(256), 0, s, d_groups, ng, nv,
-                     total_wg, lc_gemm_mx_bpw(nv));
+                     total_wg);
   return hipGetLastError();
 }
 
@@ -422,14 +512,16 @@ extern "C" hipError_t lc_launch_mfma_bf16_selftest(const float* a, const float* 
   return hipGetLastError();
 }
 
-// x: nslice slices of len complex doubles (contiguous); y: same layout in complex float.
-extern "C" hipError_t lc_launch_rescale(const void* x, void* y, long long len, long long nslice,
-                                        double* scale, hipStream_t s) {
+// x: nslice slices of len complex doubles (contiguous, len a multiple of nv^2); y_rows, y_trans:
+// the split planes (6 * len * nslice BF16 each); scale: nslice doubles.
+extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, long long len,
+                                              int nv, long long nslice, double* scale, hipStream_t s) {
   if (nslice <= 0 || len <= 0) return hipSuccess;
   hipLaunchKernelGGL(lc::lc_slice_norm_kernel, dim3((unsigned)nslice), dim3(256), 0, s,
                      (const double2*)x, len, len, scale);
   const long long n = len * nslice;
-  hipLaunchKernelGGL(lc::lc_slice_convert_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     (const double2*)x, (float2*)y, len, (const double*)scale, nslice);
+  hipLaunchKernelGGL(lc::lc_split_convert_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     (const double2*)x, (unsigned short*)y_rows, (unsigned short*)y_trans, len, nv,
+                     (const double*)scale, nslice);
   return hipGetLastError();
 }

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
 
   __shared__ T As_re[2][KT][TM + PAD], As_im[2][KT][TM + PAD];
   __shared__ T Bs_re[2][KT][TM + PAD], Bs_im[2][KT][TM + PAD];
-  __shared__ double red[8];
+  __shared__ double tred[LC_MAX_FUSED_TRACES][2][4];
 
   const long long wg = xcd_order(total);
   if (wg < 0) return;
@@ -215,39 +215,74 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
             Cplx<T> z;
             z.x = zr[m][n][q];
             z.y = zi[m][n][q];
-            C[(long long)row * nv + col] = z;
+            gstore_c(C + (long long)row * nv + col, z);
           }
         }
   }
-  for (int t = 0; t < g.ntr; t++) {
+  // Fused trace pieces sum_ij C_ij Y_ji: per trace, all 4 * MI * MI loads of this lane are issued
+  // before the first is used (one memory round trip per trace), then one combined reduction.
+  double tr_re[LC_MAX_FUSED_TRACES], tr_im[LC_MAX_FUSED_TRACES];
+#pragma unroll
+  for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
+    tr_re[t] = 0;
+    tr_im[t] = 0;
+    if (t >= g.ntr) continue;
     const BlockRef<T> Y = view_block<T>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
+    if (Y.zero) continue;
+    const bool tv = (Y.flags & LC_TV) != 0;
+    Cplx<T> yv[MI][MI][4];
+#pragma unroll
+    for (int m = 0; m < MI; m++)
+#pragma unroll
+      for (int n = 0; n < MI; n++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
+          const int col = col0 + wn * WM + n * 16 + (lane & 15);
+          const long long idx = tv ? (long long)row * nv + col : (long long)col * nv + row;
+          yv[m][n][q] = (row < nv && col < nv) ? gload_c(Y.p + idx) : Cplx<T>{T(0), T(0)};
+        }
+    const double cj = (Y.flags & LC_CONJ) ? -1.0 : 1.0;
     double sr = 0, si = 0;
-    if (!Y.zero) {
 #pragma unroll
-      for (int m = 0; m < MI; m++)
+    for (int m = 0; m < MI; m++)
 #pragma unroll
-        for (int n = 0; n < MI; n++)
+      for (int n = 0; n < MI; n++)
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
-            const int col = col0 + wn * WM + n * 16 + (lane & 15);
-            if (row < nv && col < nv) {
-              const Cplx<T> y = elem(Y, col, row, nv);
-              const double xr = zr[m][n][q], xi = zi[m][n][q];
-              sr += xr * (double)y.x - xi * (double)y.y;
-              si += xr * (double)y.y + xi * (double)y.x;
-            }
-          }
+        for (int q = 0; q < 4; q++) {
+          const double yr = yv[m][n][q].x, yi = cj * (double)yv[m][n][q].y;
+          const double xr = zr[m][n][q], xi = zi[m][n][q];
+          sr += xr * yr - xi * yi;
+          si += xr * yi + xi * yr;
+        }
+    // Y block phase, then the trace coefficient
+    const double yr = Y.sr, yi = Y.si;
+    const double pr = sr * yr - si * yi, pi = sr * yi + si * yr;
+    const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
+    tr_re[t] = ar * pr - ai * pi;
+    tr_im[t] = ar * pi + ai * pr;
+  }
+  if (g.ntr > 0) {
+    // one reduction for every trace: waves -> LDS -> threads 0..ntr-1
+#pragma unroll
+    for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
+      tr_re[t] = wave_sum(tr_re[t]);
+      tr_im[t] = wave_sum(tr_im[t]);
     }
-    block_sum2(sr, si, red);
-    if (tid == 0) {
-      // Y block phase, then the trace coefficient
-      const double yr = Y.sr, yi = Y.si;
-      const double pr = sr * yr - si * yi, pi = sr * yi + si * yr;
-      const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
+    if (lane == 0) {
+#pragma unroll
+      for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
+        tred[t][0][wave] = tr_re[t];
+        tred[t][1][wave] = tr_im[t];
+      }
+    }
+    __syncthreads();
+    if (tid < g.ntr) {
+      const int t = tid;
+      typedef double d2v __attribute__((ext_vector_type(2)));
       double* out = g.part[t] + 2 * (item * per_item + r);
-      out[0] = ar * pr - ai * pi;
-      out[1] = ar * pi + ai * pr;
+      gstore(out, d2v{tred[t][0][0] + tred[t][0][1] + tred[t][0][2] + tred[t][0][3],
+                      tred[t][1][0] + tred[t][1][1] + tred[t][1][2] + tred[t][1][3]});
     }
   }
 }

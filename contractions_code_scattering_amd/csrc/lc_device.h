@@ -11,6 +11,28 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 typedef float fx4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Cplx { T x, y; };
+
+// Device memory reached through descriptors (pointers read from a group struct) is generic to
+// the compiler, which would emit FLAT instructions; FLAT loads count against the LDS counter as
+// well, so every wait for an LDS read would also wait for the prefetch in flight. All global
+// traffic goes through these address-space-1 accessors instead (global_load / global_store).
+#define LC_GLOBAL __attribute__((address_space(1)))
+template <typename V> __device__ __forceinline__ V gload(const void* p) {
+  return *(const LC_GLOBAL V*)(p);
+}
+template <typename V> __device__ __forceinline__ void gstore(void* p, const V& v) {
+  *(LC_GLOBAL V*)(p) = v;
+}
+// complex values move as two-element vectors (a struct cannot be copied across address spaces)
+template <typename T> __device__ __forceinline__ Cplx<T> gload_c(const Cplx<T>* p) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = *(const LC_GLOBAL v2*)(p);
+  return Cplx<T>{v[0], v[1]};
+}
+template <typename T> __device__ __forceinline__ void gstore_c(Cplx<T>* p, const Cplx<T>& z) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  *(LC_GLOBAL v2*)(p) = v2{z.x, z.y};
+}
 template <typename T> struct Acc4;
 template <> struct Acc4<double> { typedef dx4 type; };
 template <> struct Acc4<float> { typedef fx4 type; };
@@ -65,7 +87,7 @@ __device__ __forceinline__ BlockRef<T> view_block(const LcView& v, long long off
 template <typename T>
 __device__ __forceinline__ Cplx<T> elem(const BlockRef<T>& r, int i, int j, int nv) {
   const long long idx = (r.flags & LC_TV) ? (long long)j * nv + i : (long long)i * nv + j;
-  Cplx<T> z = r.p[idx];
+  Cplx<T> z = gload_c(r.p + idx);
   if (r.flags & LC_CONJ) z.y = -z.y;
   return z;
 }

@@ -37,9 +37,21 @@ struct LcView {
 // (not stored when C == nullptr). Optionally fused closing traces: for t < ntr,
 //   part_t[item][tile] = tr_alpha_t * sum over the tile of C_ij * Y_t(c,a)_ji
 // i.e. the per-tile pieces of tr_alpha_t * Tr[C * Y_t]; a fixed-order reduction sums the tiles.
+//
+// Split storage (mixed-precision kernel only). A split matrix holds every complex value x as six
+// BF16 planes [piece][re/im]: x = x0 + x1 + x2 exactly for each component (three-way split of an
+// FP32 value, see gemm_mx.hip). Block (a,b) of a D-space matrix, or the single block of an
+// elemental, occupies 6 * Nv * Nv BF16 values, plane q = 2 * piece + (0 re, 1 im) at q * Nv * Nv,
+// each plane Nv x Nv row-major. Offsets and strides stay in complex elements (6 BF16 each), so a
+// split view is addressed exactly like a complex view of the same shape. The mixed kernel reads
+// its A and B operands in split storage; C may be written as FP32 complex (C) and/or split (Cs,
+// every block transposed when cs_trans: element (i,j) at j * Nv + i).
 struct LcGemmGroup {
   LcView A, B;
   void* C;
+  void* Cs;
+  int cs_trans;
+  int bpw;                  // output tiles per workgroup (mixed-precision kernel)
   long long cs[3];
   double alpha[2];
   int n[3];

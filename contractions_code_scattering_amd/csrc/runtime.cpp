@@ -38,17 +38,29 @@ void Program::barrier() {
 
 void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
                    long long cs2, int n0, int n1, int n2, double ar, double ai,
-                   const std::vector<FusedTrace>& traces) {
+                   const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
   if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
-  if (!C && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
+  if (!C && !Cs && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
+  if (Cs && mode_ != MX) fail(LC_ERR_ARG, "split output outside the mixed-precision mode");
+  if (mode_ == MX) {
+    if ((A.flags & LC_TV) || !(B.flags & LC_TV))
+      fail(LC_ERR_ARG, "mixed-precision operands must be read along contiguous K (A row-major, B transposed)");
+    for (const LcView* v : {&A, &B})
+      for (int a = 0; a < 4; a++)
+        for (const double* ph : {v->ph_r[a], v->ph_c[a]})
+          if (!((ph[1] == 0 && (ph[0] == 1 || ph[0] == -1)) || (ph[0] == 0 && (ph[1] == 1 || ph[1] == -1))))
+            fail(LC_ERR_ARG, "mixed-precision operand phase (%g, %g) is not a power of i", ph[0], ph[1]);
+  }
   Level* l = open_level(false);
   LcGemmGroup g;
   std::memset(&g, 0, sizeof g);
   g.A = A;
   g.B = B;
   g.C = C;
+  g.Cs = Cs;
+  g.cs_trans = cs_trans ? 1 : 0;
   g.cs[0] = cs0; g.cs[1] = cs1; g.cs[2] = cs2;
   g.alpha[0] = ar; g.alpha[1] = ai;
   g.n[0] = n0; g.n[1] = n1; g.n[2] = n2;
@@ -56,8 +68,13 @@ void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, lon
   g.tiles = (nv_ + tm - 1) / tm;
   g.wg_begin = l->total_wg;
   const long long items = (long long)n0 * n1 * n2;
+  const long long tt = (long long)g.tiles * g.tiles;
   // workgroups (and trace pieces) per item: one per output tile, or one per `bpw` tiles in MX mode
-  const long long per_item = 16LL * g.tiles * g.tiles / (mode_ == MX ? lc_gemm_mx_bpw(nv_) : 1);
+  long long per_item = 16 * tt;
+  if (mode_ == MX) {
+    g.bpw = tt == 1 ? 4 : 1;
+    per_item = 16 * tt / g.bpw;
+  }
   l->total_wg += items * per_item;
   l->count++;
   g.ntr = (int)traces.size();

@@ -17,13 +17,12 @@ extern "C" hipError_t lc_launch_trace(int is_double, const LcTraceGroup* d_group
                                       long long total_wg, int nv, hipStream_t s);
 extern "C" hipError_t lc_launch_reduce(const LcReduceJob* d_jobs, const long long* d_begins,
                                        int njobs, long long total, hipStream_t s);
-extern "C" int lc_gemm_mx_bpw(int nv);
 extern "C" hipError_t lc_launch_gemm_mx(const LcGemmGroup* d_groups, int ng, long long total_wg,
                                         int nv, hipStream_t s);
 extern "C" hipError_t lc_launch_mfma_bf16_selftest(const float* a, const float* b, float* d,
                                                    hipStream_t s);
-extern "C" hipError_t lc_launch_rescale(const void* x, void* y, long long len, long long nslice,
-                                        double* scale, hipStream_t s);
+extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, long long len,
+                                              int nv, long long nslice, double* scale, hipStream_t s);
 extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b,
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
@@ -131,9 +130,12 @@ class Program {
   Program(int nv, Mode m) : nv_(nv), dbl_(m == F64), mode_(m) {}
 
   // C = alpha * A * B over the batch; C may be nullptr when only fused traces are wanted.
+  // MX mode: A and B are split views (A without LC_TV, B with LC_TV, so that both are read
+  // along contiguous K) with phases in {+-1, +-i}; C is FP32 complex, and Cs (optional) receives
+  // the same product in split storage, every block transposed when cs_trans.
   void gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
             long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
-            const std::vector<FusedTrace>& traces = {});
+            const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false);
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
