@@ -116,6 +116,12 @@ __device__ __forceinline__ void fetch_split(const unsigned short* p, int line, i
   }
 }
 
+// FULL: every tile is complete (Nv a multiple of 64) and every operand and output is 16-byte
+// aligned (checked by the launcher's caller); the kernel then has no element-wise fallback
+// paths. Besides saving the bounds checks this matters for the schedule: the compiler's
+// wait-count analysis merges all paths, so registers written by a fallback load would force a
+// wait for the whole prefetch at the top of the matrix-core section.
+template <bool FULL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total) {
   typedef float T;
@@ -142,7 +148,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const long long nv2 = (long long)nv * nv;
   const long long offA = voff(g.A.s, co), offB = voff(g.B.s, co);
   const bool akron = g.A.kind == LC_KRON, bkron = g.B.kind == LC_KRON;
-  const bool vec_ok = (nv & 7) == 0;  // 16-byte plane vectors stay aligned
+  const bool vec_ok = FULL || (nv & 7) == 0;  // 16-byte plane vectors stay aligned
 
   const int nterm = (akron || bkron) ? 1 : 4;
   const int nk = (nv + MX_KT - 1) / MX_KT;
@@ -181,43 +187,81 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     c = blk & 3;
   };
 
-  // Fetch chunk ch of both operands into registers and work out how the combined view phase
-  // (carried by A) and the conjugations apply: with A' = conj?(A) the stored plane pair (R, I),
+  // Chunk cursor of the fetch pipeline: tile, term (spin index b of the product) and K chunk of
+  // the next chunk to fetch, advanced incrementally. Everything that depends only on the tile and
+  // the term (block pointers of this thread's staging line, phase and conjugation masks) is
+  // worked out once per term, not once per chunk.
+  int cur_tl = 0, cur_term = 0, cur_kc = 0;
+  int cur_tm = 0, cur_tn = 0;
+  const unsigned short* pA = nullptr;  // this thread's staging line of the A block, K = 0
+  const unsigned short* pB = nullptr;
+  int t_swap = 0;
+  unsigned t_mr = 0, t_mi = 0, t_bmi = 0;
+  bool t_alA = true, t_alB = true;  // 16-byte alignment of the block bases (general kernel)
+
+  // How the combined view phase (carried by A) and the conjugations apply: with A' = conj?(A)
+  // the stored plane pair (R, I),
   //   phase  1: (R, I)      -1: (-R, -I)      i: (-I, R)      -i: (I, -R)
-  // i.e. R goes to plane slot a_swap with sign mask a_mr, I to slot 1 - a_swap with a_mi.
-  auto fetch = [&](int ch) {
-    const int tl = ch / cpb;
-    const int rem = ch - tl * cpb;
-    const int term = rem / nk;
-    const int k0 = (rem - term * nk) * MX_KT;
-    int a, c, tm, tn;
-    tile_of(tl, a, c, tm, tn);
+  // i.e. R goes to plane slot swap with sign mask mr, I to slot 1 - swap with mask mi.
+  auto set_term = [&]() {
+    int a, c;
+    tile_of(cur_tl, a, c, cur_tm, cur_tn);
     int ab, bb, ba, bc;
     if (akron) { ab = a; bb = a; ba = a; bc = c; }
     else if (bkron) { ab = a; bb = c; ba = c; bc = c; }
-    else { ab = a; bb = term; ba = term; bc = c; }
+    else { ab = a; bb = cur_term; ba = cur_term; bc = c; }
     const SplitRef A = split_block(g.A, offA, ab, bb, nv2);
     const SplitRef B = split_block(g.B, offB, ba, bc, nv2);
     const float sr = A.sr * B.sr - A.si * B.si, si = A.sr * B.si + A.si * B.sr;
     const unsigned ca = (A.flags & LC_CONJ) ? SIGN2 : 0u;
     if (si == 0.f) {
-      a_swap = 0;
-      a_mr = sr < 0.f ? SIGN2 : 0u;
-      a_mi = a_mr ^ ca;
+      t_swap = 0;
+      t_mr = sr < 0.f ? SIGN2 : 0u;
+      t_mi = t_mr ^ ca;
     } else {
-      a_swap = 1;
+      t_swap = 1;
       // phase i: slot re <- -I', slot im <- R;  phase -i: slot re <- I', slot im <- -R
-      a_mr = si > 0.f ? 0u : SIGN2;         // sign applied to R (lands in the im slot)
-      a_mi = (si > 0.f ? SIGN2 : 0u) ^ ca;  // sign applied to I (lands in the re slot)
+      t_mr = si > 0.f ? 0u : SIGN2;         // sign applied to R (lands in the im slot)
+      t_mi = (si > 0.f ? SIGN2 : 0u) ^ ca;  // sign applied to I (lands in the re slot)
     }
-    b_mi = (B.flags & LC_CONJ) ? SIGN2 : 0u;
-    const int k = k0 + s_kg * 8;
-    const bool fullA = vec_ok && (reinterpret_cast<uintptr_t>(A.p) & 15) == 0 && (tm + 1) * MX_TM <= nv &&
-                       k0 + MX_KT <= nv;
-    const bool fullB = vec_ok && (reinterpret_cast<uintptr_t>(B.p) & 15) == 0 && (tn + 1) * MX_TM <= nv &&
-                       k0 + MX_KT <= nv;
-    fetch_split(A.p, tm * MX_TM + s_line, k, nv, nv2, fullA, va);
-    fetch_split(B.p, tn * MX_TM + s_line, k, nv, nv2, fullB, vb);
+    t_bmi = (B.flags & LC_CONJ) ? SIGN2 : 0u;
+    t_alA = (reinterpret_cast<uintptr_t>(A.p) & 15) == 0;
+    t_alB = (reinterpret_cast<uintptr_t>(B.p) & 15) == 0;
+    pA = A.p + (long long)(cur_tm * MX_TM + s_line) * nv + s_kg * 8;
+    pB = B.p + (long long)(cur_tn * MX_TM + s_line) * nv + s_kg * 8;
+  };
+  set_term();
+
+  // Fetch the chunk under the cursor into registers (with its masks), then advance the cursor.
+  auto fetch = [&]() {
+    const int k0 = cur_kc * MX_KT;
+    a_swap = t_swap;
+    a_mr = t_mr;
+    a_mi = t_mi;
+    b_mi = t_bmi;
+    if (FULL) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        va[q] = gload<u32x4>(pA + q * nv2 + k0);
+        vb[q] = gload<u32x4>(pB + q * nv2 + k0);
+      }
+    } else {
+      const bool fullA = vec_ok && t_alA && (cur_tm + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
+      const bool fullB = vec_ok && t_alB && (cur_tn + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
+      const int k = k0 + s_kg * 8;
+      fetch_split(pA - (long long)(cur_tm * MX_TM + s_line) * nv - s_kg * 8, cur_tm * MX_TM + s_line, k, nv, nv2,
+                  fullA, va);
+      fetch_split(pB - (long long)(cur_tn * MX_TM + s_line) * nv - s_kg * 8, cur_tn * MX_TM + s_line, k, nv, nv2,
+                  fullB, vb);
+    }
+    if (++cur_kc == nk) {
+      cur_kc = 0;
+      if (++cur_term == nterm) {
+        cur_term = 0;
+        ++cur_tl;
+      }
+      if (cur_tl < bpw) set_term();
+    }
   };
 
   auto store = [&]() {
@@ -235,7 +279,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 #pragma unroll
   for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) { tsr[t] = 0; tsi[t] = 0; }
   const T alr = T(g.alpha[0]), ali = T(g.alpha[1]);
-  const bool vec_c = (nv & 3) == 0;
+  const bool vec_c = FULL || (nv & 3) == 0;
 
   // Per-tile epilogue: stores, and the fused trace pieces sum_ij C_ij Y_ji in FP64. Every load
   // of the trace operands (all traces, all 16 values of this lane) is issued before the first
@@ -278,16 +322,32 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 #pragma unroll
       for (int m = 0; m < 2; m++)
 #pragma unroll
-        for (int n = 0; n < 2; n++)
+        for (int n = 0; n < 2; n++) {
+          const int row = rbase + m * 16, col = cbase + n * 16;
+          if (FULL && tv) {
+            // f(Y)(col + q, row) = Y(row, col + q): four consecutive complex values
+            const float* p = reinterpret_cast<const float*>(Y.p + (long long)row * nv + col);
+            const fx4v v01 = Y.zero ? fx4v{0.f, 0.f, 0.f, 0.f} : gload<fx4v>(p);
+            const fx4v v23 = Y.zero ? fx4v{0.f, 0.f, 0.f, 0.f} : gload<fx4v>(p + 4);
+            y.v[m][n][0] = f2v{v01[0], v01[1]};
+            y.v[m][n][1] = f2v{v01[2], v01[3]};
+            y.v[m][n][2] = f2v{v23[0], v23[1]};
+            y.v[m][n][3] = f2v{v23[2], v23[3]};
+          } else {
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int row = rbase + m * 16, col = cbase + n * 16 + q;
-            const long long idx = tv ? (long long)row * nv + col : (long long)col * nv + row;
-            y.v[m][n][q] = (!Y.zero && row < nv && col < nv) ? gload<f2v>(Y.p + idx) : f2v{0.f, 0.f};
+            for (int q = 0; q < 4; q++) {
+              const long long idx = tv ? (long long)row * nv + col + q : (long long)(col + q) * nv + row;
+              y.v[m][n][q] = (!Y.zero && (FULL || (row < nv && col + q < nv))) ? gload<f2v>(Y.p + idx)
+                                                                               : f2v{0.f, 0.f};
+            }
           }
+        }
     };
+    // every loaded value is used on every path (traces past ntr sum zeros): a load left pending
+    // on some path would make the compiler wait for all memory traffic at the next reuse of its
+    // register, i.e. for the whole prefetch at the top of the matrix-core section
     YTile ycur, ynext;
-    if (g.ntr > 0) load_y(0, ycur);
+    load_y(0, ycur);
 #pragma unroll
     for (int m = 0; m < 2; m++)
 #pragma unroll
@@ -297,11 +357,11 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         const int col = cbase + n * 16;
         const float* zr = zr_[m][n];
         const float* zi = zi_[m][n];
-        const bool rin = row < nv;
-        const bool full4 = rin && col + 3 < nv;
+        const bool rin = FULL || row < nv;
+        const bool full4 = FULL || (rin && col + 3 < nv);
         if (g.C) {
           float* C = reinterpret_cast<float*>(g.C) + 2 * (cofs + (long long)row * nv + col);
-          if (full4 && vec_c && (cofs & 1) == 0) {
+          if (FULL || (full4 && vec_c && (cofs & 1) == 0)) {
             gstore(C, fx4v{zr[0], zi[0], zr[1], zi[1]});
             gstore(C + 4, fx4v{zr[2], zi[2], zr[3], zi[3]});
           } else if (rin) {
@@ -317,7 +377,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
           split2(zi[0], zi[1], i01[0], i01[1], i01[2]);
           split2(zi[2], zi[3], i23[0], i23[1], i23[2]);
           unsigned short* S = reinterpret_cast<unsigned short*>(g.Cs) + 6 * cofs;
-          if (!g.cs_trans && full4 && vec_c && (cofs & 1) == 0) {
+          if (!g.cs_trans && (FULL || (full4 && vec_c && (cofs & 1) == 0))) {
             const long long e = (long long)row * nv + col;
 #pragma unroll
             for (int p = 0; p < 3; p++) {
@@ -343,34 +403,39 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     // traces, the loads of trace t+1 in flight while trace t is summed
 #pragma unroll
     for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
-      if (t >= g.ntr) break;
-      if (t + 1 < g.ntr) load_y(t + 1, ynext);
+      if (t + 1 < LC_MAX_FUSED_TRACES) load_y(t + 1, ynext);
+      // four-term FP32 partial sums (rounding far below that of the FP32 accumulation of the
+      // product itself), carried on in FP64
       double sr = 0, si = 0;
 #pragma unroll
       for (int m = 0; m < 2; m++)
 #pragma unroll
-        for (int n = 0; n < 2; n++)
+        for (int n = 0; n < 2; n++) {
+          float fr = 0.f, fi = 0.f;
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            const double yr = ycur.v[m][n][q][0], yi = ycur.cj * ycur.v[m][n][q][1];
-            const double xr = zr_[m][n][q], xi = zi_[m][n][q];
-            sr += xr * yr - xi * yi;
-            si += xr * yi + xi * yr;
+            const float yr = ycur.v[m][n][q][0], yi = ycur.cj * ycur.v[m][n][q][1];
+            const float xr = zr_[m][n][q], xi = zi_[m][n][q];
+            fr += xr * yr - xi * yi;
+            fi += xr * yi + xi * yr;
           }
+          sr += fr;
+          si += fi;
+        }
       // the Y block phase is uniform over the tile
       tsr[t] += sr * ycur.pr - si * ycur.pi;
       tsi[t] += sr * ycur.pi + si * ycur.pr;
-      ycur = ynext;
+      if (t + 1 < LC_MAX_FUSED_TRACES) ycur = ynext;
     }
   };
 
-  fetch(0);
+  fetch();
   store();
   __syncthreads();
   const int kq = (lane >> 4) * 8;
   for (int ch = 0; ch < nchunk; ch++) {
     const bool more = ch + 1 < nchunk;
-    if (more) fetch(ch + 1);
+    if (more) fetch();
     bf16x8 bf[2][3][2];  // [n][piece][re/im]
 #pragma unroll
     for (int n = 0; n < 2; n++)
@@ -497,12 +562,18 @@ __global__ void lc_split_convert_kernel(const double2* __restrict__ x, unsigned 
 
 }  // namespace lc
 
-extern "C" hipError_t lc_launch_gemm_mx(const LcGemmGroup* d_groups, int ng, long long total_wg, int nv,
-                                        hipStream_t s) {
+// variant 1: full-tile kernel; 0: general kernel. The planner decides which conditions every
+// group of the launch meets.
+extern "C" hipError_t lc_launch_gemm_mx(int variant, const LcGemmGroup* d_groups, int ng, long long total_wg,
+                                        int nv, hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
   const long long per = (total_wg + 7) / 8;
-  hipLaunchKernelGGL(lc::lc_gemm_mx_kernel, dim3((unsigned)(8 * per)), dim3(256), 0, s, d_groups, ng, nv,
-                     total_wg);
+  if (variant == 1)
+    hipLaunchKernelGGL(lc::lc_gemm_mx_kernel<true>, dim3((unsigned)(8 * per)), dim3(256), 0, s, d_groups, ng,
+                       nv, total_wg);
+  else
+    hipLaunchKernelGGL(lc::lc_gemm_mx_kernel<false>, dim3((unsigned)(8 * per)), dim3(256), 0, s, d_groups, ng,
+                       nv, total_wg);
   return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Host runtime: context, Program builder/launcher.
 #include "runtime.h"
 
+#include <cstdlib>
 #include <cstring>
 
 namespace lc {
@@ -19,11 +20,13 @@ void require_init() {
   if (!ctx().on) fail(LC_ERR_NOT_INIT, "contraction library is not initialised (call lc_init)");
 }
 
-Program::Level* Program::open_level(bool is_trace) {
-  if (levels_.empty() || levels_.back().closed || levels_.back().is_trace != is_trace) {
+Program::Level* Program::open_level(bool is_trace, int variant) {
+  if (levels_.empty() || levels_.back().closed || levels_.back().is_trace != is_trace ||
+      levels_.back().variant != variant) {
     Level l;
     l.is_trace = is_trace;
     l.closed = false;
+    l.variant = variant;
     l.first = is_trace ? traces_.size() : gemms_.size();
     l.count = 0;
     l.total_wg = 0;
@@ -44,6 +47,7 @@ void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, lon
   if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
   if (!C && !Cs && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
   if (Cs && mode_ != MX) fail(LC_ERR_ARG, "split output outside the mixed-precision mode");
+  int variant = 0;
   if (mode_ == MX) {
     if ((A.flags & LC_TV) || !(B.flags & LC_TV))
       fail(LC_ERR_ARG, "mixed-precision operands must be read along contiguous K (A row-major, B transposed)");
@@ -52,8 +56,22 @@ void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, lon
         for (const double* ph : {v->ph_r[a], v->ph_c[a]})
           if (!((ph[1] == 0 && (ph[0] == 1 || ph[0] == -1)) || (ph[0] == 0 && (ph[1] == 1 || ph[1] == -1))))
             fail(LC_ERR_ARG, "mixed-precision operand phase (%g, %g) is not a power of i", ph[0], ph[1]);
+    // complete 64 x 64 tiles and 16-byte alignment of every operand and output (split storage:
+    // 12 B per complex element, FP32 output 8 B): full-tile kernel
+    bool full = nv_ % 64 == 0;
+    for (const LcView* v : {&A, &B}) {
+      full = full && (reinterpret_cast<uintptr_t>(v->base) & 15) == 0;
+      for (int d = 0; d < 3; d++) full = full && v->s[d] % 4 == 0;
+    }
+    full = full && (reinterpret_cast<uintptr_t>(C) & 15) == 0 && (reinterpret_cast<uintptr_t>(Cs) & 15) == 0;
+    full = full && cs0 % 4 == 0 && cs1 % 4 == 0 && cs2 % 4 == 0;
+    for (const FusedTrace& f : traces) {  // FP32 trace operands, read four complex values at a time
+      full = full && (reinterpret_cast<uintptr_t>(f.Y.base) & 15) == 0;
+      for (int d = 0; d < 3; d++) full = full && f.Y.s[d] % 2 == 0;
+    }
+    variant = full ? 1 : 0;
   }
-  Level* l = open_level(false);
+  Level* l = open_level(false, variant);
   LcGemmGroup g;
   std::memset(&g, 0, sizeof g);
   g.A = A;
@@ -183,7 +201,7 @@ void Program::run(hipStream_t s) {
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev], s), "hipEventRecord");
     hipError_t e = l.is_trace
                        ? lc_launch_trace(dbl_, dt + l.first, l.count, l.total_wg, nv_, s)
-                       : (mode_ == MX ? lc_launch_gemm_mx(dg + l.first, l.count, l.total_wg, nv_, s)
+                       : (mode_ == MX ? lc_launch_gemm_mx(l.variant, dg + l.first, l.count, l.total_wg, nv_, s)
                                       : lc_launch_gemm(dbl_, dg + l.first, l.count, l.total_wg, nv_, s));
     hip_check(e, l.is_trace ? "trace launch" : "gemm launch");
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev + 1], s), "hipEventRecord");

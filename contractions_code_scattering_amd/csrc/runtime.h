@@ -17,8 +17,8 @@ extern "C" hipError_t lc_launch_trace(int is_double, const LcTraceGroup* d_group
                                       long long total_wg, int nv, hipStream_t s);
 extern "C" hipError_t lc_launch_reduce(const LcReduceJob* d_jobs, const long long* d_begins,
                                        int njobs, long long total, hipStream_t s);
-extern "C" hipError_t lc_launch_gemm_mx(const LcGemmGroup* d_groups, int ng, long long total_wg,
-                                        int nv, hipStream_t s);
+extern "C" hipError_t lc_launch_gemm_mx(int variant, const LcGemmGroup* d_groups, int ng,
+                                        long long total_wg, int nv, hipStream_t s);
 extern "C" hipError_t lc_launch_mfma_bf16_selftest(const float* a, const float* b, float* d,
                                                    hipStream_t s);
 extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, long long len,
@@ -151,6 +151,7 @@ class Program {
   struct Level {
     bool is_trace;
     bool closed;
+    int variant;  // MX kernel: 1 full-tile, 0 general
     size_t first;
     int count;
     long long total_wg;
@@ -172,7 +173,7 @@ class Program {
   size_t partial_bytes_ = 0;
   std::vector<Level> levels_;
   double gemm_flops_ = 0, trace_flops_ = 0;
-  Level* open_level(bool is_trace);
+  Level* open_level(bool is_trace, int variant = 0);
 };
 
 }  // namespace lc
