@@ -449,6 +449,39 @@ __global__ void lc_engine_average_kernel(const double2* __restrict__ corr, doubl
 }  // namespace lc
 
 // ------------------------------------------------------------------------------------------
+// Mixed form of the sigma / sigma-pi / rho-pi engine: multiply every raw trace of
+// raw [np][nsrc][nt][10] back by the FP64 per-slice scales of the matrices it contains:
+// s(tf,t0) of tau, s(t,t) of D(t), s(p,t,j) of elemental type j (types as in engine_sigma.cpp).
+//   sigma(l, r):  tau^2 X_l(tf) X_r(t0)
+//   triangle(ip), rho-pi(iop): tau^2 D(t0) P1(t0) P2(t0) times X_ip(tf) or P12_ip(tf)
+__global__ void lc_sigma_scale_kernel(double2* __restrict__ raw, long long n, int nsrc, int nt, int src0,
+                                      int src_step, int idir, const double* __restrict__ s_tau,
+                                      const double* __restrict__ s_el) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int tf = (int)(i % nt);
+  const int k = (int)((i / nt) % nsrc);
+  const long long p = i / ((long long)nt * nsrc);
+  const int t0 = src0 + k * src_step;
+  const double st = s_tau[(long long)tf * nt + t0], d0 = s_tau[(long long)t0 * nt + t0];
+  const double* ef = s_el + (p * nt + tf) * 11;
+  const double* e0 = s_el + (p * nt + t0) * 11;
+  const double base = st * st * d0 * e0[9] * e0[10];
+  const int ip_of_op[4] = {0, 0, 1, 2 + idir};
+  double f[10];
+  for (int l = 0; l < 2; l++)
+    for (int r = 0; r < 2; r++) f[l * 2 + r] = st * st * ef[l] * e0[r];
+  f[4] = base * ef[2];
+  f[5] = base * ef[3];
+  for (int op = 0; op < 4; op++) f[6 + op] = base * ef[4 + ip_of_op[op]];
+  double2* r = raw + i * 10;
+  for (int d = 0; d < 10; d++) {
+    const double2 v = r[d];
+    r[d] = make_double2(v.x * f[d], v.y * f[d]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // Host launchers (called from the planner).
 extern "C" int lc_gemm_tile(int nv) { return nv <= 32 ? 32 : 64; }
 
@@ -505,5 +538,13 @@ extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, voi
   const int m = np * 8 * nt;
   hipLaunchKernelGGL(lc::lc_engine_average_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
                      (const double2*)corr, (double2*)avg, np, nsrc, nt, 8, src0, src_step);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_sigma_scale(void* raw, int np, int nsrc, int nt, int src0, int src_step,
+                                            int idir, const double* s_tau, const double* s_el, hipStream_t s) {
+  const long long n = (long long)np * nsrc * nt;
+  hipLaunchKernelGGL(lc_sigma_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (double2*)raw, n,
+                     nsrc, nt, src0, src_step, idir, s_tau, s_el);
   return hipGetLastError();
 }

@@ -16,6 +16,8 @@ N_CORR = len(CORR_NAMES)
 _VP, _I, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
 register("lc_engine_rho_pipi", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
 register("lc_engine_rho_pipi_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
+register("lc_engine_sigma_set", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
+register("lc_engine_sigma_set_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
 register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
 register("lc_profile", [_I])
 register("lc_profile_read", [_VP])
@@ -51,6 +53,41 @@ def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=N
     check(fn(tau.ptr, phi.ptr, nt, nv, np_, src0, src_step, nsrc, max_scratch_bytes,
              raw.ptr, corr.ptr, avg.ptr))
     return raw, corr, avg
+
+
+SIGMA_RAW_NAMES = ["sigma_00", "sigma_01", "sigma_10", "sigma_11", "triangle_0", "triangle_1",
+                   "rhopi_0", "rhopi_1", "rhopi_2", "rhopi_3"]
+EL_TYPES = 11  # 0-1 sigma X_l, 2-3 triangle X_ip, 4-8 rho-pi P1P2 types, 9 P1, 10 P2
+
+
+def sigma_set(tau, el, idir=0, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None, mixed=False):
+    """Batched sigma / sigma-pi / rho-pi set (lc_engine_sigma_set[_mixed]).
+    tau: DeviceArray complex128 [nt][nt][4][4][nv][nv]; el: complex128 [np][nt][11][nv][nv].
+    Returns (raw [np][nsrc][nt][10], loops [np][nt][3]) DeviceArrays."""
+    import numpy as np
+    from .devmem import DeviceArray
+    if not (isinstance(tau, DeviceArray) and isinstance(el, DeviceArray)):
+        raise TypeError("engine operands must be DeviceArrays")
+    if tau.dtype != np.complex128 or el.dtype != np.complex128:
+        raise TypeError("engine operands are complex128")
+    nt, nv = tau.shape[0], tau.shape[-1]
+    if tuple(tau.shape) != (nt, nt, 4, 4, nv, nv):
+        raise ValueError("tau must be [nt][nt][4][4][nv][nv]")
+    np_ = el.shape[0]
+    if tuple(el.shape) != (np_, nt, EL_TYPES, nv, nv):
+        raise ValueError("el must be [np][nt][11][nv][nv]")
+    if nsrc is None:
+        nsrc = (nt - src0 + src_step - 1) // src_step
+    if out is None:
+        raw = DeviceArray((np_, nsrc, nt, 10), np.complex128)
+        loops = DeviceArray((np_, nt, 3), np.complex128)
+    else:
+        raw, loops = out
+        if raw.shape != (np_, nsrc, nt, 10) or loops.shape != (np_, nt, 3):
+            raise ValueError("output buffers have the wrong shape")
+    fn = lib().lc_engine_sigma_set_mixed if mixed else lib().lc_engine_sigma_set
+    check(fn(tau.ptr, el.ptr, nt, nv, np_, idir, src0, src_step, nsrc, max_scratch_bytes, raw.ptr, loops.ptr))
+    return raw, loops
 
 
 def profile(enable=True):

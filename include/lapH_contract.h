@@ -101,6 +101,27 @@ int lc_engine_rho_pipi_mixed(const void* d_tau, const void* d_phi, int nt, int n
                              int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                              void* d_corr, void* d_avg);
 
+/* Batched sigma / sigma-pi / rho-pi set (no reference counterpart: the reference forms these one
+ * sink row at a time through run_ContractSigma, run_ContractSigmaPi_triangle, run_ContractRhoPi,
+ * run_ContractLoopUnity and run_ContractSigmaPi_fish_hor). On a resident perambulator, for every
+ * momentum set p, sink tf and source t0_k = src0 + k * src_step:
+ *   d_tau   [nt][nt][4][4][nv][nv]   complex double
+ *   d_el    [np][nt][11][nv][nv]     complex double elementals by type: 0-1 sigma X_l, 2-3 sigma-pi
+ *                                    triangle X_ip, 4-8 rho-pi P1P2 types, 9 P1, 10 P2
+ *   d_raw   [np][nsrc][nt][10]       {sigma(l,r) l*2+r, triangle(0..1), rho-pi(iop 0..3)}: the
+ *                                    unweighted per-source values of the row launchers
+ *   d_loops [np][nt][3]              {Tr[D(t) X_0(t)], Tr[D(t) X_1(t)], hor_fish(t)}
+ * idir: the rho polarisation of the rho-pi operators. */
+int lc_engine_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, int idir, int src0,
+                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw, void* d_loops);
+
+/* Mixed-precision form of lc_engine_sigma_set (same arguments and outputs): per-slice rescaled
+ * FP32 values split into BF16 pieces on the BF16 matrix cores, FP64 traces and scale factors;
+ * the per-time loops and hor_fish are FP64. */
+int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int nv, int np, int idir,
+                              int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                              void* d_loops);
+
 /* Diagnostic: one v_mfma_f32_16x16x32_bf16 on A(16x32), B(32x16) given as FP32 values exact in
  * BF16; h_d as for lc_selftest_mfma (D row-major, then raw accumulators). */
 int lc_selftest_mfma_bf16(const float* h_a, const float* h_b, float* h_d);
