@@ -81,6 +81,46 @@ def traffic_from_profiles(kernel, nmom):
         return None
 
 
+def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
+    """The batched sigma / sigma-pi / rho-pi set on its own workload, both forms, one timed pass
+    each after one warm-up pass: Nv = 64 (D = 256), Lt = 64, sources every 4th time slice (16),
+    27 momentum sets of 11 elemental types each, rho-pi polarisation idir = 2."""
+    import numpy as np
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd import inputs as I
+    from contractions_code_scattering_amd.devmem import synchronize, to_device
+    g = I.rng(seed)
+    tau = to_device(I.perambulator(g, nt, nv, 0.3))
+    el = to_device(np.stack([I.cn(g, (nt, 11, nv, nv)) / np.sqrt(nv) for _ in range(nmom)]))
+    nsrc = nt // src_step
+    ncorr = nmom * nsrc * nt * 10
+    res = {}
+    outs = {}
+    for mixed in (False, True):
+        engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)
+        synchronize()
+        engine.profile(True)
+        t = time.perf_counter()
+        raw, loops = engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)
+        synchronize()
+        dt = time.perf_counter() - t
+        p = engine.profile_read()
+        engine.profile(False)
+        outs[mixed] = raw.to_numpy()
+        key = "mixed" if mixed else "fp64"
+        res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
+                        gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
+        log("sigma set %s: %.1f ms" % (key, dt * 1e3))
+    d = np.abs(outs[True] - outs[False]).reshape(-1, 10)
+    a = np.abs(outs[False]).reshape(-1, 10)
+    res["mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    res["max_normwise_err"] = float((d.max(axis=0) / a.max(axis=0)).max())
+    res["unit"] = "correlators/s"
+    res["config"] = dict(workload="sigma one_fish (4) + sigma-pi triangle (2) + rho-pi (4) per (tf, t0, p)",
+                         nv=nv, lt=nt, sources=nsrc, momentum_sets=nmom, correlators_per_pass=ncorr)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +132,9 @@ def main():
                     help="profiling aid: run only one form (the JSON line needs both)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="profiling aid: one device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--sigma-set", type=int, default=1,
+                    help="also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
+                         "27 momentum sets (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -186,6 +229,8 @@ def main():
                                               kernel="lc_gemm_kernel<double,64>",
                                               avg_launch_ms=round(s64["avg_launch_ms"], 4))),
                 effective_tflops=round(smx["effective_tflops"], 3))
+    if args.sigma_set:
+        line["sigma_set"] = sigma_set_measurement()
     if rank == 0 and args.cpu_rows > 0:
         rec, ref = cpu_baseline(tau_h, phi_h, args.cpu_rows)
         line["cpu_baseline"] = rec
