@@ -446,6 +446,27 @@ __global__ void lc_engine_average_kernel(const double2* __restrict__ corr, doubl
   avg[i] = make_double2(sr / nsrc, si / nsrc);
 }
 
+// ------------------------------------------------------------------------------------------
+// Perambulator file slab -> device layout. The file holds complex double in the order
+// [s1][v1][t1][s2][v2][t2] (lib/contract.cpp:111-128); one (s1, v1) slab is [t1][s2][v2][t2].
+// Element (t1, s2, v2, t2) goes to tau[t1][t2][s1][s2][v1][v2] (lib/contract.cpp:140,152), in FP64
+// or rounded to FP32. Threads run over the destination order (v2 fastest) so the writes coalesce.
+template <typename T>
+__global__ void lc_prop_scatter_kernel(const double2* __restrict__ slab, T* __restrict__ tau, int nt, int nv,
+                                       int s1, int v1) {
+  const long long n = (long long)nt * nt * 4 * nv;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v2 = (int)(i % nv);
+  const int s2 = (int)((i / nv) % 4);
+  const int t2 = (int)((i / (4LL * nv)) % nt);
+  const int t1 = (int)(i / (4LL * nv * nt));
+  const double2 z = slab[(((long long)t1 * 4 + s2) * nv + v2) * nt + t2];
+  const long long dst = (((((long long)t1 * nt + t2) * 4 + s1) * 4 + s2) * nv + v1) * nv + v2;
+  tau[2 * dst] = (T)z.x;
+  tau[2 * dst + 1] = (T)z.y;
+}
+
 }  // namespace lc
 
 // ------------------------------------------------------------------------------------------
@@ -546,5 +567,18 @@ extern "C" hipError_t lc_launch_sigma_scale(void* raw, int np, int nsrc, int nt,
   const long long n = (long long)np * nsrc * nt;
   hipLaunchKernelGGL(lc_sigma_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (double2*)raw, n,
                      nsrc, nt, src0, src_step, idir, s_tau, s_el);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_prop_scatter(const void* slab, void* tau, int is_double, int nt, int nv, int s1,
+                                             int v1, hipStream_t s) {
+  const long long n = (long long)nt * nt * 4 * nv;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (is_double)
+    hipLaunchKernelGGL(lc::lc_prop_scatter_kernel<double>, dim3(grid), dim3(256), 0, s, (const double2*)slab,
+                       (double*)tau, nt, nv, s1, v1);
+  else
+    hipLaunchKernelGGL(lc::lc_prop_scatter_kernel<float>, dim3(grid), dim3(256), 0, s, (const double2*)slab,
+                       (float*)tau, nt, nv, s1, v1);
   return hipGetLastError();
 }

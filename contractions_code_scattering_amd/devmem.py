@@ -71,3 +71,18 @@ def copy(dst, src):
     if dst.nbytes != src.nbytes:
         raise ValueError("size mismatch")
     check(lib().lc_memcpy_d2d(dst.ptr, src.ptr, src.nbytes))
+
+
+register("lc_read_prop_device", [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP])
+
+
+def read_perambulator(path, nt, nv, dtype=np.complex128):
+    """Perambulator file (the reference's readPropH format) -> DeviceArray [nt][nt][4][4][nv][nv],
+    permuted on the GPU (lc_read_prop_device); dtype complex128 or complex64."""
+    dtype = np.dtype(dtype)
+    if dtype not in (np.dtype(np.complex128), np.dtype(np.complex64)):
+        raise TypeError("perambulator dtype must be complex128 or complex64")
+    out = DeviceArray((nt, nt, 4, 4, nv, nv), dtype)
+    prec = 1 if dtype == np.complex128 else 0
+    check(lib().lc_read_prop_device(str(path).encode(), nt, nv, prec, out.ptr))
+    return out

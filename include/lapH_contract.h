@@ -122,6 +122,13 @@ int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int n
                               int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                               void* d_loops);
 
+/* Perambulator file -> device: the reference's readPropH (lib/contract.cpp:91-160) with the file
+ * order [s1][v1][t1][s2][v2][t2] permuted on the GPU into d_tau [nt][nt][4][4][nv][nv] (complex
+ * double, or complex float for prec = LC_SINGLE), streamed one (s1, v1) slab at a time through
+ * pinned staging buffers. LC_ERR_IO on a missing file, a header that does not match (nt, nv) or a
+ * short read. */
+int lc_read_prop_device(const char* path, int nt, int nv, int prec, void* d_tau);
+
 /* Diagnostic: one v_mfma_f32_16x16x32_bf16 on A(16x32), B(32x16) given as FP32 values exact in
  * BF16; h_d as for lc_selftest_mfma (D row-major, then raw accumulators). */
 int lc_selftest_mfma_bf16(const float* h_a, const float* h_b, float* h_d);
