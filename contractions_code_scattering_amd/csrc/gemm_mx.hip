@@ -63,7 +63,10 @@ __device__ __forceinline__ void split2(float a, float b, unsigned& p0, unsigned&
 
 constexpr int MX_TM = 64;
 constexpr int MX_KT = 32;
-constexpr int MX_LD = MX_KT + 8;  // padded LDS row length, in BF16
+// LDS line layout: a line holds its 32 K values as four 16-byte groups, group g stored at position
+// g ^ ((line / 4) & 3). A fragment read (16 consecutive lines, one K group each) then touches 16
+// distinct groups of four banks, and so does a 16-byte store of 4 lanes x 16 lines.
+__device__ __forceinline__ int swz(int line) { return (line >> 2) & 3; }
 constexpr unsigned SIGN2 = 0x80008000u;
 
 // Block (a,b) of a split view: base of its six planes, element flags, exact phase.
@@ -126,8 +129,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total) {
   typedef float T;
   // planes [piece * 2 + re/im][line][k]
-  __shared__ __attribute__((aligned(16))) __bf16 As[6][MX_TM][MX_LD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[6][MX_TM][MX_LD];
+  __shared__ __attribute__((aligned(16))) __bf16 As[6][MX_TM][MX_KT];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[6][MX_TM][MX_KT];
   __shared__ double red[8];
 
   const long long wg = xcd_order(total);
@@ -159,6 +162,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int s_line = tid >> 2, s_kg = tid & 3;  // staging: line and 8-wide K group of this thread
+  const int s_pos = s_kg ^ swz(s_line);           // its position in the LDS line
 
   fx4v bre[2][2], bim[2][2], sre[2][2], sim[2][2];  // leading (a0b0) and correction sums
   auto zero_acc = [&]() {
@@ -267,10 +271,10 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   auto store = [&]() {
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-      *reinterpret_cast<u32x4*>(&As[2 * p + a_swap][s_line][s_kg * 8]) = va[2 * p] ^ a_mr;
-      *reinterpret_cast<u32x4*>(&As[2 * p + 1 - a_swap][s_line][s_kg * 8]) = va[2 * p + 1] ^ a_mi;
-      *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_kg * 8]) = vb[2 * p];
-      *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_kg * 8]) = vb[2 * p + 1] ^ b_mi;
+      *reinterpret_cast<u32x4*>(&As[2 * p + a_swap][s_line][s_pos * 8]) = va[2 * p] ^ a_mr;
+      *reinterpret_cast<u32x4*>(&As[2 * p + 1 - a_swap][s_line][s_pos * 8]) = va[2 * p + 1] ^ a_mi;
+      *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_pos * 8]) = vb[2 * p];
+      *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_pos * 8]) = vb[2 * p + 1] ^ b_mi;
     }
   };
 
@@ -432,7 +436,15 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   fetch();
   store();
   __syncthreads();
-  const int kq = (lane >> 4) * 8;
+  // fragment lines of this lane and the LDS offsets of its K group in them
+  int al[2], ao[2], bl[2], bo[2];
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    al[i] = wm * 32 + i * 16 + (lane & 15);
+    bl[i] = wn * 32 + i * 16 + (lane & 15);
+    ao[i] = ((lane >> 4) ^ swz(al[i])) * 8;
+    bo[i] = ((lane >> 4) ^ swz(bl[i])) * 8;
+  }
   for (int ch = 0; ch < nchunk; ch++) {
     const bool more = ch + 1 < nchunk;
     if (more) fetch();
@@ -443,7 +455,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++)
-          bf[n][p][ri] = *reinterpret_cast<const bf16x8*>(&Bs[2 * p + ri][wn * 32 + n * 16 + (lane & 15)][kq]);
+          bf[n][p][ri] = *reinterpret_cast<const bf16x8*>(&Bs[2 * p + ri][bl[n]][bo[n]]);
 #pragma unroll
     for (int m = 0; m < 2; m++) {
       bf16x8 af[3][2], nai[3];
@@ -451,7 +463,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++)
-          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[2 * p + ri][wm * 32 + m * 16 + (lane & 15)][kq]);
+          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[2 * p + ri][al[m]][ao[m]]);
 #pragma unroll
       for (int p = 0; p < 3; p++)
         nai[p] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, af[p][1]) ^ SIGN2);
