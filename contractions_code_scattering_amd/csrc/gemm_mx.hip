@@ -546,29 +546,55 @@ __global__ void __launch_bounds__(256) lc_slice_norm_kernel(const double2* __res
   if (threadIdx.x == 0) scale[blockIdx.x] = s > 0 ? sqrt(s) : 1.0;
 }
 
-__global__ void lc_split_convert_kernel(const double2* __restrict__ x, unsigned short* __restrict__ yr,
-                                        unsigned short* __restrict__ yt, long long len, int nv,
-                                        const double* __restrict__ scale, long long nslice) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len * nslice) return;
+// One workgroup per 32 x 32 tile of one Nv x Nv block: the values are read and the row-major
+// planes written along rows; the pieces go through LDS so that the transposed planes are written
+// along rows of the transpose as well.
+constexpr int SC_T = 32;
+__global__ void __launch_bounds__(256) lc_split_convert_kernel(const double2* __restrict__ x,
+                                                               unsigned short* __restrict__ yr,
+                                                               unsigned short* __restrict__ yt, long long len,
+                                                               int nv, const double* __restrict__ scale) {
+  __shared__ unsigned short pc[6][SC_T][SC_T + 1];
+  const int tpb = (nv + SC_T - 1) / SC_T;  // tiles per block side
+  const long long tile = blockIdx.x;
+  const long long bi = tile / (tpb * tpb);  // global block index (slice * blocks per slice + block)
+  const int tr = (int)((tile / tpb) % tpb), tc = (int)(tile % tpb);
   const long long nv2 = (long long)nv * nv;
-  const long long bi = i / nv2;  // global block index
-  const long long e = i - bi * nv2;
-  const int row = (int)(e / nv), col = (int)(e - (long long)(e / nv) * nv);
-  const double inv = 1.0 / scale[i / len];
-  const double2 v = x[i];
-  unsigned p0, p1, p2;
-  split2((float)(v.x * inv), (float)(v.y * inv), p0, p1, p2);
-  const unsigned pc[3] = {p0, p1, p2};
+  const double inv = 1.0 / scale[(bi * nv2) / len];
+  const double2* xb = x + bi * nv2;
   unsigned short* br = yr + 6 * bi * nv2;
   unsigned short* bt = yt + 6 * bi * nv2;
-  const long long et = (long long)col * nv + row;
+  const int c = threadIdx.x & (SC_T - 1), r0 = threadIdx.x >> 5;  // 8 rows per pass
 #pragma unroll
-  for (int p = 0; p < 3; p++) {
-    br[(2 * p) * nv2 + e] = (unsigned short)(pc[p] & 0xffff);
-    br[(2 * p + 1) * nv2 + e] = (unsigned short)(pc[p] >> 16);
-    bt[(2 * p) * nv2 + et] = (unsigned short)(pc[p] & 0xffff);
-    bt[(2 * p + 1) * nv2 + et] = (unsigned short)(pc[p] >> 16);
+  for (int pass = 0; pass < SC_T / 8; pass++) {
+    const int r = r0 + pass * 8;
+    const int row = tr * SC_T + r, col = tc * SC_T + c;
+    if (row < nv && col < nv) {
+      const long long e = (long long)row * nv + col;
+      const double2 v = xb[e];
+      unsigned p0, p1, p2;
+      split2((float)(v.x * inv), (float)(v.y * inv), p0, p1, p2);
+      const unsigned q[3] = {p0, p1, p2};
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        br[(2 * p) * nv2 + e] = (unsigned short)(q[p] & 0xffff);
+        br[(2 * p + 1) * nv2 + e] = (unsigned short)(q[p] >> 16);
+        pc[2 * p][r][c] = (unsigned short)(q[p] & 0xffff);
+        pc[2 * p + 1][r][c] = (unsigned short)(q[p] >> 16);
+      }
+    }
+  }
+  __syncthreads();
+  // transposed planes: element (row, col) at col * nv + row; here thread c runs along row
+#pragma unroll
+  for (int pass = 0; pass < SC_T / 8; pass++) {
+    const int r = r0 + pass * 8;  // a column of the tile
+    const int row = tr * SC_T + c, col = tc * SC_T + r;
+    if (row < nv && col < nv) {
+      const long long et = (long long)col * nv + row;
+#pragma unroll
+      for (int q = 0; q < 6; q++) bt[q * nv2 + et] = pc[q][c][r];
+    }
   }
 }
 
@@ -602,9 +628,10 @@ extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void*
   if (nslice <= 0 || len <= 0) return hipSuccess;
   hipLaunchKernelGGL(lc::lc_slice_norm_kernel, dim3((unsigned)nslice), dim3(256), 0, s,
                      (const double2*)x, len, len, scale);
-  const long long n = len * nslice;
-  hipLaunchKernelGGL(lc::lc_split_convert_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     (const double2*)x, (unsigned short*)y_rows, (unsigned short*)y_trans, len, nv,
-                     (const double*)scale, nslice);
+  const long long nv2 = (long long)nv * nv;
+  const long long tpb = (nv + lc::SC_T - 1) / lc::SC_T;
+  const long long tiles = (len * nslice / nv2) * tpb * tpb;
+  hipLaunchKernelGGL(lc::lc_split_convert_kernel, dim3((unsigned)tiles), dim3(256), 0, s, (const double2*)x,
+                     (unsigned short*)y_rows, (unsigned short*)y_trans, len, nv, (const double*)scale);
   return hipGetLastError();
 }
