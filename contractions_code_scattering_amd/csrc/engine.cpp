@@ -110,6 +110,7 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   };
 
   Program pg(E.nv, mode);
+  pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   // level 1
   pg.gemm(phi_tf, tau_v, B, P0, P1, P2, nt, ns, pc);
   pg.gemm(phi_tf_dag, tau_v, L1, P0, P1, P2, nt, ns, pc);
@@ -209,6 +210,7 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   (void)f32;
 
   Program pg(E.nv, Program::MX);
+  pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   // level 1
   pg.gemm(phi_tf_A, tau_B, B32, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Bs);
   pg.gemm(phi_tf_dag_A, tau_B, L1_32, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, L1s);

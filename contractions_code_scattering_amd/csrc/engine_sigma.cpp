@@ -103,6 +103,7 @@ void run_chunk(const Shape& E, bool mixed, const Store& tau, const Store& el, in
 
   Builder bl(mixed, E.nv);
   Program& pg = bl.pg();
+  pg.set_last_axis_fastest(true);  // momentum sets of one (tf, t0) pair next to each other
   const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
   const long long tau0 = (long long)E.src0 * b;
   // elemental type j of momentum p0 + co2: at sink (per tf), at source (per k)

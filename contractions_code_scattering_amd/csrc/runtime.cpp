@@ -42,6 +42,28 @@ void Program::barrier() {
 void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
                    long long cs2, int n0, int n1, int n2, double ar, double ai,
                    const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
+  if (!last_axis_fastest_ || n2 <= 1) {
+    gemm_items(A, B, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans);
+    return;
+  }
+  // the same batch with its axes relabelled (2, 0, 1): workgroups then walk axis 2 fastest
+  auto rot = [](LcView v) {
+    const long long s0 = v.s[0], s1 = v.s[1], s2 = v.s[2];
+    v.s[0] = s2; v.s[1] = s0; v.s[2] = s1;
+    return v;
+  };
+  std::vector<FusedTrace> tr = traces;
+  for (FusedTrace& f : tr) {
+    f.Y = rot(f.Y);
+    const long long o0 = f.os[0], o1 = f.os[1], o2 = f.os[2];
+    f.os[0] = o2; f.os[1] = o0; f.os[2] = o1;
+  }
+  gemm_items(rot(A), rot(B), C, cs2, cs0, cs1, n2, n0, n1, ar, ai, tr, Cs, cs_trans);
+}
+
+void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
+                         long long cs2, int n0, int n1, int n2, double ar, double ai,
+                         const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
   if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");

@@ -136,6 +136,10 @@ class Program {
   void gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
             long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
             const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false);
+  // Workgroup order over a batch: by default axis 0 runs fastest. With last_axis_fastest, axis 2
+  // does (e.g. the momenta of one (sink, source) pair next to each other, so that the shared
+  // perambulator block is fetched from HBM once and from L2 for the other momenta).
+  void set_last_axis_fastest(bool on) { last_axis_fastest_ = on; }
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -174,6 +178,10 @@ class Program {
   std::vector<Level> levels_;
   double gemm_flops_ = 0, trace_flops_ = 0;
   Level* open_level(bool is_trace, int variant = 0);
+  void gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
+                  int n0, int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
+                  void* Cs, bool cs_trans);
+  bool last_axis_fastest_ = false;
 };
 
 }  // namespace lc
