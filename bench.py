@@ -70,13 +70,17 @@ def cpu_baseline(tau, phi, rows):
 
 
 def traffic_from_profiles(kernel, nmom):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (calibrated against a 1 GiB copy); None when no summary exists for this workload."""
+    """HBM bytes per launch of the kernel whose name contains `kernel`, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a 1 GiB copy); None when no summary exists
+    for this workload."""
     path = os.path.join(ROOT, "profiles", "traffic_rho_pipi_D256_Lt48_%dmom.json" % nmom)
     try:
         with open(path) as f:
-            k = json.load(f)["kernels"].get(kernel)
-        return None if k is None else round(k["bytes_per_launch"])
+            ks = json.load(f)["kernels"]
+        for name, k in ks.items():
+            if kernel in name:
+                return round(k["bytes_per_launch"])
+        return None
     except (OSError, ValueError, KeyError):
         return None
 
@@ -132,9 +136,9 @@ def main():
                     help="profiling aid: run only one form (the JSON line needs both)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="profiling aid: one device-to-device copy of this size (counter calibration)")
-    ap.add_argument("--sigma-set", type=int, default=1,
-                    help="also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
-                         "27 momentum sets (0: skip)")
+    ap.add_argument("--sigma-set", type=int, default=0,
+                    help="1: also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
+                         "27 momentum sets (reported under sigma_set)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -204,8 +208,8 @@ def main():
     max_rel = max(e["max_rel"] for e in errs.values())
     max_norm = max(e["max_normwise"] for e in errs.values())
     roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",
-                frac=round(smx["frac"], 4), traffic=traffic_from_profiles("lc::lc_gemm_mx_kernel", nmom),
-                kernel="lc_gemm_mx_kernel (BF16x3 on v_mfma_f32_16x16x32_bf16)",
+                frac=round(smx["frac"], 4), traffic=traffic_from_profiles("lc_gemm_mx_kernel", nmom),
+                kernel="lc_gemm_mx_kernel<true> (BF16x3 on v_mfma_f32_16x16x32_bf16)",
                 flops_per_launch=smx["flops_per_launch"], avg_launch_ms=round(smx["avg_launch_ms"], 4),
                 frac_of_peaks={k: round(smx["achieved"] / v, 4) for k, v in PEAK_TFLOPS.items()},
                 bf16_mfma_utilization=round(smx["achieved"] * 6.0 / PEAK_TFLOPS["bf16"], 4))
@@ -225,7 +229,7 @@ def main():
                                 roofline=dict(bound="mfma", achieved=round(s64["achieved"], 3),
                                               peak=PEAK_TFLOPS["f64"], unit="TFLOP/s",
                                               frac=round(s64["frac"], 4),
-                                              traffic=traffic_from_profiles("lc::lc_gemm_kernel<double, 64>", nmom),
+                                              traffic=traffic_from_profiles("lc_gemm_kernel<double, 64>", nmom),
                                               kernel="lc_gemm_kernel<double,64>",
                                               avg_launch_ms=round(s64["avg_launch_ms"], 4))),
                 effective_tflops=round(smx["effective_tflops"], 3))

@@ -271,7 +271,7 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     const size_t head = 2 * tau_split + 2 * phi_split + scale_bytes;
     const size_t per_p = mixed ? mx_bytes_per_momentum(E) + 20 * 256
                                : (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * 16;
-    if (max_scratch_bytes <= 0) max_scratch_bytes = 64LL << 30;
+    if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
     long long avail = max_scratch_bytes - (long long)head;
     int pc = avail > 0 ? (int)(avail / (long long)per_p) : 0;
     if (pc < 1) fail(LC_ERR_NOMEM, "scratch budget %lld B below one momentum (%zu B)", max_scratch_bytes, per_p + head);

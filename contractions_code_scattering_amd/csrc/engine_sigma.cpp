@@ -217,7 +217,7 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     const size_t loop_per_p = 2 * ((((size_t)nt * E.blk * 16) + 255) & ~size_t(255));
     const size_t per_p = bytes_per_momentum(E, mixed);
     const size_t need1 = per_p > loop_per_p ? per_p : loop_per_p;
-    if (max_scratch_bytes <= 0) max_scratch_bytes = 64LL << 30;
+    if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
     const long long avail = max_scratch_bytes - (long long)head;
     int pc = avail > 0 ? (int)(avail / (long long)need1) : 0;
     if (pc < 1) fail(LC_ERR_NOMEM, "scratch budget %lld B below one momentum set (%zu B)", max_scratch_bytes, need1 + head);

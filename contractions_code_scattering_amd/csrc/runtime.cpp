@@ -16,6 +16,18 @@ std::string& last_error() {
   return e;
 }
 
+long long default_scratch_budget() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 64LL << 30;
+  }
+  // the engine's own scratch buffer counts as free: it is reused or grown
+  const long long avail = (long long)free_b + (long long)ctx().scratch.cap;
+  const long long b = avail / 10 * 6;
+  return b > (1LL << 30) ? b : (1LL << 30);
+}
+
 void require_init() {
   if (!ctx().on) fail(LC_ERR_NOT_INIT, "contraction library is not initialised (call lc_init)");
 }

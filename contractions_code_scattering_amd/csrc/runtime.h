@@ -94,6 +94,9 @@ struct Context {
 
 Context& ctx();
 void require_init();
+// Default bound on an engine's scratch: 60% of the device memory free now (larger momentum
+// chunks share each perambulator block between more momenta), at least 1 GiB.
+long long default_scratch_budget();
 // Message of the last failed C-ABI call (shared by every entry point).
 std::string& last_error();
 

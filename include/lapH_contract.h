@@ -87,7 +87,8 @@ int lc_sigmapi_triangle_row(int prec, const void* d_row, const void* d_diag, con
  *   d_raw  [np][nsrc][nt][10]  {rho_0..2, square, dT, dTHor, star1, star2, fish1, fish2} traces
  *   d_corr [np][nsrc][nt][8]   {rho_0..2, square, dT, dTHor, star, fish} correlators
  *   d_avg  [np][8][nt]         source-averaged C(dt), FP64 on the device
- * Unweighted diagrams; max_scratch_bytes bounds the intermediate storage (<= 0: 64 GiB). */
+ * Unweighted diagrams; max_scratch_bytes bounds the intermediate storage (<= 0: 60% of the
+ * device memory free at the call). */
 int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                        void* d_corr, void* d_avg);
