@@ -10,9 +10,11 @@
 // convention of tests/compute_pipi_I0.cpp:143-146), rho from the op pair (l,k) = (0,0) of
 // lib/contract_kernels_rho.cu with Gamma = gamma_i:
 //   rho_i  = -Tr[ B^dag (g5 g_i) A (-g_i g5) ]     A = tau Phi(t0),  B = Phi(tf) tau
-//   square = Tr[ U V ],  dT = Tr[ U V' ]            U = Q (g5 tau g5), Q = Phi^dag D(tf) Phi,
-//                                                   V = R tau^dag, R = Phi^dag D(t0) Phi,
-//                                                   V' = R' tau^dag, R' = Phi D(t0) Phi^dag
+//   square = Tr[ M R ],  dT = Tr[ M R' ]            M = tau^dag U, U = Q (g5 tau g5),
+//                                                   Q = Phi^dag D(tf) Phi, R = Phi^dag D(t0) Phi,
+//                                                   R' = Phi D(t0) Phi^dag
+//     (the same traces as Tr[U (R tau^dag)] and Tr[U (R' tau^dag)], reassociated so that the two
+//      share one dense product instead of needing R tau^dag and R' tau^dag separately)
 //   dTHor  = Tr[ (L W) (Rr tau^dag) ]               L = Phi^dag tau Phi, Rr = Phi tau Phi^dag
 //   star   = Tr[B S1] * Tr[L1 S2], fish = Tr[L1 S1] * Tr[B S2]
 //                                                   S1 = Phi tau^dag, S2 = Phi^dag tau^dag,
@@ -54,18 +56,19 @@ inline const char* off(const void* p, long long elems, size_t esz) {
 inline char* off(char* p, long long elems, size_t esz) { return p + (size_t)elems * esz; }
 
 // Number of D-space scratch matrices per (tf, t0, p) item and per (t, p) item.
-const int PAIR_BUFS = 7;
+const int PAIR_BUFS = 6;
 
 // Levels (each one grouped launch; closing traces fused into the GEMM that forms their left
-// factor, so A, S1, S2, U and LW are never stored):
+// factor, so A, S1, S2, M and LW are never stored):
 //   1  B = Phi(tf) tau, L1 = Phi(tf)^dag tau;  Qa = D(tf) Phi, T1 = D(t0) Phi, T1' = D(t0) Phi^dag
 //   2  A = tau Phi(t0)          -> rho_i = -Tr[A (K'_i B^dag L'_i)]
 //      S1 = Phi(t0) tau^dag     -> star1 = Tr[S1 B], fish1 = Tr[S1 L1]
 //      S2 = Phi(t0)^dag tau^dag -> star2 = Tr[S2 L1], fish2 = Tr[S2 B]
 //      L = L1 Phi(t0), Rr = B Phi(t0)^dag;  Q = Phi^dag Qa, R = Phi^dag T1, R' = Phi T1'
-//   3  V = R tau^dag, V' = R' tau^dag, Zh = Rr tau^dag
-//   4  U = Q (g5 tau g5)        -> square = Tr[U V], dT = Tr[U V']
+//   3  Zh = Rr tau^dag, U = Q (g5 tau g5)
+//   4  M = tau^dag U            -> square = Tr[M R], dT = Tr[M R']
 //      LW = L W                 -> dTHor = Tr[LW Zh]
+// Dense D x D products per (tf, t0, p): Zh, U, M, LW (4 x 64 block GEMMs).
 // tau / phi / scratch hold complex double (mode F64) or complex float (mode MX); raw is always
 // complex double.
 void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const void* phi, int p0,
@@ -79,7 +82,7 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   char* pb[PAIR_BUFS];
   for (int i = 0; i < PAIR_BUFS; i++) pb[i] = off(scratch, (long long)i * pc * npair * b);
   char* tb0 = off(scratch, (long long)PAIR_BUFS * pc * npair * b);
-  char *B = pb[0], *L1 = pb[1], *L = pb[2], *Rr = pb[3], *V = pb[4], *Vp = pb[5], *Zh = pb[6];
+  char *B = pb[0], *L1 = pb[1], *L = pb[2], *Rr = pb[3], *U = pb[4], *Zh = pb[5];
   // per-sink buffers [pc][nt]: strides {b, 0, nt*b}; per-source buffers [pc][nsrc]: {0, b, ns*b}
   char* Qa = tb0;
   char* Q = off(tb0, (long long)pc * nt * b);
@@ -136,23 +139,23 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   pg.gemm(phi_t0, dense(T1p, 0, b, ns * b), Rp, 0, b, ns * b, 1, ns, pc);
   pg.barrier();
   // level 3
-  pg.gemm(dense(R, 0, b, ns * b), tau_dag, V, P0, P1, P2, nt, ns, pc);
-  pg.gemm(dense(Rp, 0, b, ns * b), tau_dag, Vp, P0, P1, P2, nt, ns, pc);
   pg.gemm(pair(Rr), tau_dag, Zh, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_v, g5), g5), U, P0, P1, P2, nt, ns, pc);
   pg.barrier();
   // level 4
-  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_v, g5), g5), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
-          {slot(3, pair(V)), slot(4, pair(Vp))});
+  pg.gemm(tau_dag, pair(U), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+          {slot(3, dense(R, 0, b, ns * b)), slot(4, dense(Rp, 0, b, ns * b))});
   pg.gemm(pair(L), W, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
   pg.run(ctx().stream);
 }
 
 // Per-momentum-chunk scratch of the mixed form, in bytes per (pc = 1): FP32 pair buffers B, L1,
-// V, V', Zh (8 B per complex), split pair buffers B, L1, L, Rr (12 B), split per-sink Qa, Q and
-// per-source T1, T1', R, R' buffers (12 B).
+// Zh (8 B per complex), split pair buffers B, L1, L, Rr, U (12 B), split per-sink Qa, Q and
+// per-source T1, T1' buffers (12 B), FP32 per-source R, R' (8 B).
 size_t mx_bytes_per_momentum(const EngineShape& E) {
   const long long npair = (long long)E.nt * E.nsrc;
-  return (size_t)E.blk * (size_t)(npair * (5 * 8 + 4 * 12) + (2LL * E.nt + 4LL * E.nsrc) * 12);
+  return (size_t)E.blk * (size_t)(npair * (3 * 8 + 5 * 12) + (2LL * E.nt + 2LL * E.nsrc) * 12 +
+                                  2LL * E.nsrc * 8);
 }
 
 // The same chain as run_chunk in the mixed form (Program::MX). tau and Phi are held split, twice:
@@ -174,18 +177,18 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
     cur += ((size_t)elems * es + 255) & ~size_t(255);
     return p;
   };
-  char *B32 = take(pc * npair * b, 8), *L1_32 = take(pc * npair * b, 8), *V = take(pc * npair * b, 8),
-       *Vp = take(pc * npair * b, 8), *Zh = take(pc * npair * b, 8);
+  char *B32 = take(pc * npair * b, 8), *L1_32 = take(pc * npair * b, 8), *Zh = take(pc * npair * b, 8);
   char *Bs = take(pc * npair * b, 12), *L1s = take(pc * npair * b, 12), *Ls = take(pc * npair * b, 12),
-       *Rrs = take(pc * npair * b, 12);
+       *Rrs = take(pc * npair * b, 12), *Us = take(pc * npair * b, 12);
   char *Qa = take((long long)pc * nt * b, 12), *Q = take((long long)pc * nt * b, 12);
   char *T1 = take((long long)pc * ns * b, 12), *T1p = take((long long)pc * ns * b, 12),
-       *R = take((long long)pc * ns * b, 12), *Rp = take((long long)pc * ns * b, 12);
+       *R = take((long long)pc * ns * b, 8), *Rp = take((long long)pc * ns * b, 8);
 
   const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
   // tau(tf, t0_k): A role from the row-major planes, B role from the transposed planes
   const LcView tau_A = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0);
   const LcView tau_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TV);
+  const LcView tau_dag_A = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
   const LcView tau_dag_B = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
   const LcView W_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
   const char* pr = spl(phi_r, (long long)p0 * nt * m);
@@ -232,17 +235,17 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   pg.gemm(pair(L1s), phi_t0_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Ls);
   pg.gemm(pair(Bs), phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Rrs);
   pg.gemm(phi_tf_dag_A, dense(Qa, b, 0, nt * b, LC_TV), nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Q);
-  pg.gemm(phi_t0_dag_A, dense(T1, 0, b, ns * b, LC_TV), nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, R);
-  pg.gemm(phi_t0_A, dense(T1p, 0, b, ns * b, LC_TV), nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, Rp);
+  pg.gemm(phi_t0_dag_A, dense(T1, 0, b, ns * b, LC_TV), R, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(phi_t0_A, dense(T1p, 0, b, ns * b, LC_TV), Rp, 0, b, ns * b, 1, ns, pc);
   pg.barrier();
   // level 3
-  pg.gemm(dense(R, 0, b, ns * b), tau_dag_B, V, P0, P1, P2, nt, ns, pc);
-  pg.gemm(dense(Rp, 0, b, ns * b), tau_dag_B, Vp, P0, P1, P2, nt, ns, pc);
   pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
+          {}, Us, true);
   pg.barrier();
   // level 4
-  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
-          {slot(3, pair(V)), slot(4, pair(Vp))});
+  pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+          {slot(3, dense(R, 0, b, ns * b)), slot(4, dense(Rp, 0, b, ns * b))});
   pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
   pg.run(ctx().stream);
 }

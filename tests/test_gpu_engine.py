@@ -72,7 +72,7 @@ def test_engine_chunking_is_invisible(lc):
     tau, phi = _inputs(12, nt, nv, np_)
     t, f = to_device(tau), to_device(phi)
     _, c1, _ = engine.rho_pipi(t, f)
-    per_p = (7 * nt * nt + 2 * nt + 4 * nt) * 16 * nv * nv * 16
+    per_p = (6 * nt * nt + 2 * nt + 4 * nt) * 16 * nv * nv * 16
     _, c2, _ = engine.rho_pipi(t, f, max_scratch_bytes=per_p)
     assert np.array_equal(c1.to_numpy(), c2.to_numpy())
 
