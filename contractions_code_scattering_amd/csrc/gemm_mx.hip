@@ -187,8 +187,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     const int rem = rr - blk * tt;
     tm = rem / tiles;
     tn = rem - tm * tiles;
-    a = blk >> 2;
-    c = blk & 3;
+    a = g.col_tiles ? (blk & 3) : (blk >> 2);
+    c = g.col_tiles ? (blk >> 2) : (blk & 3);
   };
 
   // Chunk cursor of the fetch pipeline: tile, term (spin index b of the product) and K chunk of

@@ -52,6 +52,7 @@ struct LcGemmGroup {
   void* Cs;
   int cs_trans;
   int bpw;                  // output tiles per workgroup (mixed-precision kernel)
+  int col_tiles;            // bpw tiles of a workgroup: 0 one block row, 1 one block column
   long long cs[3];
   double alpha[2];
   int n[3];

@@ -126,6 +126,12 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
   if (mode_ == MX) {
     g.bpw = tt == 1 ? 4 : 1;
     per_item = 16 * tt / g.bpw;
+    // Items along batch axis 0 run next to each other. When only A changes along that axis, B is
+    // shared by neighbouring items and stays in L2, while each A block is read by this item
+    // alone: a workgroup then takes one block column, so that the four workgroups of the item
+    // read each A block row at about the same time (one fetch from memory instead of one per
+    // output tile).
+    g.col_tiles = (g.bpw > 1 && A.s[0] != 0 && B.s[0] == 0) ? 1 : 0;
   }
   l->total_wg += items * per_item;
   l->count++;
