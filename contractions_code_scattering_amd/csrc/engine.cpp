@@ -250,6 +250,114 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   pg.run(ctx().stream);
 }
 
+// Scratch per momentum of the sandwich form: split pair buffers L, Rr, U (12 B per complex),
+// FP32 pair buffer Zh (8 B), split per-sink Q (12 B), FP32 per-source R, R' (8 B).
+size_t mx_sw_bytes_per_momentum(const EngineShape& E) {
+  const long long npair = (long long)E.nt * E.nsrc;
+  return (size_t)E.blk * (size_t)(npair * (3 * 12 + 8) + E.nt * 12LL + 2LL * E.nsrc * 8);
+}
+
+// The mixed form with fused sandwiches (Nv = 64): every elemental-perambulator-elemental chain is
+// one pass, Phi^(+-)(tf) tau Phi^(+-)(t0), with the middle product kept on the chip, and the
+// closing traces of the short diagrams taken against tau^dag itself (FP32 copy tau32):
+//   1  L  = Phi(tf)^dag tau Phi(t0)      -> rho_i = -Tr[L (K'_i tau^dag L'_i)] (i = 0, 1),
+//                                           fish1 = Tr[L tau^dag]
+//      Rr = Phi(tf) tau Phi(t0)^dag      -> fish2 = Tr[Rr tau^dag],
+//                                           rho_2 = -conj Tr[Rr (K'_2^dag tau^dag L'_2^dag)]
+//      X  = Phi(tf) tau Phi(t0)          -> star1 = Tr[X tau^dag]
+//      Y  = Phi(tf)^dag tau Phi(t0)^dag  -> star2 = Tr[Y tau^dag]
+//      Q = Phi^dag D(tf) Phi, R = Phi^dag D(t0) Phi, R' = Phi D(t0) Phi^dag
+//   2  Zh = Rr tau^dag, U = Q (g5 tau g5)
+//   3  M = tau^dag U -> square = Tr[M R], dT = Tr[M R'];  LW = L W -> dTHor = Tr[LW Zh]
+// These are the traces of run_chunk reassociated: with Phi (x) 1 commuting with spin matrices,
+// Tr[A (K' B^dag L')] = Tr[tau Phi(t0) K' tau^dag Phi(tf)^dag L'] = Tr[L K' tau^dag L'],
+// Tr[S1 B] = Tr[X tau^dag], Tr[S1 L1] = Tr[L tau^dag], Tr[S2 L1] = Tr[Y tau^dag], Tr[S2 B] =
+// Tr[Rr tau^dag], and conj Tr[L K' tau^dag L'] = Tr[L'^dag tau K'^dag L^dag] = Tr[Rr K'^dag tau^dag
+// L'^dag] (one rho polarisation moves to Rr so that no product carries more than three traces).
+// B and L1 are never formed.
+void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t, const char* tau32,
+                     const char* phi_r, const char* phi_t, int p0, int pc, char* scratch, void* raw) {
+  const int nt = E.nt, ns = E.nsrc;
+  const long long m = E.m, b = E.blk;
+  const long long npair = (long long)nt * ns;
+  auto f32 = [](auto p, long long e) { return ::off(p, e, 8); };
+  auto spl = [](auto p, long long e) { return ::off(p, e, 12); };
+  char* cur = scratch;
+  auto take = [&](long long elems, size_t es) {
+    char* p = cur;
+    cur += ((size_t)elems * es + 255) & ~size_t(255);
+    return p;
+  };
+  char* Zh = take(pc * npair * b, 8);
+  char *Ls = take(pc * npair * b, 12), *Rrs = take(pc * npair * b, 12), *Us = take(pc * npair * b, 12);
+  char* Q = take((long long)pc * nt * b, 12);
+  char *R = take((long long)pc * ns * b, 8), *Rp = take((long long)pc * ns * b, 8);
+
+  const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
+  const LcView tau_A = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0);
+  const LcView tau_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TV);
+  const LcView tau_dag_A = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
+  const LcView tau_dag_B = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
+  const LcView W_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
+  const LcView tau_dag_Y = dense(f32(tau32, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
+  (void)tau_A;
+  const char* pr = spl(phi_r, (long long)p0 * nt * m);
+  const char* pt = spl(phi_t, (long long)p0 * nt * m);
+  const LcView phi_tf_A = kron(pr, m, 0, nt * m);
+  const LcView phi_tf_B = kron(pt, m, 0, nt * m, LC_TV);
+  const LcView phi_tf_dag_A = kron(pt, m, 0, nt * m, LC_CONJ);
+  const LcView phi_t0_A = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m);
+  const LcView phi_t0_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
+  const LcView phi_t0_dag_A = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ);
+  const LcView phi_t0_dag_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  const LcView D_tf_B = dense(tau_t, (nt + 1) * b, 0, 0, LC_TV);
+  const LcView D_t0_B = dense(spl(tau_t, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0, LC_TV);
+  auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
+  char* rw = ::off(static_cast<char*>(raw), (long long)p0 * ns * nt * 10, 16);
+  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  auto slot = [&](int k, const LcView& Y, double ar = 1.0) {
+    FusedTrace f = fused(Y, ::off(rw, k, 16), O0, O1, O2, ar);
+    f.out_double = 1;
+    return f;
+  };
+
+  Program pg(E.nv, Program::MX);
+  pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
+  // level 1: the sandwiches
+  const Spin g5 = gamma(5);
+  std::vector<FusedTrace> tl, tr;
+  for (int i = 0; i < 3; i++) {
+    const Spin Lp = g5 * rhoj(1, i, false);
+    const Spin Kp = rhoj(1, i, true) * g5;
+    if (i < 2) {
+      tl.push_back(slot(i, right(left(tau_dag_Y, Kp), Lp), -1.0));
+    } else {
+      tr.push_back(slot(i, right(left(tau_dag_Y, Kp.adjoint()), Lp.adjoint()), -1.0));
+      tr.back().conj = true;
+    }
+  }
+  tl.push_back(slot(8, tau_dag_Y));
+  tr.push_back(slot(9, tau_dag_Y));
+  pg.sandwich(phi_tf_dag_A, tau_B, phi_t0_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tl, Ls);
+  pg.sandwich(phi_tf_A, tau_B, phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tr, Rrs);
+  pg.sandwich(phi_tf_A, tau_B, phi_t0_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(6, tau_dag_Y)});
+  pg.sandwich(phi_tf_dag_A, tau_B, phi_t0_dag_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(7, tau_dag_Y)});
+  pg.sandwich(phi_tf_dag_A, D_tf_B, phi_tf_B, nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Q);
+  pg.sandwich(phi_t0_dag_A, D_t0_B, phi_t0_B, R, 0, b, ns * b, 1, ns, pc);
+  pg.sandwich(phi_t0_A, D_t0_B, phi_t0_dag_B, Rp, 0, b, ns * b, 1, ns, pc);
+  pg.barrier();
+  // level 2
+  pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
+          {}, Us, true);
+  pg.barrier();
+  // level 3
+  pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+          {slot(3, dense(R, 0, b, ns * b)), slot(4, dense(Rp, 0, b, ns * b))});
+  pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
+  pg.run(ctx().stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -268,12 +376,16 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     const long long npair = (long long)nt * nsrc;
     // mixed form: tau and Phi split twice (row-major and transposed planes, 12 B per complex
     // each) + FP64 per-slice scale factors, ahead of the scratch area
+    // the fused-sandwich form (Nv = 64) also keeps tau in FP32 for its closing traces
+    const bool sw = mixed && nv == 64;
     const size_t tau_split = mixed ? (((size_t)nt * nt * E.blk * 12 + 255) & ~size_t(255)) : 0;
+    const size_t tau_f32 = sw ? (((size_t)nt * nt * E.blk * 8 + 255) & ~size_t(255)) : 0;
     const size_t phi_split = mixed ? (((size_t)np * nt * E.m * 12 + 255) & ~size_t(255)) : 0;
     const size_t scale_bytes = mixed ? (((size_t)(nt * nt + np * nt) * 8 + 255) & ~size_t(255)) : 0;
-    const size_t head = 2 * tau_split + 2 * phi_split + scale_bytes;
-    const size_t per_p = mixed ? mx_bytes_per_momentum(E) + 20 * 256
-                               : (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * 16;
+    const size_t head = 2 * tau_split + tau_f32 + 2 * phi_split + scale_bytes;
+    const size_t per_p = sw      ? mx_sw_bytes_per_momentum(E) + 8 * 256
+                         : mixed ? mx_bytes_per_momentum(E) + 20 * 256
+                                 : (size_t)(PAIR_BUFS * npair + 2LL * nt + 4LL * nsrc) * E.blk * 16;
     if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
     long long avail = max_scratch_bytes - (long long)head;
     int pc = avail > 0 ? (int)(avail / (long long)per_p) : 0;
@@ -287,15 +399,18 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     if (mixed) {
       char* tau_r = base;
       char* tau_t = tau_r + tau_split;
-      char* phi_r = tau_t + tau_split;
+      char* tau32 = sw ? tau_t + tau_split : nullptr;
+      char* phi_r = tau_t + tau_split + tau_f32;
       char* phi_t = phi_r + phi_split;
       scales = reinterpret_cast<double*>(phi_t + phi_split);
-      hip_check(lc_launch_split_rescale(d_tau, tau_r, tau_t, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
-      hip_check(lc_launch_split_rescale(d_phi, phi_r, phi_t, E.m, nv, (long long)np * nt, scales + (size_t)nt * nt, s),
+      hip_check(lc_launch_split_rescale(d_tau, tau_r, tau_t, tau32, E.blk, nv, (long long)nt * nt, scales, s),
+                "split tau");
+      hip_check(lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, E.m, nv, (long long)np * nt, scales + (size_t)nt * nt, s),
                 "split phi");
       for (int p0 = 0; p0 < np; p0 += pc) {
         const int n = (np - p0) < pc ? (np - p0) : pc;
-        run_chunk_mx(E, tau_r, tau_t, phi_r, phi_t, p0, n, scratch, d_raw);
+        if (sw) run_chunk_mx_sw(E, tau_r, tau_t, tau32, phi_r, phi_t, p0, n, scratch, d_raw);
+        else run_chunk_mx(E, tau_r, tau_t, phi_r, phi_t, p0, n, scratch, d_raw);
       }
     } else {
       for (int p0 = 0; p0 < np; p0 += pc) {

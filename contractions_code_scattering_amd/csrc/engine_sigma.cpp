@@ -234,8 +234,8 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
       char* er = tt + tau_split;
       char* et = er + el_split;
       scales = reinterpret_cast<double*>(et + el_split);
-      hip_check(lc_launch_split_rescale(d_tau, tr, tt, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
-      hip_check(lc_launch_split_rescale(d_el, er, et, E.m, nv, (long long)np * nt * NT_EL, scales + (size_t)nt * nt, s),
+      hip_check(lc_launch_split_rescale(d_tau, tr, tt, nullptr, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
+      hip_check(lc_launch_split_rescale(d_el, er, et, nullptr, E.m, nv, (long long)np * nt * NT_EL, scales + (size_t)nt * nt, s),
                 "split elementals");
       tau = Store{tr, tt};
       el = Store{er, et};

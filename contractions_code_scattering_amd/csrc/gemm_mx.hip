@@ -124,12 +124,19 @@ __device__ __forceinline__ void fetch_split(const unsigned short* p, int line, i
 // paths. Besides saving the bounds checks this matters for the schedule: the compiler's
 // wait-count analysis merges all paths, so registers written by a fallback load would force a
 // wait for the whole prefetch at the top of the matrix-core section.
-template <bool FULL>
+//
+// SW (sandwich, with FULL and Nv = 64): every group is C = alpha * (L (x) 1) M (R (x) 1) with L = A
+// and R = B2 elementals and M = B a D-space view. Per output block (a,c) the workgroup first
+// forms T = L M_ac (K = Nv), then, with T taken from the accumulators into LDS as the A operand
+// (rounded to FP32 and split exactly, as a stored intermediate would be), C_ac = T R. T never
+// leaves the chip. Its two K chunks are staged in two A buffers at once, so that no accumulator
+// copy has to wait in registers for the second one.
+template <bool FULL, bool SW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total) {
   typedef float T;
   // planes [piece * 2 + re/im][line][k]
-  __shared__ __attribute__((aligned(16))) __bf16 As[6][MX_TM][MX_KT];
+  __shared__ __attribute__((aligned(16))) __bf16 As[SW ? 2 : 1][6][MX_TM][MX_KT];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[6][MX_TM][MX_KT];
   __shared__ double red[8];
 
@@ -153,7 +160,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const bool akron = g.A.kind == LC_KRON, bkron = g.B.kind == LC_KRON;
   const bool vec_ok = FULL || (nv & 7) == 0;  // 16-byte plane vectors stay aligned
 
-  const int nterm = (akron || bkron) ? 1 : 4;
+  // sandwich: term 0 is L M_ac, term 1 is T R
+  const int nterm = SW ? 2 : (akron || bkron) ? 1 : 4;
   const int nk = (nv + MX_KT - 1) / MX_KT;
   const int cpb = nterm * nk;  // chunks per tile
   const int nchunk = cpb * bpw;
@@ -180,6 +188,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   u32x4 va[6], vb[6];
   int a_swap = 0;
   unsigned a_mr = 0, a_mi = 0, b_mi = 0;
+  int a_t2 = 0, a_kc = 0;  // sandwich: the chunk belongs to T R, its K chunk
+  int c_asel = 0;          // A buffer of the chunk in LDS
 
   auto tile_of = [&](int tl, int& a, int& c, int& tm, int& tn) {
     const int rr = r0 + tl;
@@ -210,6 +220,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   auto set_term = [&]() {
     int a, c;
     tile_of(cur_tl, a, c, cur_tm, cur_tn);
+    if (SW && cur_term == 1) {
+      // T R: A comes from the accumulators, B is the right elemental (no phase)
+      t_swap = 0;
+      t_mr = 0u;
+      t_mi = 0u;
+      t_bmi = (g.B2.flags & LC_CONJ) ? SIGN2 : 0u;
+      pB = reinterpret_cast<const unsigned short*>(g.B2.base) + 6 * voff(g.B2.s, co) +
+           (long long)(cur_tn * MX_TM + s_line) * nv + s_kg * 8;
+      return;
+    }
     int ab, bb, ba, bc;
     if (akron) { ab = a; bb = a; ba = a; bc = c; }
     else if (bkron) { ab = a; bb = c; ba = c; bc = c; }
@@ -243,7 +263,15 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     a_mr = t_mr;
     a_mi = t_mi;
     b_mi = t_bmi;
-    if (FULL) {
+    a_t2 = SW && cur_term == 1;
+    a_kc = cur_kc;
+    if (SW) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        if (!a_t2) va[q] = gload<u32x4>(pA + q * nv2 + k0);
+        vb[q] = gload<u32x4>(pB + q * nv2 + k0);
+      }
+    } else if (FULL) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
         va[q] = gload<u32x4>(pA + q * nv2 + k0);
@@ -269,10 +297,44 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   };
 
   auto store = [&]() {
+    c_asel = 0;
+    if (SW && a_t2) {
+      c_asel = a_kc;
+      if (a_kc == 0) {
+        // T is complete: this wave's 32 x 32 part goes to A buffer wn (T's columns are the K of
+        // T R, 32 per chunk), split, in the layout of a staged A chunk
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+          for (int n = 0; n < 2; n++) {
+            const fx4v xr = bre[m][n] + sre[m][n], xi = bim[m][n] + sim[m][n];
+            unsigned r01[3], r23[3], i01[3], i23[3];
+            split2(xr[0], xr[1], r01[0], r01[1], r01[2]);
+            split2(xr[2], xr[3], r23[0], r23[1], r23[2]);
+            split2(xi[0], xi[1], i01[0], i01[1], i01[2]);
+            split2(xi[2], xi[3], i23[0], i23[1], i23[2]);
+            const int line = wm * 32 + m * 16 + (lane & 15);
+            const int kg = 2 * n + ((lane >> 4) >> 1);  // 8-wide K group of the four values
+            const int e = ((kg ^ swz(line)) * 8) + ((lane >> 4) & 1) * 4;
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+              *reinterpret_cast<u32x2*>(&As[wn][2 * p][line][e]) = u32x2{r01[p], r23[p]};
+              *reinterpret_cast<u32x2*>(&As[wn][2 * p + 1][line][e]) = u32x2{i01[p], i23[p]};
+            }
+          }
+        zero_acc();
+      }
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_pos * 8]) = vb[2 * p];
+        *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_pos * 8]) = vb[2 * p + 1] ^ b_mi;
+      }
+      return;
+    }
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-      *reinterpret_cast<u32x4*>(&As[2 * p + a_swap][s_line][s_pos * 8]) = va[2 * p] ^ a_mr;
-      *reinterpret_cast<u32x4*>(&As[2 * p + 1 - a_swap][s_line][s_pos * 8]) = va[2 * p + 1] ^ a_mi;
+      *reinterpret_cast<u32x4*>(&As[0][2 * p + a_swap][s_line][s_pos * 8]) = va[2 * p] ^ a_mr;
+      *reinterpret_cast<u32x4*>(&As[0][2 * p + 1 - a_swap][s_line][s_pos * 8]) = va[2 * p + 1] ^ a_mi;
       *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_pos * 8]) = vb[2 * p];
       *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_pos * 8]) = vb[2 * p + 1] ^ b_mi;
     }
@@ -463,7 +525,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++)
-          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[2 * p + ri][al[m]][ao[m]]);
+          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[SW ? c_asel : 0][2 * p + ri][al[m]][ao[m]]);
 #pragma unroll
       for (int p = 0; p < 3; p++)
         nai[p] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, af[p][1]) ^ SIGN2);
@@ -531,7 +593,8 @@ __global__ void lc_mfma_bf16_selftest_kernel(const float* a, const float* b, flo
 // Inputs of the mixed path: per-slice Frobenius scale ||x_i||_F (FP64), then x_i / ||x_i||_F
 // rounded to FP32 and split into the six BF16 planes, written twice: row-major planes (y_rows)
 // and with every Nv x Nv block transposed (y_trans), so that each operand role finds its K
-// dimension contiguous. A slice is `len` complex values made of len / Nv^2 blocks.
+// dimension contiguous; optionally also the FP32 values themselves (y32, closing-trace factors).
+// A slice is `len` complex values made of len / Nv^2 blocks.
 __global__ void __launch_bounds__(256) lc_slice_norm_kernel(const double2* __restrict__ x,
                                                             long long len, long long stride,
                                                             double* __restrict__ scale) {
@@ -552,7 +615,8 @@ __global__ void __launch_bounds__(256) lc_slice_norm_kernel(const double2* __res
 constexpr int SC_T = 32;
 __global__ void __launch_bounds__(256) lc_split_convert_kernel(const double2* __restrict__ x,
                                                                unsigned short* __restrict__ yr,
-                                                               unsigned short* __restrict__ yt, long long len,
+                                                               unsigned short* __restrict__ yt,
+                                                               float2* __restrict__ y32, long long len,
                                                                int nv, const double* __restrict__ scale) {
   __shared__ unsigned short pc[6][SC_T][SC_T + 1];
   const int tpb = (nv + SC_T - 1) / SC_T;  // tiles per block side
@@ -572,8 +636,10 @@ __global__ void __launch_bounds__(256) lc_split_convert_kernel(const double2* __
     if (row < nv && col < nv) {
       const long long e = (long long)row * nv + col;
       const double2 v = xb[e];
+      const float fr = (float)(v.x * inv), fi = (float)(v.y * inv);
+      if (y32) y32[bi * nv2 + e] = make_float2(fr, fi);
       unsigned p0, p1, p2;
-      split2((float)(v.x * inv), (float)(v.y * inv), p0, p1, p2);
+      split2(fr, fi, p0, p1, p2);
       const unsigned q[3] = {p0, p1, p2};
 #pragma unroll
       for (int p = 0; p < 3; p++) {
@@ -600,18 +666,19 @@ __global__ void __launch_bounds__(256) lc_split_convert_kernel(const double2* __
 
 }  // namespace lc
 
-// variant 1: full-tile kernel; 0: general kernel. The planner decides which conditions every
-// group of the launch meets.
+// variant 2: sandwich kernel (full tiles, Nv = 64); 1: full-tile kernel; 0: general kernel. The
+// planner decides which conditions every group of the launch meets.
 extern "C" hipError_t lc_launch_gemm_mx(int variant, const LcGemmGroup* d_groups, int ng, long long total_wg,
                                         int nv, hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
   const long long per = (total_wg + 7) / 8;
-  if (variant == 1)
-    hipLaunchKernelGGL(lc::lc_gemm_mx_kernel<true>, dim3((unsigned)(8 * per)), dim3(256), 0, s, d_groups, ng,
-                       nv, total_wg);
+  const dim3 grid((unsigned)(8 * per)), block(256);
+  if (variant == 2)
+    hipLaunchKernelGGL((lc::lc_gemm_mx_kernel<true, true>), grid, block, 0, s, d_groups, ng, nv, total_wg);
+  else if (variant == 1)
+    hipLaunchKernelGGL((lc::lc_gemm_mx_kernel<true, false>), grid, block, 0, s, d_groups, ng, nv, total_wg);
   else
-    hipLaunchKernelGGL(lc::lc_gemm_mx_kernel<false>, dim3((unsigned)(8 * per)), dim3(256), 0, s, d_groups, ng,
-                       nv, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_mx_kernel<false, false>), grid, block, 0, s, d_groups, ng, nv, total_wg);
   return hipGetLastError();
 }
 
@@ -623,8 +690,9 @@ extern "C" hipError_t lc_launch_mfma_bf16_selftest(const float* a, const float* 
 
 // x: nslice slices of len complex doubles (contiguous, len a multiple of nv^2); y_rows, y_trans:
 // the split planes (6 * len * nslice BF16 each); scale: nslice doubles.
-extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, long long len,
-                                              int nv, long long nslice, double* scale, hipStream_t s) {
+extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, void* y32,
+                                              long long len, int nv, long long nslice, double* scale,
+                                              hipStream_t s) {
   if (nslice <= 0 || len <= 0) return hipSuccess;
   hipLaunchKernelGGL(lc::lc_slice_norm_kernel, dim3((unsigned)nslice), dim3(256), 0, s,
                      (const double2*)x, len, len, scale);
@@ -632,6 +700,7 @@ extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void*
   const long long tpb = (nv + lc::SC_T - 1) / lc::SC_T;
   const long long tiles = (len * nslice / nv2) * tpb * tpb;
   hipLaunchKernelGGL(lc::lc_split_convert_kernel, dim3((unsigned)tiles), dim3(256), 0, s, (const double2*)x,
-                     (unsigned short*)y_rows, (unsigned short*)y_trans, len, nv, (const double*)scale);
+                     (unsigned short*)y_rows, (unsigned short*)y_trans, (float2*)y32, len, nv,
+                     (const double*)scale);
   return hipGetLastError();
 }

@@ -350,6 +350,7 @@ __global__ void lc_reduce_kernel(const LcReduceJob* __restrict__ jobs, int njobs
     sr += p[2 * k];
     si += p[2 * k + 1];
   }
+  if (J.conj) si = -si;
   const long long o = voff(J.os, co);
   if (J.out_double) {
     double* out = reinterpret_cast<double*>(J.out) + 2 * o;

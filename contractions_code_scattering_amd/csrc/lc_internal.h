@@ -48,6 +48,7 @@ struct LcView {
 // every block transposed when cs_trans: element (i,j) at j * Nv + i).
 struct LcGemmGroup {
   LcView A, B;
+  LcView B2;                // sandwich groups (mixed-precision kernel): C = alpha * A B B2
   void* C;
   void* Cs;
   int cs_trans;
@@ -64,7 +65,8 @@ struct LcGemmGroup {
   double* part[LC_MAX_FUSED_TRACES];  // complex double [items][16*tiles*tiles]
 };
 
-// out[item] (+)= sum_{k < nk} part[item*nk + k] (fixed order); out addressed through strides.
+// out[item] (+)= sum_{k < nk} part[item*nk + k] (fixed order), complex conjugated first when
+// conj; out addressed through strides.
 struct LcReduceJob {
   const double* part;
   void* out;                // complex of the output type
@@ -73,7 +75,7 @@ struct LcReduceJob {
   int nk;
   int accumulate;
   int out_double;           // 1: out is complex double, 0: complex float
-  int pad;
+  int conj;
 };
 
 // out (+)= alpha * Tr[X * Y] over a batch. out holds complex values of the compute type.

@@ -54,8 +54,23 @@ void Program::barrier() {
 void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
                    long long cs2, int n0, int n1, int n2, double ar, double ai,
                    const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
+  batch(A, B, nullptr, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans);
+}
+
+void Program::sandwich(const LcView& L, const LcView& M, const LcView& R, void* C, long long cs0,
+                       long long cs1, long long cs2, int n0, int n1, int n2, double ar, double ai,
+                       const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
+  if (!sandwich_supported()) fail(LC_ERR_ARG, "fused sandwich needs the mixed-precision mode and Nv = 64");
+  if (L.kind != LC_KRON || M.kind != LC_DENSE || R.kind != LC_KRON)
+    fail(LC_ERR_ARG, "sandwich operands must be elemental, D-space, elemental");
+  batch(L, M, &R, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans);
+}
+
+void Program::batch(const LcView& A, const LcView& B, const LcView* B2, void* C, long long cs0, long long cs1,
+                    long long cs2, int n0, int n1, int n2, double ar, double ai,
+                    const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
   if (!last_axis_fastest_ || n2 <= 1) {
-    gemm_items(A, B, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans);
+    gemm_items(A, B, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans, B2);
     return;
   }
   // the same batch with its axes relabelled (2, 0, 1): workgroups then walk axis 2 fastest
@@ -70,12 +85,14 @@ void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, lon
     const long long o0 = f.os[0], o1 = f.os[1], o2 = f.os[2];
     f.os[0] = o2; f.os[1] = o0; f.os[2] = o1;
   }
-  gemm_items(rot(A), rot(B), C, cs2, cs0, cs1, n2, n0, n1, ar, ai, tr, Cs, cs_trans);
+  LcView r2;
+  if (B2) r2 = rot(*B2);
+  gemm_items(rot(A), rot(B), C, cs2, cs0, cs1, n2, n0, n1, ar, ai, tr, Cs, cs_trans, B2 ? &r2 : nullptr);
 }
 
 void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
                          long long cs2, int n0, int n1, int n2, double ar, double ai,
-                         const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
+                         const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans, const LcView* B2) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
   if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
@@ -85,6 +102,8 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
   if (mode_ == MX) {
     if ((A.flags & LC_TV) || !(B.flags & LC_TV))
       fail(LC_ERR_ARG, "mixed-precision operands must be read along contiguous K (A row-major, B transposed)");
+    if (B2 && (!(B2->flags & LC_TV) || B2->kind != LC_KRON))
+      fail(LC_ERR_ARG, "sandwich right factor must be an elemental read along contiguous K");
     for (const LcView* v : {&A, &B})
       for (int a = 0; a < 4; a++)
         for (const double* ph : {v->ph_r[a], v->ph_c[a]})
@@ -93,7 +112,7 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
     // complete 64 x 64 tiles and 16-byte alignment of every operand and output (split storage:
     // 12 B per complex element, FP32 output 8 B): full-tile kernel
     bool full = nv_ % 64 == 0;
-    for (const LcView* v : {&A, &B}) {
+    for (const LcView* v : {&A, &B, B2 ? B2 : &B}) {
       full = full && (reinterpret_cast<uintptr_t>(v->base) & 15) == 0;
       for (int d = 0; d < 3; d++) full = full && v->s[d] % 4 == 0;
     }
@@ -104,12 +123,17 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
       for (int d = 0; d < 3; d++) full = full && f.Y.s[d] % 2 == 0;
     }
     variant = full ? 1 : 0;
+    if (B2) {
+      if (!full || nv_ != 64) fail(LC_ERR_ARG, "fused sandwich needs Nv = 64 and 16-byte aligned operands");
+      variant = 2;
+    }
   }
   Level* l = open_level(false, variant);
   LcGemmGroup g;
   std::memset(&g, 0, sizeof g);
   g.A = A;
   g.B = B;
+  if (B2) g.B2 = *B2;
   g.C = C;
   g.Cs = Cs;
   g.cs_trans = cs_trans ? 1 : 0;
@@ -151,6 +175,7 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
     j.n[0] = n0; j.n[1] = n1; j.n[2] = n2;
     j.nk = (int)per_item;
     j.accumulate = f.accumulate ? 1 : 0;
+    j.conj = f.conj ? 1 : 0;
     j.out_double = f.out_double < 0 ? (dbl_ ? 1 : 0) : f.out_double;
     l->jobs.push_back((int)jobs_.size());
     jobs_.push_back(j);
@@ -158,7 +183,7 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
     partial_bytes_ += (bytes + 255) & ~size_t(255);
   }
   gemms_.push_back(g);
-  const int nterm = (A.kind == LC_KRON || B.kind == LC_KRON) ? 1 : 4;
+  const int nterm = B2 ? 2 : (A.kind == LC_KRON || B.kind == LC_KRON) ? 1 : 4;
   gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
 }
 

@@ -21,8 +21,9 @@ extern "C" hipError_t lc_launch_gemm_mx(int variant, const LcGemmGroup* d_groups
                                         long long total_wg, int nv, hipStream_t s);
 extern "C" hipError_t lc_launch_mfma_bf16_selftest(const float* a, const float* b, float* d,
                                                    hipStream_t s);
-extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, long long len,
-                                              int nv, long long nslice, double* scale, hipStream_t s);
+extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void* y_trans, void* y32,
+                                              long long len, int nv, long long nslice, double* scale,
+                                              hipStream_t s);
 extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b,
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
@@ -108,6 +109,7 @@ struct FusedTrace {
   double ar = 1.0, ai = 0.0;
   bool accumulate = false;
   int out_double = -1;  // -1: the program's compute type
+  bool conj = false;    // deliver conj(alpha * Tr[C * Y])
 };
 
 inline FusedTrace fused(const LcView& Y, void* out, long long os0, long long os1, long long os2,
@@ -139,6 +141,13 @@ class Program {
   void gemm(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
             long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
             const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false);
+  // Sandwich C = alpha * (L (x) 1) M (R (x) 1) over the batch, MX mode with Nv = 64 only
+  // (sandwich_supported): L an A-role elemental view, M a B-role D-space view, R a B-role
+  // elemental view. Each block is formed in one pass with L M_ac kept on the chip.
+  void sandwich(const LcView& L, const LcView& M, const LcView& R, void* C, long long cs0, long long cs1,
+                long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
+                const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false);
+  bool sandwich_supported() const { return mode_ == MX && nv_ == 64; }
   // Workgroup order over a batch: by default axis 0 runs fastest. With last_axis_fastest, axis 2
   // does (e.g. the momenta of one (sink, source) pair next to each other, so that the shared
   // perambulator block is fetched from HBM once and from L2 for the other momenta).
@@ -183,7 +192,10 @@ class Program {
   Level* open_level(bool is_trace, int variant = 0);
   void gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
                   int n0, int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
-                  void* Cs, bool cs_trans);
+                  void* Cs, bool cs_trans, const LcView* B2 = nullptr);
+  void batch(const LcView& A, const LcView& B, const LcView* B2, void* C, long long cs0, long long cs1,
+             long long cs2, int n0, int n1, int n2, double ar, double ai,
+             const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans);
   bool last_axis_fastest_ = false;
 };
 

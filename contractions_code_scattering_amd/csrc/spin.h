@@ -30,6 +30,12 @@ struct Spin {
         for (int k = 0; k < 4; k++) r.m[i][j] += m[i][k] * o.m[k][j];
     return r;
   }
+  Spin adjoint() const {
+    Spin r;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) r.m[i][j] = std::conj(m[j][i]);
+    return r;
+  }
   Spin scaled(cd f) const {
     Spin r;
     for (int i = 0; i < 4; i++)

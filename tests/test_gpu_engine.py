@@ -96,7 +96,7 @@ def test_mfma_bf16_layout(lc):
     np.testing.assert_allclose(d, ref, rtol=1e-6, atol=1e-5)
 
 
-@pytest.mark.parametrize("nt,nv,np_", [(4, 16, 2), (3, 64, 1), (4, 33, 1), (2, 128, 1)])
+@pytest.mark.parametrize("nt,nv,np_", [(4, 16, 2), (3, 64, 1), (4, 64, 3), (4, 33, 1), (2, 128, 1)])
 def test_engine_mixed_vs_fp64(lc, nt, nv, np_):
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
