@@ -338,10 +338,12 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   }
   tl.push_back(slot(8, tau_dag_Y));
   tr.push_back(slot(9, tau_dag_Y));
-  pg.sandwich(phi_tf_dag_A, tau_B, phi_t0_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tl, Ls);
-  pg.sandwich(phi_tf_A, tau_B, phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tr, Rrs);
-  pg.sandwich(phi_tf_A, tau_B, phi_t0_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(6, tau_dag_Y)});
-  pg.sandwich(phi_tf_dag_A, tau_B, phi_t0_dag_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(7, tau_dag_Y)});
+  // L and Y share Phi(tf)^dag tau, Rr and X share Phi(tf) tau: one pass each
+  const FusedTrace star1 = slot(6, tau_dag_Y), star2 = slot(7, tau_dag_Y);
+  pg.sandwich(phi_tf_dag_A, tau_B, phi_t0_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tl, Ls, false,
+              &phi_t0_dag_B, &star2);
+  pg.sandwich(phi_tf_A, tau_B, phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tr, Rrs, false,
+              &phi_t0_B, &star1);
   pg.sandwich(phi_tf_dag_A, D_tf_B, phi_tf_B, nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Q);
   pg.sandwich(phi_t0_dag_A, D_t0_B, phi_t0_B, R, 0, b, ns * b, 1, ns, pc);
   pg.sandwich(phi_t0_A, D_t0_B, phi_t0_dag_B, Rp, 0, b, ns * b, 1, ns, pc);

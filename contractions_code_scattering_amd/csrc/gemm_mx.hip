@@ -130,7 +130,9 @@ __device__ __forceinline__ void fetch_split(const unsigned short* p, int line, i
 // forms T = L M_ac (K = Nv), then, with T taken from the accumulators into LDS as the A operand
 // (rounded to FP32 and split exactly, as a stored intermediate would be), C_ac = T R. T never
 // leaves the chip. Its two K chunks are staged in two A buffers at once, so that no accumulator
-// copy has to wait in registers for the second one.
+// copy has to wait in registers for the second one. With a second right factor B3 the same T
+// also forms T B3 (third term), which is only traced (Yx): one product L M_ac serves two
+// sandwiches.
 template <bool FULL, bool SW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total) {
@@ -160,8 +162,9 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   const bool akron = g.A.kind == LC_KRON, bkron = g.B.kind == LC_KRON;
   const bool vec_ok = FULL || (nv & 7) == 0;  // 16-byte plane vectors stay aligned
 
-  // sandwich: term 0 is L M_ac, term 1 is T R
-  const int nterm = SW ? 2 : (akron || bkron) ? 1 : 4;
+  // sandwich: term 0 is L M_ac, term 1 is T R, term 2 (if any) T R'
+  const bool two_r = SW && g.B3.base != nullptr;
+  const int nterm = SW ? (two_r ? 3 : 2) : (akron || bkron) ? 1 : 4;
   const int nk = (nv + MX_KT - 1) / MX_KT;
   const int cpb = nterm * nk;  // chunks per tile
   const int nchunk = cpb * bpw;
@@ -188,7 +191,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   u32x4 va[6], vb[6];
   int a_swap = 0;
   unsigned a_mr = 0, a_mi = 0, b_mi = 0;
-  int a_t2 = 0, a_kc = 0;  // sandwich: the chunk belongs to T R, its K chunk
+  int a_t2 = 0, a_kc = 0;  // sandwich: term of the chunk (>= 1: T R or T R'), its K chunk
   int c_asel = 0;          // A buffer of the chunk in LDS
 
   auto tile_of = [&](int tl, int& a, int& c, int& tm, int& tn) {
@@ -220,13 +223,14 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   auto set_term = [&]() {
     int a, c;
     tile_of(cur_tl, a, c, cur_tm, cur_tn);
-    if (SW && cur_term == 1) {
+    if (SW && cur_term >= 1) {
       // T R: A comes from the accumulators, B is the right elemental (no phase)
+      const LcView& R = cur_term == 1 ? g.B2 : g.B3;
       t_swap = 0;
       t_mr = 0u;
       t_mi = 0u;
-      t_bmi = (g.B2.flags & LC_CONJ) ? SIGN2 : 0u;
-      pB = reinterpret_cast<const unsigned short*>(g.B2.base) + 6 * voff(g.B2.s, co) +
+      t_bmi = (R.flags & LC_CONJ) ? SIGN2 : 0u;
+      pB = reinterpret_cast<const unsigned short*>(R.base) + 6 * voff(R.s, co) +
            (long long)(cur_tn * MX_TM + s_line) * nv + s_kg * 8;
       return;
     }
@@ -263,7 +267,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     a_mr = t_mr;
     a_mi = t_mi;
     b_mi = t_bmi;
-    a_t2 = SW && cur_term == 1;
+    a_t2 = SW ? cur_term : 0;
     a_kc = cur_kc;
     if (SW) {
 #pragma unroll
@@ -300,7 +304,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     c_asel = 0;
     if (SW && a_t2) {
       c_asel = a_kc;
-      if (a_kc == 0) {
+      if (a_t2 == 1 && a_kc == 0) {
         // T is complete: this wave's 32 x 32 part goes to A buffer wn (T's columns are the K of
         // T R, 32 per chunk), split, in the layout of a staged A chunk
 #pragma unroll
@@ -344,13 +348,15 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   double tsr[LC_MAX_FUSED_TRACES], tsi[LC_MAX_FUSED_TRACES];
 #pragma unroll
   for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) { tsr[t] = 0; tsi[t] = 0; }
+  double txr = 0, txi = 0;  // trace of the second sandwich product
   const T alr = T(g.alpha[0]), ali = T(g.alpha[1]);
   const bool vec_c = FULL || (nv & 3) == 0;
 
   // Per-tile epilogue: stores, and the fused trace pieces sum_ij C_ij Y_ji in FP64. Every load
   // of the trace operands (all traces, all 16 values of this lane) is issued before the first
   // one is used, so the tile pays one memory round trip instead of one per value.
-  auto epilogue = [&](int tl) {
+  // second: the tile of T R' (sandwich term 2): no stores, one trace (Yx)
+  auto epilogue = [&](int tl, bool second) {
     // keep the scheduler from hoisting the epilogue's loads into the matrix-core section
     __builtin_amdgcn_sched_barrier(0);
     int a, c, tm, tn;
@@ -367,8 +373,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const T xr = bre[m][n][q] + sre[m][n][q], xi = bim[m][n][q] + sim[m][n][q];
-          zr_[m][n][q] = alr * xr - ali * xi;
-          zi_[m][n][q] = alr * xi + ali * xr;
+          zr_[m][n][q] = second ? xr : alr * xr - ali * xi;
+          zi_[m][n][q] = second ? xi : alr * xi + ali * xr;
         }
     zero_acc();
     // trace operand of trace t: this lane's 16 values of f(Y), element (col, row), loaded with no
@@ -382,7 +388,11 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       y.pr = 0; y.pi = 0; y.cj = 1.f;
       BlockRef<T> Y;
       Y.zero = true;
-      if (t < g.ntr) Y = view_block<T>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
+      if (second) {
+        if (t == 0) Y = view_block<T>(g.Yx, voff(g.Yx.s, co), c, a, nv2);
+      } else if (t < g.ntr) {
+        Y = view_block<T>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
+      }
       if (!Y.zero) { y.pr = Y.sr; y.pi = Y.si; y.cj = (Y.flags & LC_CONJ) ? -1.f : 1.f; }
       const bool tv = (Y.flags & LC_TV) != 0;
 #pragma unroll
@@ -425,6 +435,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         const float* zi = zi_[m][n];
         const bool rin = FULL || row < nv;
         const bool full4 = FULL || (rin && col + 3 < nv);
+        if (second) continue;
         if (g.C) {
           float* C = reinterpret_cast<float*>(g.C) + 2 * (cofs + (long long)row * nv + col);
           if (FULL || (full4 && vec_c && (cofs & 1) == 0)) {
@@ -489,8 +500,14 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
           si += fi;
         }
       // the Y block phase is uniform over the tile
-      tsr[t] += sr * ycur.pr - si * ycur.pi;
-      tsi[t] += sr * ycur.pi + si * ycur.pr;
+      const double dr = sr * ycur.pr - si * ycur.pi, di = sr * ycur.pi + si * ycur.pr;
+      if (!second) {
+        tsr[t] += dr;
+        tsi[t] += di;
+      } else if (t == 0) {
+        txr += dr;
+        txi += di;
+      }
       if (t + 1 < LC_MAX_FUSED_TRACES) ycur = ynext;
     }
   };
@@ -549,7 +566,17 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         bim[m][n] = mfma_bf16(bf[n][0][0], af[0][1], bim[m][n]);
       }
     }
-    if ((ch + 1) % cpb == 0) epilogue(ch / cpb);
+    {
+      // end of a product: of the tile (plain), of T R (sandwich term 1) or of T R' (term 2)
+      const int tl = ch / cpb, kin = ch - tl * cpb;
+      if (!SW) {
+        if (kin == cpb - 1) epilogue(tl, false);
+      } else if (kin == 2 * nk - 1) {
+        epilogue(tl, false);
+      } else if (kin == 3 * nk - 1) {
+        epilogue(tl, true);
+      }
+    }
     __syncthreads();
     if (more) {
       store();
@@ -565,6 +592,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     if (tid == 0) {
       const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
       double* out = g.part[t] + 2 * (item * wg_per_item + r0 / bpw);
+      typedef double d2v __attribute__((ext_vector_type(2)));
+      gstore(out, d2v{ar * sr - ai * si, ar * si + ai * sr});
+    }
+  }
+  if (two_r) {
+    double sr = txr, si = txi;
+    block_sum2(sr, si, red);
+    if (tid == 0) {
+      const double ar = g.trx_alpha[0], ai = g.trx_alpha[1];
+      double* out = g.partx + 2 * (item * wg_per_item + r0 / bpw);
       typedef double d2v __attribute__((ext_vector_type(2)));
       gstore(out, d2v{ar * sr - ai * si, ar * si + ai * sr});
     }

@@ -49,6 +49,11 @@ struct LcView {
 struct LcGemmGroup {
   LcView A, B;
   LcView B2;                // sandwich groups (mixed-precision kernel): C = alpha * A B B2
+  // optional second right factor of a sandwich (B3.base != nullptr): the product A B B3 is not
+  // stored, only traced, trx_alpha * Tr[(A B B3) Yx], into partx
+  LcView B3, Yx;
+  double trx_alpha[2];
+  double* partx;
   void* C;
   void* Cs;
   int cs_trans;

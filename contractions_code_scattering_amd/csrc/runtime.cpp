@@ -59,18 +59,21 @@ void Program::gemm(const LcView& A, const LcView& B, void* C, long long cs0, lon
 
 void Program::sandwich(const LcView& L, const LcView& M, const LcView& R, void* C, long long cs0,
                        long long cs1, long long cs2, int n0, int n1, int n2, double ar, double ai,
-                       const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
+                       const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans, const LcView* R2,
+                       const FusedTrace* tr2) {
   if (!sandwich_supported()) fail(LC_ERR_ARG, "fused sandwich needs the mixed-precision mode and Nv = 64");
-  if (L.kind != LC_KRON || M.kind != LC_DENSE || R.kind != LC_KRON)
+  if (L.kind != LC_KRON || M.kind != LC_DENSE || R.kind != LC_KRON || (R2 && R2->kind != LC_KRON))
     fail(LC_ERR_ARG, "sandwich operands must be elemental, D-space, elemental");
-  batch(L, M, &R, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans);
+  if ((R2 == nullptr) != (tr2 == nullptr)) fail(LC_ERR_ARG, "second sandwich factor needs its trace");
+  batch(L, M, &R, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans, R2, tr2);
 }
 
 void Program::batch(const LcView& A, const LcView& B, const LcView* B2, void* C, long long cs0, long long cs1,
                     long long cs2, int n0, int n1, int n2, double ar, double ai,
-                    const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans) {
+                    const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans, const LcView* B3,
+                    const FusedTrace* trx) {
   if (!last_axis_fastest_ || n2 <= 1) {
-    gemm_items(A, B, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans, B2);
+    gemm_items(A, B, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans, B2, B3, trx);
     return;
   }
   // the same batch with its axes relabelled (2, 0, 1): workgroups then walk axis 2 fastest
@@ -85,14 +88,23 @@ void Program::batch(const LcView& A, const LcView& B, const LcView* B2, void* C,
     const long long o0 = f.os[0], o1 = f.os[1], o2 = f.os[2];
     f.os[0] = o2; f.os[1] = o0; f.os[2] = o1;
   }
-  LcView r2;
+  LcView r2, r3;
   if (B2) r2 = rot(*B2);
-  gemm_items(rot(A), rot(B), C, cs2, cs0, cs1, n2, n0, n1, ar, ai, tr, Cs, cs_trans, B2 ? &r2 : nullptr);
+  if (B3) r3 = rot(*B3);
+  FusedTrace tx;
+  if (trx) {
+    tx = *trx;
+    tx.Y = rot(tx.Y);
+    tx.os[0] = trx->os[2]; tx.os[1] = trx->os[0]; tx.os[2] = trx->os[1];
+  }
+  gemm_items(rot(A), rot(B), C, cs2, cs0, cs1, n2, n0, n1, ar, ai, tr, Cs, cs_trans, B2 ? &r2 : nullptr,
+             B3 ? &r3 : nullptr, trx ? &tx : nullptr);
 }
 
 void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,
                          long long cs2, int n0, int n1, int n2, double ar, double ai,
-                         const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans, const LcView* B2) {
+                         const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans, const LcView* B2,
+                         const LcView* B3, const FusedTrace* trx) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
   if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
@@ -102,8 +114,9 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
   if (mode_ == MX) {
     if ((A.flags & LC_TV) || !(B.flags & LC_TV))
       fail(LC_ERR_ARG, "mixed-precision operands must be read along contiguous K (A row-major, B transposed)");
-    if (B2 && (!(B2->flags & LC_TV) || B2->kind != LC_KRON))
-      fail(LC_ERR_ARG, "sandwich right factor must be an elemental read along contiguous K");
+    for (const LcView* r : {B2, B3})
+      if (r && (!(r->flags & LC_TV) || r->kind != LC_KRON))
+        fail(LC_ERR_ARG, "sandwich right factor must be an elemental read along contiguous K");
     for (const LcView* v : {&A, &B})
       for (int a = 0; a < 4; a++)
         for (const double* ph : {v->ph_r[a], v->ph_c[a]})
@@ -112,15 +125,18 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
     // complete 64 x 64 tiles and 16-byte alignment of every operand and output (split storage:
     // 12 B per complex element, FP32 output 8 B): full-tile kernel
     bool full = nv_ % 64 == 0;
-    for (const LcView* v : {&A, &B, B2 ? B2 : &B}) {
+    for (const LcView* v : {&A, &B, B2 ? B2 : &B, B3 ? B3 : &B}) {
       full = full && (reinterpret_cast<uintptr_t>(v->base) & 15) == 0;
       for (int d = 0; d < 3; d++) full = full && v->s[d] % 4 == 0;
     }
     full = full && (reinterpret_cast<uintptr_t>(C) & 15) == 0 && (reinterpret_cast<uintptr_t>(Cs) & 15) == 0;
     full = full && cs0 % 4 == 0 && cs1 % 4 == 0 && cs2 % 4 == 0;
-    for (const FusedTrace& f : traces) {  // FP32 trace operands, read four complex values at a time
-      full = full && (reinterpret_cast<uintptr_t>(f.Y.base) & 15) == 0;
-      for (int d = 0; d < 3; d++) full = full && f.Y.s[d] % 2 == 0;
+    std::vector<const FusedTrace*> all;
+    for (const FusedTrace& f : traces) all.push_back(&f);
+    if (trx) all.push_back(trx);
+    for (const FusedTrace* f : all) {  // FP32 trace operands, read four complex values at a time
+      full = full && (reinterpret_cast<uintptr_t>(f->Y.base) & 15) == 0;
+      for (int d = 0; d < 3; d++) full = full && f->Y.s[d] % 2 == 0;
     }
     variant = full ? 1 : 0;
     if (B2) {
@@ -134,6 +150,13 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
   g.A = A;
   g.B = B;
   if (B2) g.B2 = *B2;
+  if (B3) {
+    if (!B2) fail(LC_ERR_ARG, "second right factor outside a sandwich");
+    g.B3 = *B3;
+    g.Yx = trx->Y;
+    g.trx_alpha[0] = trx->ar;
+    g.trx_alpha[1] = trx->ai;
+  }
   g.C = C;
   g.Cs = Cs;
   g.cs_trans = cs_trans ? 1 : 0;
@@ -165,26 +188,33 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
     g.Y[t] = f.Y;
     g.tr_alpha[t][0] = f.ar;
     g.tr_alpha[t][1] = f.ai;
-    g.part[t] = nullptr;
-    const size_t bytes = (size_t)items * per_item * 16;
-    patches_.push_back(Patch{gemms_.size(), t, partial_bytes_});
-    LcReduceJob j;
-    std::memset(&j, 0, sizeof j);
-    j.out = f.out;
-    for (int d = 0; d < 3; d++) j.os[d] = f.os[d];
-    j.n[0] = n0; j.n[1] = n1; j.n[2] = n2;
-    j.nk = (int)per_item;
-    j.accumulate = f.accumulate ? 1 : 0;
-    j.conj = f.conj ? 1 : 0;
-    j.out_double = f.out_double < 0 ? (dbl_ ? 1 : 0) : f.out_double;
-    l->jobs.push_back((int)jobs_.size());
-    jobs_.push_back(j);
-    job_part_off_.push_back(partial_bytes_);
-    partial_bytes_ += (bytes + 255) & ~size_t(255);
+    add_reduce(l, f, n0, n1, n2, per_item, t);
   }
+  if (trx) add_reduce(l, *trx, n0, n1, n2, per_item, LC_MAX_FUSED_TRACES);
   gemms_.push_back(g);
-  const int nterm = B2 ? 2 : (A.kind == LC_KRON || B.kind == LC_KRON) ? 1 : 4;
+  const int nterm = B2 ? (B3 ? 3 : 2) : (A.kind == LC_KRON || B.kind == LC_KRON) ? 1 : 4;
   gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
+}
+
+// Fixed-order reduction of the per-workgroup pieces of one fused trace of the group being added;
+// slot LC_MAX_FUSED_TRACES is the trace of a sandwich's second product.
+void Program::add_reduce(Level* l, const FusedTrace& f, int n0, int n1, int n2, long long per_item, int slot) {
+  const long long items = (long long)n0 * n1 * n2;
+  const size_t bytes = (size_t)items * per_item * 16;
+  patches_.push_back(Patch{gemms_.size(), slot, partial_bytes_});
+  LcReduceJob j;
+  std::memset(&j, 0, sizeof j);
+  j.out = f.out;
+  for (int d = 0; d < 3; d++) j.os[d] = f.os[d];
+  j.n[0] = n0; j.n[1] = n1; j.n[2] = n2;
+  j.nk = (int)per_item;
+  j.accumulate = f.accumulate ? 1 : 0;
+  j.conj = f.conj ? 1 : 0;
+  j.out_double = f.out_double < 0 ? (dbl_ ? 1 : 0) : f.out_double;
+  l->jobs.push_back((int)jobs_.size());
+  jobs_.push_back(j);
+  job_part_off_.push_back(partial_bytes_);
+  partial_bytes_ += (bytes + 255) & ~size_t(255);
 }
 
 void Program::trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
@@ -215,7 +245,11 @@ void Program::run(hipStream_t s) {
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   if (partial_bytes_) c.partials.ensure(partial_bytes_);
   char* pbase = static_cast<char*>(c.partials.p);
-  for (const Patch& p : patches_) gemms_[p.group].part[p.slot] = reinterpret_cast<double*>(pbase + p.part_off);
+  for (const Patch& p : patches_) {
+    double* q = reinterpret_cast<double*>(pbase + p.part_off);
+    if (p.slot < LC_MAX_FUSED_TRACES) gemms_[p.group].part[p.slot] = q;
+    else gemms_[p.group].partx = q;
+  }
   for (size_t j = 0; j < jobs_.size(); j++) jobs_[j].part = reinterpret_cast<const double*>(pbase + job_part_off_[j]);
   // reduce jobs of each level, contiguous, with their thread offsets
   std::vector<LcReduceJob> jobs;

@@ -143,10 +143,13 @@ class Program {
             const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false);
   // Sandwich C = alpha * (L (x) 1) M (R (x) 1) over the batch, MX mode with Nv = 64 only
   // (sandwich_supported): L an A-role elemental view, M a B-role D-space view, R a B-role
-  // elemental view. Each block is formed in one pass with L M_ac kept on the chip.
+  // elemental view. Each block is formed in one pass with L M_ac kept on the chip. With R2 (a
+  // second B-role elemental view) and tr2, the same pass also forms (L (x) 1) M (R2 (x) 1) from the
+  // same L M_ac and delivers only its trace tr2 (the output of tr2 is not scaled by alpha).
   void sandwich(const LcView& L, const LcView& M, const LcView& R, void* C, long long cs0, long long cs1,
                 long long cs2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
-                const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false);
+                const std::vector<FusedTrace>& traces = {}, void* Cs = nullptr, bool cs_trans = false,
+                const LcView* R2 = nullptr, const FusedTrace* tr2 = nullptr);
   bool sandwich_supported() const { return mode_ == MX && nv_ == 64; }
   // Workgroup order over a batch: by default axis 0 runs fastest. With last_axis_fastest, axis 2
   // does (e.g. the momenta of one (sink, source) pair next to each other, so that the shared
@@ -192,10 +195,13 @@ class Program {
   Level* open_level(bool is_trace, int variant = 0);
   void gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
                   int n0, int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
-                  void* Cs, bool cs_trans, const LcView* B2 = nullptr);
+                  void* Cs, bool cs_trans, const LcView* B2 = nullptr, const LcView* B3 = nullptr,
+                  const FusedTrace* trx = nullptr);
   void batch(const LcView& A, const LcView& B, const LcView* B2, void* C, long long cs0, long long cs1,
              long long cs2, int n0, int n1, int n2, double ar, double ai,
-             const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans);
+             const std::vector<FusedTrace>& traces, void* Cs, bool cs_trans, const LcView* B3 = nullptr,
+             const FusedTrace* trx = nullptr);
+  void add_reduce(Level* l, const FusedTrace& f, int n0, int n1, int n2, long long per_item, int slot);
   bool last_axis_fastest_ = false;
 };
 
