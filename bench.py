@@ -205,11 +205,12 @@ def main():
     s64 = stats(r64, PEAK_TFLOPS["f64"])
     smx = stats(rmx, peak_mx)
     errs = engine.rel_errors(rmx["avg"], r64["avg"])
+    errs_t = engine.errors_per_t(rmx["avg"], r64["avg"])
     max_rel = max(e["max_rel"] for e in errs.values())
     max_norm = max(e["max_normwise"] for e in errs.values())
     roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",
-                frac=round(smx["frac"], 4), traffic=traffic_from_profiles("lc_gemm_mx_kernel", nmom),
-                kernel="lc_gemm_mx_kernel<true> (BF16x3 on v_mfma_f32_16x16x32_bf16)",
+                frac=round(smx["frac"], 4), traffic=traffic_from_profiles("lc_gemm_mx_kernel (all variants)", nmom),
+                kernel="lc_gemm_mx_kernel, all variants (BF16x3 on v_mfma_f32_16x16x32_bf16)",
                 flops_per_launch=smx["flops_per_launch"], avg_launch_ms=round(smx["avg_launch_ms"], 4),
                 frac_of_peaks={k: round(smx["achieved"] / v, 4) for k, v in PEAK_TFLOPS.items()},
                 bf16_mfma_utilization=round(smx["achieved"] * 6.0 / PEAK_TFLOPS["bf16"], 4))
@@ -225,6 +226,7 @@ def main():
                 roofline=roof,
                 mixed_vs_fp64=round(smx["value"] / s64["value"], 3),
                 max_rel_err=max_rel, max_normwise_err=max_norm, errors_per_diagram=errs,
+                errors_per_t=errs_t,
                 plain_fp64=dict(value=round(s64["value"], 1), ms_per_step=round(s64["ms_per_step"], 3),
                                 roofline=dict(bound="mfma", achieved=round(s64["achieved"], 3),
                                               peak=PEAK_TFLOPS["f64"], unit="TFLOP/s",

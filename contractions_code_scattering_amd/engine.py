@@ -112,6 +112,19 @@ def selftest_mfma_bf16(a, b):
     return d[:256].reshape(16, 16), d[256:].reshape(64, 4)
 
 
+def errors_per_t(c_test, c_ref):
+    """Per source-time separation t: max over (momentum, diagram) of |dC(t)|/|C(t)| and of
+    |dC(t)|/max_t|C(t)|, and the fraction of all C(t) within 1e-6 relative."""
+    import numpy as np
+    d = np.abs(c_test - c_ref)          # [np][ncorr][nt]
+    a = np.abs(c_ref)
+    rel = d / np.where(a > 0, a, np.inf)
+    norm = d / a.max(axis=2, keepdims=True)
+    return dict(max_rel=[float('%.3g' % v) for v in rel.max(axis=(0, 1))],
+                max_normwise=[float('%.3g' % v) for v in norm.max(axis=(0, 1))],
+                frac_rel_le_1e6=float((rel <= 1e-6).mean()))
+
+
 def rel_errors(c_test, c_ref):
     """Per diagram: max over (momentum, t) of |dC|/|C| and of |dC|/max_t|C| (normwise)."""
     import numpy as np
