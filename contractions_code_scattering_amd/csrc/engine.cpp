@@ -140,7 +140,9 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   pg.barrier();
   // level 3
   pg.gemm(pair(Rr), tau_dag, Zh, P0, P1, P2, nt, ns, pc);
+  pg.set_second_axis(1);  // sources before sinks: Q(tf, p) shared by neighbouring workgroups
   pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_v, g5), g5), U, P0, P1, P2, nt, ns, pc);
+  pg.set_second_axis(0);
   pg.barrier();
   // level 4
   pg.gemm(tau_dag, pair(U), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
@@ -240,8 +242,10 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   pg.barrier();
   // level 3
   pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  pg.set_second_axis(1);  // sources before sinks: Q(tf, p) shared by neighbouring workgroups
   pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
           {}, Us, true);
+  pg.set_second_axis(0);
   pg.barrier();
   // level 4
   pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
@@ -348,10 +352,13 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   pg.sandwich(phi_t0_dag_A, D_t0_B, phi_t0_B, R, 0, b, ns * b, 1, ns, pc);
   pg.sandwich(phi_t0_A, D_t0_B, phi_t0_dag_B, Rp, 0, b, ns * b, 1, ns, pc);
   pg.barrier();
-  // level 2
+  // level 2; U walks the sources before the sinks, so that Q(tf, p) of the momenta in flight is
+  // shared by neighbouring workgroups
   pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  pg.set_second_axis(1);
   pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
           {}, Us, true);
+  pg.set_second_axis(0);
   pg.barrier();
   // level 3
   pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,

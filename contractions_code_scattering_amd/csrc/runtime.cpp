@@ -76,17 +76,22 @@ void Program::batch(const LcView& A, const LcView& B, const LcView* B2, void* C,
     gemm_items(A, B, C, cs0, cs1, cs2, n0, n1, n2, ar, ai, traces, Cs, cs_trans, B2, B3, trx);
     return;
   }
-  // the same batch with its axes relabelled (2, 0, 1): workgroups then walk axis 2 fastest
-  auto rot = [](LcView v) {
-    const long long s0 = v.s[0], s1 = v.s[1], s2 = v.s[2];
-    v.s[0] = s2; v.s[1] = s0; v.s[2] = s1;
+  // the same batch with its axes relabelled (2, 0, 1), or (2, 1, 0) with second_axis 1:
+  // workgroups then walk axis 2 fastest, then the second axis
+  const int o1 = second_axis_ == 1 ? 1 : 0, o2 = 1 - o1;
+  auto rot = [&](LcView v) {
+    const long long s[3] = {v.s[0], v.s[1], v.s[2]};
+    v.s[0] = s[2]; v.s[1] = s[o1]; v.s[2] = s[o2];
     return v;
+  };
+  auto rot3 = [&](const long long* x, long long* y) {
+    const long long t[3] = {x[0], x[1], x[2]};
+    y[0] = t[2]; y[1] = t[o1]; y[2] = t[o2];
   };
   std::vector<FusedTrace> tr = traces;
   for (FusedTrace& f : tr) {
     f.Y = rot(f.Y);
-    const long long o0 = f.os[0], o1 = f.os[1], o2 = f.os[2];
-    f.os[0] = o2; f.os[1] = o0; f.os[2] = o1;
+    rot3(f.os, f.os);
   }
   LcView r2, r3;
   if (B2) r2 = rot(*B2);
@@ -95,10 +100,12 @@ void Program::batch(const LcView& A, const LcView& B, const LcView* B2, void* C,
   if (trx) {
     tx = *trx;
     tx.Y = rot(tx.Y);
-    tx.os[0] = trx->os[2]; tx.os[1] = trx->os[0]; tx.os[2] = trx->os[1];
+    rot3(trx->os, tx.os);
   }
-  gemm_items(rot(A), rot(B), C, cs2, cs0, cs1, n2, n0, n1, ar, ai, tr, Cs, cs_trans, B2 ? &r2 : nullptr,
-             B3 ? &r3 : nullptr, trx ? &tx : nullptr);
+  const long long cs[3] = {cs0, cs1, cs2};
+  const int nn[3] = {n0, n1, n2};
+  gemm_items(rot(A), rot(B), C, cs[2], cs[o1], cs[o2], nn[2], nn[o1], nn[o2], ar, ai, tr, Cs, cs_trans,
+             B2 ? &r2 : nullptr, B3 ? &r3 : nullptr, trx ? &tx : nullptr);
 }
 
 void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1,

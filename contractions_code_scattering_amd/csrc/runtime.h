@@ -155,6 +155,9 @@ class Program {
   // does (e.g. the momenta of one (sink, source) pair next to each other, so that the shared
   // perambulator block is fetched from HBM once and from L2 for the other momenta).
   void set_last_axis_fastest(bool on) { last_axis_fastest_ = on; }
+  // With last_axis_fastest: which of axes 0 and 1 runs next (default 0). Axis 1 suits a product
+  // whose operand depends on axes 2 and 0 only, which neighbouring items then share.
+  void set_second_axis(int axis) { second_axis_ = axis; }
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -203,6 +206,7 @@ class Program {
              const FusedTrace* trx = nullptr);
   void add_reduce(Level* l, const FusedTrace& f, int n0, int n1, int n2, long long per_item, int slot);
   bool last_axis_fastest_ = false;
+  int second_axis_ = 0;
 };
 
 }  // namespace lc
