@@ -85,6 +85,46 @@ def traffic_from_profiles(kernel, nmom):
         return None
 
 
+def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
+    """rho + pi-pi at the largest size: Nv = 128 (D = 512), Lt = 64, all 64 sources, `nmom` momenta
+    (the first Hermitian), perambulator decay 0.25; both forms, one timed pass each after one
+    warm-up pass. At Nv = 128 the mixed form runs its four-level chain (no fused sandwiches)."""
+    import numpy as np
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import DeviceArray, synchronize, to_device
+    tau_h, phi_h = workload(nv, nt, nmom, seed, mass)
+    tau, phi = to_device(tau_h), to_device(phi_h)
+    del tau_h
+    log("d512 inputs resident: tau %.2f GB" % (tau.nbytes / 1e9))
+    ncorr = nmom * nt * nt * 8
+    res, avg = {}, {}
+    for mixed in (False, True):
+        out = (DeviceArray((nmom, nt, nt, 10), np.complex128), DeviceArray((nmom, nt, nt, 8), np.complex128),
+               DeviceArray((nmom, 8, nt), np.complex128))
+        engine.rho_pipi(tau, phi, out=out, mixed=mixed)
+        synchronize()
+        engine.profile(True)
+        t = time.perf_counter()
+        engine.rho_pipi(tau, phi, out=out, mixed=mixed)
+        synchronize()
+        dt = time.perf_counter() - t
+        p = engine.profile_read()
+        engine.profile(False)
+        avg[mixed] = out[2].to_numpy()
+        key = "mixed" if mixed else "fp64"
+        res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
+                        gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
+        log("d512 %s: %.1f ms" % (key, dt * 1e3))
+    errs = engine.rel_errors(avg[True], avg[False])
+    res["mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    res["max_normwise_err"] = max(e["max_normwise"] for e in errs.values())
+    res["max_rel_err"] = max(e["max_rel"] for e in errs.values())
+    res["unit"] = "correlators/s"
+    res["config"] = dict(workload="rho(3 pol)+pipi(5 diagrams)", nv=nv, D=4 * nv, lt=nt, sources=nt, momenta=nmom,
+                         mass=mass, seed=seed, correlators_per_pass=ncorr)
+    return res
+
+
 def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
     """The batched sigma / sigma-pi / rho-pi set on its own workload, both forms, one timed pass
     each after one warm-up pass: Nv = 64 (D = 256), Lt = 64, sources every 4th time slice (16),
@@ -136,13 +176,15 @@ def main():
                     help="profiling aid: run only one form (the JSON line needs both)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="profiling aid: one device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--d512", type=int, default=0,
+                    help="N > 0: also time rho + pi-pi at Nv=128 (D=512), Lt=64, all 64 sources, N momenta "
+                         "(reported under d512)")
     ap.add_argument("--sigma-set", type=int, default=0,
                     help="1: also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
                          "27 momentum sets (reported under sigma_set)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
@@ -237,6 +279,8 @@ def main():
                 effective_tflops=round(smx["effective_tflops"], 3))
     if args.sigma_set:
         line["sigma_set"] = sigma_set_measurement()
+    if args.d512 > 0:
+        line["d512"] = d512_measurement(args.d512)
     if rank == 0 and args.cpu_rows > 0:
         rec, ref = cpu_baseline(tau_h, phi_h, args.cpu_rows)
         line["cpu_baseline"] = rec

@@ -16,6 +16,13 @@ def rng(seed):
 
 
 def cn(g, shape, dtype=np.complex128):
+    if np.dtype(dtype) == np.complex128:
+        # the same values as below, built in place (a few GB smaller peak for large perambulators)
+        out = np.empty(shape, np.complex128)
+        out.real = g.standard_normal(shape)
+        out.imag = g.standard_normal(shape)
+        out *= np.sqrt(0.5)
+        return out
     re = g.standard_normal(shape)
     im = g.standard_normal(shape)
     return ((re + 1j * im) * np.sqrt(0.5)).astype(dtype)
