@@ -168,6 +168,85 @@ void run_chunk(const Shape& E, bool mixed, const Store& tau, const Store& el, in
   pg.run(ctx().stream);
 }
 
+// Scratch per momentum set of the sandwich form: the pair buffer Y (FP32) and the per-source R
+// (split).
+size_t sw_bytes_per_momentum(const Shape& E) {
+  const long long npair = (long long)E.nt * E.nsrc;
+  return (size_t)E.blk * ((size_t)npair * 8 + (size_t)E.nsrc * 12) + 4 * 256;
+}
+
+// The mixed form with fused sandwiches (Nv = 64). With X_l (x) 1 commuting with g5,
+//   sigma(l, r) = Tr[tau^dag X_l^dag g5 tau X_r g5] = Tr[S_lr tau^dag],  S_lr = X_l(tf)^dag (g5 tau g5) X_r(t0)
+// and one pass per l forms X_l^dag (g5 tau g5) once and both right factors X_0, X_1 (traces only,
+// against tau^dag in FP32); R = P2 D(t0) P1 is one sandwich per source. B_l is never formed.
+//   1  sandwiches S_l0, S_l1 (l = 0, 1) -> sigma;  R (per source, split)
+//   2  Y = (g5 R) tau^dag
+//   3  Z = X_ip(tf) tau -> triangle(ip) = Tr[Z (Y g5)];  Z' = P12_ip(tf) tau -> rhopi = Tr[Z' (Y G)]
+void run_chunk_sw(const Shape& E, const Store& tau, const char* tau32, const Store& el, int p0, int pc,
+                  char* scratch, void* raw) {
+  const int nt = E.nt, ns = E.nsrc;
+  const long long m = E.m, b = E.blk;
+  const long long npair = (long long)nt * ns;
+  char* cur = scratch;
+  auto take = [&](long long elems, size_t es) {
+    char* p = cur;
+    cur += ((size_t)elems * es + 255) & ~size_t(255);
+    return p;
+  };
+  char* Y = take(pc * npair * b, 8);
+  char* R = take((long long)pc * ns * b, 12);
+
+  Builder bl(true, E.nv);
+  Program& pg = bl.pg();
+  pg.set_last_axis_fastest(true);  // momentum sets of one (tf, t0) pair next to each other
+  const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
+  const long long tau0 = (long long)E.src0 * b;
+  const long long eP = (long long)nt * NT_EL * m;
+  auto el_tf = [&](int j) { return (long long)p0 * eP + (long long)j * m; };
+  auto el_t0 = [&](int j) { return (long long)p0 * eP + (long long)E.src0 * NT_EL * m + (long long)j * m; };
+  const long long eTF = NT_EL * m, eK = (long long)E.step * NT_EL * m;
+  auto pair = [&](char* p, int flags = 0) { return make_view(p, P0, P1, P2, LC_DENSE, flags); };
+  char* rw = static_cast<char*>(raw) + (size_t)p0 * ns * nt * N_RAW * 16;
+  auto slot = [&](int k, const LcView& Yv) {
+    FusedTrace f = fused(Yv, rw + (size_t)k * 16, N_RAW, (long long)nt * N_RAW, (long long)ns * nt * N_RAW);
+    f.out_double = 1;
+    return f;
+  };
+  const Spin g5 = gamma(5);
+  const LcView tau_dag_Y = make_view(tau32 + (size_t)tau0 * 8, nt * b, E.step * b, 0, LC_DENSE, LC_DAG);
+  const LcView g5tg5 = right(left(bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE), g5), g5);
+
+  // level 1
+  const LcView X1_t0 = bl.B(el, el_t0(EL_SIG + 1), 0, eK, eP, LC_KRON);
+  for (int l = 0; l < 2; l++) {
+    const FusedTrace s1 = slot(l * 2 + 1, tau_dag_Y);
+    pg.sandwich(bl.A(el, el_tf(EL_SIG + l), eTF, 0, eP, LC_KRON, LC_CONJ | LC_TV), g5tg5,
+                bl.B(el, el_t0(EL_SIG + 0), 0, eK, eP, LC_KRON), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+                {slot(l * 2 + 0, tau_dag_Y)}, nullptr, false, &X1_t0, &s1);
+  }
+  pg.sandwich(bl.A(el, el_t0(EL_P2), 0, eK, eP, LC_KRON),
+              bl.B(tau, (long long)E.src0 * (nt + 1) * b, 0, E.step * (nt + 1) * b, 0, LC_DENSE),
+              bl.B(el, el_t0(EL_P1), 0, eK, eP, LC_KRON), nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, R);
+  pg.barrier();
+  // level 2
+  pg.gemm(left(make_view(R, 0, b, ns * b, LC_DENSE, 0), g5), bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE, LC_DAG),
+          Y, P0, P1, P2, nt, ns, pc);
+  pg.barrier();
+  // level 3
+  for (int ip = 0; ip < 2; ip++)
+    pg.gemm(bl.A(el, el_tf(EL_TRI + ip), eTF, 0, eP, LC_KRON), bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE),
+            nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(4 + ip, right(pair(Y), g5))});
+  const int ips[3] = {0, 1, 2 + E.idir};
+  for (int j = 0; j < 3; j++) {
+    std::vector<FusedTrace> t4;
+    for (int iop = 0; iop < 4; iop++)
+      if (rhopi_ip(E.idir, iop) == ips[j]) t4.push_back(slot(6 + iop, right(pair(Y), g5 * rhoj(iop + 1, E.idir, false))));
+    pg.gemm(bl.A(el, el_tf(EL_P12 + ips[j]), eTF, 0, eP, LC_KRON), bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE),
+            nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, t4);
+  }
+  pg.run(ctx().stream);
+}
+
 // Per-time loops and hor_fish, FP64 on the FP64 inputs (a few products per time slice).
 void run_loops(const Shape& E, const void* tau, const void* el, int p0, int pc, char* scratch, void* loops) {
   const int nt = E.nt;
@@ -210,12 +289,15 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     Shape E{nt, nv, np, src0, src_step, nsrc, idir, (long long)nv * nv, 16LL * nv * nv};
     hipStream_t s = ctx().stream;
     // mixed form: tau and the elementals split twice + FP64 per-slice scales, ahead of the scratch
+    // the fused-sandwich form (Nv = 64) also keeps tau in FP32 for its closing traces
+    const bool sw = mixed && nv == 64;
     const size_t tau_split = mixed ? (((size_t)nt * nt * E.blk * 12 + 255) & ~size_t(255)) : 0;
+    const size_t tau_f32 = sw ? (((size_t)nt * nt * E.blk * 8 + 255) & ~size_t(255)) : 0;
     const size_t el_split = mixed ? (((size_t)np * nt * NT_EL * E.m * 12 + 255) & ~size_t(255)) : 0;
     const size_t scale_bytes = mixed ? (((size_t)nt * nt + (size_t)np * nt * NT_EL) * 8 + 255) & ~size_t(255) : 0;
-    const size_t head = 2 * tau_split + 2 * el_split + scale_bytes;
+    const size_t head = 2 * tau_split + tau_f32 + 2 * el_split + scale_bytes;
     const size_t loop_per_p = 2 * ((((size_t)nt * E.blk * 16) + 255) & ~size_t(255));
-    const size_t per_p = bytes_per_momentum(E, mixed);
+    const size_t per_p = sw ? sw_bytes_per_momentum(E) : bytes_per_momentum(E, mixed);
     const size_t need1 = per_p > loop_per_p ? per_p : loop_per_p;
     if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
     const long long avail = max_scratch_bytes - (long long)head;
@@ -226,15 +308,17 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     char* base = static_cast<char*>(ctx().scratch.p);
     char* scratch = base + head;
     double* scales = nullptr;
+    char* tau32 = nullptr;
     Store tau{static_cast<const char*>(d_tau), static_cast<const char*>(d_tau)};
     Store el{static_cast<const char*>(d_el), static_cast<const char*>(d_el)};
     if (mixed) {
       char* tr = base;
       char* tt = tr + tau_split;
-      char* er = tt + tau_split;
+      tau32 = sw ? tt + tau_split : nullptr;
+      char* er = tt + tau_split + tau_f32;
       char* et = er + el_split;
       scales = reinterpret_cast<double*>(et + el_split);
-      hip_check(lc_launch_split_rescale(d_tau, tr, tt, nullptr, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
+      hip_check(lc_launch_split_rescale(d_tau, tr, tt, tau32, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
       hip_check(lc_launch_split_rescale(d_el, er, et, nullptr, E.m, nv, (long long)np * nt * NT_EL, scales + (size_t)nt * nt, s),
                 "split elementals");
       tau = Store{tr, tt};
@@ -242,7 +326,8 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     }
     for (int p0 = 0; p0 < np; p0 += pc) {
       const int n = (np - p0) < pc ? (np - p0) : pc;
-      run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
+      if (sw) run_chunk_sw(E, tau, tau32, el, p0, n, scratch, d_raw);
+      else run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
     }
     if (mixed)
       hip_check(lc_launch_sigma_scale(d_raw, np, nsrc, nt, src0, src_step, idir, scales, scales + (size_t)nt * nt, s),

@@ -67,7 +67,7 @@ def test_sigma_set_vs_oracle(lc, nt, nv, np_, idir, src0, step):
         assert relerr(loops[..., d], ref_loops[..., d]) <= TOL, d
 
 
-@pytest.mark.parametrize("nt,nv,np_", [(3, 16, 2), (3, 64, 1), (3, 33, 1), (2, 128, 1)])
+@pytest.mark.parametrize("nt,nv,np_", [(3, 16, 2), (3, 64, 1), (4, 64, 3), (3, 33, 1), (2, 128, 1)])
 def test_sigma_set_mixed_vs_fp64(lc, nt, nv, np_):
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
