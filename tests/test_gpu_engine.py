@@ -96,14 +96,16 @@ def test_mfma_bf16_layout(lc):
     np.testing.assert_allclose(d, ref, rtol=1e-6, atol=1e-5)
 
 
-@pytest.mark.parametrize("nt,nv,np_", [(4, 16, 2), (3, 64, 1), (4, 64, 3), (4, 33, 1), (2, 128, 1)])
-def test_engine_mixed_vs_fp64(lc, nt, nv, np_):
+@pytest.mark.parametrize("nt,nv,np_,src0,step", [(4, 16, 2, 0, 1), (3, 64, 1, 0, 1), (4, 64, 3, 0, 1),
+                                                  (5, 64, 2, 1, 2), (4, 33, 1, 0, 1), (2, 128, 1, 0, 1)])
+def test_engine_mixed_vs_fp64(lc, nt, nv, np_, src0, step):
+    """Nv = 64 runs the fused-sandwich chain, other Nv the four-level chain."""
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
     tau, phi = _inputs(13, nt, nv, np_)
     t, f = to_device(tau), to_device(phi)
-    _, c64, a64 = engine.rho_pipi(t, f)
-    _, cmx, amx = engine.rho_pipi(t, f, mixed=True)
+    _, c64, a64 = engine.rho_pipi(t, f, src0=src0, src_step=step)
+    _, cmx, amx = engine.rho_pipi(t, f, mixed=True, src0=src0, src_step=step)
     c64, cmx = c64.to_numpy(), cmx.to_numpy()
     errs = engine.rel_errors(amx.to_numpy(), a64.to_numpy())
     print("mixed vs fp64:", errs)

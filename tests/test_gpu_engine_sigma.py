@@ -67,14 +67,16 @@ def test_sigma_set_vs_oracle(lc, nt, nv, np_, idir, src0, step):
         assert relerr(loops[..., d], ref_loops[..., d]) <= TOL, d
 
 
-@pytest.mark.parametrize("nt,nv,np_", [(3, 16, 2), (3, 64, 1), (4, 64, 3), (3, 33, 1), (2, 128, 1)])
-def test_sigma_set_mixed_vs_fp64(lc, nt, nv, np_):
+@pytest.mark.parametrize("nt,nv,np_,src0,step", [(3, 16, 2, 0, 1), (3, 64, 1, 0, 1), (4, 64, 3, 0, 1),
+                                                  (5, 64, 2, 1, 2), (3, 33, 1, 0, 1), (2, 128, 1, 0, 1)])
+def test_sigma_set_mixed_vs_fp64(lc, nt, nv, np_, src0, step):
+    """Nv = 64 runs the fused-sandwich chain, other Nv the four-level chain."""
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
     tau, el = _inputs(22, nt, nv, np_)
     t, e = to_device(tau), to_device(el)
-    r64, l64 = engine.sigma_set(t, e, idir=1)
-    rmx, lmx = engine.sigma_set(t, e, idir=1, mixed=True)
+    r64, l64 = engine.sigma_set(t, e, idir=1, src0=src0, src_step=step)
+    rmx, lmx = engine.sigma_set(t, e, idir=1, mixed=True, src0=src0, src_step=step)
     r64, rmx = r64.to_numpy(), rmx.to_numpy()
     errs = {engine.SIGMA_RAW_NAMES[d]: relerr(rmx[..., d], r64[..., d]) for d in range(10)}
     print("sigma set, mixed vs fp64 (normwise):", errs)
