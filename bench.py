@@ -125,6 +125,37 @@ def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
     return res
 
 
+def elementals_measurement(L=24, nv=64, nt=48, nmom=19, seed=6):
+    """Plain momentum elementals Phi(p, t) = V_t^dag e^{-ip.x} V_t on the GPU for an L^3 lattice with 3
+    colours (K = 3 L^3), Nv eigenvectors per time slice (random), the 19 momenta |p|^2 <= 2; one timed
+    call after a warm-up call."""
+    import numpy as np
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd import inputs as I
+    from contractions_code_scattering_amd.devmem import synchronize, to_device
+    g = I.rng(seed)
+    nsite = 3 * L ** 3
+    V = to_device(I.cn(g, (nt, nv, nsite)))
+    moms = [(0, 0, 0)] + [p for p in [(1, 0, 0), (0, 1, 0), (0, 0, 1), (-1, 0, 0), (0, -1, 0), (0, 0, -1),
+                                      (1, 1, 0), (1, -1, 0), (-1, 1, 0), (-1, -1, 0), (1, 0, 1), (1, 0, -1),
+                                      (-1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, 1, -1), (0, -1, 1), (0, -1, -1)]]
+    moms = moms[:nmom]
+    x = np.indices((L, L, L)).reshape(3, -1)
+    ph = np.stack([np.repeat(np.exp(-2j * np.pi * (np.array(p) @ x) / L), 3) for p in moms])
+    phd = to_device(np.ascontiguousarray(ph))
+    out = engine.build_elementals(V, phd)
+    synchronize()
+    t = time.perf_counter()
+    engine.build_elementals(V, phd, out=out)
+    synchronize()
+    dt = time.perf_counter() - t
+    flops = 8.0 * nmom * nt * nv * nv * nsite
+    log("elementals: %.2f ms" % (dt * 1e3))
+    return dict(ms=round(dt * 1e3, 3), tflops=round(flops / dt / 1e12, 2),
+                frac_f64_peak=round(flops / dt / 1e12 / PEAK_TFLOPS["f64"], 3),
+                config=dict(L=L, colours=3, nsite=nsite, nv=nv, lt=nt, momenta=nmom, dtype="f64"))
+
+
 def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
     """The batched sigma / sigma-pi / rho-pi set on its own workload, both forms, one timed pass
     each after one warm-up pass: Nv = 64 (D = 256), Lt = 64, sources every 4th time slice (16),
@@ -179,6 +210,9 @@ def main():
     ap.add_argument("--d512", type=int, default=0,
                     help="N > 0: also time rho + pi-pi at Nv=128 (D=512), Lt=64, all 64 sources, N momenta "
                          "(reported under d512)")
+    ap.add_argument("--elementals", type=int, default=0,
+                    help="1: also time the GPU construction of the momentum elementals at L=24, Nv=64, Lt=48, "
+                         "19 momenta (reported under elementals)")
     ap.add_argument("--sigma-set", type=int, default=0,
                     help="1: also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
                          "27 momentum sets (reported under sigma_set)")
@@ -281,6 +315,8 @@ def main():
         line["sigma_set"] = sigma_set_measurement()
     if args.d512 > 0:
         line["d512"] = d512_measurement(args.d512)
+    if args.elementals:
+        line["elementals"] = elementals_measurement()
     if rank == 0 and args.cpu_rows > 0:
         rec, ref = cpu_baseline(tau_h, phi_h, args.cpu_rows)
         line["cpu_baseline"] = rec

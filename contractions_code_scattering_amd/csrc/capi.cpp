@@ -335,6 +335,10 @@ void triangle_row(const Shape& S, const void* row, const void* diag, const void*
 }  // namespace
 
 // =========================================================================== C-ABI entry points
+extern "C" hipError_t lc_launch_phi(const void* V, const void* ph, int nt, int nv, long long nsite, int np, int nks,
+                                    void* part, void* phi, hipStream_t s);
+extern "C" long long lc_phi_partial_bytes(int nt, int nv, int np, int nks);
+
 extern "C" {
 
 const char* lc_last_error(void) { return last_error().c_str(); }
@@ -515,6 +519,26 @@ int lc_synchronize(void) {
   return guarded([&] {
     require_init();
     hip_check(hipStreamSynchronize(ctx().stream), "sync");
+  });
+}
+
+int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv, long long nsite, int np,
+                        void* d_phi) {
+  return guarded([&] {
+    require_init();
+    if (nt < 1 || nv < 1 || nv > 1024 || np < 1 || nsite < 1)
+      fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d np=%d nsite=%lld", nt, nv, np, nsite);
+    check_ptr(d_evec, "d_evec"); check_ptr(d_phase, "d_phase"); check_ptr(d_phi, "d_phi");
+    // enough K slices to give every CU several workgroups; at least 256 entries per slice
+    const long long tiles = (nv + 63) / 64;
+    const long long tiles_total = (long long)np * nt * tiles * tiles;
+    long long nks = (2048 + tiles_total - 1) / tiles_total;
+    if (nks > nsite / 256) nks = nsite / 256;
+    if (nks < 1) nks = 1;
+    ctx().partials.ensure((size_t)lc_phi_partial_bytes(nt, nv, np, (int)nks));
+    hipStream_t s = ctx().stream;
+    hip_check(lc_launch_phi(d_evec, d_phase, nt, nv, nsite, np, (int)nks, ctx().partials.p, d_phi, s), "phi launch");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   });
 }
 

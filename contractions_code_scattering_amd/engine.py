@@ -19,6 +19,7 @@ register("lc_engine_rho_pipi_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP
 register("lc_engine_sigma_set", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
 register("lc_engine_sigma_set_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
 register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
+register("lc_build_elementals", [_VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_profile", [_I])
 register("lc_profile_read", [_VP])
 
@@ -110,6 +111,27 @@ def selftest_mfma_bf16(a, b):
     d = np.zeros(512, np.float32)
     check(lib().lc_selftest_mfma_bf16(a.ctypes.data, b.ctypes.data, d.ctypes.data))
     return d[:256].reshape(16, 16), d[256:].reshape(64, 4)
+
+
+def build_elementals(evec, phase, out=None):
+    """Plain momentum elementals on the GPU (lc_build_elementals): evec [nt][nv][nsite] and phase
+    [np][nsite] DeviceArrays (complex128) -> DeviceArray [np][nt][nv][nv] with
+    Phi[p][t][r][c] = sum_k conj(evec[t][r][k]) phase[p][k] evec[t][c][k]
+    (the drivers' host loop, tests/compute_rho.cpp:116-129)."""
+    import numpy as np
+    from .devmem import DeviceArray
+    if not (isinstance(evec, DeviceArray) and isinstance(phase, DeviceArray)):
+        raise TypeError("operands must be DeviceArrays")
+    if evec.dtype != np.complex128 or phase.dtype != np.complex128:
+        raise TypeError("operands are complex128")
+    nt, nv, nsite = evec.shape
+    np_ = phase.shape[0]
+    if tuple(phase.shape) != (np_, nsite):
+        raise ValueError("phase must be [np][nsite] with the eigenvectors' nsite")
+    if out is None:
+        out = DeviceArray((np_, nt, nv, nv), np.complex128)
+    check(lib().lc_build_elementals(evec.ptr, phase.ptr, nt, nv, nsite, np_, out.ptr))
+    return out
 
 
 def errors_per_t(c_test, c_ref):

@@ -130,6 +130,15 @@ int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int n
  * short read. */
 int lc_read_prop_device(const char* path, int nt, int nv, int prec, void* d_tau);
 
+/* Plain momentum elementals from Laplacian eigenvectors, on the GPU: replaces the drivers' host loop
+ * matp[t](r,c) = cscalar_product(evec[r], phase * evec[c]) (tests/compute_rho.cpp:116-129):
+ *   d_phi[p][t][r][c] = sum_k conj(d_evec[t][r][k]) * d_phase[p][k] * d_evec[t][c][k]
+ * d_evec [nt][nv][nsite] complex double (k over site and colour, in the order the phases follow),
+ * d_phase [np][nsite] complex double (exp(-2 pi i p.x / L) at the site of entry k),
+ * d_phi [np][nt][nv][nv] complex double: the type-0 layout of the contraction inputs. FP64. */
+int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv, long long nsite, int np,
+                        void* d_phi);
+
 /* Diagnostic: one v_mfma_f32_16x16x32_bf16 on A(16x32), B(32x16) given as FP32 values exact in
  * BF16; h_d as for lc_selftest_mfma (D row-major, then raw accumulators). */
 int lc_selftest_mfma_bf16(const float* h_a, const float* h_b, float* h_d);
