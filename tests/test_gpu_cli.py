@@ -1,0 +1,92 @@
+"""The command-line driver end to end (perambulator file, momentum list, eigenvectors or elemental
+files -> text and FP64 side channel), against the same flows called directly through the C-ABI.
+
+* pi-pi from eigenvectors: the CLI forms P1..P4 on the GPU. The direct call gets them from numpy,
+  with the conventions of tests/compute_pipi_I0.cpp:133-146 (P1, P2 with exp(-i p.x), P3, P4 with
+  exp(+i p.x), cscalar_product conjugating the left eigenvector). FP64 values agree to 1e-12.
+* rho from an elemental file: two momentum combinations, bit-identical to two direct calls.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from contractions_code_scattering_amd import _capi, check, cli
+from contractions_code_scattering_amd import inputs as I
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+NT, NV, NX = 4, 8, 4
+
+
+def _write_prop(path, tau):
+    nt, nv = tau.shape[0], tau.shape[-1]
+    body = np.transpose(tau, (2, 4, 0, 3, 5, 1))  # [s1][v1][t1][s2][v2][t2]
+    with open(path, "wb") as f:
+        f.write(b"synthetic".ljust(100, b"\0"))
+        f.write(np.array([nt, nv], "<u4").tobytes())
+        f.write(np.ascontiguousarray(body).astype("<c16").tobytes())
+
+
+def _direct(tmp_path, name, fn):
+    """Run fn(L, prop_ptr, outfile) between init and destroy; returns (text, FP64 values)."""
+    L = _capi.lib()
+    out = str(tmp_path / name)
+    check(L.lc_contract_init(NV, NT, 0))
+    check(L.lc_set_binary_output(1))
+    prop = ctypes.c_void_p()
+    check(L.lc_read_prop(str(tmp_path / "prop.bin").encode(), 1, ctypes.byref(prop)))
+    fn(L, prop, out.encode())
+    L.lc_free_host(prop)
+    check(L.lc_set_binary_output(0))
+    check(L.lc_contract_destroy())
+    return open(out).read(), np.fromfile(out + ".bin", dtype="<f8")
+
+
+def _p(x):
+    return ctypes.c_void_p(x.ctypes.data)
+
+
+def test_cli_pipi_from_eigenvectors(lc, tmp_path):
+    g = I.rng(41)
+    _write_prop(tmp_path / "prop.bin", I.perambulator(g, NT, NV, 0.4))
+    nsite = 3 * NX ** 3
+    ev = I.cn(g, (NT, NV, nsite))
+    ev.astype("<c16").tofile(tmp_path / "evec.bin")
+    moms = [[(1, 0, 0), (-1, 0, 0), (1, 0, 0), (-1, 0, 0)], [(0, 1, 1), (0, -1, -1), (0, 1, 1), (0, -1, -1)]]
+    with open(tmp_path / "mom.dat", "w") as f:
+        for m in moms:
+            f.write("\t".join(" ".join(str(c) for c in p) for p in m) + "\n")
+    out = tmp_path / "cli.dat"
+    assert cli.main(["pipi_I0", "--propname", str(tmp_path / "prop.bin"), "--eigename", str(tmp_path / "evec.bin"),
+                     "--momname", str(tmp_path / "mom.dat"), "--Nmom", "4", "--momComb", "2", "--Nx", str(NX),
+                     "--Ny", str(NX), "--Nz", str(NX), "--Nt", str(NT), "--Nvec", str(NV), "--outname", str(out),
+                     "--binary"]) == 0
+    text, vals = open(out).read(), np.fromfile(str(out) + ".bin", dtype="<f8")
+
+    def fn(L, prop, o):
+        for m in moms:
+            ph = cli.site_phases(m, [-1, -1, 1, 1], NX, NX, NX)
+            ps = [np.ascontiguousarray(np.einsum("trk,k,tck->trc", np.conj(ev), ph[i], ev)) for i in range(4)]
+            check(L.lc_calculate_pipi(NV, NT, prop, *[_p(x) for x in ps], 1, o, 0))
+    text_ref, vals_ref = _direct(tmp_path, "direct.dat", fn)
+    assert text.count("\n") == text_ref.count("\n")
+    assert vals.shape == vals_ref.shape
+    assert relerr(vals, vals_ref) < 1e-12
+
+
+def test_cli_rho_from_elemental_file(lc, tmp_path):
+    g = I.rng(42)
+    _write_prop(tmp_path / "prop.bin", I.perambulator(g, NT, NV, 0.4))
+    recs = [np.ascontiguousarray(I.cn(g, (5, NT, NV, NV))) for _ in range(2)]
+    np.concatenate([r.ravel() for r in recs]).astype("<c16").tofile(tmp_path / "elem.bin")
+    out = tmp_path / "cli.dat"
+    assert cli.main(["rho", "--propname", str(tmp_path / "prop.bin"), "--elemname", str(tmp_path / "elem.bin"),
+                     "--momComb", "2", "--Nt", str(NT), "--Nvec", str(NV), "--outname", str(out), "--binary"]) == 0
+
+    def fn(L, prop, o):
+        for r in recs:
+            check(L.lc_calculate_rho(NV, NT, prop, _p(r), 1, o))
+    text_ref, vals_ref = _direct(tmp_path, "direct.dat", fn)
+    assert open(out).read() == text_ref
+    assert np.array_equal(np.fromfile(str(out) + ".bin", dtype="<f8"), vals_ref)
