@@ -281,7 +281,7 @@ def main():
     s64 = stats(r64, PEAK_TFLOPS["f64"])
     smx = stats(rmx, peak_mx)
     errs = engine.rel_errors(rmx["avg"], r64["avg"])
-    errs_t = engine.errors_per_t(rmx["avg"], r64["avg"])
+    errs_t = engine.errors_per_t(rmx["avg"], r64["avg"], corr_ref=r64["corr"])
     max_rel = max(e["max_rel"] for e in errs.values())
     max_norm = max(e["max_normwise"] for e in errs.values())
     roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",

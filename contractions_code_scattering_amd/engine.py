@@ -134,17 +134,32 @@ def build_elementals(evec, phase, out=None):
     return out
 
 
-def errors_per_t(c_test, c_ref):
+def errors_per_t(c_test, c_ref, corr_ref=None, src0=0, src_step=1):
     """Per source-time separation t: max over (momentum, diagram) of |dC(t)|/|C(t)| and of
-    |dC(t)|/max_t|C(t)|, and the fraction of all C(t) within 1e-6 relative."""
+    |dC(t)|/max_t|C(t)|, and the fraction of all C(t) within 1e-6 relative. With corr_ref (the
+    reference per-source correlators [np][nsrc][nt][ncorr]) also |dC(t)| / S(t), where S(t) is the
+    source average of |c(t0 + t, t0)|: the scale of the summed terms, which a C(t) that happens to
+    cancel between sources does not have."""
     import numpy as np
     d = np.abs(c_test - c_ref)          # [np][ncorr][nt]
     a = np.abs(c_ref)
     rel = d / np.where(a > 0, a, np.inf)
     norm = d / a.max(axis=2, keepdims=True)
-    return dict(max_rel=[float('%.3g' % v) for v in rel.max(axis=(0, 1))],
-                max_normwise=[float('%.3g' % v) for v in norm.max(axis=(0, 1))],
-                frac_rel_le_1e6=float((rel <= 1e-6).mean()))
+    out = dict(max_rel=[float('%.3g' % v) for v in rel.max(axis=(0, 1))],
+               max_normwise=[float('%.3g' % v) for v in norm.max(axis=(0, 1))],
+               frac_rel_le_1e6=float((rel <= 1e-6).mean()))
+    if corr_ref is not None:
+        np_, nsrc, nt, nc = corr_ref.shape
+        scale = np.zeros((np_, nc, nt))
+        for k in range(nsrc):
+            t0 = src0 + k * src_step
+            for dt in range(nt):
+                scale[:, :, dt] += np.abs(corr_ref[:, k, (t0 + dt) % nt, :])
+        scale /= nsrc
+        rs = d / np.where(scale > 0, scale, np.inf)
+        out["max_rel_source_scale"] = [float('%.3g' % v) for v in rs.max(axis=(0, 1))]
+        out["max_rel_source_scale_all"] = float(rs.max())
+    return out
 
 
 def rel_errors(c_test, c_ref):
