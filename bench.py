@@ -101,8 +101,9 @@ def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
     for mixed in (False, True):
         out = (DeviceArray((nmom, nt, nt, 10), np.complex128), DeviceArray((nmom, nt, nt, 8), np.complex128),
                DeviceArray((nmom, 8, nt), np.complex128))
-        engine.rho_pipi(tau, phi, out=out, mixed=mixed)
+        engine.rho_pipi(tau, phi, out=out, mixed=mixed)  # warm-up (the mixed form tunes its tile shape)
         synchronize()
+        tune = engine.autotune_report()
         engine.profile(True)
         t = time.perf_counter()
         engine.rho_pipi(tau, phi, out=out, mixed=mixed)
@@ -114,6 +115,9 @@ def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
         key = "mixed" if mixed else "fp64"
         res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
                         gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
+        if mixed:
+            res[key]["tiles_per_wg"] = tune["tiles_per_wg"]
+            res[key]["tile_candidates_ms"] = {k: round(v, 2) for k, v in tune["candidate_ms"].items()}
         log("d512 %s: %.1f ms" % (key, dt * 1e3))
     errs = engine.rel_errors(avg[True], avg[False])
     res["mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
@@ -172,8 +176,9 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
     res = {}
     outs = {}
     for mixed in (False, True):
-        engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)
+        engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)  # warm-up (mixed: tunes the tile shape)
         synchronize()
+        tune = engine.autotune_report()
         engine.profile(True)
         t = time.perf_counter()
         raw, loops = engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)
@@ -185,6 +190,9 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
         key = "mixed" if mixed else "fp64"
         res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
                         gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
+        if mixed:
+            res[key]["tiles_per_wg"] = tune["tiles_per_wg"]
+            res[key]["tile_candidates_ms"] = {k: round(v, 2) for k, v in tune["candidate_ms"].items()}
         log("sigma set %s: %.1f ms" % (key, dt * 1e3))
     d = np.abs(outs[True] - outs[False]).reshape(-1, 10)
     a = np.abs(outs[False]).reshape(-1, 10)
@@ -229,6 +237,7 @@ def main():
     nv, nt, nmom = 64, 48, args.nmom
     tau_h, phi_h = workload(nv, nt, nmom)
     rows.init(0)
+    engine.autotune(True)  # the mixed form picks its tile shape on its first (untimed) call per shape
     tau = to_device(tau_h)
     phi = to_device(phi_h)
     nsrc = nt
@@ -239,6 +248,13 @@ def main():
         out = (DeviceArray((nmom, nsrc, nt, 10), np.complex128),
                DeviceArray((nmom, nsrc, nt, 8), np.complex128),
                DeviceArray((nmom, 8, nt), np.complex128))
+        tune = None
+        if mixed:
+            # untimed: the autotuner times its tile-shape candidates on this call
+            engine.rho_pipi(tau, phi, out=out, mixed=mixed)
+            tune = engine.autotune_report()
+            log("mixed tile shape: %d tiles per workgroup (%s; candidates %s ms)"
+                % (tune["tiles_per_wg"], tune["how"], {k: round(v, 2) for k, v in tune["candidate_ms"].items()}))
         for i in range(args.warmup):
             engine.rho_pipi(tau, phi, out=out, mixed=mixed)
         synchronize()
@@ -251,7 +267,8 @@ def main():
         prof = engine.profile_read()
         engine.profile(False)
         log("%s: %d steps in %.3f s" % ("mixed" if mixed else "fp64", args.steps, elapsed))
-        return dict(elapsed=elapsed, prof=prof, corr=out[1].to_numpy(), avg=out[2].to_numpy())
+        return dict(elapsed=elapsed, prof=prof, corr=out[1].to_numpy(), avg=out[2].to_numpy(),
+                    tune=tune if mixed else None)
 
     if args.calib_copy_mib > 0:
         from contractions_code_scattering_amd.devmem import copy
@@ -310,7 +327,9 @@ def main():
                                               traffic=traffic_from_profiles("lc_gemm_kernel<double, 64>", nmom),
                                               kernel="lc_gemm_kernel<double,64>",
                                               avg_launch_ms=round(s64["avg_launch_ms"], 4))),
-                effective_tflops=round(smx["effective_tflops"], 3))
+                effective_tflops=round(smx["effective_tflops"], 3),
+                autotune=dict(tiles_per_wg=rmx["tune"]["tiles_per_wg"], how=rmx["tune"]["how"],
+                              candidate_ms={k: round(v, 2) for k, v in rmx["tune"]["candidate_ms"].items()}))
     if args.sigma_set:
         line["sigma_set"] = sigma_set_measurement()
     if args.d512 > 0:

@@ -376,10 +376,42 @@ int lc_finalize(void) {
     c.groups.release();
     c.scratch.release();
     c.out.release();
+    c.partials.release();
+    for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
+    c.ev_pool.clear();
+    c.tuned.clear();
+    c.last_tune = TuneRecord();
     (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.on = false;
     c.device = -1;
+  });
+}
+
+int lc_set_tile_params(int tiles_per_wg) {
+  return guarded([&] {
+    if (tiles_per_wg != 0 && tiles_per_wg != 1 && tiles_per_wg != 2 && tiles_per_wg != 4)
+      fail(LC_ERR_ARG, "lc_set_tile_params: tiles per workgroup must be 0, 1, 2 or 4 (got %d)", tiles_per_wg);
+    ctx().tile_bpw = tiles_per_wg;
+  });
+}
+
+int lc_autotune(int enable) {
+  return guarded([&] {
+    if (enable < -1 || enable > 1) fail(LC_ERR_ARG, "lc_autotune: enable must be -1, 0 or 1 (got %d)", enable);
+    Context& c = ctx();
+    if (enable < 0) c.tuned.clear();
+    c.autotune = enable == 1;
+  });
+}
+
+int lc_autotune_report(double* out5) {
+  return guarded([&] {
+    if (!out5) fail(LC_ERR_ARG, "lc_autotune_report: null output");
+    const TuneRecord& r = ctx().last_tune;
+    out5[0] = r.bpw;
+    out5[1] = r.how;
+    for (int i = 0; i < 3; i++) out5[2 + i] = r.ms[i];
   });
 }
 

@@ -403,34 +403,41 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     ctx().scratch.ensure(head + per_p * pc + 4096);
     char* base = static_cast<char*>(ctx().scratch.p);
     hipStream_t s = ctx().stream;
-    double* scales = nullptr;
     char* scratch = base + head;
-    if (mixed) {
-      char* tau_r = base;
-      char* tau_t = tau_r + tau_split;
-      char* tau32 = sw ? tau_t + tau_split : nullptr;
-      char* phi_r = tau_t + tau_split + tau_f32;
-      char* phi_t = phi_r + phi_split;
-      scales = reinterpret_cast<double*>(phi_t + phi_split);
-      hip_check(lc_launch_split_rescale(d_tau, tau_r, tau_t, tau32, E.blk, nv, (long long)nt * nt, scales, s),
-                "split tau");
-      hip_check(lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, E.m, nv, (long long)np * nt, scales + (size_t)nt * nt, s),
-                "split phi");
-      for (int p0 = 0; p0 < np; p0 += pc) {
-        const int n = (np - p0) < pc ? (np - p0) : pc;
-        if (sw) run_chunk_mx_sw(E, tau_r, tau_t, tau32, phi_r, phi_t, p0, n, scratch, d_raw);
-        else run_chunk_mx(E, tau_r, tau_t, phi_r, phi_t, p0, n, scratch, d_raw);
+    auto call = [&] {
+      double* scales = nullptr;
+      if (mixed) {
+        char* tau_r = base;
+        char* tau_t = tau_r + tau_split;
+        char* tau32 = sw ? tau_t + tau_split : nullptr;
+        char* phi_r = tau_t + tau_split + tau_f32;
+        char* phi_t = phi_r + phi_split;
+        scales = reinterpret_cast<double*>(phi_t + phi_split);
+        hip_check(lc_launch_split_rescale(d_tau, tau_r, tau_t, tau32, E.blk, nv, (long long)nt * nt, scales, s),
+                  "split tau");
+        hip_check(lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, E.m, nv, (long long)np * nt,
+                                          scales + (size_t)nt * nt, s),
+                  "split phi");
+        for (int p0 = 0; p0 < np; p0 += pc) {
+          const int n = (np - p0) < pc ? (np - p0) : pc;
+          if (sw) run_chunk_mx_sw(E, tau_r, tau_t, tau32, phi_r, phi_t, p0, n, scratch, d_raw);
+          else run_chunk_mx(E, tau_r, tau_t, phi_r, phi_t, p0, n, scratch, d_raw);
+        }
+      } else {
+        for (int p0 = 0; p0 < np; p0 += pc) {
+          const int n = (np - p0) < pc ? (np - p0) : pc;
+          run_chunk(E, Program::F64, d_tau, d_phi, p0, n, scratch, d_raw);
+        }
       }
-    } else {
-      for (int p0 = 0; p0 < np; p0 += pc) {
-        const int n = (np - p0) < pc ? (np - p0) : pc;
-        run_chunk(E, Program::F64, d_tau, d_phi, p0, n, scratch, d_raw);
-      }
-    }
-    hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales,
-                                        scales ? scales + (size_t)nt * nt : nullptr, s),
-              "engine epilogue");
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+      hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales,
+                                          scales ? scales + (size_t)nt * nt : nullptr, s),
+                "engine epilogue");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    };
+    char key[160];
+    std::snprintf(key, sizeof key, "rho_pipi nt=%d nv=%d np=%d src0=%d step=%d nsrc=%d chunk=%d", nt, nv, np, src0,
+                  src_step, nsrc, pc);
+    run_with_tiles(key, mixed, call);
     return LC_OK;
   } catch (const Error& e) {
     last_error() = e.msg;

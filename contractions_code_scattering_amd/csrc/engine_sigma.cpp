@@ -307,36 +307,43 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     ctx().scratch.ensure(head + need1 * pc + 4096);
     char* base = static_cast<char*>(ctx().scratch.p);
     char* scratch = base + head;
-    double* scales = nullptr;
-    char* tau32 = nullptr;
-    Store tau{static_cast<const char*>(d_tau), static_cast<const char*>(d_tau)};
-    Store el{static_cast<const char*>(d_el), static_cast<const char*>(d_el)};
-    if (mixed) {
-      char* tr = base;
-      char* tt = tr + tau_split;
-      tau32 = sw ? tt + tau_split : nullptr;
-      char* er = tt + tau_split + tau_f32;
-      char* et = er + el_split;
-      scales = reinterpret_cast<double*>(et + el_split);
-      hip_check(lc_launch_split_rescale(d_tau, tr, tt, tau32, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
-      hip_check(lc_launch_split_rescale(d_el, er, et, nullptr, E.m, nv, (long long)np * nt * NT_EL, scales + (size_t)nt * nt, s),
-                "split elementals");
-      tau = Store{tr, tt};
-      el = Store{er, et};
-    }
-    for (int p0 = 0; p0 < np; p0 += pc) {
-      const int n = (np - p0) < pc ? (np - p0) : pc;
-      if (sw) run_chunk_sw(E, tau, tau32, el, p0, n, scratch, d_raw);
-      else run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
-    }
-    if (mixed)
-      hip_check(lc_launch_sigma_scale(d_raw, np, nsrc, nt, src0, src_step, idir, scales, scales + (size_t)nt * nt, s),
-                "rescale traces");
-    for (int p0 = 0; p0 < np; p0 += pc) {
-      const int n = (np - p0) < pc ? (np - p0) : pc;
-      run_loops(E, d_tau, d_el, p0, n, scratch, d_loops);
-    }
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    auto call = [&] {
+      double* scales = nullptr;
+      char* tau32 = nullptr;
+      Store tau{static_cast<const char*>(d_tau), static_cast<const char*>(d_tau)};
+      Store el{static_cast<const char*>(d_el), static_cast<const char*>(d_el)};
+      if (mixed) {
+        char* tr = base;
+        char* tt = tr + tau_split;
+        tau32 = sw ? tt + tau_split : nullptr;
+        char* er = tt + tau_split + tau_f32;
+        char* et = er + el_split;
+        scales = reinterpret_cast<double*>(et + el_split);
+        hip_check(lc_launch_split_rescale(d_tau, tr, tt, tau32, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
+        hip_check(lc_launch_split_rescale(d_el, er, et, nullptr, E.m, nv, (long long)np * nt * NT_EL,
+                                          scales + (size_t)nt * nt, s),
+                  "split elementals");
+        tau = Store{tr, tt};
+        el = Store{er, et};
+      }
+      for (int p0 = 0; p0 < np; p0 += pc) {
+        const int n = (np - p0) < pc ? (np - p0) : pc;
+        if (sw) run_chunk_sw(E, tau, tau32, el, p0, n, scratch, d_raw);
+        else run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
+      }
+      if (mixed)
+        hip_check(lc_launch_sigma_scale(d_raw, np, nsrc, nt, src0, src_step, idir, scales, scales + (size_t)nt * nt, s),
+                  "rescale traces");
+      for (int p0 = 0; p0 < np; p0 += pc) {
+        const int n = (np - p0) < pc ? (np - p0) : pc;
+        run_loops(E, d_tau, d_el, p0, n, scratch, d_loops);
+      }
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    };
+    char key[160];
+    std::snprintf(key, sizeof key, "sigma_set nt=%d nv=%d np=%d idir=%d src0=%d step=%d nsrc=%d chunk=%d", nt, nv, np,
+                  idir, src0, src_step, nsrc, pc);
+    run_with_tiles(key, mixed, call);
     return LC_OK;
   } catch (const Error& e) {
     last_error() = e.msg;

@@ -16,6 +16,65 @@ std::string& last_error() {
   return e;
 }
 
+void run_with_tiles(const std::string& key, bool tunable, const std::function<void()>& call) {
+  Context& c = ctx();
+  struct Restore {
+    Context& c;
+    ~Restore() { c.active_bpw = 0; }
+  } restore{c};
+  TuneRecord rec;
+  rec.bpw = c.tile_bpw;
+  if (!tunable || !c.autotune || c.tile_bpw > 0) {
+    c.active_bpw = c.tile_bpw;
+    call();
+    c.last_tune = rec;
+    return;
+  }
+  auto hit = c.tuned.find(key);
+  if (hit != c.tuned.end()) {
+    rec.bpw = hit->second;
+    rec.how = 1;
+    c.active_bpw = rec.bpw;
+    call();
+    c.last_tune = rec;
+    return;
+  }
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  hip_check(hipEventCreate(&ev.e[0]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev.e[1]), "hipEventCreate");
+  int best = 0;
+  for (int i = 0; i < 3; i++) {
+    double t_min = 0;
+    for (int rep = 0; rep < 2; rep++) {
+      c.active_bpw = kTileCandidates[i];
+      hip_check(hipEventRecord(ev.e[0], c.stream), "hipEventRecord");
+      call();
+      hip_check(hipEventRecord(ev.e[1], c.stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(ev.e[1]), "hipEventSynchronize");
+      float ms = 0;
+      hip_check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "hipEventElapsedTime");
+      if (rep == 0 || ms < t_min) t_min = ms;
+    }
+    rec.ms[i] = t_min;
+    if (rec.ms[i] < rec.ms[best]) best = i;
+  }
+  // the outputs hold the last candidate's results; run the fastest again unless it was that one
+  if (best != 2) {
+    c.active_bpw = kTileCandidates[best];
+    call();
+  }
+  c.tuned[key] = kTileCandidates[best];
+  rec.bpw = kTileCandidates[best];
+  rec.how = 2;
+  c.last_tune = rec;
+}
+
 long long default_scratch_budget() {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
@@ -178,7 +237,10 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
   // workgroups (and trace pieces) per item: one per output tile, or one per `bpw` tiles in MX mode
   long long per_item = 16 * tt;
   if (mode_ == MX) {
-    g.bpw = tt == 1 ? 4 : 1;
+    // 64 x 64 output tiles per workgroup: the engine call's setting, else 4 when a spin block is
+    // one tile (Nv <= 64) and 1 otherwise
+    const int want = ctx().active_bpw;
+    g.bpw = want > 0 ? want : (tt == 1 ? 4 : 1);
     per_item = 16 * tt / g.bpw;
     // Items along batch axis 0 run next to each other. When only A changes along that axis, B is
     // shared by neighbouring items and stays in L2, while each A block is read by this item

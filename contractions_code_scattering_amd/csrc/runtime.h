@@ -5,6 +5,8 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -83,6 +85,14 @@ struct KernelTimes {
   void clear() { *this = KernelTimes(); }
 };
 
+// What the last engine call ran with: output tiles per workgroup of the mixed GEMM (0: the
+// built-in rule), how they were chosen, and the candidate timings when this call tuned them.
+struct TuneRecord {
+  int bpw = 0;
+  int how = 0;  // 0 fixed (rule or lc_set_tile_params), 1 cached choice, 2 tuned by this call
+  double ms[3] = {0, 0, 0};  // candidates 4, 2, 1 tiles per workgroup (best of two runs each)
+};
+
 struct Context {
   bool on = false;
   int device = -1;
@@ -91,7 +101,24 @@ struct Context {
   bool profile = false;
   KernelTimes times;
   std::vector<hipEvent_t> ev_pool;
+  // mixed-GEMM tiles per workgroup: the caller's override (0: rule), the value in force while an
+  // engine call plans its launches, and the autotuner's switch, choices and last record
+  int tile_bpw = 0;
+  int active_bpw = 0;
+  bool autotune = false;
+  std::map<std::string, int> tuned;
+  TuneRecord last_tune;
 };
+
+// Tiles-per-workgroup candidates of the autotuner, in timing order.
+constexpr int kTileCandidates[3] = {4, 2, 1};
+
+// Runs one engine call under the tile parameters for `key` (the engine and its shape). A
+// tunable (mixed-form) call with the autotuner on, no override and no choice cached for `key`
+// times every candidate on the call itself, twice each: an engine call overwrites all of its
+// outputs, so repeating it gives the same results. The fastest is cached for `key` and its
+// results are the ones left in the outputs.
+void run_with_tiles(const std::string& key, bool tunable, const std::function<void()>& call);
 
 Context& ctx();
 void require_init();

@@ -21,6 +21,9 @@ register("lc_engine_sigma_set_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL
 register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
 register("lc_build_elementals", [_VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_profile", [_I])
+register("lc_set_tile_params", [_I])
+register("lc_autotune", [_I])
+register("lc_autotune_report", [_VP])
 register("lc_profile_read", [_VP])
 
 
@@ -101,6 +104,28 @@ def profile_read():
     check(lib().lc_profile_read(buf.ctypes.data))
     return dict(gemm_ms=buf[0], gemm_launches=int(buf[1]), gemm_flops=buf[2],
                 trace_ms=buf[3], trace_launches=int(buf[4]), trace_flops=buf[5])
+
+
+def set_tile_params(tiles_per_wg=0):
+    """Output tiles (64 x 64) per workgroup of the mixed GEMM: 1, 2 or 4; 0 = built-in rule."""
+    check(lib().lc_set_tile_params(int(tiles_per_wg)))
+
+
+def autotune(enable=True, forget=False):
+    """Turn the in-project autotuner on or off (forget=True also drops its cached choices)."""
+    check(lib().lc_autotune(-1 if forget else (1 if enable else 0)))
+    if forget and enable:
+        check(lib().lc_autotune(1))
+
+
+def autotune_report():
+    """Tile shape of the last engine call: tiles per workgroup (0: rule), how it was chosen
+    ("fixed", "cached", "tuned") and the candidate timings in ms (4, 2, 1 tiles per workgroup)."""
+    import numpy as np
+    buf = np.zeros(5)
+    check(lib().lc_autotune_report(buf.ctypes.data))
+    return dict(tiles_per_wg=int(buf[0]), how=("fixed", "cached", "tuned")[int(buf[1])],
+                candidate_ms={4: buf[2], 2: buf[3], 1: buf[4]})
 
 
 def selftest_mfma_bf16(a, b):

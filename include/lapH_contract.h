@@ -123,6 +123,26 @@ int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int n
                               int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                               void* d_loops);
 
+/* Tile shape of the mixed-precision GEMM launches (no reference counterpart): 64 x 64 output tiles
+ * per workgroup, 1, 2 or 4; 0 restores the built-in rule (4 when Nv <= 64, else 1). Applies to the
+ * *_mixed engines (the FP64 kernels take one tile per workgroup) and takes precedence over the
+ * autotuner. Results do not depend on it. Needs no lc_init. */
+int lc_set_tile_params(int tiles_per_wg);
+
+/* In-project autotuner (no reference counterpart). enable = 1: the first *_mixed engine call of
+ * each shape (engine, nt, nv, np, idir, source list, momentum chunk) runs itself for every
+ * candidate tile shape (4, 2 and 1 tiles per workgroup, twice each, timed with HIP events; the
+ * engines overwrite their outputs, so the repeats give the same results), keeps the fastest for
+ * the later calls of that shape and leaves the fastest's results in the outputs. enable = 0 stops
+ * tuning and uses the built-in rule (choices are kept); -1 also forgets them. lc_finalize forgets
+ * them too. Needs no lc_init. */
+int lc_autotune(int enable);
+
+/* What the last engine call ran with: out5 = {tiles per workgroup (0: built-in rule), how they
+ * were chosen (0 fixed, 1 cached choice, 2 tuned by that call), ms of the candidates 4, 2 and 1
+ * (best of two runs; 0 unless tuned by that call)}. */
+int lc_autotune_report(double* out5);
+
 /* Perambulator file -> device: the reference's readPropH (lib/contract.cpp:91-160) with the file
  * order [s1][v1][t1][s2][v2][t2] permuted on the GPU into d_tau [nt][nt][4][4][nv][nv] (complex
  * double, or complex float for prec = LC_SINGLE), streamed one (s1, v1) slab at a time through

@@ -38,6 +38,22 @@ def test_not_initialised_errors():
     assert L.lc_contract_destroy() == -1
 
 
+def test_tile_params_and_autotune_arguments():
+    """The tile-shape override and the autotuner switch validate their arguments without a GPU."""
+    L = _capi.lib()
+    for bad in (3, -1, 8):
+        assert L.lc_set_tile_params(bad) == -2
+        assert b"tiles per workgroup" in L.lc_last_error()
+    for ok in (1, 2, 4, 0):
+        assert L.lc_set_tile_params(ok) == 0
+    assert L.lc_autotune(2) == -2
+    assert L.lc_autotune(-1) == 0 and L.lc_autotune(0) == 0
+    assert L.lc_autotune_report(None) == -2
+    out = (ctypes.c_double * 5)()
+    assert L.lc_autotune_report(out) == 0
+    assert out[0] in (0.0, 1.0, 2.0, 4.0) and out[1] in (0.0, 1.0, 2.0)
+
+
 @pytest.mark.skipif(not _no_gpu() if os.path.exists(_capi.LIB_PATH) else True, reason="a GPU is visible")
 def test_no_device_errors():
     L = _capi.lib()
