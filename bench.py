@@ -224,6 +224,10 @@ def main():
     ap.add_argument("--sigma-set", type=int, default=0,
                     help="1: also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
                          "27 momentum sets (reported under sigma_set)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: the mixed form tunes its tile shape on an untimed first call per workload; 0: the "
+                         "built-in rule (4 tiles per workgroup at Nv=64, the tuned choice there), no extra calls "
+                         "(for profiles: every traced launch is then a measured one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path", file=sys.stderr)
@@ -237,7 +241,8 @@ def main():
     nv, nt, nmom = 64, 48, args.nmom
     tau_h, phi_h = workload(nv, nt, nmom)
     rows.init(0)
-    engine.autotune(True)  # the mixed form picks its tile shape on its first (untimed) call per shape
+    # the mixed form picks its tile shape on its first (untimed) call per shape
+    engine.autotune(bool(args.autotune))
     tau = to_device(tau_h)
     phi = to_device(phi_h)
     nsrc = nt
@@ -249,7 +254,7 @@ def main():
                DeviceArray((nmom, nsrc, nt, 8), np.complex128),
                DeviceArray((nmom, 8, nt), np.complex128))
         tune = None
-        if mixed:
+        if mixed and args.autotune:
             # untimed: the autotuner times its tile-shape candidates on this call
             engine.rho_pipi(tau, phi, out=out, mixed=mixed)
             tune = engine.autotune_report()
@@ -267,6 +272,8 @@ def main():
         prof = engine.profile_read()
         engine.profile(False)
         log("%s: %d steps in %.3f s" % ("mixed" if mixed else "fp64", args.steps, elapsed))
+        if mixed and tune is None:
+            tune = engine.autotune_report()
         return dict(elapsed=elapsed, prof=prof, corr=out[1].to_numpy(), avg=out[2].to_numpy(),
                     tune=tune if mixed else None)
 
