@@ -217,7 +217,9 @@ void pipi_row(const Shape& S, int iso, const void* row, const void* diag, const 
   p.gemm(dense(R, b), tau_dag, V, b, 0, 0, nt, 1, 1);
   p.gemm(dense(Rp, b), tau_dag, Vp, b, 0, 0, nt, 1, 1);
   if (hor) {
-    p.gemm(dense(L, b), dense(row, b, 0, 0, LC_CONJ | LC_TS), LW, b, 0, 0, nt, 1, 1);
+    // W_ab = conj(tau_ba), no eigenvector transpose (lib/calculate_pipi_doubleTriangleHor_core.h:126), or
+    // tau^dag under lc_set_dthor_dagger
+    p.gemm(dense(L, b), dense(row, b, 0, 0, ctx().dthor_dagger ? LC_DAG : (LC_CONJ | LC_TS)), LW, b, 0, 0, nt, 1, 1);
     p.gemm(dense(Rr, b), tau_dag, Zh, b, 0, 0, nt, 1, 1);
   }
   p.barrier();
@@ -385,6 +387,13 @@ int lc_finalize(void) {
     c.stream = nullptr;
     c.on = false;
     c.device = -1;
+  });
+}
+
+int lc_set_dthor_dagger(int on) {
+  return guarded([&] {
+    if (on != 0 && on != 1) fail(LC_ERR_ARG, "lc_set_dthor_dagger: on must be 0 or 1 (got %d)", on);
+    ctx().dthor_dagger = on == 1;
   });
 }
 

@@ -94,7 +94,9 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
   const LcView tau_v = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0);
   const LcView tau_dag = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
-  const LcView W = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
+  // doubleTriangleHor's W: the reference's element-wise conj(tau_ba), or tau^dag (lc_set_dthor_dagger)
+  const LcView W = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0,
+                         ctx().dthor_dagger ? LC_DAG : (LC_CONJ | LC_TS));
   const char* phip = off(phi, (long long)p0 * nt * m);
   const LcView phi_tf = kron(phip, m, 0, nt * m);
   const LcView phi_tf_dag = kron(phip, m, 0, nt * m, LC_CONJ | LC_TV);
@@ -192,7 +194,11 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   const LcView tau_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TV);
   const LcView tau_dag_A = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
   const LcView tau_dag_B = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
-  const LcView W_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
+  // doubleTriangleHor's W as a B operand: the reference's element-wise conj(tau_ba) from the
+  // transposed planes, or tau^dag from the row-major ones (lc_set_dthor_dagger)
+  const LcView W_B = ctx().dthor_dagger
+                         ? dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG)
+                         : dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
   const char* pr = spl(phi_r, (long long)p0 * nt * m);
   const char* pt = spl(phi_t, (long long)p0 * nt * m);
   const LcView phi_tf_A = kron(pr, m, 0, nt * m);
@@ -302,7 +308,11 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   const LcView tau_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TV);
   const LcView tau_dag_A = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
   const LcView tau_dag_B = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
-  const LcView W_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
+  // doubleTriangleHor's W as a B operand: the reference's element-wise conj(tau_ba) from the
+  // transposed planes, or tau^dag from the row-major ones (lc_set_dthor_dagger)
+  const LcView W_B = ctx().dthor_dagger
+                         ? dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG)
+                         : dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
   const LcView tau_dag_Y = dense(f32(tau32, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
   (void)tau_A;
   const char* pr = spl(phi_r, (long long)p0 * nt * m);

@@ -103,6 +103,9 @@ struct Context {
   std::vector<hipEvent_t> ev_pool;
   // mixed-GEMM tiles per workgroup: the caller's override (0: rule), the value in force while an
   // engine call plans its launches, and the autotuner's switch, choices and last record
+  // doubleTriangleHor with tau^dag (conjugate transpose) in place of the reference's element-wise
+  // conjugate (lc_set_dthor_dagger)
+  bool dthor_dagger = false;
   int tile_bpw = 0;
   int active_bpw = 0;
   bool autotune = false;

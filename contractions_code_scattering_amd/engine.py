@@ -22,6 +22,7 @@ register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
 register("lc_build_elementals", [_VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_profile", [_I])
 register("lc_set_tile_params", [_I])
+register("lc_set_dthor_dagger", [_I])
 register("lc_autotune", [_I])
 register("lc_autotune_report", [_VP])
 register("lc_profile_read", [_VP])
@@ -104,6 +105,13 @@ def profile_read():
     check(lib().lc_profile_read(buf.ctypes.data))
     return dict(gemm_ms=buf[0], gemm_launches=int(buf[1]), gemm_flops=buf[2],
                 trace_ms=buf[3], trace_launches=int(buf[4]), trace_flops=buf[5])
+
+
+def set_dthor_dagger(on=True):
+    """doubleTriangleHor with the conjugate transpose tau^dag instead of the reference's element-wise
+    conjugate (lib/calculate_pipi_doubleTriangleHor_core.h:126); applies to the pi-pi row launcher,
+    calculatePiPi and the rho + pi-pi engines. Off by default."""
+    check(lib().lc_set_dthor_dagger(1 if on else 0))
 
 
 def set_tile_params(tiles_per_wg=0):

@@ -123,6 +123,13 @@ int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int n
                               int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                               void* d_loops);
 
+/* doubleTriangleHor with the conjugate transpose tau^dag (no reference counterpart). The reference's
+ * kernel uses W_ab = conj(tau_ba) element-wise, without transposing the eigenvector indices
+ * (lib/calculate_pipi_doubleTriangleHor_core.h:126); that stays the default (on = 0) of lc_pipi_row,
+ * calculatePiPi and the rho + pi-pi engines. on = 1 switches all of them to W = tau^dag. Needs no
+ * lc_init. */
+int lc_set_dthor_dagger(int on);
+
 /* Tile shape of the mixed-precision GEMM launches (no reference counterpart): 64 x 64 output tiles
  * per workgroup, 1, 2 or 4; 0 restores the built-in rule (4 when Nv <= 64, else 1). Applies to the
  * *_mixed engines (the FP64 kernels take one tile per workgroup) and takes precedence over the
