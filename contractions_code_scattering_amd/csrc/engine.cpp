@@ -298,10 +298,13 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
     cur += ((size_t)elems * es + 255) & ~size_t(255);
     return p;
   };
-  char* Zh = take(pc * npair * b, 8);
+  // Zh, R and R' are only closing-trace factors: they are stored transposed, so that the fused
+  // traces read four consecutive values per 16-byte load (ZhT = conj(tau) Rr^T, RT = Phi^T D^T conj(Phi),
+  // RpT = conj(Phi) D^T Phi^T)
+  char* ZhT = take(pc * npair * b, 8);
   char *Ls = take(pc * npair * b, 12), *Rrs = take(pc * npair * b, 12), *Us = take(pc * npair * b, 12);
   char* Q = take((long long)pc * nt * b, 12);
-  char *R = take((long long)pc * ns * b, 8), *Rp = take((long long)pc * ns * b, 8);
+  char *RT = take((long long)pc * ns * b, 8), *RpT = take((long long)pc * ns * b, 8);
 
   const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
   const LcView tau_A = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0);
@@ -326,6 +329,13 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   const LcView phi_t0_dag_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
   const LcView D_tf_B = dense(tau_t, (nt + 1) * b, 0, 0, LC_TV);
   const LcView D_t0_B = dense(spl(tau_t, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0, LC_TV);
+  // transposes as operands: Phi^T (A) and conj(Phi) (A), D^T (B), conj(Phi) and Phi^T (B), conj(tau) (A)
+  const LcView phi_t0_T_A = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m);
+  const LcView phi_t0_cj_A = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ);
+  const LcView D_t0_T_B = dense(spl(tau_r, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0, LC_TS | LC_TV);
+  const LcView phi_t0_cj_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  const LcView phi_t0_T_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
+  const LcView tau_cj_A = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ);
   auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
   char* rw = ::off(static_cast<char*>(raw), (long long)p0 * ns * nt * 10, 16);
   const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
@@ -359,12 +369,12 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   pg.sandwich(phi_tf_A, tau_B, phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, tr, Rrs, false,
               &phi_t0_B, &star1);
   pg.sandwich(phi_tf_dag_A, D_tf_B, phi_tf_B, nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Q);
-  pg.sandwich(phi_t0_dag_A, D_t0_B, phi_t0_B, R, 0, b, ns * b, 1, ns, pc);
-  pg.sandwich(phi_t0_A, D_t0_B, phi_t0_dag_B, Rp, 0, b, ns * b, 1, ns, pc);
+  pg.sandwich(phi_t0_T_A, D_t0_T_B, phi_t0_cj_B, RT, 0, b, ns * b, 1, ns, pc);
+  pg.sandwich(phi_t0_cj_A, D_t0_T_B, phi_t0_T_B, RpT, 0, b, ns * b, 1, ns, pc);
   pg.barrier();
   // level 2; U walks the sources before the sinks, so that Q(tf, p) of the momenta in flight is
   // shared by neighbouring workgroups
-  pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  pg.gemm(tau_cj_A, pair(Rrs, LC_TV | LC_TS), ZhT, P0, P1, P2, nt, ns, pc);
   pg.set_second_axis(1);
   pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
           {}, Us, true);
@@ -372,8 +382,8 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   pg.barrier();
   // level 3
   pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
-          {slot(3, dense(R, 0, b, ns * b)), slot(4, dense(Rp, 0, b, ns * b))});
-  pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
+          {slot(3, dense(RT, 0, b, ns * b, LC_TV | LC_TS)), slot(4, dense(RpT, 0, b, ns * b, LC_TV | LC_TS))});
+  pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(ZhT, LC_TV | LC_TS))});
   pg.run(ctx().stream);
 }
 
