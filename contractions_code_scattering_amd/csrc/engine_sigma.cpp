@@ -193,7 +193,9 @@ void run_chunk_sw(const Shape& E, const Store& tau, const char* tau32, const Sto
     cur += ((size_t)elems * es + 255) & ~size_t(255);
     return p;
   };
-  char* Y = take(pc * npair * b, 8);
+  // Y = (g5 R) tau^dag is only a closing-trace factor: it is formed transposed, YT = conj(tau) R^T g5,
+  // so that the fused traces read four consecutive values per 16-byte load
+  char* YT = take(pc * npair * b, 8);
   char* R = take((long long)pc * ns * b, 12);
 
   Builder bl(true, E.nv);
@@ -229,18 +231,19 @@ void run_chunk_sw(const Shape& E, const Store& tau, const char* tau32, const Sto
               bl.B(el, el_t0(EL_P1), 0, eK, eP, LC_KRON), nullptr, 0, b, ns * b, 1, ns, pc, 1.0, 0.0, {}, R);
   pg.barrier();
   // level 2
-  pg.gemm(left(make_view(R, 0, b, ns * b, LC_DENSE, 0), g5), bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE, LC_DAG),
-          Y, P0, P1, P2, nt, ns, pc);
+  pg.gemm(bl.A(tau, tau0, nt * b, E.step * b, 0, LC_DENSE, LC_CONJ),
+          right(make_view(R, 0, b, ns * b, LC_DENSE, LC_TV | LC_TS), g5), YT, P0, P1, P2, nt, ns, pc);
   pg.barrier();
   // level 3
   for (int ip = 0; ip < 2; ip++)
     pg.gemm(bl.A(el, el_tf(EL_TRI + ip), eTF, 0, eP, LC_KRON), bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE),
-            nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(4 + ip, right(pair(Y), g5))});
+            nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(4 + ip, right(pair(YT, LC_TV | LC_TS), g5))});
   const int ips[3] = {0, 1, 2 + E.idir};
   for (int j = 0; j < 3; j++) {
     std::vector<FusedTrace> t4;
     for (int iop = 0; iop < 4; iop++)
-      if (rhopi_ip(E.idir, iop) == ips[j]) t4.push_back(slot(6 + iop, right(pair(Y), g5 * rhoj(iop + 1, E.idir, false))));
+      if (rhopi_ip(E.idir, iop) == ips[j])
+        t4.push_back(slot(6 + iop, right(pair(YT, LC_TV | LC_TS), g5 * rhoj(iop + 1, E.idir, false))));
     pg.gemm(bl.A(el, el_tf(EL_P12 + ips[j]), eTF, 0, eP, LC_KRON), bl.B(tau, tau0, nt * b, E.step * b, 0, LC_DENSE),
             nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, t4);
   }
