@@ -181,12 +181,12 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
     cur += ((size_t)elems * es + 255) & ~size_t(255);
     return p;
   };
-  char *B32 = take(pc * npair * b, 8), *L1_32 = take(pc * npair * b, 8), *Zh = take(pc * npair * b, 8);
+  char *B32 = take(pc * npair * b, 8), *L1_32 = take(pc * npair * b, 8), *ZhT = take(pc * npair * b, 8);
   char *Bs = take(pc * npair * b, 12), *L1s = take(pc * npair * b, 12), *Ls = take(pc * npair * b, 12),
        *Rrs = take(pc * npair * b, 12), *Us = take(pc * npair * b, 12);
   char *Qa = take((long long)pc * nt * b, 12), *Q = take((long long)pc * nt * b, 12);
   char *T1 = take((long long)pc * ns * b, 12), *T1p = take((long long)pc * ns * b, 12),
-       *R = take((long long)pc * ns * b, 8), *Rp = take((long long)pc * ns * b, 8);
+       *RT = take((long long)pc * ns * b, 8), *RpT = take((long long)pc * ns * b, 8);
 
   const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
   // tau(tf, t0_k): A role from the row-major planes, B role from the transposed planes
@@ -243,11 +243,18 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   pg.gemm(pair(L1s), phi_t0_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Ls);
   pg.gemm(pair(Bs), phi_t0_dag_B, nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Rrs);
   pg.gemm(phi_tf_dag_A, dense(Qa, b, 0, nt * b, LC_TV), nullptr, b, 0, nt * b, nt, 1, pc, 1.0, 0.0, {}, Q);
-  pg.gemm(phi_t0_dag_A, dense(T1, 0, b, ns * b, LC_TV), R, 0, b, ns * b, 1, ns, pc);
-  pg.gemm(phi_t0_A, dense(T1p, 0, b, ns * b, LC_TV), Rp, 0, b, ns * b, 1, ns, pc);
+  // R = Phi^dag T1 and R' = Phi T1' are only closing-trace factors: formed transposed,
+  // RT = T1^T conj(Phi) and RpT = T1'^T Phi^T, so that their fused traces read 16-byte vectors
+  // (T1, T1' are stored block-transposed: a spin-transposed view of them is T1^T read by rows)
+  const LcView phi_t0_cj_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  const LcView phi_t0_T_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
+  pg.gemm(dense(T1, 0, b, ns * b, LC_TS), phi_t0_cj_B, RT, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(dense(T1p, 0, b, ns * b, LC_TS), phi_t0_T_B, RpT, 0, b, ns * b, 1, ns, pc);
   pg.barrier();
   // level 3
-  pg.gemm(pair(Rrs), tau_dag_B, Zh, P0, P1, P2, nt, ns, pc);
+  // Zh = Rr tau^dag, a closing-trace factor, formed transposed: ZhT = conj(tau) Rr^T
+  pg.gemm(dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ), pair(Rrs, LC_TV | LC_TS), ZhT, P0,
+          P1, P2, nt, ns, pc);
   pg.set_second_axis(1);  // sources before sinks: Q(tf, p) shared by neighbouring workgroups
   pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
           {}, Us, true);
@@ -255,8 +262,8 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   pg.barrier();
   // level 4
   pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
-          {slot(3, dense(R, 0, b, ns * b)), slot(4, dense(Rp, 0, b, ns * b))});
-  pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(Zh))});
+          {slot(3, dense(RT, 0, b, ns * b, LC_TV | LC_TS)), slot(4, dense(RpT, 0, b, ns * b, LC_TV | LC_TS))});
+  pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(ZhT, LC_TV | LC_TS))});
   pg.run(ctx().stream);
 }
 
