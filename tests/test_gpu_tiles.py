@@ -1,4 +1,5 @@
-"""The mixed GEMM's tile shape (output tiles per workgroup) and the in-project autotuner.
+"""The mixed GEMM's tile shape (output tiles per workgroup), the in-project autotuner, and momentum
+chunking of the mixed engines.
 
 The tile shape only regroups the same products, so every setting must give the results of the
 built-in rule, to FP64 rounding of the regrouped trace partial sums (1e-13 normwise; the mixed
@@ -114,3 +115,38 @@ def test_sigma_set_tile_shapes_agree(tiles):
     raw, loops = (x.to_numpy() for x in tiles.sigma_set(t, e, idir=1, mixed=True))
     assert tiles.autotune_report()["how"] == "tuned"
     assert relerr(raw, raw0) < 1e-13 and relerr(loops, loops0) < 1e-13
+
+
+def _one_momentum_budget(run):
+    """The scratch budget of one momentum per chunk, read from the engine's own refusal."""
+    import re
+    from contractions_code_scattering_amd._capi import LcError
+    with pytest.raises(LcError) as e:
+        run(1)
+    m = re.search(r"below one momentum(?: set)? \((\d+) B\)", str(e.value))
+    assert m, str(e.value)
+    return int(m.group(1))
+
+
+def test_mixed_momentum_chunks_are_invisible(tiles):
+    """One momentum per chunk (every view offset by the chunk's first momentum) gives the results
+    of one chunk for all momenta, bit for bit, in the fused-sandwich rho + pi-pi chain and the
+    sigma set."""
+    from contractions_code_scattering_amd.devmem import to_device
+    nt, nv, np_ = 3, 64, 3
+    tau, phi = _rho_pipi_inputs(25, nt, nv, np_)
+    t, f = to_device(tau), to_device(phi)
+    whole = _run(tiles, t, f, src0=1, src_step=1)
+    budget = _one_momentum_budget(lambda b: tiles.rho_pipi(t, f, mixed=True, src0=1, src_step=1,
+                                                           max_scratch_bytes=b))
+    chunked = _run(tiles, t, f, src0=1, src_step=1, max_scratch_bytes=budget)
+    for x, y in zip(chunked, whole):
+        assert np.array_equal(x, y)
+    g = I.rng(26)
+    el = np.ascontiguousarray(np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(11)], axis=1)
+                                        for _ in range(np_)]))
+    e = to_device(el)
+    raw0, loops0 = (x.to_numpy() for x in tiles.sigma_set(t, e, idir=0, mixed=True))
+    budget = _one_momentum_budget(lambda b: tiles.sigma_set(t, e, idir=0, mixed=True, max_scratch_bytes=b))
+    raw1, loops1 = (x.to_numpy() for x in tiles.sigma_set(t, e, idir=0, mixed=True, max_scratch_bytes=budget))
+    assert np.array_equal(raw0, raw1) and np.array_equal(loops0, loops1)
