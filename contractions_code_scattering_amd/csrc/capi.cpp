@@ -375,6 +375,8 @@ int lc_finalize(void) {
     Context& c = ctx();
     if (!c.on) fail(LC_ERR_NOT_INIT, "cannot finalize: library is not initialised");
     (void)hipStreamSynchronize(c.stream);
+    drop_graphs();
+    c.graphs = false;
     c.groups.release();
     c.scratch.release();
     c.out.release();
@@ -387,6 +389,25 @@ int lc_finalize(void) {
     c.stream = nullptr;
     c.on = false;
     c.device = -1;
+  });
+}
+
+int lc_engine_graph(int enable) {
+  return guarded([&] {
+    if (enable != 0 && enable != 1) fail(LC_ERR_ARG, "lc_engine_graph: enable must be 0 or 1 (got %d)", enable);
+    Context& c = ctx();
+    if (!enable && c.on) {
+      (void)hipStreamSynchronize(c.stream);
+      drop_graphs();
+    }
+    c.graphs = enable == 1;
+  });
+}
+
+int lc_engine_graph_report(int* state) {
+  return guarded([&] {
+    if (!state) fail(LC_ERR_ARG, "lc_engine_graph_report: null output");
+    *state = ctx().last_graph;
   });
 }
 

@@ -440,11 +440,14 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
         char* phi_r = tau_t + tau_split + tau_f32;
         char* phi_t = phi_r + phi_split;
         scales = reinterpret_cast<double*>(phi_t + phi_split);
-        hip_check(lc_launch_split_rescale(d_tau, tau_r, tau_t, tau32, E.blk, nv, (long long)nt * nt, scales, s),
-                  "split tau");
-        hip_check(lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, E.m, nv, (long long)np * nt,
-                                          scales + (size_t)nt * nt, s),
-                  "split phi");
+        const long long blk = E.blk, m = E.m;
+        launch([=](hipStream_t st) { return lc_launch_split_rescale(d_tau, tau_r, tau_t, tau32, blk, nv, (long long)nt * nt, scales, st); },
+               "split tau");
+        launch([=](hipStream_t st) {
+                 return lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, m, nv, (long long)np * nt,
+                                                scales + (size_t)nt * nt, st);
+               },
+               "split phi");
         for (int p0 = 0; p0 < np; p0 += pc) {
           const int n = (np - p0) < pc ? (np - p0) : pc;
           if (sw) run_chunk_mx_sw(E, tau_r, tau_t, tau32, phi_r, phi_t, p0, n, scratch, d_raw);
@@ -456,15 +459,20 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
           run_chunk(E, Program::F64, d_tau, d_phi, p0, n, scratch, d_raw);
         }
       }
-      hip_check(lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales,
-                                          scales ? scales + (size_t)nt * nt : nullptr, s),
-                "engine epilogue");
+      const double* s_phi = scales ? scales + (size_t)nt * nt : nullptr;
+      launch([=](hipStream_t st) {
+               return lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales, s_phi, st);
+             },
+             "engine epilogue");
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     };
-    char key[160];
+    char key[160], gkey[320];
     std::snprintf(key, sizeof key, "rho_pipi nt=%d nv=%d np=%d src0=%d step=%d nsrc=%d chunk=%d", nt, nv, np, src0,
                   src_step, nsrc, pc);
-    run_with_tiles(key, mixed, call);
+    std::snprintf(gkey, sizeof gkey, "%s mx=%d tau=%p phi=%p raw=%p corr=%p avg=%p", key, mixed ? 1 : 0, d_tau, d_phi,
+                  d_raw, d_corr, d_avg);
+    const std::string graph_key(gkey);
+    run_with_tiles(key, mixed, [&] { run_engine_call(graph_key, call); });
     return LC_OK;
   } catch (const Error& e) {
     last_error() = e.msg;

@@ -322,10 +322,16 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
         char* er = tt + tau_split + tau_f32;
         char* et = er + el_split;
         scales = reinterpret_cast<double*>(et + el_split);
-        hip_check(lc_launch_split_rescale(d_tau, tr, tt, tau32, E.blk, nv, (long long)nt * nt, scales, s), "split tau");
-        hip_check(lc_launch_split_rescale(d_el, er, et, nullptr, E.m, nv, (long long)np * nt * NT_EL,
-                                          scales + (size_t)nt * nt, s),
-                  "split elementals");
+        const long long blk = E.blk, m = E.m;
+        char* t32 = tau32;
+        double* sc = scales;
+        launch([=](hipStream_t st) { return lc_launch_split_rescale(d_tau, tr, tt, t32, blk, nv, (long long)nt * nt, sc, st); },
+               "split tau");
+        launch([=](hipStream_t st) {
+                 return lc_launch_split_rescale(d_el, er, et, nullptr, m, nv, (long long)np * nt * NT_EL,
+                                                sc + (size_t)nt * nt, st);
+               },
+               "split elementals");
         tau = Store{tr, tt};
         el = Store{er, et};
       }
@@ -334,19 +340,26 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
         if (sw) run_chunk_sw(E, tau, tau32, el, p0, n, scratch, d_raw);
         else run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
       }
-      if (mixed)
-        hip_check(lc_launch_sigma_scale(d_raw, np, nsrc, nt, src0, src_step, idir, scales, scales + (size_t)nt * nt, s),
-                  "rescale traces");
+      if (mixed) {
+        double* sc = scales;
+        launch([=](hipStream_t st) {
+                 return lc_launch_sigma_scale(d_raw, np, nsrc, nt, src0, src_step, idir, sc, sc + (size_t)nt * nt, st);
+               },
+               "rescale traces");
+      }
       for (int p0 = 0; p0 < np; p0 += pc) {
         const int n = (np - p0) < pc ? (np - p0) : pc;
         run_loops(E, d_tau, d_el, p0, n, scratch, d_loops);
       }
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     };
-    char key[160];
+    char key[160], gkey[320];
     std::snprintf(key, sizeof key, "sigma_set nt=%d nv=%d np=%d idir=%d src0=%d step=%d nsrc=%d chunk=%d", nt, nv, np,
                   idir, src0, src_step, nsrc, pc);
-    run_with_tiles(key, mixed, call);
+    std::snprintf(gkey, sizeof gkey, "%s mx=%d tau=%p el=%p raw=%p loops=%p", key, mixed ? 1 : 0, d_tau, d_el, d_raw,
+                  d_loops);
+    const std::string graph_key(gkey);
+    run_with_tiles(key, mixed, [&] { run_engine_call(graph_key, call); });
     return LC_OK;
   } catch (const Error& e) {
     last_error() = e.msg;

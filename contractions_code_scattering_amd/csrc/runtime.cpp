@@ -48,6 +48,11 @@ void run_with_tiles(const std::string& key, bool tunable, const std::function<vo
   } ev;
   hip_check(hipEventCreate(&ev.e[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev.e[1]), "hipEventCreate");
+  struct Tuning {
+    Context& c;
+    ~Tuning() { c.tuning = false; }
+  } tuning{c};
+  c.tuning = true;
   int best = 0;
   for (int i = 0; i < 3; i++) {
     double t_min = 0;
@@ -64,6 +69,7 @@ void run_with_tiles(const std::string& key, bool tunable, const std::function<vo
     rec.ms[i] = t_min;
     if (rec.ms[i] < rec.ms[best]) best = i;
   }
+  c.tuning = false;
   // the outputs hold the last candidate's results; run the fastest again unless it was that one
   if (best != 2) {
     c.active_bpw = kTileCandidates[best];
@@ -73,6 +79,83 @@ void run_with_tiles(const std::string& key, bool tunable, const std::function<vo
   rec.bpw = kTileCandidates[best];
   rec.how = 2;
   c.last_tune = rec;
+}
+
+void launch(const std::function<hipError_t(hipStream_t)>& f, const char* what) {
+  Context& c = ctx();
+  hip_check(f(c.stream), what);
+  if (c.log) c.log->launches.push_back(f);
+}
+
+static void destroy_entry(GraphEntry& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+  for (DevBuf& b : g.bufs) b.release();
+  g.bufs.clear();
+}
+
+void drop_graphs() {
+  Context& c = ctx();
+  for (auto& kv : c.graph_cache) destroy_entry(kv.second);
+  c.graph_cache.clear();
+}
+
+void run_engine_call(const std::string& key, const std::function<void()>& call) {
+  Context& c = ctx();
+  c.last_graph = 0;
+  if (!c.graphs || c.profile || c.tuning) {
+    call();
+    return;
+  }
+  char tail[64];
+  std::snprintf(tail, sizeof tail, " bpw=%d dthor=%d", c.active_bpw, c.dthor_dagger ? 1 : 0);
+  const std::string k = key + tail;
+  auto hit = c.graph_cache.find(k);
+  if (hit != c.graph_cache.end()) {
+    GraphEntry& g = hit->second;
+    if (g.scratch_gen == c.scratch.gen && g.partials_gen == c.partials.gen) {
+      hip_check(hipGraphLaunch(g.exec, c.stream), "hipGraphLaunch");
+      hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+      c.last_graph = 2;
+      return;
+    }
+    destroy_entry(g);
+    c.graph_cache.erase(hit);
+  }
+  // record: run eagerly (the results of this call) while logging the launches
+  GraphEntry g;
+  LaunchLog log;
+  c.log = &log;
+  try {
+    call();
+  } catch (...) {
+    c.log = nullptr;
+    for (DevBuf& b : log.bufs) b.release();
+    throw;
+  }
+  c.log = nullptr;
+  g.bufs = std::move(log.bufs);
+  g.scratch_gen = c.scratch.gen;
+  g.partials_gen = c.partials.gen;
+  // capture the logged launches (kernel launches only: no copies, fills or host waits)
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (auto& f : log.launches) {
+      e = f(c.stream);
+      if (e != hipSuccess) break;
+    }
+    const hipError_t e2 = hipStreamEndCapture(c.stream, &graph);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    destroy_entry(g);
+    hip_check(e, "graph capture");
+  }
+  c.graph_cache[k] = std::move(g);
+  c.last_graph = 1;
 }
 
 long long default_scratch_budget() {
@@ -341,8 +424,15 @@ void Program::run(hipStream_t s) {
   const size_t toff = (gb + 255) & ~size_t(255);
   const size_t joff = (toff + tb + 255) & ~size_t(255);
   const size_t boff = (joff + jb + 255) & ~size_t(255);
-  c.groups.ensure(boff + bb + 256);
-  char* base = static_cast<char*>(c.groups.p);
+  // descriptors: the shared buffer, or, while an engine call is recorded for a graph, a buffer of
+  // this program's own that lives as long as the graph
+  DevBuf* dbuf = &c.groups;
+  if (c.log) {
+    c.log->bufs.emplace_back();
+    dbuf = &c.log->bufs.back();
+  }
+  dbuf->ensure(boff + bb + 256);
+  char* base = static_cast<char*>(dbuf->p);
   if (gb) hip_check(hipMemcpyAsync(base, gemms_.data(), gb, hipMemcpyHostToDevice, s), "upload gemm groups");
   if (tb) hip_check(hipMemcpyAsync(base + toff, traces_.data(), tb, hipMemcpyHostToDevice, s), "upload trace groups");
   if (jb) hip_check(hipMemcpyAsync(base + joff, jobs.data(), jb, hipMemcpyHostToDevice, s), "upload reduce jobs");
@@ -367,17 +457,27 @@ void Program::run(hipStream_t s) {
     if (l.count == 0) continue;
     if (l.total_wg > 0x7ffffff0LL) fail(LC_ERR_ARG, "launch too large (%lld workgroups)", l.total_wg);
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev], s), "hipEventRecord");
-    hipError_t e = l.is_trace
-                       ? lc_launch_trace(dbl_, dt + l.first, l.count, l.total_wg, nv_, s)
-                       : (mode_ == MX ? lc_launch_gemm_mx(l.variant, dg + l.first, l.count, l.total_wg, nv_, s)
-                                      : lc_launch_gemm(dbl_, dg + l.first, l.count, l.total_wg, nv_, s));
-    hip_check(e, l.is_trace ? "trace launch" : "gemm launch");
+    const bool dbl = dbl_, is_trace = l.is_trace, mx = mode_ == MX;
+    const int variant = l.variant, count = l.count, nv = nv_;
+    const long long total_wg = l.total_wg;
+    const LcTraceGroup* tg = dt + l.first;
+    const LcGemmGroup* gg = dg + l.first;
+    launch(
+        [=](hipStream_t st) {
+          return is_trace ? lc_launch_trace(dbl, tg, count, total_wg, nv, st)
+                          : (mx ? lc_launch_gemm_mx(variant, gg, count, total_wg, nv, st)
+                                : lc_launch_gemm(dbl, gg, count, total_wg, nv, st));
+        },
+        l.is_trace ? "trace launch" : "gemm launch");
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev + 1], s), "hipEventRecord");
     nev += 2;
-    if (!l.jobs.empty())
-      hip_check(lc_launch_reduce(dj + level_jobs[li].first, db + level_jobs[li].first, (int)l.jobs.size(),
-                                 level_jobs[li].second, s),
-                "reduce launch");
+    if (!l.jobs.empty()) {
+      const LcReduceJob* rj = dj + level_jobs[li].first;
+      const long long* rb = db + level_jobs[li].first;
+      const int nj = (int)l.jobs.size();
+      const long long nthr = level_jobs[li].second;
+      launch([=](hipStream_t st) { return lc_launch_reduce(rj, rb, nj, nthr, st); }, "reduce launch");
+    }
   }
   if (prof) {
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");

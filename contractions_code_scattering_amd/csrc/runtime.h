@@ -50,12 +50,15 @@ inline void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) fail(LC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-// Grow-only device buffer.
+// Grow-only device buffer. `gen` counts its allocations, so that a recorded graph can tell
+// whether the memory it launches on is still the same.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  long long gen = 0;
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
+    gen++;
     if (p) {
       hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
       hip_check(hipFree(p), "hipFree");
@@ -93,6 +96,20 @@ struct TuneRecord {
   double ms[3] = {0, 0, 0};  // candidates 4, 2, 1 tiles per workgroup (best of two runs each)
 };
 
+// Launches of an engine call being recorded for graph replay, and the descriptor buffers they read.
+struct LaunchLog {
+  std::vector<std::function<hipError_t(hipStream_t)>> launches;
+  std::vector<DevBuf> bufs;
+};
+
+// A recorded engine call: the instantiated graph, its descriptor buffers and the allocations of
+// the shared scratch and partial buffers it was recorded on.
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  std::vector<DevBuf> bufs;
+  long long scratch_gen = 0, partials_gen = 0;
+};
+
 struct Context {
   bool on = false;
   int device = -1;
@@ -111,7 +128,29 @@ struct Context {
   bool autotune = false;
   std::map<std::string, int> tuned;
   TuneRecord last_tune;
+  // hipGraph replay of engine calls (lc_engine_graph): the switch, the log of the call being
+  // recorded, the graphs by call and what the last engine call did (0 eager, 1 recorded,
+  // 2 replayed)
+  bool graphs = false;
+  bool tuning = false;  // the autotuner's candidate runs are never recorded
+  LaunchLog* log = nullptr;
+  std::map<std::string, GraphEntry> graph_cache;
+  int last_graph = 0;
 };
+
+// Runs one kernel launch on the library stream; while an engine call is being recorded the launch
+// is also kept for the graph.
+void launch(const std::function<hipError_t(hipStream_t)>& f, const char* what);
+
+// Runs one engine call (`call`, which ends with a stream synchronisation). With graphs on and no
+// profiling, the first call of `key` (the engine, its arguments and the tile parameters) runs
+// eagerly while its launches are logged and then captured into a graph; a later call with the same
+// key replays the graph, unless the shared scratch or partial buffers were reallocated since, in
+// which case it is recorded again.
+void run_engine_call(const std::string& key, const std::function<void()>& call);
+
+// Destroys every recorded graph.
+void drop_graphs();
 
 // Tiles-per-workgroup candidates of the autotuner, in timing order.
 constexpr int kTileCandidates[3] = {4, 2, 1};

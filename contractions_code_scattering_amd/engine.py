@@ -23,6 +23,8 @@ register("lc_build_elementals", [_VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_profile", [_I])
 register("lc_set_tile_params", [_I])
 register("lc_set_dthor_dagger", [_I])
+register("lc_engine_graph", [_I])
+register("lc_engine_graph_report", [_VP])
 register("lc_autotune", [_I])
 register("lc_autotune_report", [_VP])
 register("lc_profile_read", [_VP])
@@ -105,6 +107,19 @@ def profile_read():
     check(lib().lc_profile_read(buf.ctypes.data))
     return dict(gemm_ms=buf[0], gemm_launches=int(buf[1]), gemm_flops=buf[2],
                 trace_ms=buf[3], trace_launches=int(buf[4]), trace_flops=buf[5])
+
+
+def graph(enable=True):
+    """hipGraph replay of the engines: the first call with given arguments is recorded, later calls
+    with the same arguments replay it (lc_engine_graph); False destroys the graphs."""
+    check(lib().lc_engine_graph(1 if enable else 0))
+
+
+def graph_report():
+    """What the last engine call did: "eager", "recorded" or "replayed"."""
+    v = ctypes.c_int(0)
+    check(lib().lc_engine_graph_report(ctypes.byref(v)))
+    return ("eager", "recorded", "replayed")[v.value]
 
 
 def set_dthor_dagger(on=True):

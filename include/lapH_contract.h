@@ -123,6 +123,17 @@ int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int n
                               int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                               void* d_loops);
 
+/* hipGraph replay of the batched engines (no reference counterpart). enable = 1: the first
+ * lc_engine_* call with a given set of arguments (pointers, sizes, source list, scratch budget) and
+ * tile parameters runs eagerly and is recorded; later calls with the same arguments replay the
+ * recorded launches as one graph (the inputs are read again, so new values in the same buffers are
+ * used). A call is recorded again after the library's shared scratch or partial buffers were
+ * reallocated. Profiled calls (lc_profile) and the autotuner's candidate runs are never recorded.
+ * enable = 0 destroys the graphs. Needs no lc_init. */
+int lc_engine_graph(int enable);
+/* What the last engine call did: *state = 0 eager, 1 recorded, 2 replayed. */
+int lc_engine_graph_report(int* state);
+
 /* doubleTriangleHor with the conjugate transpose tau^dag (no reference counterpart). The reference's
  * kernel uses W_ab = conj(tau_ba) element-wise, without transposing the eigenvector indices
  * (lib/calculate_pipi_doubleTriangleHor_core.h:126); that stays the default (on = 0) of lc_pipi_row,

@@ -1,0 +1,135 @@
+"""hipGraph replay of the batched engines (lc_engine_graph).
+
+Every replay is checked against the eager result of the same inputs, bit for bit (the replay runs
+the same kernels on the same descriptors). Between replays the program does eager work of its own
+(other engine calls, host copies, synchronisations). New values written into the same input
+buffers must be read by the replay. A call is recorded again after the library's scratch grows.
+"""
+import numpy as np
+import pytest
+
+from contractions_code_scattering_amd import inputs as I
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def eng(lc):
+    from contractions_code_scattering_amd import engine
+    engine.graph(False)
+    yield engine
+    engine.graph(False)
+
+
+def _outs(np_, nsrc, nt):
+    from contractions_code_scattering_amd.devmem import DeviceArray
+    return (DeviceArray((np_, nsrc, nt, 10), np.complex128), DeviceArray((np_, nsrc, nt, 8), np.complex128),
+            DeviceArray((np_, 8, nt), np.complex128))
+
+
+def _host(out):
+    return [x.to_numpy() for x in out]
+
+
+def _same(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("nv,mixed", [(16, False), (64, True), (40, True)])
+def test_rho_pipi_replays(eng, nv, mixed):
+    from contractions_code_scattering_amd.devmem import copy, to_device
+    nt, np_ = 3, 2
+    g = I.rng(51)
+    tau_a = I.perambulator(g, nt, nv, 0.3)
+    tau_b = I.perambulator(g, nt, nv, 0.3)
+    phi = np.stack([I.random_elemental(g, nt, nv) for _ in range(np_)])
+    t, ta, tb, f = to_device(tau_a), to_device(tau_a), to_device(tau_b), to_device(phi)
+    out = _outs(np_, nt, nt)
+    eng.rho_pipi(ta, f, out=out, mixed=mixed)
+    ref_a = _host(out)
+    eng.rho_pipi(tb, f, out=out, mixed=mixed)
+    ref_b = _host(out)
+    assert eng.graph_report() == "eager"
+    eng.graph(True)
+    eng.rho_pipi(t, f, out=out, mixed=mixed)
+    assert eng.graph_report() == "recorded" and _same(_host(out), ref_a)
+    for rep in range(3):
+        eng.rho_pipi(t, f, out=out, mixed=mixed)
+        assert eng.graph_report() == "replayed" and _same(_host(out), ref_a), rep
+        # eager work in between: another engine call on other buffers (recorded separately)
+        other = _outs(np_, nt, nt)
+        eng.rho_pipi(tb, f, out=other, mixed=mixed)
+        assert _same(_host(other), ref_b)
+    # new values in the same input buffer are read by the replay
+    copy(t, tb)
+    eng.rho_pipi(t, f, out=out, mixed=mixed)
+    assert eng.graph_report() == "replayed" and _same(_host(out), ref_b)
+    copy(t, ta)
+    eng.rho_pipi(t, f, out=out, mixed=mixed)
+    assert eng.graph_report() == "replayed" and _same(_host(out), ref_a)
+    # profiled calls run eagerly
+    eng.profile(True)
+    eng.rho_pipi(t, f, out=out, mixed=mixed)
+    eng.profile(False)
+    assert eng.graph_report() == "eager" and _same(_host(out), ref_a)
+
+
+def test_rerecorded_after_scratch_grows(eng):
+    from contractions_code_scattering_amd.devmem import to_device
+    nt, nv, np_ = 3, 64, 1
+    g = I.rng(52)
+    tau = to_device(I.perambulator(g, nt, nv, 0.3))
+    f = to_device(np.stack([I.random_elemental(g, nt, nv) for _ in range(np_)]))
+    out = _outs(np_, nt, nt)
+    eng.rho_pipi(tau, f, out=out, mixed=True)
+    ref = _host(out)
+    eng.graph(True)
+    eng.rho_pipi(tau, f, out=out, mixed=True)
+    assert eng.graph_report() == "recorded"
+    # a much larger call grows the shared scratch (more momenta, larger Nv)
+    big_t = to_device(I.perambulator(g, 4, 128, 0.3))
+    big_f = to_device(np.stack([I.random_elemental(g, 4, 128) for _ in range(6)]))
+    eng.rho_pipi(big_t, big_f, mixed=True)
+    eng.rho_pipi(tau, f, out=out, mixed=True)
+    assert eng.graph_report() == "recorded" and _same(_host(out), ref)
+    eng.rho_pipi(tau, f, out=out, mixed=True)
+    assert eng.graph_report() == "replayed" and _same(_host(out), ref)
+
+
+def test_sigma_set_replays_and_autotune(eng):
+    from contractions_code_scattering_amd.devmem import DeviceArray, to_device
+    nt, nv, np_ = 3, 64, 2
+    g = I.rng(53)
+    tau = to_device(I.perambulator(g, nt, nv, 0.3))
+    el = to_device(np.ascontiguousarray(np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(11)], axis=1)
+                                                  for _ in range(np_)])))
+    out = (DeviceArray((np_, nt, nt, 10), np.complex128), DeviceArray((np_, nt, 3), np.complex128))
+    for mixed in (False, True):
+        eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
+        ref = _host(out)
+        eng.graph(True)
+        eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
+        assert eng.graph_report() == "recorded" and _same(_host(out), ref)
+        eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
+        assert eng.graph_report() == "replayed" and _same(_host(out), ref)
+        eng.graph(False)
+    # with the autotuner on, the candidate runs are not recorded; the chosen shape is
+    try:
+        eng.autotune(True, forget=True)
+        eng.graph(True)
+        eng.sigma_set(tau, el, idir=1, mixed=True, out=out)
+        assert eng.autotune_report()["how"] == "tuned"
+        assert eng.graph_report() in ("recorded", "eager")
+        # the tuning call recorded its final run unless the last candidate won (then the next call does)
+        eng.sigma_set(tau, el, idir=1, mixed=True, out=out)
+        assert eng.autotune_report()["how"] == "cached" and eng.graph_report() in ("recorded", "replayed")
+        eng.sigma_set(tau, el, idir=1, mixed=True, out=out)
+        assert eng.autotune_report()["how"] == "cached" and eng.graph_report() == "replayed"
+        assert relerr_ok(_host(out), ref)
+    finally:
+        eng.autotune(False, forget=True)
+
+
+def relerr_ok(a, b):
+    """The tuned tile shape regroups the trace sums: equal to 1e-13 normwise."""
+    return all(np.abs(x - y).max() <= 1e-13 * np.abs(y).max() for x, y in zip(a, b))
