@@ -39,8 +39,8 @@ def test_not_initialised_errors():
 
 
 def test_tile_params_and_autotune_arguments():
-    """The tile-shape override, the autotuner switch and the dTHor option validate their arguments
-    without a GPU."""
+    """The tile-shape override, the autotuner and graph switches and the dTHor option validate their
+    arguments without a GPU."""
     L = _capi.lib()
     for bad in (3, -1, 8):
         assert L.lc_set_tile_params(bad) == -2
@@ -52,6 +52,10 @@ def test_tile_params_and_autotune_arguments():
     assert L.lc_autotune_report(None) == -2
     assert L.lc_set_dthor_dagger(2) == -2
     assert L.lc_set_dthor_dagger(1) == 0 and L.lc_set_dthor_dagger(0) == 0
+    assert L.lc_engine_graph(2) == -2
+    assert L.lc_engine_graph_report(None) == -2
+    st = ctypes.c_int(-1)
+    assert L.lc_engine_graph_report(ctypes.byref(st)) == 0 and st.value in (0, 1, 2)
     out = (ctypes.c_double * 5)()
     assert L.lc_autotune_report(out) == 0
     assert out[0] in (0.0, 1.0, 2.0, 4.0) and out[1] in (0.0, 1.0, 2.0)
