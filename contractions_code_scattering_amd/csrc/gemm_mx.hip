@@ -69,6 +69,15 @@ constexpr int MX_KT = 32;
 __device__ __forceinline__ int swz(int line) { return (line >> 2) & 3; }
 constexpr unsigned SIGN2 = 0x80008000u;
 
+// Position of element (line, k) in one Nv x Nv plane of split storage. For Nv a multiple of 64 the
+// plane is kept in K panels of 32: panel k / 32 holds the 32 values of every line, line after
+// line, so that the 64 x 32 chunk of a tile that the main loop stages is one contiguous 4 KB run
+// per plane (whole cache lines). Otherwise lines are contiguous (row-major). A line is a row of a
+// row-major operand or a column of a transposed one.
+__device__ __forceinline__ long long plane_idx(int line, int k, int nv) {
+  return (nv & 63) == 0 ? ((long long)(k >> 5) * nv + line) * 32 + (k & 31) : (long long)line * nv + k;
+}
+
 // Block (a,b) of a split view: base of its six planes, element flags, exact phase.
 struct SplitRef {
   const unsigned short* p;
@@ -95,13 +104,14 @@ __device__ __forceinline__ SplitRef split_block(const LcView& v, long long off, 
 }
 
 // Six planes of 8 consecutive-K values of one line: element (line, k + e) of a plane sits at
-// line * nv + k + e (A rows / B columns are contiguous along K, enforced by the planner).
+// plane_idx(line, k + e) (A rows / B columns are contiguous along K, enforced by the planner; k is a
+// multiple of 8, so the 8 values never straddle a K panel).
 __device__ __forceinline__ void fetch_split(const unsigned short* p, int line, int k, int nv, long long nv2,
                                             bool vec, u32x4 (&v)[6]) {
   if (vec) {
 #pragma unroll
     for (int q = 0; q < 6; q++)
-      v[q] = gload<u32x4>(p + q * nv2 + (long long)line * nv + k);
+      v[q] = gload<u32x4>(p + q * nv2 + plane_idx(line, k, nv));
   } else {
 #pragma unroll
     for (int q = 0; q < 6; q++) {
@@ -109,9 +119,9 @@ __device__ __forceinline__ void fetch_split(const unsigned short* p, int line, i
 #pragma unroll
       for (int h = 0; h < 4; h++) {
         unsigned lo = 0, hi = 0;
-        if (line < nv && k + 2 * h < nv) lo = gload<unsigned short>(p + q * nv2 + (long long)line * nv + k + 2 * h);
+        if (line < nv && k + 2 * h < nv) lo = gload<unsigned short>(p + q * nv2 + plane_idx(line, k + 2 * h, nv));
         if (line < nv && k + 2 * h + 1 < nv)
-          hi = gload<unsigned short>(p + q * nv2 + (long long)line * nv + k + 2 * h + 1);
+          hi = gload<unsigned short>(p + q * nv2 + plane_idx(line, k + 2 * h + 1, nv));
         w[h] = lo | (hi << 16);
       }
       v[q] = u32x4{w[0], w[1], w[2], w[3]};
@@ -231,7 +241,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       t_mi = 0u;
       t_bmi = (R.flags & LC_CONJ) ? SIGN2 : 0u;
       pB = reinterpret_cast<const unsigned short*>(R.base) + 6 * voff(R.s, co) +
-           (long long)(cur_tn * MX_TM + s_line) * nv + s_kg * 8;
+           plane_idx(cur_tn * MX_TM + s_line, s_kg * 8, nv);
       return;
     }
     int ab, bb, ba, bc;
@@ -255,8 +265,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     t_bmi = (B.flags & LC_CONJ) ? SIGN2 : 0u;
     t_alA = (reinterpret_cast<uintptr_t>(A.p) & 15) == 0;
     t_alB = (reinterpret_cast<uintptr_t>(B.p) & 15) == 0;
-    pA = A.p + (long long)(cur_tm * MX_TM + s_line) * nv + s_kg * 8;
-    pB = B.p + (long long)(cur_tn * MX_TM + s_line) * nv + s_kg * 8;
+    pA = A.p + plane_idx(cur_tm * MX_TM + s_line, s_kg * 8, nv);
+    pB = B.p + plane_idx(cur_tn * MX_TM + s_line, s_kg * 8, nv);
   };
   set_term();
 
@@ -269,26 +279,29 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     b_mi = t_bmi;
     a_t2 = SW ? cur_term : 0;
     a_kc = cur_kc;
+    // chunk kc starts k0 values further along each line: in the K-panel layout (Nv a multiple of
+    // 64) that is k0 / 32 panels of 32 Nv values further
+    const long long kofs = (nv & 63) == 0 ? (long long)k0 * nv : k0;
     if (SW) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        if (!a_t2) va[q] = gload<u32x4>(pA + q * nv2 + k0);
-        vb[q] = gload<u32x4>(pB + q * nv2 + k0);
+        if (!a_t2) va[q] = gload<u32x4>(pA + q * nv2 + kofs);
+        vb[q] = gload<u32x4>(pB + q * nv2 + kofs);
       }
     } else if (FULL) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        va[q] = gload<u32x4>(pA + q * nv2 + k0);
-        vb[q] = gload<u32x4>(pB + q * nv2 + k0);
+        va[q] = gload<u32x4>(pA + q * nv2 + kofs);
+        vb[q] = gload<u32x4>(pB + q * nv2 + kofs);
       }
     } else {
       const bool fullA = vec_ok && t_alA && (cur_tm + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
       const bool fullB = vec_ok && t_alB && (cur_tn + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
       const int k = k0 + s_kg * 8;
-      fetch_split(pA - (long long)(cur_tm * MX_TM + s_line) * nv - s_kg * 8, cur_tm * MX_TM + s_line, k, nv, nv2,
-                  fullA, va);
-      fetch_split(pB - (long long)(cur_tn * MX_TM + s_line) * nv - s_kg * 8, cur_tn * MX_TM + s_line, k, nv, nv2,
-                  fullB, vb);
+      fetch_split(pA - plane_idx(cur_tm * MX_TM + s_line, s_kg * 8, nv), cur_tm * MX_TM + s_line, k, nv, nv2, fullA,
+                  va);
+      fetch_split(pB - plane_idx(cur_tn * MX_TM + s_line, s_kg * 8, nv), cur_tn * MX_TM + s_line, k, nv, nv2, fullB,
+                  vb);
     }
     if (++cur_kc == nk) {
       cur_kc = 0;
@@ -455,7 +468,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
           split2(zi[2], zi[3], i23[0], i23[1], i23[2]);
           unsigned short* S = reinterpret_cast<unsigned short*>(g.Cs) + 6 * cofs;
           if (!g.cs_trans && (FULL || (full4 && vec_c && (cofs & 1) == 0))) {
-            const long long e = (long long)row * nv + col;
+            const long long e = plane_idx(row, col, nv);  // col % 4 == 0: four values in one K panel
 #pragma unroll
             for (int p = 0; p < 3; p++) {
               gstore(S + (2 * p) * nv2 + e, u32x2{r01[p], r23[p]});
@@ -465,7 +478,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 #pragma unroll
             for (int q = 0; q < 4; q++) {
               if (col + q >= nv) continue;
-              const long long e = g.cs_trans ? (long long)(col + q) * nv + row : (long long)row * nv + col + q;
+              const long long e = g.cs_trans ? plane_idx(col + q, row, nv) : plane_idx(row, col + q, nv);
               const int sh = (q & 1) * 16;
 #pragma unroll
               for (int p = 0; p < 3; p++) {
@@ -671,30 +684,31 @@ __global__ void __launch_bounds__(256) lc_split_convert_kernel(const double2* __
     const int r = r0 + pass * 8;
     const int row = tr * SC_T + r, col = tc * SC_T + c;
     if (row < nv && col < nv) {
-      const long long e = (long long)row * nv + col;
+      const long long e = (long long)row * nv + col;  // FP64 input and FP32 copy: row-major
       const double2 v = xb[e];
       const float fr = (float)(v.x * inv), fi = (float)(v.y * inv);
       if (y32) y32[bi * nv2 + e] = make_float2(fr, fi);
       unsigned p0, p1, p2;
       split2(fr, fi, p0, p1, p2);
       const unsigned q[3] = {p0, p1, p2};
+      const long long es = plane_idx(row, col, nv);
 #pragma unroll
       for (int p = 0; p < 3; p++) {
-        br[(2 * p) * nv2 + e] = (unsigned short)(q[p] & 0xffff);
-        br[(2 * p + 1) * nv2 + e] = (unsigned short)(q[p] >> 16);
+        br[(2 * p) * nv2 + es] = (unsigned short)(q[p] & 0xffff);
+        br[(2 * p + 1) * nv2 + es] = (unsigned short)(q[p] >> 16);
         pc[2 * p][r][c] = (unsigned short)(q[p] & 0xffff);
         pc[2 * p + 1][r][c] = (unsigned short)(q[p] >> 16);
       }
     }
   }
   __syncthreads();
-  // transposed planes: element (row, col) at col * nv + row; here thread c runs along row
+  // transposed planes: element (row, col) is element (line col, k row); here thread c runs along row
 #pragma unroll
   for (int pass = 0; pass < SC_T / 8; pass++) {
     const int r = r0 + pass * 8;  // a column of the tile
     const int row = tr * SC_T + c, col = tc * SC_T + r;
     if (row < nv && col < nv) {
-      const long long et = (long long)col * nv + row;
+      const long long et = plane_idx(col, row, nv);
 #pragma unroll
       for (int q = 0; q < 6; q++) bt[q * nv2 + et] = pc[q][c][r];
     }
