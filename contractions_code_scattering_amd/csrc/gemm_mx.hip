@@ -501,20 +501,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int n = 0; n < 2; n++) {
-          // the four real products summed apart; the conjugation of Y (cj = -1) only changes how
-          // they combine: x y = (rr - ii) + i (ri + ir), x conj(y) = (rr + ii) + i (ir - ri)
-          float rr = 0.f, ii = 0.f, ri = 0.f, ir = 0.f;
+          float fr = 0.f, fi = 0.f;
 #pragma unroll
           for (int q = 0; q < 4; q++) {
-            const float yr = ycur.v[m][n][q][0], yi = ycur.v[m][n][q][1];
+            const float yr = ycur.v[m][n][q][0], yi = ycur.cj * ycur.v[m][n][q][1];
             const float xr = zr_[m][n][q], xi = zi_[m][n][q];
-            rr += xr * yr;
-            ii += xi * yi;
-            ri += xr * yi;
-            ir += xi * yr;
+            fr += xr * yr - xi * yi;
+            fi += xr * yi + xi * yr;
           }
-          sr += rr - ycur.cj * ii;
-          si += ir + ycur.cj * ri;
+          sr += fr;
+          si += fi;
         }
       // the Y block phase is uniform over the tile
       const double dr = sr * ycur.pr - si * ycur.pi, di = sr * ycur.pi + si * ycur.pr;
