@@ -382,13 +382,15 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   // level 2; U walks the sources before the sinks, so that Q(tf, p) of the momenta in flight is
   // shared by neighbouring workgroups
   pg.gemm(tau_cj_A, pair(Rrs, LC_TV | LC_TS), ZhT, P0, P1, P2, nt, ns, pc);
+  // U = Q (g5 tau g5) is formed transposed, UT = (g5 tau^T g5) Q^T, so that its split store is
+  // row-major (tau^T read by rows is the transposed planes with the spin blocks exchanged)
   pg.set_second_axis(1);
-  pg.gemm(dense(Q, b, 0, nt * b), right(left(tau_B, g5), g5), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0,
-          {}, Us, true);
+  pg.gemm(right(left(dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TS), g5), g5),
+          dense(Q, b, 0, nt * b, LC_TV | LC_TS), nullptr, P0, P1, P2, nt, ns, pc, 1.0, 0.0, {}, Us, false);
   pg.set_second_axis(0);
   pg.barrier();
   // level 3
-  pg.gemm(tau_dag_A, pair(Us, LC_TV), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
+  pg.gemm(tau_dag_A, pair(Us, LC_TV | LC_TS), nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0,
           {slot(3, dense(RT, 0, b, ns * b, LC_TV | LC_TS)), slot(4, dense(RpT, 0, b, ns * b, LC_TV | LC_TS))});
   pg.gemm(pair(Ls), W_B, nullptr, 0, 0, 0, nt, ns, pc, 1.0, 0.0, {slot(5, pair(ZhT, LC_TV | LC_TS))});
   pg.run(ctx().stream);
