@@ -78,6 +78,15 @@ __device__ __forceinline__ long long plane_idx(int line, int k, int nv) {
   return (nv & 63) == 0 ? ((long long)(k >> 5) * nv + line) * 32 + (k & 31) : (long long)line * nv + k;
 }
 
+// Column of a 64-wide B tile held by LDS line `ph`. A wave's 16 x 16 MFMA block n reads the 16
+// consecutive lines 32 wn + 16 n + i (i = 0..15, conflict-free with the LDS swizzle) and produces
+// the columns those lines hold. Line 32 wn + 16 n + 4 a + b holds column 32 wn + 8 a + 4 n + b, so
+// that the four columns a lane gets from block 0 and the four from block 1 are eight adjacent
+// columns, and a lane stores each piece plane of a row as one 16-byte vector.
+__device__ __forceinline__ int col_of_line(int ph) {
+  return (ph & 32) + 8 * ((ph >> 2) & 3) + 4 * ((ph >> 4) & 1) + (ph & 3);
+}
+
 // Block (a,b) of a split view: base of its six planes, element flags, exact phase.
 struct SplitRef {
   const unsigned short* p;
@@ -241,7 +250,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       t_mi = 0u;
       t_bmi = (R.flags & LC_CONJ) ? SIGN2 : 0u;
       pB = reinterpret_cast<const unsigned short*>(R.base) + 6 * voff(R.s, co) +
-           plane_idx(cur_tn * MX_TM + s_line, s_kg * 8, nv);
+           plane_idx(cur_tn * MX_TM + col_of_line(s_line), s_kg * 8, nv);
       return;
     }
     int ab, bb, ba, bc;
@@ -266,7 +275,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     t_alA = (reinterpret_cast<uintptr_t>(A.p) & 15) == 0;
     t_alB = (reinterpret_cast<uintptr_t>(B.p) & 15) == 0;
     pA = A.p + plane_idx(cur_tm * MX_TM + s_line, s_kg * 8, nv);
-    pB = B.p + plane_idx(cur_tn * MX_TM + s_line, s_kg * 8, nv);
+    pB = B.p + plane_idx(cur_tn * MX_TM + col_of_line(s_line), s_kg * 8, nv);
   };
   set_term();
 
@@ -300,8 +309,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       const int k = k0 + s_kg * 8;
       fetch_split(pA - plane_idx(cur_tm * MX_TM + s_line, s_kg * 8, nv), cur_tm * MX_TM + s_line, k, nv, nv2, fullA,
                   va);
-      fetch_split(pB - plane_idx(cur_tn * MX_TM + s_line, s_kg * 8, nv), cur_tn * MX_TM + s_line, k, nv, nv2, fullB,
-                  vb);
+      fetch_split(pB - plane_idx(cur_tn * MX_TM + col_of_line(s_line), s_kg * 8, nv),
+                  cur_tn * MX_TM + col_of_line(s_line), k, nv, nv2, fullB, vb);
     }
     if (++cur_kc == nk) {
       cur_kc = 0;
@@ -321,24 +330,25 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         // T is complete: this wave's 32 x 32 part goes to A buffer wn (T's columns are the K of
         // T R, 32 per chunk), split, in the layout of a staged A chunk
 #pragma unroll
-        for (int m = 0; m < 2; m++)
+        for (int m = 0; m < 2; m++) {
+          // the lane's eight adjacent columns of T (blocks n = 0, 1) are one 8-wide K group
+          unsigned r01[2][3], r23[2][3], i01[2][3], i23[2][3];
 #pragma unroll
           for (int n = 0; n < 2; n++) {
             const fx4v xr = bre[m][n] + sre[m][n], xi = bim[m][n] + sim[m][n];
-            unsigned r01[3], r23[3], i01[3], i23[3];
-            split2(xr[0], xr[1], r01[0], r01[1], r01[2]);
-            split2(xr[2], xr[3], r23[0], r23[1], r23[2]);
-            split2(xi[0], xi[1], i01[0], i01[1], i01[2]);
-            split2(xi[2], xi[3], i23[0], i23[1], i23[2]);
-            const int line = wm * 32 + m * 16 + (lane & 15);
-            const int kg = 2 * n + ((lane >> 4) >> 1);  // 8-wide K group of the four values
-            const int e = ((kg ^ swz(line)) * 8) + ((lane >> 4) & 1) * 4;
-#pragma unroll
-            for (int p = 0; p < 3; p++) {
-              *reinterpret_cast<u32x2*>(&As[wn][2 * p][line][e]) = u32x2{r01[p], r23[p]};
-              *reinterpret_cast<u32x2*>(&As[wn][2 * p + 1][line][e]) = u32x2{i01[p], i23[p]};
-            }
+            split2(xr[0], xr[1], r01[n][0], r01[n][1], r01[n][2]);
+            split2(xr[2], xr[3], r23[n][0], r23[n][1], r23[n][2]);
+            split2(xi[0], xi[1], i01[n][0], i01[n][1], i01[n][2]);
+            split2(xi[2], xi[3], i23[n][0], i23[n][1], i23[n][2]);
           }
+          const int line = wm * 32 + m * 16 + (lane & 15);
+          const int e = ((lane >> 4) ^ swz(line)) * 8;
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            *reinterpret_cast<u32x4*>(&As[wn][2 * p][line][e]) = u32x4{r01[0][p], r23[0][p], r01[1][p], r23[1][p]};
+            *reinterpret_cast<u32x4*>(&As[wn][2 * p + 1][line][e]) = u32x4{i01[0][p], i23[0][p], i01[1][p], i23[1][p]};
+          }
+        }
         zero_acc();
       }
 #pragma unroll
@@ -376,7 +386,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     tile_of(tl, a, c, tm, tn);
     const long long cofs = voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
     const int rbase = tm * MX_TM + wm * 32 + (lane & 15);
-    const int cbase = tn * MX_TM + wn * 32 + 4 * (lane >> 4);
+    // this lane's columns: cbase + 4 n + q for block n (eight adjacent columns, see col_of_line)
+    const int cbase = tn * MX_TM + wn * 32 + 8 * (lane >> 4);
     // the tile's values (alpha applied); the accumulators are free from here on
     float zr_[2][2][4], zi_[2][2][4];
 #pragma unroll
@@ -412,7 +423,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int n = 0; n < 2; n++) {
-          const int row = rbase + m * 16, col = cbase + n * 16;
+          const int row = rbase + m * 16, col = cbase + 4 * n;
           if (FULL && tv) {
             // f(Y)(col + q, row) = Y(row, col + q): four consecutive complex values
             const float* p = reinterpret_cast<const float*>(Y.p + (long long)row * nv + col);
@@ -438,12 +449,15 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     YTile ycur, ynext;
     load_y(0, ycur);
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < 2; m++) {
+      // full tiles: the split pieces of both blocks, stored together below (16 bytes per plane)
+      unsigned mr01[2][3], mr23[2][3], mi01[2][3], mi23[2][3];
+      const bool merged = FULL && !second && g.Cs && !g.cs_trans;
 #pragma unroll
       for (int n = 0; n < 2; n++) {
         // this lane's values: row `row`, columns col..col+3
         const int row = rbase + m * 16;
-        const int col = cbase + n * 16;
+        const int col = cbase + 4 * n;
         const float* zr = zr_[m][n];
         const float* zi = zi_[m][n];
         const bool rin = FULL || row < nv;
@@ -467,7 +481,15 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
           split2(zi[0], zi[1], i01[0], i01[1], i01[2]);
           split2(zi[2], zi[3], i23[0], i23[1], i23[2]);
           unsigned short* S = reinterpret_cast<unsigned short*>(g.Cs) + 6 * cofs;
-          if (!g.cs_trans && (FULL || (full4 && vec_c && (cofs & 1) == 0))) {
+          if (merged) {
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+              mr01[n][p] = r01[p];
+              mr23[n][p] = r23[p];
+              mi01[n][p] = i01[p];
+              mi23[n][p] = i23[p];
+            }
+          } else if (!g.cs_trans && (FULL || (full4 && vec_c && (cofs & 1) == 0))) {
             const long long e = plane_idx(row, col, nv);  // col % 4 == 0: four values in one K panel
 #pragma unroll
             for (int p = 0; p < 3; p++) {
@@ -490,6 +512,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
           }
         }
       }
+      if (merged) {
+        // columns cbase..cbase+7 of row rbase + 16 m: one K panel, 16-byte aligned
+        unsigned short* S = reinterpret_cast<unsigned short*>(g.Cs) + 6 * cofs + plane_idx(rbase + m * 16, cbase, nv);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          gstore(S + (2 * p) * nv2, u32x4{mr01[0][p], mr23[0][p], mr01[1][p], mr23[1][p]});
+          gstore(S + (2 * p + 1) * nv2, u32x4{mi01[0][p], mi23[0][p], mi01[1][p], mi23[1][p]});
+        }
+      }
+    }
     // traces, the loads of trace t+1 in flight while trace t is summed
 #pragma unroll
     for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
