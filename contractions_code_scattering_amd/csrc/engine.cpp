@@ -314,28 +314,23 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
   char *RT = take((long long)pc * ns * b, 8), *RpT = take((long long)pc * ns * b, 8);
 
   const long long P0 = b, P1 = nt * b, P2 = (long long)ns * nt * b;  // pair strides
-  const LcView tau_A = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0);
   const LcView tau_B = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_TV);
   const LcView tau_dag_A = dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS);
   const LcView tau_dag_B = dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
   // doubleTriangleHor's W as a B operand: the reference's element-wise conj(tau_ba) from the
   // transposed planes, or tau^dag from the row-major ones (lc_set_dthor_dagger)
   const LcView W_B = ctx().dthor_dagger
-                         ? dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG)
+                         ? tau_dag_B
                          : dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
   const LcView tau_dag_Y = dense(f32(tau32, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG);
-  (void)tau_A;
   const char* pr = spl(phi_r, (long long)p0 * nt * m);
   const char* pt = spl(phi_t, (long long)p0 * nt * m);
   const LcView phi_tf_A = kron(pr, m, 0, nt * m);
   const LcView phi_tf_B = kron(pt, m, 0, nt * m, LC_TV);
   const LcView phi_tf_dag_A = kron(pt, m, 0, nt * m, LC_CONJ);
-  const LcView phi_t0_A = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m);
   const LcView phi_t0_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
-  const LcView phi_t0_dag_A = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ);
   const LcView phi_t0_dag_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
   const LcView D_tf_B = dense(tau_t, (nt + 1) * b, 0, 0, LC_TV);
-  const LcView D_t0_B = dense(spl(tau_t, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0, LC_TV);
   // transposes as operands: Phi^T (A) and conj(Phi) (A), D^T (B), conj(Phi) and Phi^T (B), conj(tau) (A)
   const LcView phi_t0_T_A = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m);
   const LcView phi_t0_cj_A = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ);
