@@ -30,8 +30,8 @@
 
 extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
                                                 int nsrc, int nt, int src0, int src_step,
-                                                const double* s_tau, const double* s_phi,
-                                                hipStream_t s);
+                                                const double* s_tau, const double* const* s_p,
+                                                long long ps, hipStream_t s);
 
 using namespace lc;
 
@@ -40,6 +40,18 @@ namespace {
 struct EngineShape {
   int nt, nv, np, src0, step, nsrc;
   long long m, blk;
+};
+
+// One elemental factor P1..P4 of the pi-pi chains (the m1..m4 of run_ContractPiPi,
+// lib/contract_kernels_pipi.cu): the stored matrices S (FP64 array, or the row-major `r` and
+// block-transposed `t` split planes of the mixed form), whether the chain uses S or its adjoint,
+// and the stride between momentum items in complex elements. Back-to-back pions (the
+// rho + pi-pi engine) use one array Phi for all four: P1 = P4 = Phi, P2 = P3 = Phi^dag.
+struct Role {
+  const char* r;
+  const char* t;
+  bool dag;
+  long long stride;
 };
 
 inline LcView dense(const void* p, long long s0, long long s1, long long s2, int flags = 0) {
@@ -71,7 +83,7 @@ const int PAIR_BUFS = 6;
 // Dense D x D products per (tf, t0, p): Zh, U, M, LW (4 x 64 block GEMMs).
 // tau / phi / scratch hold complex double (mode F64) or complex float (mode MX); raw is always
 // complex double.
-void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const void* phi, int p0,
+void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const Role* P, int p0,
                int pc, char* scratch, void* raw) {
   const size_t es = mode == Program::F64 ? 16 : 8;
   auto off = [es](auto p, long long elems) { return ::off(p, elems, es); };
@@ -97,11 +109,17 @@ void run_chunk(const EngineShape& E, Program::Mode mode, const void* tau, const 
   // doubleTriangleHor's W: the reference's element-wise conj(tau_ba), or tau^dag (lc_set_dthor_dagger)
   const LcView W = dense(off(tau, (long long)E.src0 * b), nt * b, E.step * b, 0,
                          ctx().dthor_dagger ? LC_DAG : (LC_CONJ | LC_TS));
-  const char* phip = off(phi, (long long)p0 * nt * m);
-  const LcView phi_tf = kron(phip, m, 0, nt * m);
-  const LcView phi_tf_dag = kron(phip, m, 0, nt * m, LC_CONJ | LC_TV);
-  const LcView phi_t0 = kron(off(phip, (long long)E.src0 * m), 0, E.step * m, nt * m);
-  const LcView phi_t0_dag = kron(off(phip, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  // role k as a view over the sinks (P3, P4) or the sources (P1, P2) of the chunk's items
+  auto role = [&](int k, bool sink) {
+    const Role& R = P[k];
+    const char* base = off(R.r, (long long)p0 * R.stride + (sink ? 0 : (long long)E.src0 * m));
+    const int fl = R.dag ? (LC_CONJ | LC_TV) : 0;
+    return sink ? kron(base, m, 0, R.stride, fl) : kron(base, 0, E.step * m, R.stride, fl);
+  };
+  const LcView phi_tf = role(3, true);       // P4(tf)
+  const LcView phi_tf_dag = role(2, true);   // P3(tf)
+  const LcView phi_t0 = role(0, false);      // P1(t0)
+  const LcView phi_t0_dag = role(1, false);  // P2(t0)
   const LcView D_tf = dense(tau, (nt + 1) * b, 0, 0);
   const LcView D_t0 = dense(off(tau, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0);
   auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
@@ -168,8 +186,8 @@ size_t mx_bytes_per_momentum(const EngineShape& E) {
 // copy carries LC_TV toggled. Intermediates are written by their producer directly in the form
 // their consumer reads: split row-major when used as A, split transposed when used as B, FP32
 // when used as a closing-trace factor.
-void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, const char* phi_r,
-                  const char* phi_t, int p0, int pc, char* scratch, void* raw) {
+void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, const Role* P, int p0, int pc,
+                  char* scratch, void* raw) {
   const int nt = E.nt, ns = E.nsrc;
   const long long m = E.m, b = E.blk;
   const long long npair = (long long)nt * ns;
@@ -199,15 +217,25 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   const LcView W_B = ctx().dthor_dagger
                          ? dense(spl(tau_r, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_DAG)
                          : dense(spl(tau_t, (long long)E.src0 * b), nt * b, E.step * b, 0, LC_CONJ | LC_TS | LC_TV);
-  const char* pr = spl(phi_r, (long long)p0 * nt * m);
-  const char* pt = spl(phi_t, (long long)p0 * nt * m);
-  const LcView phi_tf_A = kron(pr, m, 0, nt * m);
-  const LcView phi_tf_B = kron(pt, m, 0, nt * m, LC_TV);
-  const LcView phi_tf_dag_A = kron(pt, m, 0, nt * m, LC_CONJ);
-  const LcView phi_t0_A = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m);
-  const LcView phi_t0_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
-  const LcView phi_t0_dag_A = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ);
-  const LcView phi_t0_dag_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
+  // Role k as an operand over the sinks (P3, P4) or the sources (P1, P2). The operand is X or X^T
+  // with X = S or S^dag: S, S^T, S^dag = conj(S^T) or conj(S). The A role reads the operand by
+  // rows, the B role reads its transpose by rows (flag TV); S by rows is the row-major planes,
+  // S^T by rows the transposed ones.
+  auto role = [&](int k, bool sink, bool brole, bool transposed) {
+    const Role& R = P[k];
+    const int op = R.dag ? (transposed ? 3 : 2) : (transposed ? 1 : 0);
+    const bool t_rows = (op == 1 || op == 2) != brole;  // read the transposed planes
+    const int fl = ((op >= 2) ? LC_CONJ : 0) | (brole ? LC_TV : 0);
+    const char* base = spl(t_rows ? R.t : R.r, (long long)p0 * R.stride + (sink ? 0 : (long long)E.src0 * m));
+    return sink ? kron(base, m, 0, R.stride, fl) : kron(base, 0, E.step * m, R.stride, fl);
+  };
+  const LcView phi_tf_A = role(3, true, false, false);       // P4(tf)
+  const LcView phi_tf_B = role(3, true, true, false);
+  const LcView phi_tf_dag_A = role(2, true, false, false);   // P3(tf)
+  const LcView phi_t0_A = role(0, false, false, false);      // P1(t0)
+  const LcView phi_t0_B = role(0, false, true, false);
+  const LcView phi_t0_dag_A = role(1, false, false, false);  // P2(t0)
+  const LcView phi_t0_dag_B = role(1, false, true, false);
   const LcView D_tf = dense(tau_r, (nt + 1) * b, 0, 0);
   const LcView D_t0 = dense(spl(tau_r, (long long)E.src0 * (nt + 1) * b), 0, E.step * (nt + 1) * b, 0);
   auto pair = [&](char* p, int flags = 0) { return dense(p, P0, P1, P2, flags); };
@@ -246,10 +274,8 @@ void run_chunk_mx(const EngineShape& E, const char* tau_r, const char* tau_t, co
   // R = Phi^dag T1 and R' = Phi T1' are only closing-trace factors: formed transposed,
   // RT = T1^T conj(Phi) and RpT = T1'^T Phi^T, so that their fused traces read 16-byte vectors
   // (T1, T1' are stored block-transposed: a spin-transposed view of them is T1^T read by rows)
-  const LcView phi_t0_cj_B = kron(spl(pt, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_CONJ | LC_TV);
-  const LcView phi_t0_T_B = kron(spl(pr, (long long)E.src0 * m), 0, E.step * m, nt * m, LC_TV);
-  pg.gemm(dense(T1, 0, b, ns * b, LC_TS), phi_t0_cj_B, RT, 0, b, ns * b, 1, ns, pc);
-  pg.gemm(dense(T1p, 0, b, ns * b, LC_TS), phi_t0_T_B, RpT, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(dense(T1, 0, b, ns * b, LC_TS), role(1, false, true, true), RT, 0, b, ns * b, 1, ns, pc);
+  pg.gemm(dense(T1p, 0, b, ns * b, LC_TS), role(0, false, true, true), RpT, 0, b, ns * b, 1, ns, pc);
   pg.barrier();
   // level 3
   // Zh = Rr tau^dag, a closing-trace factor, formed transposed: ZhT = conj(tau) Rr^T
@@ -396,7 +422,9 @@ void run_chunk_mx_sw(const EngineShape& E, const char* tau_r, const char* tau_t,
 extern "C" {
 
 // Engine entry points; see include/lapH_contract.h.
-static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt, int nv, int np,
+// sets = false: d_phi [np][nt][nv][nv], back-to-back pions P1 = P4 = Phi, P2 = P3 = Phi^dag.
+// sets = true:  d_phi [np][4][nt][nv][nv], the four elementals P1..P4 of each of np momentum sets.
+static int engine_impl(bool mixed, bool sets, const void* d_tau, const void* d_phi, int nt, int nv, int np,
                        int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                        void* d_corr, void* d_avg) {
   try {
@@ -410,11 +438,14 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     // mixed form: tau and Phi split twice (row-major and transposed planes, 12 B per complex
     // each) + FP64 per-slice scale factors, ahead of the scratch area
     // the fused-sandwich form (Nv = 64) also keeps tau in FP32 for its closing traces
-    const bool sw = mixed && nv == 64;
+    // (the sandwich chain pairs Phi with Phi^dag through shared passes: back-to-back only)
+    const bool sw = mixed && nv == 64 && !sets;
+    const int nel = sets ? 4 : 1;  // elemental arrays per momentum item
+    const long long nphi = (long long)np * nel * nt;  // elemental matrices
     const size_t tau_split = mixed ? (((size_t)nt * nt * E.blk * 12 + 255) & ~size_t(255)) : 0;
     const size_t tau_f32 = sw ? (((size_t)nt * nt * E.blk * 8 + 255) & ~size_t(255)) : 0;
-    const size_t phi_split = mixed ? (((size_t)np * nt * E.m * 12 + 255) & ~size_t(255)) : 0;
-    const size_t scale_bytes = mixed ? (((size_t)(nt * nt + np * nt) * 8 + 255) & ~size_t(255)) : 0;
+    const size_t phi_split = mixed ? (((size_t)nphi * E.m * 12 + 255) & ~size_t(255)) : 0;
+    const size_t scale_bytes = mixed ? (((size_t)(nt * nt + nphi) * 8 + 255) & ~size_t(255)) : 0;
     const size_t head = 2 * tau_split + tau_f32 + 2 * phi_split + scale_bytes;
     const size_t per_p = sw      ? mx_sw_bytes_per_momentum(E) + 8 * 256
                          : mixed ? mx_bytes_per_momentum(E) + 20 * 256
@@ -428,6 +459,16 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
     char* base = static_cast<char*>(ctx().scratch.p);
     hipStream_t s = ctx().stream;
     char* scratch = base + head;
+    // roles P1..P4 over arrays r (and t) whose item p sits at p * stride elements
+    auto roles = [&](const char* r, const char* t, size_t esz, Role* P) {
+      for (int k = 0; k < 4; k++) {
+        if (sets)
+          P[k] = Role{r + (size_t)k * nt * E.m * esz, t ? t + (size_t)k * nt * E.m * esz : nullptr, false,
+                      4LL * nt * E.m};
+        else
+          P[k] = Role{r, t, k == 1 || k == 2, (long long)nt * E.m};
+      }
+    };
     auto call = [&] {
       double* scales = nullptr;
       if (mixed) {
@@ -441,31 +482,40 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
         launch([=](hipStream_t st) { return lc_launch_split_rescale(d_tau, tau_r, tau_t, tau32, blk, nv, (long long)nt * nt, scales, st); },
                "split tau");
         launch([=](hipStream_t st) {
-                 return lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, m, nv, (long long)np * nt,
-                                                scales + (size_t)nt * nt, st);
+                 return lc_launch_split_rescale(d_phi, phi_r, phi_t, nullptr, m, nv, nphi, scales + (size_t)nt * nt,
+                                                st);
                },
                "split phi");
+        Role P[4];
+        roles(phi_r, phi_t, 12, P);
         for (int p0 = 0; p0 < np; p0 += pc) {
           const int n = (np - p0) < pc ? (np - p0) : pc;
           if (sw) run_chunk_mx_sw(E, tau_r, tau_t, tau32, phi_r, phi_t, p0, n, scratch, d_raw);
-          else run_chunk_mx(E, tau_r, tau_t, phi_r, phi_t, p0, n, scratch, d_raw);
+          else run_chunk_mx(E, tau_r, tau_t, P, p0, n, scratch, d_raw);
         }
       } else {
+        Role P[4];
+        roles(static_cast<const char*>(d_phi), nullptr, 16, P);
         for (int p0 = 0; p0 < np; p0 += pc) {
           const int n = (np - p0) < pc ? (np - p0) : pc;
-          run_chunk(E, Program::F64, d_tau, d_phi, p0, n, scratch, d_raw);
+          run_chunk(E, Program::F64, d_tau, P, p0, n, scratch, d_raw);
         }
       }
+      // per-role scale factors of the elementals: item p, time t at s_p[k][p * ps + t]
       const double* s_phi = scales ? scales + (size_t)nt * nt : nullptr;
+      const double* sp[4];
+      for (int k = 0; k < 4; k++) sp[k] = s_phi ? s_phi + (sets ? (size_t)k * nt : 0) : nullptr;
+      const long long ps = (long long)nel * nt;
       launch([=](hipStream_t st) {
-               return lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales, s_phi, st);
+               return lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, scales, sp, ps,
+                                                st);
              },
              "engine epilogue");
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     };
     char key[160], gkey[320];
-    std::snprintf(key, sizeof key, "rho_pipi nt=%d nv=%d np=%d src0=%d step=%d nsrc=%d chunk=%d", nt, nv, np, src0,
-                  src_step, nsrc, pc);
+    std::snprintf(key, sizeof key, "%s nt=%d nv=%d np=%d src0=%d step=%d nsrc=%d chunk=%d",
+                  sets ? "pipi_sets" : "rho_pipi", nt, nv, np, src0, src_step, nsrc, pc);
     std::snprintf(gkey, sizeof gkey, "%s mx=%d tau=%p phi=%p raw=%p corr=%p avg=%p", key, mixed ? 1 : 0, d_tau, d_phi,
                   d_raw, d_corr, d_avg);
     const std::string graph_key(gkey);
@@ -483,14 +533,27 @@ static int engine_impl(bool mixed, const void* d_tau, const void* d_phi, int nt,
 int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                        void* d_corr, void* d_avg) {
-  return engine_impl(false, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw,
+  return engine_impl(false, false, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw,
                      d_corr, d_avg);
 }
 
 int lc_engine_rho_pipi_mixed(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
                              int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                              void* d_corr, void* d_avg) {
-  return engine_impl(true, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw,
+  return engine_impl(true, false, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw,
+                     d_corr, d_avg);
+}
+
+int lc_engine_pipi_sets(const void* d_tau, const void* d_elem, int nt, int nv, int nset, int src0, int src_step,
+                        int nsrc, long long max_scratch_bytes, void* d_raw, void* d_corr, void* d_avg) {
+  return engine_impl(false, true, d_tau, d_elem, nt, nv, nset, src0, src_step, nsrc, max_scratch_bytes, d_raw,
+                     d_corr, d_avg);
+}
+
+int lc_engine_pipi_sets_mixed(const void* d_tau, const void* d_elem, int nt, int nv, int nset, int src0,
+                              int src_step, int nsrc, long long max_scratch_bytes, void* d_raw, void* d_corr,
+                              void* d_avg) {
+  return engine_impl(true, true, d_tau, d_elem, nt, nv, nset, src0, src_step, nsrc, max_scratch_bytes, d_raw,
                      d_corr, d_avg);
 }
 

@@ -393,10 +393,18 @@ __global__ void lc_mfma_selftest_kernel(const T* a, const T* b, T* d) {
 // factors of the matrices it contains: s(tf,t0) of tau, s(t,t) of D(t) = tau(t,t), s(p,t) of
 // Phi. rho, star and fish traces hold tau^2 Phi(tf) Phi(t0); square and doubleTriangle
 // tau^2 D(tf) D(t0) Phi(tf)^2 Phi(t0)^2; doubleTriangleHor tau^4 Phi(tf)^2 Phi(t0)^2.
+// Scale factors (mixed form) of each raw value: per-slice scales of tau (s_tau [t][t0]) and of the
+// four elemental roles P1..P4 (s_p[k][item * ps + t]); which factors a diagram carries follows
+// its chain: rho and star1 P4(tf) tau P1(t0) tau^dag, star2 P3 tau P2 tau^dag, fish1 P3 . P1,
+// fish2 P4 . P2, square and dT all four with D(tf) and D(t0), dTHor all four with four tau.
+struct LcRoleScales {
+  const double* s[4];
+  long long ps;
+};
+
 __global__ void lc_engine_combine_kernel(const double2* __restrict__ raw, double2* __restrict__ corr,
                                          long long n, int nsrc, int nt, int src0, int src_step,
-                                         const double* __restrict__ s_tau,
-                                         const double* __restrict__ s_phi) {
+                                         const double* __restrict__ s_tau, LcRoleScales sp) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double2* r = raw + i * 10;
@@ -411,14 +419,16 @@ __global__ void lc_engine_combine_kernel(const double2* __restrict__ raw, double
     const int t0 = src0 + k * src_step;
     const double st = s_tau[(long long)tf * nt + t0];
     const double dtf = s_tau[(long long)tf * nt + tf], dt0 = s_tau[(long long)t0 * nt + t0];
-    const double pf = s_phi[p * nt + tf], p0 = s_phi[p * nt + t0];
-    const double f1 = st * st * pf * p0;
-    const double f2 = st * st * dtf * dt0 * pf * pf * p0 * p0;
-    const double f3 = st * st * st * st * pf * pf * p0 * p0;
-    f[0] = f[1] = f[2] = f1;
-    f[3] = f[4] = f2;
-    f[5] = f3;
-    f[6] = f[7] = f[8] = f[9] = f1;
+    const double s1 = sp.s[0][p * sp.ps + t0], s2 = sp.s[1][p * sp.ps + t0];
+    const double s3 = sp.s[2][p * sp.ps + tf], s4 = sp.s[3][p * sp.ps + tf];
+    const double t2 = st * st, all4 = s1 * s2 * s3 * s4;
+    f[0] = f[1] = f[2] = t2 * s4 * s1;
+    f[3] = f[4] = t2 * dtf * dt0 * all4;
+    f[5] = t2 * t2 * all4;
+    f[6] = t2 * s4 * s1;
+    f[7] = t2 * s3 * s2;
+    f[8] = t2 * s3 * s1;
+    f[9] = t2 * s4 * s2;
   }
   double2 v[10];
 #pragma unroll
@@ -550,13 +560,17 @@ extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, cons
   return hipGetLastError();
 }
 
+// s_p: scales of the roles P1..P4 (item p, time t at s_p[k][p * ps + t]); unused without s_tau.
 extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
                                                 int nsrc, int nt, int src0, int src_step,
-                                                const double* s_tau, const double* s_phi,
-                                                hipStream_t s) {
+                                                const double* s_tau, const double* const* s_p,
+                                                long long ps, hipStream_t s) {
   const long long n = (long long)np * nsrc * nt;
+  lc::LcRoleScales sp;
+  for (int k = 0; k < 4; k++) sp.s[k] = s_tau ? s_p[k] : nullptr;
+  sp.ps = ps;
   hipLaunchKernelGGL(lc::lc_engine_combine_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     (const double2*)raw, (double2*)corr, n, nsrc, nt, src0, src_step, s_tau, s_phi);
+                     (const double2*)raw, (double2*)corr, n, nsrc, nt, src0, src_step, s_tau, sp);
   const int m = np * 8 * nt;
   hipLaunchKernelGGL(lc::lc_engine_average_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
                      (const double2*)corr, (double2*)avg, np, nsrc, nt, 8, src0, src_step);

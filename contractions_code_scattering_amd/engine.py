@@ -16,6 +16,8 @@ N_CORR = len(CORR_NAMES)
 _VP, _I, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong
 register("lc_engine_rho_pipi", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
 register("lc_engine_rho_pipi_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
+register("lc_engine_pipi_sets", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
+register("lc_engine_pipi_sets_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP, _VP])
 register("lc_engine_sigma_set", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
 register("lc_engine_sigma_set_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
 register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
@@ -58,6 +60,40 @@ def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=N
             raise ValueError("output buffers have the wrong shape")
     fn = lib().lc_engine_rho_pipi_mixed if mixed else lib().lc_engine_rho_pipi
     check(fn(tau.ptr, phi.ptr, nt, nv, np_, src0, src_step, nsrc, max_scratch_bytes,
+             raw.ptr, corr.ptr, avg.ptr))
+    return raw, corr, avg
+
+
+def pipi_sets(tau, elem, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None, mixed=False):
+    """Pi-pi over explicit momentum sets (lc_engine_pipi_sets[_mixed]).
+    tau: DeviceArray complex128 [nt][nt][4][4][nv][nv]; elem: complex128 [nset][4][nt][nv][nv],
+    the elementals P1, P2 (source) and P3, P4 (sink) of each set, the momP1..momP4 of
+    contractPiPi (lib/contract.cpp:748). Returns (raw, corr, avg) as rho_pipi with np = nset."""
+    import numpy as np
+    from .devmem import DeviceArray
+    if not (isinstance(tau, DeviceArray) and isinstance(elem, DeviceArray)):
+        raise TypeError("engine operands must be DeviceArrays")
+    if tau.dtype != np.complex128 or elem.dtype != np.complex128:
+        raise TypeError("engine operands are complex128")
+    nt, nv = tau.shape[0], tau.shape[-1]
+    if tuple(tau.shape) != (nt, nt, 4, 4, nv, nv):
+        raise ValueError("tau must be [nt][nt][4][4][nv][nv]")
+    ns_ = elem.shape[0]
+    if tuple(elem.shape) != (ns_, 4, nt, nv, nv):
+        raise ValueError("elem must be [nset][4][nt][nv][nv]")
+    if nsrc is None:
+        nsrc = (nt - src0 + src_step - 1) // src_step
+    if out is None:
+        raw = DeviceArray((ns_, nsrc, nt, 10), np.complex128)
+        corr = DeviceArray((ns_, nsrc, nt, N_CORR), np.complex128)
+        avg = DeviceArray((ns_, N_CORR, nt), np.complex128)
+    else:
+        raw, corr, avg = out
+        if (raw.shape != (ns_, nsrc, nt, 10) or corr.shape != (ns_, nsrc, nt, N_CORR)
+                or avg.shape != (ns_, N_CORR, nt)):
+            raise ValueError("output buffers have the wrong shape")
+    fn = lib().lc_engine_pipi_sets_mixed if mixed else lib().lc_engine_pipi_sets
+    check(fn(tau.ptr, elem.ptr, nt, nv, ns_, src0, src_step, nsrc, max_scratch_bytes,
              raw.ptr, corr.ptr, avg.ptr))
     return raw, corr, avg
 

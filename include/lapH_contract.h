@@ -102,6 +102,25 @@ int lc_engine_rho_pipi_mixed(const void* d_tau, const void* d_phi, int nt, int n
                              int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                              void* d_corr, void* d_avg);
 
+/* Batched pi-pi over explicit momentum sets: the engine of lc_engine_rho_pipi for arbitrary
+ * elementals P1..P4 per set, the four Mom_matrix arguments of
+ * Contract_pipi_pipi::contractPiPi (lib/contract.cpp:748-779; the driver's momentum
+ * combinations px1..px4, tests/compute_pipi_I0.cpp:130-150), batched over every set, sink and
+ * source. P1, P2 sit at the source t0 and P3, P4 at the sink tf, as in run_ContractPiPi
+ * (lib/contract_kernels_pipi.cu:521).
+ *   d_elem [nset][4][nt][nv][nv]   {P1, P2, P3, P4} of each set, complex double
+ *   d_raw / d_corr / d_avg         as lc_engine_rho_pipi, with np = nset
+ * The pi-pi slots (square, dT, dTHor, star, fish) are the unweighted diagrams of the set; the
+ * rho slots hold -Tr[(P4(tf) tau)^dag (g5 g_i) (tau P1(t0)) (-g_i g5)]. */
+int lc_engine_pipi_sets(const void* d_tau, const void* d_elem, int nt, int nv, int nset, int src0,
+                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw, void* d_corr,
+                        void* d_avg);
+
+/* Mixed-precision form of lc_engine_pipi_sets (as lc_engine_rho_pipi_mixed). */
+int lc_engine_pipi_sets_mixed(const void* d_tau, const void* d_elem, int nt, int nv, int nset, int src0,
+                              int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
+                              void* d_corr, void* d_avg);
+
 /* Batched sigma / sigma-pi / rho-pi set (no reference counterpart: the reference forms these one
  * sink row at a time through run_ContractSigma, run_ContractSigmaPi_triangle, run_ContractRhoPi,
  * run_ContractLoopUnity and run_ContractSigmaPi_fish_hor). On a resident perambulator, for every

@@ -1,0 +1,121 @@
+"""GPU parity of the pi-pi engine over explicit momentum sets (lc_engine_pipi_sets[_mixed]).
+
+Each set carries four independent elementals P1, P2 (source) and P3, P4 (sink), the momP1..momP4
+of contractPiPi (lib/contract.cpp:748). Checks:
+* the pi-pi slots against the oracle's restatement of run_ContractPiPi_I0 (weights removed) for
+  every (set, sink, source);
+* the rho slots, -Tr[(P4(tf) tau)^dag (g5 g_i) (tau P1(t0)) (-g_i g5)], against the oracle's dense
+  brute force for every pair with tf != t0 (the brute force takes one elemental array for both
+  ends, which the diagonal would need to hold both P1(t) and P4(t));
+* sets fed {Phi, Phi^dag, Phi^dag, Phi} against the back-to-back engine, all eight slots.
+FP64 bar 1e-12 relative per array; mixed 3e-6 normwise against FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from contractions_code_scattering_amd import inputs as I
+from conftest import elemerr, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _sets(seed, nt, nv, nset, spread=0.0):
+    """[nset][4][nt][nv][nv] independent elementals; spread > 0 scales each (role, time) slice by
+    10**u, u uniform in [-spread, spread], so that per-role scale factors matter."""
+    g = I.rng(seed)
+    el = np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(4)]) for _ in range(nset)])
+    if spread:
+        el = el * (10.0 ** g.uniform(-spread, spread, size=(nset, 4, nt, 1, 1)))
+    return el
+
+
+def _oracle_pipi(tau, el, src):
+    """[nset][nsrc][nt][5] {square, dT, dTHor, star, fish} from the oracle's I0 rows."""
+    nt = tau.shape[0]
+    diag = O.diag_of(tau)
+    out = np.zeros((el.shape[0], len(src), nt, 5), complex)
+    w = np.array([-3.0, -3.0, -3.0 / 5.0, 1.0, 1.0])
+    for s in range(el.shape[0]):
+        P1, P2, P3, P4 = el[s]
+        for tf in range(nt):
+            q = O.pipi_row(0, tau[tf], diag, P1, P2, P3, P4, tf)  # [ti][sq, dT, star, fish, hor]
+            for k, t0 in enumerate(src):
+                out[s, k, tf] = q[t0, [0, 1, 4, 2, 3]] * w
+    return out
+
+
+@pytest.mark.parametrize("nt,nv,nset,src0,step", [(3, 4, 2, 0, 1), (4, 8, 1, 1, 2), (3, 16, 2, 0, 1),
+                                                   (2, 64, 1, 0, 1)])
+def test_sets_vs_oracle(lc, nt, nv, nset, src0, step):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    g = I.rng(70 + nv)
+    tau = I.perambulator(g, nt, nv, 0.3)
+    el = _sets(71 + nv, nt, nv, nset)
+    src = list(range(src0, nt, step))
+    print("pipi_sets FP64 nt=%d nv=%d nset=%d src0=%d step=%d" % (nt, nv, nset, src0, step), flush=True)
+    _, corr, avg = engine.pipi_sets(to_device(tau), to_device(el), src0=src0, src_step=step)
+    corr, avg = corr.to_numpy(), avg.to_numpy()
+    ref = _oracle_pipi(tau, el, src)
+    for s in range(nset):
+        for d in range(5):
+            assert relerr(corr[s, :, :, 3 + d], ref[s, :, :, d]) < 1e-12, (s, d)
+            assert elemerr(corr[s, :, :, 3 + d], ref[s, :, :, d]) < 1e-12, (s, d)
+        # rho slots against the dense brute force, off the diagonal tf = t0
+        P1, P4 = el[s, 0], el[s, 3]
+        for k, t0 in enumerate(src):
+            for tf in range(nt):
+                if tf == t0:
+                    continue
+                mom5 = np.zeros((nt, 5, nv, nv), complex)
+                mom5[t0, 0] = P1[t0]
+                mom5[tf, 0] = P4[tf]
+                r = O.dense.rho(tau[tf, t0], mom5, tf, t0)[0]
+                assert relerr(corr[s, k, tf, 0:3], r) < 1e-12, (s, k, tf)
+        for d in range(8):
+            c = np.array([np.mean([corr[s, k, (t0 + dt) % nt, d] for k, t0 in enumerate(src)])
+                          for dt in range(nt)])
+            assert relerr(avg[s, d], c) < 1e-13
+
+
+@pytest.mark.parametrize("nt,nv", [(3, 8), (2, 64)])
+def test_back_to_back_sets_equal_engine(lc, nt, nv):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    g = I.rng(80)
+    tau = I.perambulator(g, nt, nv, 0.3)
+    phi = np.stack([I.random_elemental(g, nt, nv) for _ in range(2)])
+    el = np.stack([np.stack([P, I.dagger(P), I.dagger(P), P]) for P in phi])
+    t = to_device(tau)
+    _, c_b2b, _ = engine.rho_pipi(t, to_device(phi))
+    _, c_set, _ = engine.pipi_sets(t, to_device(el))
+    assert relerr(c_set.to_numpy(), c_b2b.to_numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("nt,nv,nset", [(3, 16, 2), (2, 64, 2)])
+def test_sets_mixed(lc, nt, nv, nset):
+    """Mixed form against FP64, with elementals whose (role, time) slices differ in scale by up to
+    10^3, so that every per-role scale factor is exercised."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    g = I.rng(90 + nv)
+    tau = I.perambulator(g, nt, nv, 0.3)
+    el = _sets(91 + nv, nt, nv, nset, spread=1.5)
+    t, e = to_device(tau), to_device(el)
+    print("pipi_sets mixed nt=%d nv=%d nset=%d" % (nt, nv, nset), flush=True)
+    _, c64, a64 = engine.pipi_sets(t, e)
+    _, cmx, amx = engine.pipi_sets(t, e, mixed=True)
+    c64, cmx = c64.to_numpy(), cmx.to_numpy()
+    for d in range(8):
+        assert relerr(cmx[..., d], c64[..., d]) < 3e-6, (d, relerr(cmx[..., d], c64[..., d]))
+    assert relerr(amx.to_numpy(), a64.to_numpy()) < 3e-6
+
+
+def test_sets_rejects_bad_shapes(lc):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    g = I.rng(99)
+    tau = I.perambulator(g, 2, 4, 0.3)
+    with pytest.raises(ValueError, match="nset"):
+        engine.pipi_sets(to_device(tau), to_device(np.zeros((1, 3, 2, 4, 4), complex)))
