@@ -340,6 +340,11 @@ void triangle_row(const Shape& S, const void* row, const void* diag, const void*
 extern "C" hipError_t lc_launch_phi(const void* V, const void* ph, int nt, int nv, long long nsite, int np, int nks,
                                     void* part, void* phi, hipStream_t s);
 extern "C" long long lc_phi_partial_bytes(int nt, int nv, int np, int nks);
+extern "C" hipError_t lc_launch_phi_bilinear(const void* X, const void* Y, const void* ph, int nt, int nv,
+                                             long long nsite, int np, int nks, void* part, double alpha,
+                                             int accumulate, void* phi, long long pstride, hipStream_t s);
+extern "C" hipError_t lc_launch_covderiv(const void* U, const void* V, int nt, int nv, int nx, int ny, int nz,
+                                         void* dV, hipStream_t s);
 
 extern "C" {
 
@@ -584,6 +589,18 @@ int lc_synchronize(void) {
   });
 }
 
+// K slices of the elemental GEMMs: enough to give every CU several workgroups, at least 256
+// entries per slice; the partial-tile buffer sized for them.
+static int phi_slices(int nt, int nv, int np, long long nsite) {
+  const long long tiles = (nv + 63) / 64;
+  const long long tiles_total = (long long)np * nt * tiles * tiles;
+  long long nks = (2048 + tiles_total - 1) / tiles_total;
+  if (nks > nsite / 256) nks = nsite / 256;
+  if (nks < 1) nks = 1;
+  ctx().partials.ensure((size_t)lc_phi_partial_bytes(nt, nv, np, (int)nks));
+  return (int)nks;
+}
+
 int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv, long long nsite, int np,
                         void* d_phi) {
   return guarded([&] {
@@ -591,15 +608,55 @@ int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv,
     if (nt < 1 || nv < 1 || nv > 1024 || np < 1 || nsite < 1)
       fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d np=%d nsite=%lld", nt, nv, np, nsite);
     check_ptr(d_evec, "d_evec"); check_ptr(d_phase, "d_phase"); check_ptr(d_phi, "d_phi");
-    // enough K slices to give every CU several workgroups; at least 256 entries per slice
-    const long long tiles = (nv + 63) / 64;
-    const long long tiles_total = (long long)np * nt * tiles * tiles;
-    long long nks = (2048 + tiles_total - 1) / tiles_total;
-    if (nks > nsite / 256) nks = nsite / 256;
-    if (nks < 1) nks = 1;
-    ctx().partials.ensure((size_t)lc_phi_partial_bytes(nt, nv, np, (int)nks));
+    const int nks = phi_slices(nt, nv, np, nsite);
     hipStream_t s = ctx().stream;
-    hip_check(lc_launch_phi(d_evec, d_phase, nt, nv, nsite, np, (int)nks, ctx().partials.p, d_phi, s), "phi launch");
+    hip_check(lc_launch_phi(d_evec, d_phase, nt, nv, nsite, np, nks, ctx().partials.p, d_phi, s), "phi launch");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+int lc_covariant_derivative(const void* d_links, const void* d_evec, int nt, int nv, int nx, int ny, int nz,
+                            void* d_dvec) {
+  return guarded([&] {
+    require_init();
+    if (nt < 1 || nv < 1 || nv > 1024 || nx < 1 || ny < 1 || nz < 1 ||
+        3LL * nt * nv * nx * ny * nz > 0xffffffffLL * 256)
+      fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d lattice %dx%dx%d", nt, nv, nx, ny, nz);
+    check_ptr(d_links, "d_links"); check_ptr(d_evec, "d_evec"); check_ptr(d_dvec, "d_dvec");
+    hipStream_t s = ctx().stream;
+    hip_check(lc_launch_covderiv(d_links, d_evec, nt, nv, nx, ny, nz, d_dvec, s), "covariant derivative launch");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+int lc_build_elementals_types(const void* d_evec, const void* d_dvec, const void* d_phase, int nt, int nv,
+                              long long nsite, int np, void* d_mom5) {
+  return guarded([&] {
+    require_init();
+    if (nt < 1 || nv < 1 || nv > 1024 || np < 1 || nsite < 1)
+      fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d np=%d nsite=%lld", nt, nv, np, nsite);
+    check_ptr(d_evec, "d_evec"); check_ptr(d_dvec, "d_dvec"); check_ptr(d_phase, "d_phase");
+    check_ptr(d_mom5, "d_mom5");
+    const int nks = phi_slices(nt, nv, np, nsite);
+    hipStream_t s = ctx().stream;
+    void* part = ctx().partials.p;
+    const size_t vec_bytes = (size_t)nt * nv * nsite * 16;  // one vector set [nt][nv][nsite]
+    const size_t type_bytes = (size_t)nt * nv * nv * 16;    // one type [nt][nv][nv]
+    const long long pstride = 5LL * nt;                     // matrices per momentum in d_mom5
+    auto dv = [&](int j) { return static_cast<const char*>(d_dvec) + j * vec_bytes; };
+    auto type = [&](int k) { return static_cast<char*>(d_mom5) + k * type_bytes; };
+    auto bil = [&](const void* X, const void* Y, double alpha, int acc, int k) {
+      hip_check(lc_launch_phi_bilinear(X, Y, d_phase, nt, nv, nsite, np, nks, part, alpha, acc, type(k), pstride, s),
+                "elemental launch");
+    };
+    // type 0: V^dag ph V; type 1: sum_j (nabla_j V)^dag ph (nabla_j V);
+    // type 2 + j: V^dag ph (nabla_j V) - (nabla_j V)^dag ph V
+    bil(d_evec, d_evec, 1.0, 0, 0);
+    for (int j = 0; j < 3; j++) bil(dv(j), dv(j), 1.0, j > 0, 1);
+    for (int j = 0; j < 3; j++) {
+      bil(d_evec, dv(j), 1.0, 0, 2 + j);
+      bil(dv(j), d_evec, -1.0, 1, 2 + j);
+    }
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   });
 }

@@ -11,13 +11,23 @@
 // ph_p[k] V_t[c][k]. A workgroup (4 wave64, 2 x 2) owns a 64 x 64 tile of Phi and one K slice,
 // staging 16-wide K chunks through double-buffered LDS (the phase is applied while staging), and
 // writes its partial tile; a second kernel sums the K slices in a fixed order.
+//
+// The kernels take the left and right vector sets separately, X^dag ph Y, so that the derivative
+// types of the drivers (tests/compute_rho.cpp:131-202) are sums of such bilinears over the
+// covariantly differentiated eigenvectors; the reduction adds alpha * (X^dag ph Y) into, or in place
+// of, its destination.
+//
+// The covariant derivative (lc_covderiv_kernel) is a nearest-neighbour stencil on the spatial
+// lattice of one time slice: (nabla_j v)(x) = 1/2 [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)],
+// periodic, one thread per (direction, time, vector, site) with its three colours.
 #include "lc_device.h"
 
 namespace lc {
 
 constexpr int PH_TM = 64, PH_KT = 16;
 
-__global__ void __launch_bounds__(256) lc_phi_partial_kernel(const double2* __restrict__ V,
+__global__ void __launch_bounds__(256) lc_phi_partial_kernel(const double2* __restrict__ X,
+                                                             const double2* __restrict__ Y,
                                                              const double2* __restrict__ ph, int nt, int nv,
                                                              long long nsite, int tiles, int nks,
                                                              long long kslice, double2* __restrict__ part) {
@@ -35,7 +45,8 @@ __global__ void __launch_bounds__(256) lc_phi_partial_kernel(const double2* __re
   const long long k_end = k_begin + kslice < nsite ? k_begin + kslice : nsite;
   const int nchunk = k_end > k_begin ? (int)((k_end - k_begin + PH_KT - 1) / PH_KT) : 0;
 
-  const double2* Vt = V + (long long)t * nv * nsite;
+  const double2* Xt = X + (long long)t * nv * nsite;
+  const double2* Yt = Y + (long long)t * nv * nsite;
   const double2* php = ph + p * nsite;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -50,9 +61,9 @@ __global__ void __launch_bounds__(256) lc_phi_partial_kernel(const double2* __re
     for (int e = 0; e < 4; e++) {
       const long long k = k0 + e;
       const bool kin = k < k_end;
-      ra[e] = (kin && r < nv) ? Vt[(long long)r * nsite + k] : double2{0.0, 0.0};
+      ra[e] = (kin && r < nv) ? Xt[(long long)r * nsite + k] : double2{0.0, 0.0};
       if (kin && c < nv) {
-        const double2 v = Vt[(long long)c * nsite + k], f = php[k];
+        const double2 v = Yt[(long long)c * nsite + k], f = php[k];
         rb[e] = double2{f.x * v.x - f.y * v.y, f.x * v.y + f.y * v.x};
       } else {
         rb[e] = double2{0.0, 0.0};
@@ -63,7 +74,7 @@ __global__ void __launch_bounds__(256) lc_phi_partial_kernel(const double2* __re
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       As_re[buf][s_k + e][s_line] = ra[e].x;
-      As_im[buf][s_k + e][s_line] = -ra[e].y;  // conj(V)
+      As_im[buf][s_k + e][s_line] = -ra[e].y;  // conj(X)
       Bs_re[buf][s_k + e][s_line] = rb[e].x;
       Bs_im[buf][s_k + e][s_line] = rb[e].y;
     }
@@ -124,9 +135,12 @@ __global__ void __launch_bounds__(256) lc_phi_partial_kernel(const double2* __re
       }
 }
 
-// Phi[item][r][c] = sum over K slices of the partial tiles, in slice order.
+// Phi[dst(item)][r][c] (+)= alpha * sum over K slices of the partial tiles, in slice order; item =
+// p * nt + t goes to matrix p * pstride + t of phi.
 __global__ void __launch_bounds__(256) lc_phi_reduce_kernel(const double2* __restrict__ part, int nv, int tiles,
-                                                            int nks, long long items, double2* __restrict__ phi) {
+                                                            int nks, long long items, int nt, long long pstride,
+                                                            double alpha, int accumulate,
+                                                            double2* __restrict__ phi) {
   const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
   const long long nv2 = (long long)nv * nv;
   if (gid >= items * nv2) return;
@@ -142,7 +156,60 @@ __global__ void __launch_bounds__(256) lc_phi_reduce_kernel(const double2* __res
     sr += v.x;
     si += v.y;
   }
-  phi[gid] = double2{sr, si};
+  const long long ip = item / nt;
+  double2* d = phi + (ip * pstride + (item - ip * nt)) * nv2 + e;
+  if (accumulate) {
+    const double2 o = *d;
+    *d = double2{o.x + alpha * sr, o.y + alpha * si};
+  } else {
+    *d = double2{alpha * sr, alpha * si};
+  }
+}
+
+// dV[j][t][c][3 * site + a] = (nabla_j V[t][c])(site)_a; U [nt][3][nsite][3][3] row-major SU(3)
+// links U_j(x) (colour index a, b), V [nt][nv][3 * nsite], sites x + nx (y + ny z).
+__global__ void __launch_bounds__(256) lc_covderiv_kernel(const double2* __restrict__ U,
+                                                          const double2* __restrict__ V, int nt, int nv, int nx,
+                                                          int ny, int nz, double2* __restrict__ dV) {
+  const long long nsite = (long long)nx * ny * nz;
+  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long per_j = (long long)nt * nv * nsite;
+  if (gid >= 3 * per_j) return;
+  const int j = (int)(gid / per_j);
+  const long long rest = gid - j * per_j;
+  const long long vec = rest / nsite;  // t * nv + c
+  const long long x = rest - vec * nsite;
+  const int t = (int)(vec / nv);
+  const int ix = (int)(x % nx), iy = (int)((x / nx) % ny), iz = (int)(x / ((long long)nx * ny));
+  int fx = ix, fy = iy, fz = iz, bx = ix, by = iy, bz = iz;
+  if (j == 0) { fx = ix + 1 == nx ? 0 : ix + 1; bx = ix == 0 ? nx - 1 : ix - 1; }
+  if (j == 1) { fy = iy + 1 == ny ? 0 : iy + 1; by = iy == 0 ? ny - 1 : iy - 1; }
+  if (j == 2) { fz = iz + 1 == nz ? 0 : iz + 1; bz = iz == 0 ? nz - 1 : iz - 1; }
+  const long long xf = fx + (long long)nx * (fy + (long long)ny * fz);
+  const long long xb = bx + (long long)nx * (by + (long long)ny * bz);
+  const double2* v = V + vec * 3 * nsite;
+  const double2* Uf = U + (((long long)t * 3 + j) * nsite + x) * 9;   // U_j(x)
+  const double2* Ub = U + (((long long)t * 3 + j) * nsite + xb) * 9;  // U_j(x - j)
+  double2 vf[3], vb[3];
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    vf[b] = v[3 * xf + b];
+    vb[b] = v[3 * xb + b];
+  }
+  double2* out = dV + (long long)j * per_j * 3 + (vec * nsite + x) * 3;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double re = 0, im = 0;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const double2 f = Uf[3 * a + b], g = Ub[3 * b + a];  // U_j(x)_ab, U_j(x - j)_ba
+      re += f.x * vf[b].x - f.y * vf[b].y;
+      im += f.x * vf[b].y + f.y * vf[b].x;
+      re -= g.x * vb[b].x + g.y * vb[b].y;  // conj(g) * vb
+      im -= g.x * vb[b].y - g.y * vb[b].x;
+    }
+    out[a] = double2{0.5 * re, 0.5 * im};
+  }
 }
 
 }  // namespace lc
@@ -153,17 +220,32 @@ extern "C" long long lc_phi_partial_bytes(int nt, int nv, int np, int nks) {
   return (long long)np * nt * tiles * tiles * nks * lc::PH_TM * lc::PH_TM * 16;
 }
 
-extern "C" hipError_t lc_launch_phi(const void* V, const void* ph, int nt, int nv, long long nsite, int np, int nks,
-                                    void* part, void* phi, hipStream_t s) {
+// phi[p * pstride + t] (+)= alpha * X_t^dag diag(ph_p) Y_t for every (p < np, t < nt).
+extern "C" hipError_t lc_launch_phi_bilinear(const void* X, const void* Y, const void* ph, int nt, int nv,
+                                             long long nsite, int np, int nks, void* part, double alpha,
+                                             int accumulate, void* phi, long long pstride, hipStream_t s) {
   const int tiles = (nv + lc::PH_TM - 1) / lc::PH_TM;
   const long long items = (long long)np * nt;
   long long kslice = (nsite + nks - 1) / nks;
   kslice = (kslice + lc::PH_KT - 1) / lc::PH_KT * lc::PH_KT;
   const long long nwg = items * tiles * tiles * nks;
-  hipLaunchKernelGGL(lc::lc_phi_partial_kernel, dim3((unsigned)nwg), dim3(256), 0, s, (const double2*)V,
-                     (const double2*)ph, nt, nv, nsite, tiles, nks, kslice, (double2*)part);
+  hipLaunchKernelGGL(lc::lc_phi_partial_kernel, dim3((unsigned)nwg), dim3(256), 0, s, (const double2*)X,
+                     (const double2*)Y, (const double2*)ph, nt, nv, nsite, tiles, nks, kslice, (double2*)part);
   const long long n = items * nv * nv;
   hipLaunchKernelGGL(lc::lc_phi_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     (const double2*)part, nv, tiles, nks, items, (double2*)phi);
+                     (const double2*)part, nv, tiles, nks, items, nt, pstride, alpha, accumulate, (double2*)phi);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_phi(const void* V, const void* ph, int nt, int nv, long long nsite, int np, int nks,
+                                    void* part, void* phi, hipStream_t s) {
+  return lc_launch_phi_bilinear(V, V, ph, nt, nv, nsite, np, nks, part, 1.0, 0, phi, nt, s);
+}
+
+extern "C" hipError_t lc_launch_covderiv(const void* U, const void* V, int nt, int nv, int nx, int ny, int nz,
+                                         void* dV, hipStream_t s) {
+  const long long n = 3LL * nt * nv * nx * ny * nz;
+  hipLaunchKernelGGL(lc::lc_covderiv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     (const double2*)U, (const double2*)V, nt, nv, nx, ny, nz, (double2*)dV);
   return hipGetLastError();
 }

@@ -22,6 +22,8 @@ register("lc_engine_sigma_set", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP,
 register("lc_engine_sigma_set_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL, _VP, _VP])
 register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
 register("lc_build_elementals", [_VP, _VP, _I, _I, _LL, _I, _VP])
+register("lc_covariant_derivative", [_VP, _VP, _I, _I, _I, _I, _I, _VP])
+register("lc_build_elementals_types", [_VP, _VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_profile", [_I])
 register("lc_set_tile_params", [_I])
 register("lc_set_dthor_dagger", [_I])
@@ -215,6 +217,56 @@ def build_elementals(evec, phase, out=None):
     if out is None:
         out = DeviceArray((np_, nt, nv, nv), np.complex128)
     check(lib().lc_build_elementals(evec.ptr, phase.ptr, nt, nv, nsite, np_, out.ptr))
+    return out
+
+
+def covariant_derivative(links, evec, nx, ny, nz, out=None):
+    """Covariant derivatives of the eigenvectors on the GPU (lc_covariant_derivative):
+    links [nt][3][nx*ny*nz][3][3], evec [nt][nv][3*nx*ny*nz] DeviceArrays (complex128) ->
+    DeviceArray [3][nt][nv][3*nx*ny*nz], (nabla_j v)(x) = 1/2 [U_j(x) v(x+j) - U_j(x-j)^dag v(x-j)]."""
+    import numpy as np
+    from .devmem import DeviceArray
+    if not (isinstance(links, DeviceArray) and isinstance(evec, DeviceArray)):
+        raise TypeError("operands must be DeviceArrays")
+    if links.dtype != np.complex128 or evec.dtype != np.complex128:
+        raise TypeError("operands are complex128")
+    nt, nv, nk = evec.shape
+    ns = nx * ny * nz
+    if nk != 3 * ns:
+        raise ValueError("evec must be [nt][nv][3*nx*ny*nz]")
+    if tuple(links.shape) != (nt, 3, ns, 3, 3):
+        raise ValueError("links must be [nt][3][nx*ny*nz][3][3]")
+    if out is None:
+        out = DeviceArray((3, nt, nv, nk), np.complex128)
+    elif tuple(out.shape) != (3, nt, nv, nk):
+        raise ValueError("out must be [3][nt][nv][3*nx*ny*nz]")
+    check(lib().lc_covariant_derivative(links.ptr, evec.ptr, nt, nv, nx, ny, nz, out.ptr))
+    return out
+
+
+def build_elementals_types(evec, dvec, phase, out=None):
+    """The drivers' five elemental types (lc_build_elementals_types): evec [nt][nv][nsite], dvec
+    [3][nt][nv][nsite] (covariant_derivative), phase [np][nsite] -> DeviceArray
+    [np][5][nt][nv][nv]: plain, smeared sum_j dV_j^dag ph dV_j, and V^dag ph dV_j - dV_j^dag ph V
+    (tests/compute_rho.cpp:116-229)."""
+    import numpy as np
+    from .devmem import DeviceArray
+    for x in (evec, dvec, phase):
+        if not isinstance(x, DeviceArray):
+            raise TypeError("operands must be DeviceArrays")
+        if x.dtype != np.complex128:
+            raise TypeError("operands are complex128")
+    nt, nv, nsite = evec.shape
+    np_ = phase.shape[0]
+    if tuple(dvec.shape) != (3, nt, nv, nsite):
+        raise ValueError("dvec must be [3][nt][nv][nsite]")
+    if tuple(phase.shape) != (np_, nsite):
+        raise ValueError("phase must be [np][nsite] with the eigenvectors' nsite")
+    if out is None:
+        out = DeviceArray((np_, 5, nt, nv, nv), np.complex128)
+    elif tuple(out.shape) != (np_, 5, nt, nv, nv):
+        raise ValueError("out must be [np][5][nt][nv][nv]")
+    check(lib().lc_build_elementals_types(evec.ptr, dvec.ptr, phase.ptr, nt, nv, nsite, np_, out.ptr))
     return out
 
 

@@ -196,6 +196,27 @@ int lc_read_prop_device(const char* path, int nt, int nv, int prec, void* d_tau)
 int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv, long long nsite, int np,
                         void* d_phi);
 
+/* Covariant derivatives of the Laplacian eigenvectors, the drivers' lm.covderiv(j, evec, dl[j])
+ * (tests/compute_rho.cpp:140-141, from the external gwu-qcd library), on the GPU:
+ *   d_dvec[j][t][c][3x + a] = 1/2 [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)]_a,  v = d_evec[t][c],
+ * periodic in the spatial directions j = 0, 1, 2 (x, y, z). Sites x = ix + nx (iy + ny iz), colour
+ * fastest: d_evec [nt][nv][3 nx ny nz], d_links [nt][3][nx ny nz][3][3] (row-major SU(3) U_j(x)),
+ * d_dvec [3][nt][nv][3 nx ny nz], complex double. The stencil normalisation of gwu-qcd's covderiv is
+ * not known here: another overall factor c scales type 1 of lc_build_elementals_types by c^2 and
+ * types 2-4 by c. */
+int lc_covariant_derivative(const void* d_links, const void* d_evec, int nt, int nv, int nx, int ny, int nz,
+                            void* d_dvec);
+
+/* The five elemental types of the rho / rho-pi drivers (tests/compute_rho.cpp:116-229, the tmpMom
+ * layout) for np momenta, from eigenvectors and their covariant derivatives:
+ *   type 0      sum_k conj(V[r][k]) ph[k] V[c][k]                         (plain)
+ *   type 1      sum_j sum_k conj(dV_j[r][k]) ph[k] dV_j[c][k]              (smeared)
+ *   type 2 + j  sum_k conj(V[r][k]) ph[k] dV_j[c][k] - conj(dV_j[r][k]) ph[k] V[c][k]
+ * d_evec [nt][nv][nsite], d_dvec [3][nt][nv][nsite] (lc_covariant_derivative), d_phase [np][nsite],
+ * d_mom5 [np][5][nt][nv][nv], complex double, FP64 matrix cores. */
+int lc_build_elementals_types(const void* d_evec, const void* d_dvec, const void* d_phase, int nt, int nv,
+                              long long nsite, int np, void* d_mom5);
+
 /* Diagnostic: one v_mfma_f32_16x16x32_bf16 on A(16x32), B(32x16) given as FP32 values exact in
  * BF16; h_d as for lc_selftest_mfma (D row-major, then raw accumulators). */
 int lc_selftest_mfma_bf16(const float* h_a, const float* h_b, float* h_d);

@@ -284,6 +284,41 @@ def read_prop_file(path, nt, nv):
     return np.ascontiguousarray(b.transpose(2, 5, 0, 3, 1, 4))
 
 
+# ------------------------------------------------------------------ elemental types
+def covariant_derivative(U, V, nx, ny, nz):
+    """(nabla_j v)(x) = 1/2 [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)], periodic, j = x, y, z: the
+    symmetric lattice covariant derivative standing in for gwu-qcd's lapmult::covderiv, which the
+    drivers call (tests/compute_rho.cpp:140-141) and which is not part of the reference; its
+    normalisation is therefore not pinned to gwu-qcd. U [nt][3][nsite][3][3], V [nt][nv][3 nsite]
+    with sites x + nx (y + ny z), colour fastest; returns [3][nt][nv][3 nsite]."""
+    nt, nv = V.shape[:2]
+    v = V.reshape(nt, nv, nz, ny, nx, 3)
+    u = U.reshape(nt, 3, nz, ny, nx, 3, 3)
+    out = np.empty((3, nt, nv, nz, ny, nx, 3), complex)
+    for j, ax in enumerate((4, 3, 2)):          # x, y, z axes of v; of u[:, j] they are ax - 1
+        uj = u[:, j]
+        ub = np.roll(uj, 1, axis=ax - 1)        # U_j(x - j)
+        fwd = np.einsum("tzyxab,tczyxb->tczyxa", uj, np.roll(v, -1, axis=ax))
+        bwd = np.einsum("tzyxba,tczyxb->tczyxa", ub.conj(), np.roll(v, 1, axis=ax))
+        out[j] = 0.5 * (fwd - bwd)
+    return out.reshape(3, nt, nv, -1)
+
+
+def elemental_types(V, dV, ph):
+    """The drivers' five momentum-matrix types per momentum, tmpMom [type][t][r][c]
+    (tests/compute_rho.cpp:116-229; cscalar_product(a, b) = sum conj(a) b):
+    0 plain V^dag ph V (:116-129), 1 smeared sum_j dV_j^dag ph dV_j (:131-162),
+    2 + j commutator V^dag ph dV_j - dV_j^dag ph V (:164-202). Returns [np][5][nt][nv][nv]."""
+    def bil(X, Y):
+        return np.einsum("trk,pk,tck->ptrc", X.conj(), ph, Y)
+    out = np.empty((ph.shape[0], 5) + V.shape[:2] + (V.shape[1],), complex)
+    out[:, 0] = bil(V, V)
+    out[:, 1] = sum(bil(dV[j], dV[j]) for j in range(3))
+    for j in range(3):
+        out[:, 2 + j] = bil(V, dV[j]) - bil(dV[j], V)
+    return out
+
+
 # ------------------------------------------------------------------ dense brute force
 class dense:
     """Explicit (4Nv)x(4Nv) spin x eigenvector matrices; index (s, v) -> s*Nv + v."""
