@@ -11,6 +11,10 @@ Three independent pieces:
   restated in numpy below: elemental transposes, the per-sink-row loops, the source average of
   ``dumpData`` and its ``%+e`` text output, and the perambulator file permutation of
   ``readPropH``.
+* ``covariant_derivative`` / ``elemental_types``: the drivers' elemental construction
+  (tests/compute_rho.cpp:116-229) from links and eigenvectors. The derivative stands in for
+  gwu-qcd's covderiv, so its normalisation is parity unpinned against gwu-qcd; the rest is pinned
+  by gauge covariance and free-field closed forms.
 * ``dense``: a brute-force evaluator that builds explicit 4Nv x 4Nv matrices and evaluates
   every diagram as a trace of gamma-matrix / elemental / perambulator products in spin x
   eigenvector space. It shares no code with the row restatement, so agreement between the two
