@@ -2,7 +2,7 @@
 (lib/catchOpts.cpp:45-66, tests/run_compute_*.sh).
 
 The perambulator is read from --propname (the readPropH file format). The elementals either come
-from --elemname, or, for pi-pi, are built on the GPU:
+from --elemname or are built on the GPU from eigenvectors (and, except for pi-pi, gauge links):
 
 * --elemname: complex double driver arrays, one record per momentum combination, in the order of
   the calculate* arguments:
@@ -15,13 +15,23 @@ from --elemname, or, for pi-pi, are built on the GPU:
   | sigma | mom [4][Nt][Nvec][Nvec] |
   | sigmapi | P1, P2, P1P2 [2][Nt][Nvec][Nvec] |
 
-* pi-pi from eigenvectors: --eigename (complex double [Nt][Nvec][Nz][Ny][Nx][3]) and --momname (the
-  reference's momentum list: Nmom * 3 integers per line, momComb lines). As in
-  tests/compute_pipi_I0.cpp:133-146:
-  * P1 and P2 take the phase exp(-2 pi i p.x / L);
-  * P3 and P4 take exp(+2 pi i p.x / L).
+* from eigenvectors: --eigename (complex double [Nt][Nvec][Nz][Ny][Nx][3]) and --momname (the
+  reference's momentum list: Nmom * 3 integers per line, momComb lines; with Nmom 2 the third
+  momentum is p1 + p2, lib/catchOpts.cpp:197-209). The derivative types need --linkname: the
+  smeared gauge links with their boundary conditions applied, complex double
+  [Nt][3][Nz][Ny][Nx][3][3] (U_j(x), j = x, y, z). Each record follows its driver, with
+  e(+-p) = exp(+-2 pi i p.x / L) and T_k(e) the five types of lc_build_elementals_types:
 
-  The elementals are formed on the GPU (lc_build_elementals).
+  | diagram | Nmom | record | driver |
+  |---|---|---|---|
+  | rho | 1 | T_0..4(e(-p1)) | tests/compute_rho.cpp:116-229 |
+  | rhopi | 2 | T_0(e(-p1)), T_0(e(-p2)), T_0..4(e(+p3)) | tests/compute_rhopi.cpp:125-245 |
+  | pipi | 4 | T_0(e(-p1)), T_0(e(-p2)), T_0(e(+p3)), T_0(e(+p4)) | tests/compute_pipi_I0.cpp:133-146 |
+  | sigma | 1 | T_0(e(+p1)), T_0(e(-p1)), T_1(e(+p1)), T_1(e(-p1)) | tests/compute_sigma.cpp:113-195 |
+  | sigmapi | 2 | T_0(e(-p1)), T_0(e(-p2)), T_0..1(e(+p3)) | tests/compute_sigmapi.cpp:115-180 |
+
+  Elementals and covariant derivatives are formed on the GPU (lc_build_elementals,
+  lc_covariant_derivative, lc_build_elementals_types).
 
 Each momentum combination appends its text to --outname, as one call of the reference's
 calculate* does. With --binary, a raw FP64 side channel is also written to <outname>.bin.
@@ -42,6 +52,7 @@ from ._capi import check
 
 DIAGRAMS = ("rho", "rhopi", "pipi_I0", "pipi_I1", "sigma", "sigmapi")
 # elemental arrays of one record, in units of [Nt][Nvec][Nvec]
+NMOM = {"rho": 1, "rhopi": 2, "pipi_I0": 4, "pipi_I1": 4, "sigma": 1, "sigmapi": 2}
 RECORD = {"rho": [5], "rhopi": [1, 1, 5], "pipi_I0": [1, 1, 1, 1], "pipi_I1": [1, 1, 1, 1], "sigma": [4],
           "sigmapi": [1, 1, 2]}
 
@@ -64,11 +75,17 @@ def parse(argv=None):
     ap.add_argument("--eigename", default="")
     ap.add_argument("--momname", default="")
     ap.add_argument("--elemname", default="")
+    ap.add_argument("--linkname", default="", help="smeared gauge links (derivative elemental types)")
     ap.add_argument("--outname", default="out.dat")
     ap.add_argument("--binary", action="store_true", help="also write <outname>.bin (FP64 values)")
     a = ap.parse_args(argv)
-    if not a.elemname and not (a.diagram.startswith("pipi") and a.eigename and a.momname):
-        ap.error("give --elemname, or (pi-pi) --eigename and --momname")
+    if not a.elemname:
+        if not (a.eigename and a.momname):
+            ap.error("give --elemname, or --eigename and --momname")
+        if not a.diagram.startswith("pipi") and not a.linkname:
+            ap.error("%s from eigenvectors needs --linkname (its derivative elemental types)" % a.diagram)
+        if a.Nmom != NMOM[a.diagram]:
+            ap.error("%s needs --Nmom %d" % (a.diagram, NMOM[a.diagram]))
     if a.idir not in (0, 1, 2):
         ap.error("--idir must be 0, 1 or 2")
     return a
@@ -108,6 +125,28 @@ def pipi_elementals(evec_dev, moms, a):
     return [np.ascontiguousarray(phi[i]) for i in range(4)]
 
 
+def driver_elementals(evec_dev, dvec_dev, moms, a):
+    """The record of one momentum combination for rho, rhopi, sigma or sigmapi, as its driver forms
+    it (table in the module docstring); moms are the combination's Nmom momenta."""
+    from .engine import build_elementals_types
+    from .devmem import to_device
+
+    def types(plist, signs):  # [len(plist)][5][Nt][Nvec][Nvec]
+        ph = to_device(site_phases(plist, signs, a.Nx, a.Ny, a.Nz))
+        return build_elementals_types(evec_dev, dvec_dev, ph).to_numpy()
+    c = np.ascontiguousarray
+    d = a.diagram
+    if d == "rho":
+        return [c(types([moms[0]], [-1])[0])]
+    if d == "sigma":
+        T = types([moms[0], moms[0]], [+1, -1])
+        return [c(np.stack([T[0, 0], T[1, 0], T[0, 1], T[1, 1]]))]
+    p3 = tuple(x + y for x, y in zip(moms[0], moms[1]))
+    T = types([moms[0], moms[1], p3], [-1, -1, +1])
+    last = T[2] if d == "rhopi" else T[2, :2]
+    return [c(T[0, 0]), c(T[1, 0]), c(last)]
+
+
 def _p(x):
     return ctypes.c_void_p(x.ctypes.data)
 
@@ -141,8 +180,6 @@ def main(argv=None):
                     records.append(arrs)
             else:
                 from .devmem import to_device
-                if a.Nmom != 4:
-                    raise SystemExit("pi-pi needs --Nmom 4")
                 moms = read_momenta(a.momname, a.Nmom, a.momComb)
                 nsite = 3 * a.Nx * a.Ny * a.Nz
                 ev = np.fromfile(a.eigename, dtype="<c16")
@@ -150,8 +187,22 @@ def main(argv=None):
                     raise SystemExit("%s holds %d complex values, expected Nt*Nvec*3*Nx*Ny*Nz = %d"
                                      % (a.eigename, ev.size, nt * nv * nsite))
                 evec = to_device(ev.reshape(nt, nv, nsite))
-                for m in moms:
-                    records.append(pipi_elementals(evec, m, a))
+                if a.diagram.startswith("pipi"):
+                    for m in moms:
+                        records.append(pipi_elementals(evec, m, a))
+                else:
+                    from .engine import covariant_derivative
+                    nlink = nt * 3 * (nsite // 3) * 9
+                    lk = np.fromfile(a.linkname, dtype="<c16")
+                    if lk.size != nlink:
+                        raise SystemExit("%s holds %d complex values, expected Nt*3*Nx*Ny*Nz*9 = %d"
+                                         % (a.linkname, lk.size, nlink))
+                    links = to_device(lk.reshape(nt, 3, nsite // 3, 3, 3))
+                    dvec = covariant_derivative(links, evec, a.Nx, a.Ny, a.Nz)
+                    del links
+                    for m in moms:
+                        records.append(driver_elementals(evec, dvec, m, a))
+                    del dvec
                 del evec
             for arrs in records:
                 ptrs = [_p(x) for x in arrs]

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg as the checker. Nothing in the product package imports it.
 
-Three independent pieces:
+Four independent pieces:
 
 * ``oracle/_build/liboracle.so`` (oracle.cpp): a row-by-row restatement of the reference's
   CUDA cores and host launchers, one table row at a time, FP64, OpenMP.

@@ -32,3 +32,12 @@ def test_site_phases():
 def test_parse_needs_elementals():
     with pytest.raises(SystemExit):
         cli.parse(["rho", "--propname", "p"])
+    # the derivative types need links; each diagram its driver's number of momenta
+    with pytest.raises(SystemExit):
+        cli.parse(["rho", "--propname", "p", "--eigename", "e", "--momname", "m"])
+    with pytest.raises(SystemExit):
+        cli.parse(["sigmapi", "--propname", "p", "--eigename", "e", "--momname", "m", "--linkname", "l"])
+    a = cli.parse(["sigmapi", "--propname", "p", "--eigename", "e", "--momname", "m", "--linkname", "l",
+                   "--Nmom", "2"])
+    assert a.linkname == "l" and a.Nmom == 2
+    assert cli.parse(["pipi_I1", "--propname", "p", "--eigename", "e", "--momname", "m", "--Nmom", "4"]).Nmom == 4

@@ -5,6 +5,10 @@ files -> text and FP64 side channel), against the same flows called directly thr
   with the conventions of tests/compute_pipi_I0.cpp:133-146 (P1, P2 with exp(-i p.x), P3, P4 with
   exp(+i p.x), cscalar_product conjugating the left eigenvector). FP64 values agree to 1e-12.
 * rho from an elemental file: two momentum combinations, bit-identical to two direct calls.
+* rho, rhopi, sigma and sigmapi from eigenvectors and gauge links: the CLI forms the covariant
+  derivatives and the drivers' elemental types on the GPU; the direct calls get records built by
+  the oracle's restatement of the drivers' loops with the phase signs of each driver. FP64 values
+  agree to 1e-12.
 """
 import ctypes
 
@@ -90,3 +94,85 @@ def test_cli_rho_from_elemental_file(lc, tmp_path):
     text_ref, vals_ref = _direct(tmp_path, "direct.dat", fn)
     assert open(out).read() == text_ref
     assert np.array_equal(np.fromfile(str(out) + ".bin", dtype="<f8"), vals_ref)
+
+
+@pytest.mark.parametrize("diagram,moms", [
+    ("rho", [[(1, 0, 0)], [(0, 1, -1)]]),
+    ("rhopi", [[(1, 0, 0), (0, 0, 1)]]),
+    ("sigma", [[(0, 1, 0)]]),
+    ("sigmapi", [[(1, 1, 0), (-1, 0, 0)]]),
+])
+def test_cli_from_eigenvectors_and_links(lc, tmp_path, diagram, moms):
+    import oracle as O
+    from test_elemental_types import _su3
+    g = I.rng(43)
+    _write_prop(tmp_path / "prop.bin", I.perambulator(g, NT, NV, 0.4))
+    ns = NX ** 3
+    ev = I.cn(g, (NT, NV, 3 * ns))
+    ev.astype("<c16").tofile(tmp_path / "evec.bin")
+    U = _su3(np.random.default_rng(44), NT * 3 * ns).reshape(NT, 3, ns, 3, 3)
+    U.astype("<c16").tofile(tmp_path / "links.bin")
+    with open(tmp_path / "mom.dat", "w") as f:
+        for m in moms:
+            f.write("\t".join(" ".join(str(c) for c in p) for p in m) + "\n")
+    out = tmp_path / "cli.dat"
+    print("cli %s from eigenvectors and links" % diagram, flush=True)
+    assert cli.main([diagram, "--propname", str(tmp_path / "prop.bin"), "--eigename", str(tmp_path / "evec.bin"),
+                     "--linkname", str(tmp_path / "links.bin"), "--momname", str(tmp_path / "mom.dat"),
+                     "--Nmom", str(len(moms[0])), "--momComb", str(len(moms)), "--Nx", str(NX), "--Ny", str(NX),
+                     "--Nz", str(NX), "--Nt", str(NT), "--Nvec", str(NV), "--outname", str(out), "--binary"]) == 0
+    vals = np.fromfile(str(out) + ".bin", dtype="<f8")
+    dV = O.covariant_derivative(U, ev, NX, NX, NX)
+
+    def T(p, sign):  # [5][Nt][Nv][Nv]
+        return O.elemental_types(ev, dV, cli.site_phases([p], [sign], NX, NX, NX))[0]
+
+    # the records the CLI forms, stage by stage, against the oracle
+    import argparse
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    check(_capi.lib().lc_contract_init(NV, NT, 0))
+    try:
+        a = argparse.Namespace(diagram=diagram, Nx=NX, Ny=NX, Nz=NX)
+        d_ev = to_device(ev)
+        d_dv = engine.covariant_derivative(to_device(U), d_ev, NX, NX, NX)
+        assert relerr(d_dv.to_numpy(), dV) < 1e-13
+        recs = [cli.driver_elementals(d_ev, d_dv, m, a) for m in moms]
+    finally:
+        check(_capi.lib().lc_contract_destroy())
+    for m, rec in zip(moms, recs):
+        if diagram == "rho":
+            want = [T(m[0], -1)]
+        elif diagram == "sigma":
+            tp, tm = T(m[0], +1), T(m[0], -1)
+            want = [np.stack([tp[0], tm[0], tp[1], tm[1]])]
+        else:
+            p3 = tuple(x + y for x, y in zip(m[0], m[1]))
+            want = [T(m[0], -1)[0], T(m[1], -1)[0], T(p3, +1) if diagram == "rhopi" else T(p3, +1)[:2]]
+        assert len(rec) == len(want)
+        for r, w in zip(rec, want):
+            assert r.shape == w.shape and relerr(r, w) < 1e-12, (r.shape, w.shape, relerr(r, w))
+
+    def fn(L, prop, o):
+        # every array handed over by pointer stays referenced until its call returns
+        c = np.ascontiguousarray
+        for m in moms:
+            if diagram == "rho":
+                r0 = c(T(m[0], -1))
+                check(L.lc_calculate_rho(NV, NT, prop, _p(r0), 1, o))
+            elif diagram == "sigma":
+                tp, tm = T(m[0], +1), T(m[0], -1)
+                r0 = c(np.stack([tp[0], tm[0], tp[1], tm[1]]))
+                check(L.lc_calculate_sigma(NV, NT, prop, _p(r0), 1, o))
+            else:
+                p3 = tuple(x + y for x, y in zip(m[0], m[1]))
+                a1, a2 = c(T(m[0], -1)[0]), c(T(m[1], -1)[0])
+                a3 = c(T(p3, +1)) if diagram == "rhopi" else c(T(p3, +1)[:2])
+                if diagram == "rhopi":
+                    check(L.lc_calculate_rhopi(0, NV, NT, prop, _p(a1), _p(a2), _p(a3), 1, o))
+                else:
+                    check(L.lc_calculate_sigmapi(NV, NT, prop, _p(a1), _p(a2), _p(a3), 1, o))
+    text_ref, vals_ref = _direct(tmp_path, "direct.dat", fn)
+    assert open(out).read().count("\n") == text_ref.count("\n")
+    assert vals.shape == vals_ref.shape and vals.size > 0
+    assert relerr(vals, vals_ref) < 1e-12, relerr(vals, vals_ref)
