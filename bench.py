@@ -85,10 +85,41 @@ def traffic_from_profiles(kernel, nmom):
         return None
 
 
+def _two_forms(label, call, ncorr, collect):
+    """FP64 then mixed: call(mixed) once untimed (the mixed form tunes its tile shape), once timed.
+    Returns ({fp64, mixed, mixed_vs_fp64, unit}, {False: collect(outputs), True: ...})."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import synchronize
+    res, outs = {}, {}
+    for mixed in (False, True):
+        call(mixed)
+        synchronize()
+        tune = engine.autotune_report()
+        engine.profile(True)
+        t = time.perf_counter()
+        out = call(mixed)
+        synchronize()
+        dt = time.perf_counter() - t
+        p = engine.profile_read()
+        engine.profile(False)
+        outs[mixed] = collect(out)
+        key = "mixed" if mixed else "fp64"
+        res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
+                        gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
+        if mixed:
+            res[key]["tiles_per_wg"] = tune["tiles_per_wg"]
+        log("%s %s: %.1f ms" % (label, key, dt * 1e3))
+    res["mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    res["unit"] = "correlators/s"
+    return res, outs
+
+
 def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
     """rho + pi-pi at the largest size: Nv = 128 (D = 512), Lt = 64, all 64 sources, `nmom` momenta
     (the first Hermitian), perambulator decay 0.25; both forms, one timed pass each after one
-    warm-up pass. At Nv = 128 the mixed form runs its four-level chain (no fused sandwiches)."""
+    warm-up pass. At Nv = 128 the mixed form runs its four-level chain (no fused sandwiches).
+    On the same resident perambulator, also a moving frame's pi-pi over `nmom` explicit momentum
+    sets and the sigma / sigma-pi / rho-pi set for `nmom` momentum sets (pipi_sets, sigma_set)."""
     import numpy as np
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import DeviceArray, synchronize, to_device
@@ -126,6 +157,32 @@ def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
     res["unit"] = "correlators/s"
     res["config"] = dict(workload="rho(3 pol)+pipi(5 diagrams)", nv=nv, D=4 * nv, lt=nt, sources=nt, momenta=nmom,
                          mass=mass, seed=seed, correlators_per_pass=ncorr)
+    del phi
+    from contractions_code_scattering_amd import inputs as I
+    g = I.rng(seed + 100)
+    # a moving frame (total momentum (0,0,1)): its pion pairs are not back-to-back, so each momentum
+    # set carries four independent elementals P1..P4 (random, distributed as the workload's Phi)
+    el = to_device(np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(4)]) for _ in range(nmom)]))
+    r, o = _two_forms("d512 pipi sets", lambda mixed: engine.pipi_sets(tau, el, mixed=mixed), ncorr,
+                      lambda out: out[2].to_numpy())
+    e = engine.rel_errors(o[True], o[False])
+    r["max_normwise_err"] = max(v["max_normwise"] for v in e.values())
+    r["config"] = dict(workload="pipi(5 diagrams)+rho slots over explicit momentum sets P1..P4", sets=nmom,
+                       correlators_per_pass=ncorr)
+    res["pipi_sets"] = r
+    del el
+    # the sigma / sigma-pi / rho-pi set on the same perambulator, all 64 sources, idir = 2
+    el = to_device(np.stack([I.cn(g, (nt, 11, nv, nv)) / np.sqrt(nv) for _ in range(nmom)]))
+    nc = nmom * nt * nt * 10
+    r, o = _two_forms("d512 sigma set", lambda mixed: engine.sigma_set(tau, el, idir=2, mixed=mixed), nc,
+                      lambda out: out[0].to_numpy())
+    d = np.abs(o[True] - o[False]).reshape(-1, 10)
+    a = np.abs(o[False]).reshape(-1, 10)
+    r["max_normwise_err"] = float((d.max(axis=0) / a.max(axis=0)).max())
+    r["config"] = dict(workload="sigma one_fish (4) + sigma-pi triangle (2) + rho-pi (4) per (tf, t0, p)",
+                       momentum_sets=nmom, correlators_per_pass=nc)
+    res["sigma_set"] = r
+    del el
     return res
 
 
@@ -155,9 +212,34 @@ def elementals_measurement(L=24, nv=64, nt=48, nmom=19, seed=6):
     dt = time.perf_counter() - t
     flops = 8.0 * nmom * nt * nv * nv * nsite
     log("elementals: %.2f ms" % (dt * 1e3))
-    return dict(ms=round(dt * 1e3, 3), tflops=round(flops / dt / 1e12, 2),
-                frac_f64_peak=round(flops / dt / 1e12 / PEAK_TFLOPS["f64"], 3),
-                config=dict(L=L, colours=3, nsite=nsite, nv=nv, lt=nt, momenta=nmom, dtype="f64"))
+    res = dict(ms=round(dt * 1e3, 3), tflops=round(flops / dt / 1e12, 2),
+               frac_f64_peak=round(flops / dt / 1e12 / PEAK_TFLOPS["f64"], 3),
+               config=dict(L=L, colours=3, nsite=nsite, nv=nv, lt=nt, momenta=nmom, dtype="f64"))
+    # the derivative types: covariant derivatives (links random complex 3x3; the stencil's cost does
+    # not depend on them being SU(3)), then the five types per momentum (10 bilinears each)
+    U = to_device(I.cn(g, (nt, 3, L ** 3, 3, 3)))
+    dV = engine.covariant_derivative(U, V, L, L, L)
+    synchronize()
+    t = time.perf_counter()
+    engine.covariant_derivative(U, V, L, L, L, out=dV)
+    synchronize()
+    dt_d = time.perf_counter() - t
+    vbytes = nt * nv * nsite * 16
+    dbytes = 4 * vbytes + U.nbytes  # V read once, U once, 3 V written
+    T = engine.build_elementals_types(V, dV, phd)
+    synchronize()
+    t = time.perf_counter()
+    engine.build_elementals_types(V, dV, phd, out=T)
+    synchronize()
+    dt_t = time.perf_counter() - t
+    log("covariant derivative: %.2f ms; five types: %.2f ms" % (dt_d * 1e3, dt_t * 1e3))
+    res["covariant_derivative"] = dict(ms=round(dt_d * 1e3, 3), gbs=round(dbytes / dt_d / 1e9, 1),
+                                       frac_hbm_peak=round(dbytes / dt_d / 1e9 / HBM_PEAK_GBS, 3),
+                                       algorithmic_bytes=dbytes)
+    res["types"] = dict(ms=round(dt_t * 1e3, 3), tflops=round(10 * flops / dt_t / 1e12, 2),
+                        frac_f64_peak=round(10 * flops / dt_t / 1e12 / PEAK_TFLOPS["f64"], 3),
+                        bilinears_per_momentum=10)
+    return res
 
 
 def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
@@ -216,8 +298,9 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="profiling aid: one device-to-device copy of this size (counter calibration)")
     ap.add_argument("--d512", type=int, default=0,
-                    help="N > 0: also time rho + pi-pi at Nv=128 (D=512), Lt=64, all 64 sources, N momenta "
-                         "(reported under d512)")
+                    help="N > 0: also time rho + pi-pi at Nv=128 (D=512), Lt=64, all 64 sources, N momenta, "
+                         "then N moving-frame pi-pi momentum sets and N sigma-set momentum sets on the same "
+                         "perambulator (reported under d512)")
     ap.add_argument("--elementals", type=int, default=0,
                     help="1: also time the GPU construction of the momentum elementals at L=24, Nv=64, Lt=48, "
                          "19 momenta (reported under elementals)")
