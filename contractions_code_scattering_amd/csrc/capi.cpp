@@ -620,7 +620,7 @@ int lc_covariant_derivative(const void* d_links, const void* d_evec, int nt, int
   return guarded([&] {
     require_init();
     if (nt < 1 || nv < 1 || nv > 1024 || nx < 1 || ny < 1 || nz < 1 ||
-        3LL * nt * nv * nx * ny * nz > 0xffffffffLL * 256)
+        3LL * nt * (((long long)nx * ny * nz + 255) / 256) > 0xffffffffLL)
       fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d lattice %dx%dx%d", nt, nv, nx, ny, nz);
     check_ptr(d_links, "d_links"); check_ptr(d_evec, "d_evec"); check_ptr(d_dvec, "d_dvec");
     hipStream_t s = ctx().stream;

@@ -19,7 +19,7 @@
 //
 // The covariant derivative (lc_covderiv_kernel) is a nearest-neighbour stencil on the spatial
 // lattice of one time slice: (nabla_j v)(x) = 1/2 [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)],
-// periodic, one thread per (direction, time, vector, site) with its three colours.
+// periodic, one thread per (time, direction, site) with its three colours, looping over the vectors.
 #include "lc_device.h"
 
 namespace lc {
@@ -168,18 +168,20 @@ __global__ void __launch_bounds__(256) lc_phi_reduce_kernel(const double2* __res
 
 // dV[j][t][c][3 * site + a] = (nabla_j V[t][c])(site)_a; U [nt][3][nsite][3][3] row-major SU(3)
 // links U_j(x) (colour index a, b), V [nt][nv][3 * nsite], sites x + nx (y + ny z).
+// One thread per (t, j, site) keeps U_j(x) and U_j(x - j) in registers and walks the nv vectors,
+// so the links are read once; the three directions of a site chunk are neighbouring workgroups,
+// so the vector reads of one direction find the others' in L2.
 __global__ void __launch_bounds__(256) lc_covderiv_kernel(const double2* __restrict__ U,
                                                           const double2* __restrict__ V, int nt, int nv, int nx,
                                                           int ny, int nz, double2* __restrict__ dV) {
   const long long nsite = (long long)nx * ny * nz;
-  const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
-  const long long per_j = (long long)nt * nv * nsite;
-  if (gid >= 3 * per_j) return;
-  const int j = (int)(gid / per_j);
-  const long long rest = gid - j * per_j;
-  const long long vec = rest / nsite;  // t * nv + c
-  const long long x = rest - vec * nsite;
-  const int t = (int)(vec / nv);
+  const long long nchunk = (nsite + 255) / 256;
+  const long long b = blockIdx.x;
+  const int j = (int)(b % 3);
+  const long long chunk = (b / 3) % nchunk;
+  const int t = (int)(b / (3 * nchunk));
+  const long long x = chunk * 256 + threadIdx.x;
+  if (x >= nsite) return;
   const int ix = (int)(x % nx), iy = (int)((x / nx) % ny), iz = (int)(x / ((long long)nx * ny));
   int fx = ix, fy = iy, fz = iz, bx = ix, by = iy, bz = iz;
   if (j == 0) { fx = ix + 1 == nx ? 0 : ix + 1; bx = ix == 0 ? nx - 1 : ix - 1; }
@@ -187,28 +189,38 @@ __global__ void __launch_bounds__(256) lc_covderiv_kernel(const double2* __restr
   if (j == 2) { fz = iz + 1 == nz ? 0 : iz + 1; bz = iz == 0 ? nz - 1 : iz - 1; }
   const long long xf = fx + (long long)nx * (fy + (long long)ny * fz);
   const long long xb = bx + (long long)nx * (by + (long long)ny * bz);
-  const double2* v = V + vec * 3 * nsite;
   const double2* Uf = U + (((long long)t * 3 + j) * nsite + x) * 9;   // U_j(x)
   const double2* Ub = U + (((long long)t * 3 + j) * nsite + xb) * 9;  // U_j(x - j)
-  double2 vf[3], vb[3];
+  double2 f[9], g[9];
 #pragma unroll
-  for (int b = 0; b < 3; b++) {
-    vf[b] = v[3 * xf + b];
-    vb[b] = v[3 * xb + b];
+  for (int e = 0; e < 9; e++) {
+    f[e] = Uf[e];
+    g[e] = Ub[e];
   }
-  double2* out = dV + (long long)j * per_j * 3 + (vec * nsite + x) * 3;
+  const long long vstride = 3 * nsite;
+  const double2* v = V + (long long)t * nv * vstride;
+  double2* out = dV + ((long long)j * nt + t) * nv * vstride + 3 * x;
+  for (int c = 0; c < nv; c++) {
+    const double2* vc = v + (long long)c * vstride;
+    double2 vf[3], vb[3];
 #pragma unroll
-  for (int a = 0; a < 3; a++) {
-    double re = 0, im = 0;
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      const double2 f = Uf[3 * a + b], g = Ub[3 * b + a];  // U_j(x)_ab, U_j(x - j)_ba
-      re += f.x * vf[b].x - f.y * vf[b].y;
-      im += f.x * vf[b].y + f.y * vf[b].x;
-      re -= g.x * vb[b].x + g.y * vb[b].y;  // conj(g) * vb
-      im -= g.x * vb[b].y - g.y * vb[b].x;
+    for (int q = 0; q < 3; q++) {
+      vf[q] = vc[3 * xf + q];
+      vb[q] = vc[3 * xb + q];
     }
-    out[a] = double2{0.5 * re, 0.5 * im};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      double re = 0, im = 0;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const double2 u = f[3 * a + q], w = g[3 * q + a];  // U_j(x)_aq, U_j(x - j)_qa
+        re += u.x * vf[q].x - u.y * vf[q].y;
+        im += u.x * vf[q].y + u.y * vf[q].x;
+        re -= w.x * vb[q].x + w.y * vb[q].y;  // conj(w) * vb
+        im -= w.x * vb[q].y - w.y * vb[q].x;
+      }
+      out[(long long)c * vstride + a] = double2{0.5 * re, 0.5 * im};
+    }
   }
 }
 
@@ -244,8 +256,8 @@ extern "C" hipError_t lc_launch_phi(const void* V, const void* ph, int nt, int n
 
 extern "C" hipError_t lc_launch_covderiv(const void* U, const void* V, int nt, int nv, int nx, int ny, int nz,
                                          void* dV, hipStream_t s) {
-  const long long n = 3LL * nt * nv * nx * ny * nz;
-  hipLaunchKernelGGL(lc::lc_covderiv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+  const long long nchunk = ((long long)nx * ny * nz + 255) / 256;
+  hipLaunchKernelGGL(lc::lc_covderiv_kernel, dim3((unsigned)(3LL * nt * nchunk)), dim3(256), 0, s,
                      (const double2*)U, (const double2*)V, nt, nv, nx, ny, nz, (double2*)dV);
   return hipGetLastError();
 }
