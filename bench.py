@@ -286,6 +286,67 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
     return res
 
 
+def reader_measurement(nv=64, nt=48):
+    """The perambulator file of readPropH (lib/contract.cpp:91-160) at Nv = 64, Lt = 48 (2.4 GB),
+    written to a temporary file whose complex values are their own file index (so a sample of
+    the device layout checks the permutation), then read page-cached: by lc_read_prop_device
+    (slabs through pinned buffers, permuted on the GPU) and by the host restatement of readPropH
+    (lc_read_prop). Runs last: the host reader's destroyContract finalises the library."""
+    import ctypes
+    import shutil
+    import tempfile
+    import numpy as np
+    from contractions_code_scattering_amd import _capi, check
+    from contractions_code_scattering_amd.devmem import DeviceArray, synchronize
+    L = _capi.lib()
+    d = tempfile.mkdtemp(dir="/tmp")
+    path = os.path.join(d, "prop.bin")
+    n = 16 * nv * nv * nt * nt
+    try:
+        with open(path, "wb") as f:
+            f.write(b"synthetic".ljust(100, b"\0"))
+            f.write(np.array([nt, nv], "<u4").tobytes())
+            for s0 in range(0, n, 1 << 24):
+                v = np.zeros(min(n, s0 + (1 << 24)) - s0, "<c16")
+                v.real = np.arange(s0, s0 + v.size)
+                f.write(v.tobytes())
+        log("reader: wrote %.2f GB" % (n * 16 / 1e9))
+        out = DeviceArray((nt, nt, 4, 4, nv, nv), np.complex128)
+        res = {}
+        for k in range(2):  # the first call pages the file in
+            t = time.perf_counter()
+            check(L.lc_read_prop_device(path.encode(), nt, nv, 1, out.ptr))
+            synchronize()
+            dt = time.perf_counter() - t
+        res["device"] = dict(ms=round(dt * 1e3, 1), gbs=round(n * 16 / dt / 1e9, 2))
+        tau = out.to_numpy()
+        del out
+        g = np.random.default_rng(0)
+        ok = True
+        for _ in range(64):
+            t1, t2, v1, v2 = g.integers(0, nt), g.integers(0, nt), g.integers(0, nv), g.integers(0, nv)
+            s1, s2 = g.integers(0, 4), g.integers(0, 4)
+            idx = ((((s1 * nv + v1) * nt + t1) * 4 + s2) * nv + v2) * nt + t2
+            ok &= tau[t1, t2, s1, s2, v1, v2] == idx
+        res["permutation_sample_ok"] = bool(ok)
+        del tau
+        check(L.lc_contract_init(nv, nt, 0))
+        try:
+            ptr = ctypes.c_void_p()
+            t = time.perf_counter()
+            check(L.lc_read_prop(path.encode(), 1, ctypes.byref(ptr)))
+            dt = time.perf_counter() - t
+            L.lc_free_host(ptr)
+        finally:
+            check(L.lc_contract_destroy())
+        res["host_readPropH"] = dict(ms=round(dt * 1e3, 1), gbs=round(n * 16 / dt / 1e9, 2))
+        log("reader: device %.1f ms, host %.1f ms" % (res["device"]["ms"], res["host_readPropH"]["ms"]))
+        res["config"] = dict(nv=nv, lt=nt, file_bytes=n * 16 + 108, page_cached=True)
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +362,9 @@ def main():
                     help="N > 0: also time rho + pi-pi at Nv=128 (D=512), Lt=64, all 64 sources, N momenta, "
                          "then N moving-frame pi-pi momentum sets and N sigma-set momentum sets on the same "
                          "perambulator (reported under d512)")
+    ap.add_argument("--reader", type=int, default=0,
+                    help="1: also time the perambulator file reader (GPU and host) at Nv=64, Lt=48 from a "
+                         "page-cached temporary file (reported under reader; runs last)")
     ap.add_argument("--elementals", type=int, default=0,
                     help="1: also time the GPU construction of the momentum elementals at L=24, Nv=64, Lt=48, "
                          "19 momenta (reported under elementals)")
@@ -431,6 +495,8 @@ def main():
         line["cpu_baseline"] = rec
         got = r64["corr"][0][:, :args.cpu_rows, :].transpose(1, 0, 2)  # [tf][t0][8]
         line["fp64_vs_oracle_max_rel"] = float(np.abs(got - ref).max() / np.abs(ref).max())
+    if args.reader:
+        line["reader"] = reader_measurement()
     if rank == 0:
         print(json.dumps(line), flush=True)
 
