@@ -119,3 +119,22 @@ def test_sets_rejects_bad_shapes(lc):
     tau = I.perambulator(g, 2, 4, 0.3)
     with pytest.raises(ValueError, match="nset"):
         engine.pipi_sets(to_device(tau), to_device(np.zeros((1, 3, 2, 4, 4), complex)))
+
+
+@pytest.mark.parametrize("nv,mixed", [(16, False), (64, False), (64, True), (40, True)])
+def test_sets_chunks_are_invisible(lc, nv, mixed):
+    """One momentum set per chunk (every role view offset by the chunk's first set, stride
+    4 nt Nv^2) gives the results of one chunk for all sets, bit for bit."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    from test_gpu_tiles import _one_momentum_budget
+    nt, nset = 3, 3
+    g = I.rng(60 + nv)
+    t = to_device(I.perambulator(g, nt, nv, 0.3))
+    e = to_device(_sets(61 + nv, nt, nv, nset, spread=0.5))
+    print("pipi_sets chunks nv=%d mixed=%d" % (nv, mixed), flush=True)
+    whole = [x.to_numpy() for x in engine.pipi_sets(t, e, src0=1, mixed=mixed)]
+    budget = _one_momentum_budget(lambda b: engine.pipi_sets(t, e, src0=1, mixed=mixed, max_scratch_bytes=b))
+    chunked = [x.to_numpy() for x in engine.pipi_sets(t, e, src0=1, mixed=mixed, max_scratch_bytes=budget)]
+    for x, y in zip(chunked, whole):
+        assert np.array_equal(x, y)
