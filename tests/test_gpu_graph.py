@@ -133,3 +133,34 @@ def test_sigma_set_replays_and_autotune(eng):
 def relerr_ok(a, b):
     """The tuned tile shape regroups the trace sums: equal to 1e-13 normwise."""
     return all(np.abs(x - y).max() <= 1e-13 * np.abs(y).max() for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("nv,mixed", [(16, False), (64, True)])
+def test_pipi_sets_replays(eng, nv, mixed):
+    """The momentum-set engine: recorded once, replayed three times with eager engine calls in
+    between, each replay bit-identical to the eager result; new elementals in the same buffer are
+    read by the replay."""
+    from contractions_code_scattering_amd.devmem import copy, to_device
+    nt, ns = 3, 2
+    g = I.rng(53)
+    t = to_device(I.perambulator(g, nt, nv, 0.3))
+    el_a = np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(4)]) for _ in range(ns)])
+    el_b = np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(4)]) for _ in range(ns)])
+    e, ea, eb = to_device(el_a), to_device(el_a), to_device(el_b)
+    out = _outs(ns, nt, nt)
+    eng.pipi_sets(t, ea, out=out, mixed=mixed)
+    ref_a = _host(out)
+    eng.pipi_sets(t, eb, out=out, mixed=mixed)
+    ref_b = _host(out)
+    eng.graph(True)
+    eng.pipi_sets(t, e, out=out, mixed=mixed)
+    assert eng.graph_report() == "recorded" and _same(_host(out), ref_a)
+    for rep in range(3):
+        eng.pipi_sets(t, e, out=out, mixed=mixed)
+        assert eng.graph_report() == "replayed" and _same(_host(out), ref_a), rep
+        other = _outs(ns, nt, nt)
+        eng.pipi_sets(t, eb, out=other, mixed=mixed)
+        assert _same(_host(other), ref_b)
+    copy(e, eb)
+    eng.pipi_sets(t, e, out=out, mixed=mixed)
+    assert eng.graph_report() == "replayed" and _same(_host(out), ref_b)
