@@ -431,8 +431,16 @@ def main():
         copy(dst, src)
         log("calibration copy of %d MiB done" % args.calib_copy_mib)
     if args.mode != "both":
+        # profiling aid: one form, then only the optional legs asked for (results on stderr)
         r = run(args.mode == "mixed")
         log("single-mode run: %.1f correlators/s" % (ncorr * args.steps / r["elapsed"]))
+        extra = {}
+        if args.elementals:
+            extra["elementals"] = elementals_measurement()
+        if args.reader:
+            extra["reader"] = reader_measurement()
+        if extra:
+            log(json.dumps(extra))
         return
     r64 = run(False)
     rmx = run(True)
