@@ -14,7 +14,9 @@
 // produces a matrix writes its pieces once, and the inputs are split once by
 // lc_split_convert_kernel. The main loop therefore only moves 16-byte BF16 vectors from global
 // memory to LDS and feeds the matrix cores; view phases (powers of i) and conjugation are sign
-// flips and a re/im plane exchange applied on the way into LDS. Output is FP32 complex and/or
+// flips and a re/im plane exchange, applied on the way into LDS, or, where the chunk is staged by
+// direct global -> LDS loads (dense products, Nv a multiple of 64), to the fragments read from
+// LDS. Output is FP32 complex and/or
 // split pieces; fused closing traces accumulate in FP64 with the same descriptors and per-tile
 // partials as the FP64 kernel.
 //
@@ -206,6 +208,10 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   };
   zero_acc();
 
+  // Dense products with no bounds checks stage through the 16-byte global -> LDS loads (lane l
+  // of a wave writes LDS bytes 16 l .. 16 l + 15 past the wave's base), so no staging registers
+  // and no LDS stores: the phase / conjugation masks are applied to the fragments instead.
+  constexpr bool DMA = FULL && !SW;
   // staging state of the chunk in flight
   u32x4 va[6], vb[6];
   int a_swap = 0;
@@ -274,8 +280,11 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     t_bmi = (B.flags & LC_CONJ) ? SIGN2 : 0u;
     t_alA = (reinterpret_cast<uintptr_t>(A.p) & 15) == 0;
     t_alB = (reinterpret_cast<uintptr_t>(B.p) & 15) == 0;
-    pA = A.p + plane_idx(cur_tm * MX_TM + s_line, s_kg * 8, nv);
-    pB = B.p + plane_idx(cur_tn * MX_TM + col_of_line(s_line), s_kg * 8, nv);
+    // DMA: the lane's 16 bytes land at LDS position s_kg of its line, which holds K group
+    // s_kg ^ swz(line) = s_pos
+    const int kg = DMA ? s_pos : s_kg;
+    pA = A.p + plane_idx(cur_tm * MX_TM + s_line, kg * 8, nv);
+    pB = B.p + plane_idx(cur_tn * MX_TM + col_of_line(s_line), kg * 8, nv);
   };
   set_term();
 
@@ -291,7 +300,15 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     // chunk kc starts k0 values further along each line: in the K-panel layout (Nv a multiple of
     // 64) that is k0 / 32 panels of 32 Nv values further
     const long long kofs = (nv & 63) == 0 ? (long long)k0 * nv : k0;
-    if (SW) {
+    if (DMA) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        __builtin_amdgcn_global_load_lds((const void*)(pA + q * nv2 + kofs),
+                                         (__attribute__((address_space(3))) void*)&As[0][q][wave * 16][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(pB + q * nv2 + kofs),
+                                         (__attribute__((address_space(3))) void*)&Bs[q][wave * 16][0], 16, 0, 0);
+      }
+    } else if (SW) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
         if (!a_t2) va[q] = gload<u32x4>(pA + q * nv2 + kofs);
@@ -324,6 +341,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 
   auto store = [&]() {
     c_asel = 0;
+    if (DMA) return;
     if (SW && a_t2) {
       c_asel = a_kc;
       if (a_t2 == 1 && a_kc == 0) {
@@ -569,7 +587,70 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     ao[i] = ((lane >> 4) ^ swz(al[i])) * 8;
     bo[i] = ((lane >> 4) ^ swz(bl[i])) * 8;
   }
-  for (int ch = 0; ch < nchunk; ch++) {
+  if (DMA) {
+    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): this wave's LDS DMA has landed
+    __syncthreads();
+  }
+  for (int ch = 0; DMA && ch < nchunk; ch++) {
+    const bool more = ch + 1 < nchunk;
+    // all fragments of the chunk, with its phase / conjugation masks, before the buffer is refilled
+    bf16x8 bf[2][3][2], af[2][3][2];  // [n or m][piece][re/im]
+    const int pre = a_swap;
+    const unsigned m0 = pre ? a_mi : a_mr, m1 = pre ? a_mr : a_mi;
+#pragma unroll
+    for (int n = 0; n < 2; n++)
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        bf[n][p][0] = *reinterpret_cast<const bf16x8*>(&Bs[2 * p][bl[n]][bo[n]]);
+        bf[n][p][1] = __builtin_bit_cast(
+            bf16x8, __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&Bs[2 * p + 1][bl[n]][bo[n]])) ^ b_mi);
+      }
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+      for (int p = 0; p < 3; p++) {
+        af[m][p][0] = __builtin_bit_cast(
+            bf16x8, __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&As[0][2 * p + pre][al[m]][ao[m]])) ^ m0);
+        af[m][p][1] = __builtin_bit_cast(
+            bf16x8,
+            __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&As[0][2 * p + 1 - pre][al[m]][ao[m]])) ^ m1);
+      }
+    __syncthreads();  // every wave holds its fragments: the buffer may be refilled
+    if (more) fetch();
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      bf16x8 nai[3];
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+        nai[p] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, af[m][p][1]) ^ SIGN2);
+#pragma unroll
+      for (int n = 0; n < 2; n++) {
+        const int PI[5] = {2, 1, 0, 1, 0};
+        const int PJ[5] = {0, 1, 2, 0, 1};
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+          const int i = PI[s], j = PJ[s];
+          sre[m][n] = mfma_bf16(bf[n][j][0], af[m][i][0], sre[m][n]);
+          sre[m][n] = mfma_bf16(bf[n][j][1], nai[i], sre[m][n]);
+          sim[m][n] = mfma_bf16(bf[n][j][1], af[m][i][0], sim[m][n]);
+          sim[m][n] = mfma_bf16(bf[n][j][0], af[m][i][1], sim[m][n]);
+        }
+        bre[m][n] = mfma_bf16(bf[n][0][0], af[m][0][0], bre[m][n]);
+        bre[m][n] = mfma_bf16(bf[n][0][1], nai[0], bre[m][n]);
+        bim[m][n] = mfma_bf16(bf[n][0][1], af[m][0][0], bim[m][n]);
+        bim[m][n] = mfma_bf16(bf[n][0][0], af[m][0][1], bim[m][n]);
+      }
+    }
+    {
+      const int tl = ch / cpb, kin = ch - tl * cpb;
+      if (kin == cpb - 1) epilogue(tl, false);
+    }
+    if (more) {
+      __builtin_amdgcn_s_waitcnt(0xF70);
+      __syncthreads();
+    }
+  }
+  for (int ch = 0; !DMA && ch < nchunk; ch++) {
     const bool more = ch + 1 < nchunk;
     if (more) fetch();
     bf16x8 bf[2][3][2];  // [n][piece][re/im]
