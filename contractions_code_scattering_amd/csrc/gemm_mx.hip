@@ -15,8 +15,8 @@
 // lc_split_convert_kernel. The main loop therefore only moves 16-byte BF16 vectors from global
 // memory to LDS and feeds the matrix cores; view phases (powers of i) and conjugation are sign
 // flips and a re/im plane exchange, applied on the way into LDS, or, where the chunk is staged by
-// direct global -> LDS loads (dense products, Nv a multiple of 64), to the fragments read from
-// LDS. Output is FP32 complex and/or
+// direct global -> LDS loads (Nv a multiple of 64), to the fragments read from LDS. Output is
+// FP32 complex and/or
 // split pieces; fused closing traces accumulate in FP64 with the same descriptors and per-tile
 // partials as the FP64 kernel.
 //
@@ -208,10 +208,11 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
   };
   zero_acc();
 
-  // Dense products with no bounds checks stage through the 16-byte global -> LDS loads (lane l
-  // of a wave writes LDS bytes 16 l .. 16 l + 15 past the wave's base), so no staging registers
-  // and no LDS stores: the phase / conjugation masks are applied to the fragments instead.
-  constexpr bool DMA = FULL && !SW;
+  // Kernels with no bounds checks stage operand chunks through the 16-byte global -> LDS loads
+  // (lane l of a wave writes LDS bytes 16 l .. 16 l + 15 past the wave's base), so no staging
+  // registers and no LDS stores: the phase / conjugation masks are applied to the fragments
+  // instead. The sandwich's T still goes from the accumulators to LDS by ordinary stores.
+  constexpr bool DMA = FULL;
   // staging state of the chunk in flight
   u32x4 va[6], vb[6];
   int a_swap = 0;
@@ -256,7 +257,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       t_mi = 0u;
       t_bmi = (R.flags & LC_CONJ) ? SIGN2 : 0u;
       pB = reinterpret_cast<const unsigned short*>(R.base) + 6 * voff(R.s, co) +
-           plane_idx(cur_tn * MX_TM + col_of_line(s_line), s_kg * 8, nv);
+           plane_idx(cur_tn * MX_TM + col_of_line(s_line), (DMA ? s_pos : s_kg) * 8, nv);
       return;
     }
     int ab, bb, ba, bc;
@@ -303,8 +304,9 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     if (DMA) {
 #pragma unroll
       for (int q = 0; q < 6; q++) {
-        __builtin_amdgcn_global_load_lds((const void*)(pA + q * nv2 + kofs),
-                                         (__attribute__((address_space(3))) void*)&As[0][q][wave * 16][0], 16, 0, 0);
+        if (!SW || !a_t2)
+          __builtin_amdgcn_global_load_lds((const void*)(pA + q * nv2 + kofs),
+                                           (__attribute__((address_space(3))) void*)&As[0][q][wave * 16][0], 16, 0, 0);
         __builtin_amdgcn_global_load_lds((const void*)(pB + q * nv2 + kofs),
                                          (__attribute__((address_space(3))) void*)&Bs[q][wave * 16][0], 16, 0, 0);
       }
@@ -341,7 +343,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 
   auto store = [&]() {
     c_asel = 0;
-    if (DMA) return;
+    if (DMA && !SW) return;
     if (SW && a_t2) {
       c_asel = a_kc;
       if (a_t2 == 1 && a_kc == 0) {
@@ -369,13 +371,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         }
         zero_acc();
       }
+      if (!DMA) {
 #pragma unroll
-      for (int p = 0; p < 3; p++) {
-        *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_pos * 8]) = vb[2 * p];
-        *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_pos * 8]) = vb[2 * p + 1] ^ b_mi;
+        for (int p = 0; p < 3; p++) {
+          *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_pos * 8]) = vb[2 * p];
+          *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_pos * 8]) = vb[2 * p + 1] ^ b_mi;
+        }
       }
       return;
     }
+    if (DMA) return;
 #pragma unroll
     for (int p = 0; p < 3; p++) {
       *reinterpret_cast<u32x4*>(&As[0][2 * p + a_swap][s_line][s_pos * 8]) = va[2 * p] ^ a_mr;
@@ -610,10 +615,11 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
 #pragma unroll
       for (int p = 0; p < 3; p++) {
         af[m][p][0] = __builtin_bit_cast(
-            bf16x8, __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&As[0][2 * p + pre][al[m]][ao[m]])) ^ m0);
+            bf16x8,
+            __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&As[SW ? c_asel : 0][2 * p + pre][al[m]][ao[m]])) ^ m0);
         af[m][p][1] = __builtin_bit_cast(
             bf16x8,
-            __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&As[0][2 * p + 1 - pre][al[m]][ao[m]])) ^ m1);
+            __builtin_bit_cast(u32x4, *reinterpret_cast<const bf16x8*>(&As[SW ? c_asel : 0][2 * p + 1 - pre][al[m]][ao[m]])) ^ m1);
       }
     __syncthreads();  // every wave holds its fragments: the buffer may be refilled
     if (more) fetch();
@@ -643,9 +649,16 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
     }
     {
       const int tl = ch / cpb, kin = ch - tl * cpb;
-      if (kin == cpb - 1) epilogue(tl, false);
+      if (!SW) {
+        if (kin == cpb - 1) epilogue(tl, false);
+      } else if (kin == 2 * nk - 1) {
+        epilogue(tl, false);
+      } else if (kin == 3 * nk - 1) {
+        epilogue(tl, true);
+      }
     }
     if (more) {
+      store();  // sandwich: T of the finished term 0 into the A buffers (no global traffic)
       __builtin_amdgcn_s_waitcnt(0xF70);
       __syncthreads();
     }
