@@ -310,19 +310,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
         __builtin_amdgcn_global_load_lds((const void*)(pB + q * nv2 + kofs),
                                          (__attribute__((address_space(3))) void*)&Bs[q][wave * 16][0], 16, 0, 0);
       }
-    } else if (SW) {
-#pragma unroll
-      for (int q = 0; q < 6; q++) {
-        if (!a_t2) va[q] = gload<u32x4>(pA + q * nv2 + kofs);
-        vb[q] = gload<u32x4>(pB + q * nv2 + kofs);
-      }
-    } else if (FULL) {
-#pragma unroll
-      for (int q = 0; q < 6; q++) {
-        va[q] = gload<u32x4>(pA + q * nv2 + kofs);
-        vb[q] = gload<u32x4>(pB + q * nv2 + kofs);
-      }
     } else {
+      // general kernel (SW and FULL kernels always take the branch above)
       const bool fullA = vec_ok && t_alA && (cur_tm + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
       const bool fullB = vec_ok && t_alB && (cur_tn + 1) * MX_TM <= nv && k0 + MX_KT <= nv;
       const int k = k0 + s_kg * 8;
@@ -370,13 +359,6 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
           }
         }
         zero_acc();
-      }
-      if (!DMA) {
-#pragma unroll
-        for (int p = 0; p < 3; p++) {
-          *reinterpret_cast<u32x4*>(&Bs[2 * p][s_line][s_pos * 8]) = vb[2 * p];
-          *reinterpret_cast<u32x4*>(&Bs[2 * p + 1][s_line][s_pos * 8]) = vb[2 * p + 1] ^ b_mi;
-        }
       }
       return;
     }
@@ -681,7 +663,7 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       for (int p = 0; p < 3; p++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++)
-          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[SW ? c_asel : 0][2 * p + ri][al[m]][ao[m]]);
+          af[p][ri] = *reinterpret_cast<const bf16x8*>(&As[0][2 * p + ri][al[m]][ao[m]]);
 #pragma unroll
       for (int p = 0; p < 3; p++)
         nai[p] = __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, af[p][1]) ^ SIGN2);
@@ -706,15 +688,8 @@ lc_gemm_mx_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long l
       }
     }
     {
-      // end of a product: of the tile (plain), of T R (sandwich term 1) or of T R' (term 2)
       const int tl = ch / cpb, kin = ch - tl * cpb;
-      if (!SW) {
-        if (kin == cpb - 1) epilogue(tl, false);
-      } else if (kin == 2 * nk - 1) {
-        epilogue(tl, false);
-      } else if (kin == 3 * nk - 1) {
-        epilogue(tl, true);
-      }
+      if (kin == cpb - 1) epilogue(tl, false);
     }
     __syncthreads();
     if (more) {
