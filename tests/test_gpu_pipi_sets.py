@@ -93,7 +93,7 @@ def test_back_to_back_sets_equal_engine(lc, nt, nv):
     assert relerr(c_set.to_numpy(), c_b2b.to_numpy()) < 1e-13
 
 
-@pytest.mark.parametrize("nt,nv,nset", [(3, 16, 2), (2, 64, 2)])
+@pytest.mark.parametrize("nt,nv,nset", [(3, 16, 2), (2, 64, 2), (2, 128, 1)])
 def test_sets_mixed(lc, nt, nv, nset):
     """Mixed form against FP64, with elementals whose (role, time) slices differ in scale by up to
     10^3, so that every per-role scale factor is exercised."""
