@@ -62,3 +62,33 @@ def test_types_reject_bad_shapes(lc):
         engine.covariant_derivative(to_device(U[:, :2].copy()), to_device(V), 2, 2, 2)
     with pytest.raises(ValueError, match="evec"):
         engine.covariant_derivative(to_device(U), to_device(V), 2, 2, 3)
+
+
+def test_types_gauge_invariant_full_lattice(lc):
+    """At the benchmark's lattice (L = 24, Nv = 64; Lt = 2 to keep the host transformation short):
+    the five types of a random SU(3) gauge transform of links and eigenvectors equal those of the
+    untransformed ones, and zero momentum keeps type 1 Hermitian and types 2-4 anti-Hermitian."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    nt, nv, L = 2, 64, 24
+    U, V, ph = _lattice(40, nt, nv, L, L, L, 2)
+    ph[1] = 1.0  # second momentum: p = 0
+    ns = L ** 3
+    G = _su3(np.random.default_rng(41), nt * ns).reshape(nt, ns, 3, 3)
+    Ug = np.empty_like(U)
+    for j in range(3):
+        Ug[:, j] = G @ U[:, j] @ np.conj(np.swapaxes(G[:, _neighbour(L, L, L, j, +1)], -1, -2))
+    Vg = np.ascontiguousarray(np.einsum("tsab,tcsb->tcsa", G, V.reshape(nt, nv, ns, 3)).reshape(nt, nv, 3 * ns))
+    print("elemental types gauge invariance L=%d nv=%d" % (L, nv), flush=True)
+    out = []
+    for u, v in ((U, V), (Ug, Vg)):
+        dv = to_device(v)
+        out.append(engine.build_elementals_types(dv, engine.covariant_derivative(to_device(u), dv, L, L, L),
+                                                 to_device(ph)).to_numpy())
+    for k in range(5):
+        assert relerr(out[1][:, k], out[0][:, k]) < 1e-12, (k, relerr(out[1][:, k], out[0][:, k]))
+    T = out[0][1]  # p = 0
+    for t in range(nt):
+        assert relerr(T[1, t], np.conj(T[1, t].T)) < 1e-12
+        for j in range(3):
+            assert relerr(T[2 + j, t], -np.conj(T[2 + j, t].T)) < 1e-12
