@@ -29,6 +29,12 @@ struct LcView {
   int flags;                // LC_CONJ | LC_TV | LC_TS
   int perm_r[4], perm_c[4];
   double ph_r[4][2], ph_c[4][2];
+  // digit storage only (exact form, gemm_oz.hip): the power-of-two exponents of the stored lines,
+  // line i of stored block row R at ex[voff(es) + R * ebs + i] (ebs = 0: one exponent per line
+  // index shared by all blocks of the matrix)
+  const int* ex;
+  long long es[3];
+  int ebs;
 };
 
 #define LC_MAX_FUSED_TRACES 3
@@ -92,4 +98,39 @@ struct LcTraceGroup {
   int n[3];
   int accumulate;
   long long wg_begin;
+};
+
+// Exact form (gemm_oz.hip). A digit matrix holds every real value x of a stored line as four
+// balanced base-128 digits D0..D3 in [-64, 64] (int8) under the line's exponent E:
+//   x = 2^(E - 27) * (D0 * 128^3 + D1 * 128^2 + D2 * 128 + D3),   |x| <= 2^E,
+// i.e. 27 bits below the largest magnitude of the line. Blocks are P x P (P = Nv rounded up to a
+// multiple of 32, zero padded); block (a,b) of a D-space matrix sits at (a*4+b) * 8 * P^2 bytes,
+// plane q = part * 4 + digit (part 0 real, 1 imaginary) at q * P^2 bytes of the block, and inside
+// a plane element (line, k) at ((k / 32) * P + line) * 32 + k % 32: each 32-line x 32-K chunk is
+// one contiguous kilobyte. Offsets and strides count 8-byte "complex digit elements" (P^2 per
+// block), so a digit view is addressed like a complex-float view of P x P blocks.
+//
+// C = alpha * A * B over a batch with A, B digit views (A read by rows, B by columns, as in the
+// mixed form). Digit products of total order <= 3 accumulate exactly in int32 (one accumulator
+// per order), and the FP64 epilogue combines them, so the only rounding is the digit truncation
+// of the operands. Outputs: FP64 complex C (blocks of nv x nv), and/or digits Cd with one exponent
+// per D-row written to Ce ([4][P] ints per matrix; needs the workgroup strip to cover whole rows,
+// i.e. 4 P <= 256), and fused closing traces as in the other kernels.
+struct LcOzGroup {
+  LcView A, B;
+  void* C;
+  long long cs[3];
+  void* Cd;
+  int* Ce;
+  long long cds[3];         // in complex digit elements
+  long long ces[3];         // in ints
+  double alpha[2];
+  int n[3];
+  int kind;                 // 0 dense x dense, 1 elemental x dense, 2 dense x elemental
+  int ntr;
+  int nseg;                 // column segments of the workgroup strip per D-row
+  long long wg_begin;
+  LcView Y[LC_MAX_FUSED_TRACES];
+  double tr_alpha[LC_MAX_FUSED_TRACES][2];
+  double* part[LC_MAX_FUSED_TRACES];
 };

@@ -181,7 +181,7 @@ Program::Level* Program::open_level(bool is_trace, int variant) {
     l.is_trace = is_trace;
     l.closed = false;
     l.variant = variant;
-    l.first = is_trace ? traces_.size() : gemms_.size();
+    l.first = is_trace ? traces_.size() : variant == kOzGemm ? ozs_.size() : gemms_.size();
     l.count = 0;
     l.total_wg = 0;
     levels_.push_back(l);
@@ -369,6 +369,117 @@ void Program::add_reduce(Level* l, const FusedTrace& f, int n0, int n1, int n2, 
   partial_bytes_ += (bytes + 255) & ~size_t(255);
 }
 
+void Program::gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
+                      const DigitOut* dout, int n0, int n1, int n2, double ar, double ai,
+                      const std::vector<FusedTrace>& traces) {
+  if (mode_ != OZ) fail(LC_ERR_ARG, "exact-form product outside the exact mode");
+  const long long cs[3] = {cs0, cs1, cs2};
+  if (!last_axis_fastest_ || n2 <= 1) {
+    oz_items(A, B, C, cs, dout, n0, n1, n2, ar, ai, traces);
+    return;
+  }
+  // the same batch with its axes relabelled as in batch(): axis 2 fastest, then the second axis
+  const int o1 = second_axis_ == 1 ? 1 : 0, o2 = 1 - o1;
+  auto rot3 = [&](const long long* x, long long* y) {
+    const long long t[3] = {x[0], x[1], x[2]};
+    y[0] = t[2]; y[1] = t[o1]; y[2] = t[o2];
+  };
+  auto rot = [&](LcView v) {
+    rot3(v.s, v.s);
+    rot3(v.es, v.es);
+    return v;
+  };
+  std::vector<FusedTrace> tr = traces;
+  for (FusedTrace& f : tr) {
+    f.Y = rot(f.Y);
+    rot3(f.os, f.os);
+  }
+  long long rc[3];
+  rot3(cs, rc);
+  DigitOut d;
+  if (dout) {
+    d = *dout;
+    rot3(dout->ds, d.ds);
+    rot3(dout->es, d.es);
+  }
+  const int nn[3] = {n0, n1, n2};
+  oz_items(rot(A), rot(B), C, rc, dout ? &d : nullptr, nn[2], nn[o1], nn[o2], ar, ai, tr);
+}
+
+void Program::oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
+                       int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces) {
+  if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+  const int P = oz_pad(nv_);
+  if (nv_ > 128) fail(LC_ERR_ARG, "exact form supports nv <= 128 (got %d)", nv_);
+  if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
+  if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
+  if (!C && !dout && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
+  if ((A.flags & LC_TV) || !(B.flags & LC_TV))
+    fail(LC_ERR_ARG, "exact-form operands must be read along contiguous K (A by rows, B by columns)");
+  if (!A.ex || !B.ex) fail(LC_ERR_ARG, "exact-form operand without exponents");
+  for (const LcView* v : {&A, &B})
+    for (int a = 0; a < 4; a++)
+      for (const double* ph : {v->ph_r[a], v->ph_c[a]})
+        if (!((ph[1] == 0 && (ph[0] == 1 || ph[0] == -1)) || (ph[0] == 0 && (ph[1] == 1 || ph[1] == -1))))
+          fail(LC_ERR_ARG, "exact-form operand phase (%g, %g) is not a power of i", ph[0], ph[1]);
+  const int kind = A.kind == LC_KRON ? 1 : B.kind == LC_KRON ? 2 : 0;
+  // a dense K sum runs over the four blocks of a line: per-row exponents must then be those of
+  // one stored block row (A without, B with the spin-block transpose)
+  if (kind == 0 && ((A.ebs && (A.flags & LC_TS)) || (B.ebs && !(B.flags & LC_TS))))
+    fail(LC_ERR_ARG, "exact-form dense operand lines cross stored block rows");
+  const int NW = P == 32 ? 4 : 8;
+  const int nseg = (4 * P) / (NW * 32);
+  if (dout && nseg != 1) fail(LC_ERR_ARG, "exact-form digit output needs 4 P <= 256 (P = %d)", P);
+  Level* l = open_level(false, kOzGemm);
+  LcOzGroup g;
+  std::memset(&g, 0, sizeof g);
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  for (int d = 0; d < 3; d++) g.cs[d] = cs[d];
+  if (dout) {
+    g.Cd = dout->d;
+    g.Ce = dout->e;
+    for (int d = 0; d < 3; d++) {
+      g.cds[d] = dout->ds[d];
+      g.ces[d] = dout->es[d];
+    }
+  }
+  g.alpha[0] = ar;
+  g.alpha[1] = ai;
+  g.n[0] = n0; g.n[1] = n1; g.n[2] = n2;
+  g.kind = kind;
+  g.nseg = nseg;
+  g.wg_begin = l->total_wg;
+  const long long items = (long long)n0 * n1 * n2;
+  const long long per_item = (long long)(4 * P / 32) * nseg;
+  l->total_wg += items * per_item;
+  l->count++;
+  g.ntr = (int)traces.size();
+  for (int t = 0; t < g.ntr; t++) {
+    const FusedTrace& f = traces[t];
+    g.Y[t] = f.Y;
+    g.tr_alpha[t][0] = f.ar;
+    g.tr_alpha[t][1] = f.ai;
+    add_reduce(l, f, n0, n1, n2, per_item, t);
+    patches_.back().group = ozs_.size();
+    patches_.back().oz = true;
+  }
+  ozs_.push_back(g);
+  const int nterm = kind == 0 ? 4 : 1;
+  gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
+}
+
+void Program::to_digits(const void* x, long long stride, long long nmat, void* y, int* e) {
+  if (mode_ != OZ) fail(LC_ERR_ARG, "digit conversion outside the exact mode");
+  if (nmat <= 0) return;
+  Level* l = open_level(false, kOzConvert);
+  l->first = converts_.size();
+  l->count = 1;
+  converts_.push_back(Convert{x, stride, nmat, y, e});
+  l->closed = true;
+}
+
 void Program::trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
                     long long os2, int n0, int n1, int n2, double ar, double ai, bool accumulate) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
@@ -399,7 +510,8 @@ void Program::run(hipStream_t s) {
   char* pbase = static_cast<char*>(c.partials.p);
   for (const Patch& p : patches_) {
     double* q = reinterpret_cast<double*>(pbase + p.part_off);
-    if (p.slot < LC_MAX_FUSED_TRACES) gemms_[p.group].part[p.slot] = q;
+    if (p.oz) ozs_[p.group].part[p.slot] = q;
+    else if (p.slot < LC_MAX_FUSED_TRACES) gemms_[p.group].part[p.slot] = q;
     else gemms_[p.group].partx = q;
   }
   for (size_t j = 0; j < jobs_.size(); j++) jobs_[j].part = reinterpret_cast<const double*>(pbase + job_part_off_[j]);
@@ -421,9 +533,11 @@ void Program::run(hipStream_t s) {
   const size_t tb = traces_.size() * sizeof(LcTraceGroup);
   const size_t jb = jobs.size() * sizeof(LcReduceJob);
   const size_t bb = begins.size() * sizeof(long long);
+  const size_t ob = ozs_.size() * sizeof(LcOzGroup);
   const size_t toff = (gb + 255) & ~size_t(255);
   const size_t joff = (toff + tb + 255) & ~size_t(255);
   const size_t boff = (joff + jb + 255) & ~size_t(255);
+  const size_t ooff = (boff + bb + 255) & ~size_t(255);
   // descriptors: the shared buffer, or, while an engine call is recorded for a graph, a buffer of
   // this program's own that lives as long as the graph
   DevBuf* dbuf = &c.groups;
@@ -431,12 +545,14 @@ void Program::run(hipStream_t s) {
     c.log->bufs.emplace_back();
     dbuf = &c.log->bufs.back();
   }
-  dbuf->ensure(boff + bb + 256);
+  dbuf->ensure(ooff + ob + 256);
   char* base = static_cast<char*>(dbuf->p);
   if (gb) hip_check(hipMemcpyAsync(base, gemms_.data(), gb, hipMemcpyHostToDevice, s), "upload gemm groups");
   if (tb) hip_check(hipMemcpyAsync(base + toff, traces_.data(), tb, hipMemcpyHostToDevice, s), "upload trace groups");
   if (jb) hip_check(hipMemcpyAsync(base + joff, jobs.data(), jb, hipMemcpyHostToDevice, s), "upload reduce jobs");
   if (bb) hip_check(hipMemcpyAsync(base + boff, begins.data(), bb, hipMemcpyHostToDevice, s), "upload reduce offsets");
+  if (ob) hip_check(hipMemcpyAsync(base + ooff, ozs_.data(), ob, hipMemcpyHostToDevice, s), "upload exact-form groups");
+  const LcOzGroup* dozs = reinterpret_cast<const LcOzGroup*>(base + ooff);
   const LcReduceJob* dj = reinterpret_cast<const LcReduceJob*>(base + joff);
   const long long* db = reinterpret_cast<const long long*>(base + boff);
   // host vectors may be pageable: make sure the copies have landed before they can change
@@ -460,15 +576,28 @@ void Program::run(hipStream_t s) {
     const bool dbl = dbl_, is_trace = l.is_trace, mx = mode_ == MX;
     const int variant = l.variant, count = l.count, nv = nv_;
     const long long total_wg = l.total_wg;
-    const LcTraceGroup* tg = dt + l.first;
-    const LcGemmGroup* gg = dg + l.first;
-    launch(
-        [=](hipStream_t st) {
-          return is_trace ? lc_launch_trace(dbl, tg, count, total_wg, nv, st)
-                          : (mx ? lc_launch_gemm_mx(variant, gg, count, total_wg, nv, st)
-                                : lc_launch_gemm(dbl, gg, count, total_wg, nv, st));
-        },
-        l.is_trace ? "trace launch" : "gemm launch");
+    if (variant == kOzConvert) {
+      const Convert cv = converts_[l.first];
+      const int P = oz_pad(nv);
+      launch([=](hipStream_t st) {
+               return lc_launch_oz_convert(cv.x, cv.stride, cv.nmat, nv, P, 16, 1, cv.e, nullptr, cv.y, nullptr, st);
+             },
+             "digit conversion");
+    } else if (variant == kOzGemm) {
+      const LcOzGroup* og = dozs + l.first;
+      const int P = oz_pad(nv);
+      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, total_wg, nv, P, st); }, "exact gemm launch");
+    } else {
+      const LcTraceGroup* tg = dt + l.first;
+      const LcGemmGroup* gg = dg + l.first;
+      launch(
+          [=](hipStream_t st) {
+            return is_trace ? lc_launch_trace(dbl, tg, count, total_wg, nv, st)
+                            : (mx ? lc_launch_gemm_mx(variant, gg, count, total_wg, nv, st)
+                                  : lc_launch_gemm(dbl, gg, count, total_wg, nv, st));
+          },
+          l.is_trace ? "trace launch" : "gemm launch");
+    }
     if (prof) hip_check(hipEventRecord(c.ev_pool[nev + 1], s), "hipEventRecord");
     nev += 2;
     if (!l.jobs.empty()) {

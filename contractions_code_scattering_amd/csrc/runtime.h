@@ -29,6 +29,13 @@ extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void*
 extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b,
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
+extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long long total_wg, int nv, int P,
+                                        hipStream_t s);
+extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
+                                           int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
+                                           hipStream_t s);
+extern "C" hipError_t lc_launch_oz_decode(const void* y, const int* ex, int nv, int P, int nblk, int mode,
+                                          long long nmat, void* out, hipStream_t s);
 
 namespace lc {
 
@@ -192,6 +199,18 @@ inline FusedTrace fused(const LcView& Y, void* out, long long os0, long long os1
   return f;
 }
 
+// Padded block side of the exact form's digit storage: Nv rounded up to 32, 64 or 128.
+inline int oz_pad(int nv) { return nv <= 32 ? 32 : nv <= 64 ? 64 : 128; }
+
+// Digit output of an exact-form product: digit planes (strides in complex digit elements, P^2 per
+// block) and the per-D-row exponents ([4][P] ints per matrix, strides in ints).
+struct DigitOut {
+  void* d = nullptr;
+  int* e = nullptr;
+  long long ds[3] = {0, 0, 0};
+  long long es[3] = {0, 0, 0};
+};
+
 // A sequence of levels; each level is one grouped GEMM launch or one grouped trace launch
 // (GEMM levels with fused traces are followed by their fixed-order reduction).
 // Groups inside a level are independent; levels run in order on the library stream.
@@ -199,9 +218,11 @@ class Program {
  public:
   // F64 / F32: FP64 or FP32 matrix cores on FP64 / FP32 data. MX: FP32 data, GEMMs on the BF16
   // matrix cores with the exact three-way split (gemm_mx.hip); traces accumulate in FP64.
-  enum Mode { F64, F32, MX };
+  // OZ: digit operands on the int8 matrix cores with exact accumulation (gemm_oz.hip), FP64
+  // outputs, traces and closing factors.
+  enum Mode { F64, F32, MX, OZ };
   explicit Program(int nv, bool dbl) : nv_(nv), dbl_(dbl), mode_(dbl ? F64 : F32) {}
-  Program(int nv, Mode m) : nv_(nv), dbl_(m == F64), mode_(m) {}
+  Program(int nv, Mode m) : nv_(nv), dbl_(m == F64 || m == OZ), mode_(m) {}
 
   // C = alpha * A * B over the batch; C may be nullptr when only fused traces are wanted.
   // MX mode: A and B are split views (A without LC_TV, B with LC_TV, so that both are read
@@ -227,6 +248,15 @@ class Program {
   // With last_axis_fastest: which of axes 0 and 1 runs next (default 0). Axis 1 suits a product
   // whose operand depends on axes 2 and 0 only, which neighbouring items then share.
   void set_second_axis(int axis) { second_axis_ = axis; }
+  // OZ mode: C = alpha * A * B with A, B digit views (A read by rows, B by columns: B carries
+  // LC_TV), phases powers of i. Outputs: FP64 C (optional, strides cs in complex elements of nv^2
+  // blocks), digits with per-D-row exponents (optional; needs 4 P <= 256), fused traces (FP64 Y).
+  void gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
+               const DigitOut* dout, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
+               const std::vector<FusedTrace>& traces = {});
+  // OZ mode: FP64 D-space matrices (nmat of them, `stride` complex elements apart) to digits with
+  // per-D-row exponents (the form gemm_oz writes), as a level of its own.
+  void to_digits(const void* x, long long stride, long long nmat, void* y, int* e);
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -239,10 +269,16 @@ class Program {
   double trace_flops() const { return trace_flops_; }
 
  private:
+  struct Convert {
+    const void* x;
+    long long stride, nmat;
+    void* y;
+    int* e;
+  };
   struct Level {
     bool is_trace;
     bool closed;
-    int variant;  // MX kernel: 1 full-tile, 0 general
+    int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzConvert
     size_t first;
     int count;
     long long total_wg;
@@ -252,7 +288,13 @@ class Program {
     size_t group;
     int slot;
     size_t part_off;  // bytes into the partials buffer
+    bool oz = false;
   };
+  static constexpr int kOzGemm = 10, kOzConvert = 11;
+  std::vector<LcOzGroup> ozs_;
+  std::vector<Convert> converts_;
+  void oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
+                int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces);
   int nv_;
   bool dbl_;
   Mode mode_;

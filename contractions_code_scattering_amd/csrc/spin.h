@@ -114,6 +114,9 @@ inline LcView make_view(const void* base, long long s0, long long s1, long long 
   v.s[0] = s0; v.s[1] = s1; v.s[2] = s2;
   v.kind = kind;
   v.flags = flags;
+  v.ex = nullptr;
+  v.es[0] = v.es[1] = v.es[2] = 0;
+  v.ebs = 0;
   for (int a = 0; a < 4; a++) {
     v.perm_r[a] = a; v.perm_c[a] = a;
     v.ph_r[a][0] = 1; v.ph_r[a][1] = 0;

@@ -262,6 +262,19 @@ int lc_synchronize(void);
  * lane l at 256 + 4l + q. Used by the tests to pin the layout. No reference counterpart. */
 int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d);
 
+/* Diagnostic of the exact form (int8 digit storage, exact integer accumulation): one product
+ * C = A' B' of host FP64 operands, D-space [16][nv][nv] or elemental [nv][nv] (ka / kb = 1),
+ * A' = G(gam4[0]) f_fa(A) G(gam4[1]), B' = G(gam4[2]) f_fb(B) G(gam4[3]) with f a combination of
+ * LC_VIEW_CONJ | LC_VIEW_TV | LC_VIEW_TS (element conjugate, eigenvector transpose, spin-block
+ * transpose) and G(mu) = gamma_mu (0: none; D-space operands only). h_c receives C
+ * [16][nv][nv] in FP64; h_cd (optional, nv <= 64) C after its round trip through the product's
+ * own digit output. nv <= 128. No reference counterpart. */
+#define LC_VIEW_CONJ 1
+#define LC_VIEW_TV 2
+#define LC_VIEW_TS 4
+int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a,
+                        const void* h_b, void* h_c, void* h_cd);
+
 #ifdef __cplusplus
 }
 #endif

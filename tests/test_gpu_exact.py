@@ -1,0 +1,158 @@
+"""Exact form (int8 digit storage, exact integer accumulation) of one product, against numpy.
+
+C = A' B' with A' = G f(A) G', B' = G f(B) G' for D-space or elemental operands, every view flag
+(conjugate, eigenvector transpose, spin-block transpose) and gamma phases of powers of i. The
+integer sums and power-of-two scalings of the exact form are exact, so the product and its digit
+output must equal the numpy model of the format (tests/oz_model.py) bit for bit; against FP64
+the format keeps a product of random operands within about 2e-8 of max |C| (tested at 5e-8).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oz_model as M
+
+from contractions_code_scattering_amd import _capi
+
+pytestmark = pytest.mark.gpu
+
+CONJ, TV, TS = 1, 2, 4
+
+TOL = 5e-8
+
+_capi.register("lc_selftest_oz_gemm", [ctypes.c_int] * 5 + [ctypes.c_void_p] * 5)
+
+I = 1j
+GAMMA = {
+    1: np.array([[0, 0, 0, -I], [0, 0, -I, 0], [0, I, 0, 0], [I, 0, 0, 0]]),
+    2: np.array([[0, 0, 0, 1], [0, 0, -1, 0], [0, -1, 0, 0], [1, 0, 0, 0]], complex),
+    3: np.array([[0, 0, -I, 0], [0, 0, 0, I], [I, 0, 0, 0], [0, -I, 0, 0]]),
+    4: np.array([[0, 0, -1, 0], [0, 0, 0, -1], [-1, 0, 0, 0], [0, -1, 0, 0]], complex),
+    5: np.diag([1, 1, -1, -1]).astype(complex),
+}
+
+
+def _dense(blocks):
+    """[4][4][nv][nv] blocks -> (4 nv) x (4 nv) matrix."""
+    nv = blocks.shape[-1]
+    return blocks.transpose(0, 2, 1, 3).reshape(4 * nv, 4 * nv)
+
+
+def _blocks(m, nv):
+    return m.reshape(4, nv, 4, nv).transpose(0, 2, 1, 3)
+
+
+def _operand(x, kron, flags, gl, gr, nv):
+    if kron:
+        m = x.reshape(nv, nv)
+        if flags & CONJ:
+            m = m.conj()
+        if flags & TV:
+            m = m.T
+        return np.kron(np.eye(4), m)
+    b = x.reshape(4, 4, nv, nv)
+    if flags & CONJ:
+        b = b.conj()
+    if flags & TV:
+        b = b.transpose(0, 1, 3, 2)
+    if flags & TS:
+        b = b.transpose(1, 0, 2, 3)
+    m = _dense(b)
+    if gl:
+        m = np.kron(GAMMA[gl], np.eye(nv)) @ m
+    if gr:
+        m = m @ np.kron(GAMMA[gr], np.eye(nv))
+    return m
+
+
+def _run(nv, ka, kb, fa, fb, gam, seed, spread=0):
+    from contractions_code_scattering_amd import rows
+    rows.init(0)
+    g = np.random.default_rng(seed)
+    na = nv * nv * (1 if ka else 16)
+    nb = nv * nv * (1 if kb else 16)
+    a = (g.standard_normal(na) + 1j * g.standard_normal(na)).astype(np.complex128)
+    b = (g.standard_normal(nb) + 1j * g.standard_normal(nb)).astype(np.complex128)
+    sa = sb = np.ones(4 * nv)
+    if spread:
+        # lines of A (rows) and of B (columns) spread over 2^-spread .. 2^spread, the same for the
+        # line of every block (an input's exponents are per line index, shared by its 16 blocks)
+        sa = np.tile(np.exp2(g.integers(-spread, spread, nv)).astype(float), 4)
+        sb = np.tile(np.exp2(g.integers(-spread, spread, nv)).astype(float), 4)
+        a = _blocks(sa[:, None] * _dense(a.reshape(4, 4, nv, nv)), nv).ravel().copy()
+        b = _blocks(_dense(b.reshape(4, 4, nv, nv)) * sb[None, :], nv).ravel().copy()
+    c = np.zeros(16 * nv * nv, np.complex128)
+    cd = np.zeros(16 * nv * nv, np.complex128)
+    g4 = (ctypes.c_int * 4)(*gam)
+    _capi.check(_capi.lib().lc_selftest_oz_gemm(nv, ka, kb, fa, fb, g4, a.ctypes.data, b.ctypes.data,
+                                               c.ctypes.data, cd.ctypes.data))
+    ref = _operand(a, ka, fa, gam[0], gam[1], nv) @ _operand(b, kb, fb, gam[2], gam[3], nv)
+    got = _dense(c.reshape(4, 4, nv, nv))
+    dig = _dense(cd.reshape(4, 4, nv, nv))
+    if spread:
+        un = 1.0 / (sa[:, None] * sb[None, :])
+        return got * un, dig, ref * un
+    return got, dig, ref
+
+
+def _check(got, ref, tol):
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < tol, err
+    return err
+
+
+def _model(nv, ka, kb, seed):
+    """The model's product of the operands _run draws (plain views): row exponents of A per row
+    index over its blocks, column exponents of B per column index over its blocks."""
+    g = np.random.default_rng(seed)
+    na = nv * nv * (1 if ka else 16)
+    nb = nv * nv * (1 if kb else 16)
+    a = (g.standard_normal(na) + 1j * g.standard_normal(na)).astype(np.complex128)
+    b = (g.standard_normal(nb) + 1j * g.standard_normal(nb)).astype(np.complex128)
+    A = _operand(a, ka, 0, 0, 0, nv)
+    B = _operand(b, kb, 0, 0, 0, nv)
+    ab = A.reshape(4, nv, 4 * nv).transpose(1, 0, 2).reshape(nv, -1)
+    bb = B.reshape(4 * nv, 4, nv).transpose(2, 0, 1).reshape(nv, -1)
+    EA = np.tile(M.line_exps(ab, 1), 4)
+    EB = np.tile(M.line_exps(bb, 1), 4)
+    return M.product(A, B, EA, EB)
+
+
+@pytest.mark.parametrize("nv", [8, 25, 32, 33, 64, 100, 128])
+def test_dense_product(nv):
+    got, dig, ref = _run(nv, 0, 0, 0, 0, (0, 0, 0, 0), 1)
+    assert np.array_equal(got, _model(nv, 0, 0, 1))
+    _check(got, ref, TOL)
+    if nv <= 64:
+        # the digit output: one exponent per D-row, bit for bit as the model writes it
+        assert np.array_equal(dig, M.round_trip(got))
+
+
+@pytest.mark.parametrize("ka,kb", [(1, 0), (0, 1)])
+@pytest.mark.parametrize("nv", [16, 64, 128])
+def test_elemental_products(nv, ka, kb):
+    got, dig, ref = _run(nv, ka, kb, 0, 0, (0, 0, 0, 0), 2)
+    assert np.array_equal(got, _model(nv, ka, kb, 2))
+    _check(got, ref, TOL)
+
+
+@pytest.mark.parametrize("fa,fb", [(CONJ, 0), (0, CONJ), (CONJ | TV | TS, 0), (0, CONJ | TV | TS),
+                                   (TV, TV), (CONJ | TS, TS), (0, CONJ | TS | TV)])
+def test_view_flags(fa, fb):
+    # per-line exponents are shared by the blocks of an input, so every flag combination is exact
+    got, _, ref = _run(32, 0, 0, fa, fb, (0, 0, 0, 0), 3)
+    _check(got, ref, TOL)
+
+
+@pytest.mark.parametrize("gam", [(5, 5, 0, 0), (0, 0, 5, 5), (1, 0, 0, 2), (3, 0, 4, 1), (0, 2, 3, 0)])
+def test_gamma_phases(gam):
+    got, _, ref = _run(32, 0, 0, CONJ, 0, gam, 4)
+    _check(got, ref, TOL)
+
+
+def test_lines_over_many_decades():
+    # rows of A and columns of B scaled by powers of two over 80 binades keep the relative
+    # accuracy of every element's own scale
+    got, _, ref = _run(64, 0, 0, 0, 0, (0, 0, 0, 0), 5, spread=40)
+    _check(got, ref, TOL)
