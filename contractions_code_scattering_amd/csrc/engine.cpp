@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "engine_oz.h"
 #include "runtime.h"
 #include "spin.h"
 
@@ -433,6 +434,11 @@ static int engine_impl(bool mixed, bool sets, const void* d_tau, const void* d_p
     if (nsrc < 1 || src_step < 1 || src0 < 0 || src0 + (long long)(nsrc - 1) * src_step >= nt)
       fail(LC_ERR_ARG, "bad source list src0=%d step=%d nsrc=%d (nt=%d)", src0, src_step, nsrc, nt);
     if (!d_tau || !d_phi || !d_raw || !d_corr || !d_avg) fail(LC_ERR_ARG, "null pointer");
+    if (mixed && ctx().mixed_form == 0) {
+      oz_engine_rho_pipi(sets, d_tau, d_phi, nt, nv, np, src0, src_step, nsrc, max_scratch_bytes, d_raw, d_corr,
+                         d_avg);
+      return LC_OK;
+    }
     EngineShape E{nt, nv, np, src0, src_step, nsrc, (long long)nv * nv, 16LL * nv * nv};
     const long long npair = (long long)nt * nsrc;
     // mixed form: tau and Phi split twice (row-major and transposed planes, 12 B per complex
@@ -555,6 +561,15 @@ int lc_engine_pipi_sets_mixed(const void* d_tau, const void* d_elem, int nt, int
                               void* d_avg) {
   return engine_impl(true, true, d_tau, d_elem, nt, nv, nset, src0, src_step, nsrc, max_scratch_bytes, d_raw,
                      d_corr, d_avg);
+}
+
+int lc_set_mixed_form(int form) {
+  if (form != LC_MIXED_EXACT && form != LC_MIXED_BF16X3) {
+    last_error() = "mixed form must be LC_MIXED_EXACT or LC_MIXED_BF16X3";
+    return LC_ERR_ARG;
+  }
+  ctx().mixed_form = form;
+  return LC_OK;
 }
 
 int lc_profile(int enable) {

@@ -6,8 +6,14 @@
 #include <string>
 #include <vector>
 
+#include "engine_oz.h"
 #include "runtime.h"
 #include "spin.h"
+
+extern "C" hipError_t lc_launch_engine_epilogue(const void* raw, void* corr, void* avg, int np,
+                                                int nsrc, int nt, int src0, int src_step,
+                                                const double* s_tau, const double* const* s_p,
+                                                long long ps, hipStream_t s);
 
 using namespace lc;
 
@@ -43,7 +49,346 @@ struct Scoped {
   char* c() const { return static_cast<char*>(p); }
 };
 
+// ------------------------------------------------------------------------------------------
+// Buffers and views of the exact form. Digit matrices are counted in complex digit elements
+// (8 bytes each, P^2 per block); exponents in ints.
+struct OzShape {
+  int nt, nv, P, src0, step, nsrc;
+  long long m, b;      // FP64 elemental / D-space matrix, complex elements (nv^2, 16 nv^2)
+  long long md, bd;    // digit elemental / D-space matrix, digit elements (P^2, 16 P^2)
+};
+
+// A digit matrix array: planes and exponents (exponents per D-row: 4 P ints per D-space matrix,
+// or per line index: P ints per matrix).
+struct DArr {
+  char* d;
+  int* e;
+};
+
+inline char* bump(char*& cur, size_t bytes) {
+  char* p = cur;
+  cur += (bytes + 255) & ~size_t(255);
+  return p;
+}
+
+inline LcView with_ex(LcView v, const int* ex, long long e0, long long e1, long long e2, int ebs) {
+  v.ex = ex;
+  v.es[0] = e0; v.es[1] = e1; v.es[2] = e2;
+  v.ebs = ebs;
+  return v;
+}
+
+// One elemental factor P1..P4 of the chains in digit storage: row-line planes r (exponents er)
+// and column-line planes t (et) of each matrix, whether the chain uses S or its adjoint, and the
+// stride between momentum items in matrices.
+struct OzRole {
+  const char* r;
+  const char* t;
+  const int* er;
+  const int* et;
+  bool dag;
+  long long stride;
+};
+
+// Where the traced and stored products of one level go. A product stored in digits is written by
+// the GEMM itself when the workgroup strip covers whole D-rows (4 P <= 256); otherwise (P = 128)
+// it is written in FP64 into a staging area and converted after the level.
+struct Staging {
+  char* base = nullptr;
+  char* cur = nullptr;
+  struct Pending {
+    const void* x;
+    long long nmat;
+    DArr y;
+  };
+  std::vector<Pending> pending;
+};
+
+// rho + pi-pi chain of one momentum chunk in the exact form. Per (tf, t0, p), with P1 = P1(t0),
+// P2 = P2(t0), P3 = P3(tf), P4 = P4(tf) and tau = tau(tf, t0):
+//   1  B = P4 tau, L1 = P3 tau (digits); sets only: B4 = P4^dag tau. Per sink: Qa^T = P4^T D(tf)^T;
+//      per source: T1^T = P1^T D(t0)^T, T1'^T = P2^T D(t0)^T (digits, transposed so that the
+//      consumers read them along their stored lines)
+//   2  L = L1 P1 (digits)   -> fish1 = Tr[L tau^dag]; back-to-back also rho_0,1 = -Tr[L K' tau^dag L']
+//      Y = L1 P2            -> star2 = Tr[Y tau^dag]
+//      Rr = B P2 (digits)   -> fish2 = Tr[Rr tau^dag]; back-to-back also
+//                              rho_2 = -conj Tr[Rr K'^dag tau^dag L'^dag]
+//      X = B P1             -> star1 = Tr[X tau^dag]
+//      sets: Lr = B4 P1     -> rho_i = -Tr[Lr K'_i tau^dag L'_i]
+//      Q = P3 Qa (digits, per sink); R^T = T1^T P2^T, R'^T = T1'^T P1^T (FP64, per source)
+//   3  Zh^T = conj(tau) Rr^T (FP64), U^T = (g5 tau^T g5) Q^T (digits)
+//   4  M = tau^dag U        -> square = Tr[M R], dT = Tr[M R']
+//      LW = L W             -> dTHor = Tr[LW Zh]
+// These are the traces of the reference's pi-pi cores (lib/calculate_pipi_*_core.h,
+// lib/contract_kernels_pipi.cu) reassociated as in the D-space rows (capi.cpp): with P (x) 1
+// commuting with spin matrices, Tr[S1 B] = Tr[X tau^dag] with S1 = P1 tau^dag, and so on; the
+// closing factor of the short diagrams is the FP64 perambulator itself.
+void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* tau_r, const char* tau_t,
+                  const int* tex_r, const int* tex_t, const OzRole* R, int p0, int pc, char* scratch,
+                  size_t stage_bytes, void* raw) {
+  const int nt = E.nt, ns = E.nsrc, P = E.P;
+  const long long b = E.b, bd = E.bd, md = E.md;
+  const long long npair = (long long)nt * ns;
+  const long long P4i = 4LL * P;  // exponents per D-space matrix
+  const bool direct = 4 * P <= 256;  // digit outputs written by the GEMMs themselves
+  char* cur = scratch;
+  auto darr = [&](long long nmat) {
+    DArr a;
+    a.d = bump(cur, (size_t)nmat * bd * 8);
+    a.e = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 4));
+    return a;
+  };
+  const long long pairs = (long long)pc * npair;
+  DArr Bd = darr(pairs), L1d = darr(pairs), Ld = darr(pairs), Rrd = darr(pairs), UTd = darr(pairs);
+  DArr B4d = sets ? darr(pairs) : DArr{nullptr, nullptr};
+  DArr Qat = darr((long long)pc * nt), Qd = darr((long long)pc * nt);
+  DArr T1t = darr((long long)pc * ns), T1pt = darr((long long)pc * ns);
+  char* ZhT = bump(cur, (size_t)pairs * b * 16);
+  char* RT = bump(cur, (size_t)pc * ns * b * 16);
+  char* RpT = bump(cur, (size_t)pc * ns * b * 16);
+  Staging stg;
+  stg.base = stg.cur = direct ? nullptr : bump(cur, stage_bytes * pc);
+
+  // batch strides of the item (axis 0 sink tf, axis 1 source k, axis 2 momentum p) for the pair,
+  // per-sink and per-source buffers, in matrices
+  const long long PR[3] = {1, nt, npair}, SK[3] = {1, 0, nt}, SR[3] = {0, 1, ns};
+  auto dview = [&](const DArr& a, const long long* st, int flags) {
+    LcView v = make_view(a.d, st[0] * bd, st[1] * bd, st[2] * bd, LC_DENSE, flags);
+    return with_ex(v, a.e, st[0] * P4i, st[1] * P4i, st[2] * P4i, P);
+  };
+  auto fview = [&](const void* p, const long long* st, int flags) {
+    return make_view(p, st[0] * b, st[1] * b, st[2] * b, LC_DENSE, flags);
+  };
+  auto cs_of = [&](const long long* st, long long* cs) {
+    for (int d = 0; d < 3; d++) cs[d] = st[d] * b;
+  };
+  Program pg(E.nv, Program::OZ);
+  pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
+  // digit output of a product into `a` (items laid out with strides st)
+  auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
+                   const std::vector<FusedTrace>& tr = {}) {
+    if (direct) {
+      DigitOut d;
+      d.d = a.d;
+      d.e = a.e;
+      for (int k = 0; k < 3; k++) {
+        d.ds[k] = st[k] * bd;
+        d.es[k] = st[k] * P4i;
+      }
+      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr);
+      return;
+    }
+    const long long nmat = (long long)n0 * n1 * n2;
+    char* x = bump(stg.cur, (size_t)nmat * b * 16);
+    long long cs[3];
+    cs_of(st, cs);
+    pg.gemm_oz(A, Bv, x, cs[0], cs[1], cs[2], nullptr, n0, n1, n2, 1.0, 0.0, tr);
+    stg.pending.push_back(Staging::Pending{x, nmat, a});
+  };
+  // close a level: a barrier, then the digit conversions of its FP64-staged products
+  auto close_level = [&] {
+    pg.barrier();
+    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e);
+    stg.pending.clear();
+    stg.cur = stg.base;
+  };
+
+  // perambulator views: tau(tf, t0_k) from the row-line (A role) or column-line (B role) planes
+  auto tauv = [&](bool tplanes, int flags) {
+    LcView v = make_view((tplanes ? tau_t : tau_r) + (size_t)E.src0 * bd * 8, nt * bd, E.step * bd, 0, LC_DENSE, flags);
+    return with_ex(v, (tplanes ? tex_t : tex_r) + (long long)E.src0 * P, (long long)nt * P, (long long)E.step * P, 0, 0);
+  };
+  const LcView tau_B = tauv(true, LC_TV);
+  const LcView tau_dag_A = tauv(true, LC_CONJ | LC_TS);
+  const LcView tau_cj_A = tauv(false, LC_CONJ);
+  // doubleTriangleHor's W as a B operand: the reference's element-wise conj(tau_ba) from the
+  // column-line planes, or tau^dag (lc_set_dthor_dagger)
+  const LcView W_B = ctx().dthor_dagger ? tauv(false, LC_DAG) : tauv(true, LC_CONJ | LC_TS | LC_TV);
+  // D(t)^T as a B operand (its columns are the rows of D): per sink and per source
+  LcView Dtf_TB = make_view(tau_r, (nt + 1) * bd, 0, 0, LC_DENSE, LC_TV | LC_TS);
+  Dtf_TB = with_ex(Dtf_TB, tex_r, (long long)(nt + 1) * P, 0, 0, 0);
+  LcView Dt0_TB = make_view(tau_r + (size_t)E.src0 * (nt + 1) * bd * 8, 0, (long long)E.step * (nt + 1) * bd, 0,
+                            LC_DENSE, LC_TV | LC_TS);
+  Dt0_TB = with_ex(Dt0_TB, tex_r + (long long)E.src0 * (nt + 1) * P, 0, (long long)E.step * (nt + 1) * P, 0, 0);
+  // FP64 tau^dag, the closing factor of the short diagrams
+  const LcView tau_dag_Y = make_view(static_cast<const char*>(tau64) + (size_t)E.src0 * b * 16, nt * b, E.step * b, 0,
+                                     LC_DENSE, LC_DAG);
+  // role k (P1..P4) as an operand: X or X^T with X = S or S^dag (adj toggles the adjoint); A
+  // reads the operand by rows, B by columns
+  auto role = [&](int k, bool sink, bool brole, bool transposed, bool adj = false) {
+    const OzRole& Rk = R[k];
+    const bool dag = Rk.dag != adj;
+    const int op = dag ? (transposed ? 3 : 2) : (transposed ? 1 : 0);
+    const bool tl = (op == 1 || op == 2) != brole;  // read the column-line planes
+    const int fl = ((op >= 2) ? LC_CONJ : 0) | (brole ? LC_TV : 0);
+    const long long m0 = (long long)p0 * Rk.stride + (sink ? 0 : E.src0);
+    const char* base = (tl ? Rk.t : Rk.r) + (size_t)m0 * md * 8;
+    const int* ex = (tl ? Rk.et : Rk.er) + m0 * P;
+    LcView v = sink ? make_view(base, md, 0, Rk.stride * md, LC_KRON, fl)
+                    : make_view(base, 0, E.step * md, Rk.stride * md, LC_KRON, fl);
+    return sink ? with_ex(v, ex, P, 0, Rk.stride * P, 0) : with_ex(v, ex, 0, (long long)E.step * P, Rk.stride * P, 0);
+  };
+  char* rw = static_cast<char*>(raw) + (size_t)p0 * ns * nt * 10 * 16;
+  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  auto slot = [&](int k, const LcView& Y, double ar = 1.0) {
+    FusedTrace f = lc::fused(Y, rw + (size_t)k * 16, O0, O1, O2, ar);
+    f.out_double = 1;
+    return f;
+  };
+  const Spin g5 = gamma(5);
+  auto rho_traces = [&](bool conj_form, std::vector<int> pols) {
+    std::vector<FusedTrace> v;
+    for (int i : pols) {
+      const Spin Lp = g5 * rhoj(1, i, false);
+      const Spin Kp = rhoj(1, i, true) * g5;
+      if (!conj_form) {
+        v.push_back(slot(i, right(left(tau_dag_Y, Kp), Lp), -1.0));
+      } else {
+        v.push_back(slot(i, right(left(tau_dag_Y, Kp.adjoint()), Lp.adjoint()), -1.0));
+        v.back().conj = true;
+      }
+    }
+    return v;
+  };
+
+  // level 1
+  dgemm(role(3, true, false, false), tau_B, Bd, PR, nt, ns, pc);
+  dgemm(role(2, true, false, false), tau_B, L1d, PR, nt, ns, pc);
+  if (sets) dgemm(role(3, true, false, false, true), tau_B, B4d, PR, nt, ns, pc);
+  dgemm(role(3, true, false, true), Dtf_TB, Qat, SK, nt, 1, pc);
+  dgemm(role(0, false, false, true), Dt0_TB, T1t, SR, 1, ns, pc);
+  dgemm(role(1, false, false, true), Dt0_TB, T1pt, SR, 1, ns, pc);
+  close_level();
+  // level 2
+  {
+    std::vector<FusedTrace> tl = sets ? std::vector<FusedTrace>{} : rho_traces(false, {0, 1});
+    tl.push_back(slot(8, tau_dag_Y));
+    dgemm(dview(L1d, PR, 0), role(0, false, true, false), Ld, PR, nt, ns, pc, tl);
+    std::vector<FusedTrace> tr = {slot(9, tau_dag_Y)};
+    if (!sets) {
+      const std::vector<FusedTrace> r2 = rho_traces(true, {2});
+      tr.push_back(r2[0]);
+    }
+    dgemm(dview(Bd, PR, 0), role(1, false, true, false), Rrd, PR, nt, ns, pc, tr);
+    pg.gemm_oz(dview(L1d, PR, 0), role(1, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
+               {slot(7, tau_dag_Y)});
+    pg.gemm_oz(dview(Bd, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
+               {slot(6, tau_dag_Y)});
+    if (sets)
+      pg.gemm_oz(dview(B4d, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
+                 rho_traces(false, {0, 1, 2}));
+    dgemm(role(2, true, false, false), dview(Qat, SK, LC_TV | LC_TS), Qd, SK, nt, 1, pc);
+    long long cs[3];
+    cs_of(SR, cs);
+    pg.gemm_oz(dview(T1t, SR, 0), role(1, false, true, true), RT, cs[0], cs[1], cs[2], nullptr, 1, ns, pc);
+    pg.gemm_oz(dview(T1pt, SR, 0), role(0, false, true, true), RpT, cs[0], cs[1], cs[2], nullptr, 1, ns, pc);
+  }
+  close_level();
+  // level 3; U walks the sources before the sinks, so that Q(tf, p) of the momenta in flight is
+  // shared by neighbouring workgroups
+  {
+    long long cs[3];
+    cs_of(PR, cs);
+    pg.gemm_oz(tau_cj_A, dview(Rrd, PR, LC_TV | LC_TS), ZhT, cs[0], cs[1], cs[2], nullptr, nt, ns, pc);
+    pg.set_second_axis(1);
+    dgemm(right(left(tauv(true, LC_TS), g5), g5), dview(Qd, SK, LC_TV | LC_TS), UTd, PR, nt, ns, pc);
+    pg.set_second_axis(0);
+  }
+  close_level();
+  // level 4
+  pg.gemm_oz(tau_dag_A, dview(UTd, PR, LC_TV | LC_TS), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
+             {slot(3, fview(RT, SR, LC_TV | LC_TS)), slot(4, fview(RpT, SR, LC_TV | LC_TS))});
+  pg.gemm_oz(dview(Ld, PR, 0), W_B, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
+             {slot(5, fview(ZhT, PR, LC_TV | LC_TS))});
+  pg.run(ctx().stream);
+}
+
+// Scratch of one momentum (pc = 1): digit pair buffers B, L1, L, Rr, U^T (and B4 for sets), per
+// sink Qa^T, Q, per source T1^T, T1'^T; FP64 Zh^T per pair and R^T, R'^T per source; for P = 128
+// the FP64 staging of the largest level (level 1: three pair products, one per sink, two per
+// source).
+size_t oz_bytes_per_momentum(const OzShape& E, bool sets, size_t* stage) {
+  const long long npair = (long long)E.nt * E.nsrc;
+  const size_t dm = (size_t)E.bd * 8 + 4 * (size_t)E.P * 4 + 512;  // one digit matrix + exponents
+  const size_t fm = (size_t)E.b * 16;
+  const size_t st = 4 * E.P <= 256 ? 0 : ((size_t)(sets ? 3 : 2) * npair + E.nt + 2 * E.nsrc) * fm + 16 * 256;
+  if (stage) *stage = st;
+  return (size_t)(npair * (sets ? 6 : 5) + 2LL * E.nt + 2LL * E.nsrc) * dm + (size_t)(npair + 2LL * E.nsrc) * fm +
+         st + 16 * 256;
+}
+
 }  // namespace
+
+namespace lc {
+
+void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw, void* d_corr, void* d_avg) {
+  if (nv > 128) fail(LC_ERR_ARG, "the exact mixed form supports nv <= 128 (got %d)", nv);
+  const int P = oz_pad(nv);
+  OzShape E{nt, nv, P, src0, src_step, nsrc, (long long)nv * nv, 16LL * nv * nv, (long long)P * P, 16LL * P * P};
+  const int nel = sets ? 4 : 1;
+  const long long nphi = (long long)np * nel * nt;
+  // inputs in digits, row-line and column-line planes, exponents per line index
+  const size_t tau_d = ((size_t)nt * nt * E.bd * 8 + 255) & ~size_t(255);
+  const size_t tau_e = ((size_t)nt * nt * P * 4 + 255) & ~size_t(255);
+  const size_t phi_d = ((size_t)nphi * E.md * 8 + 255) & ~size_t(255);
+  const size_t phi_e = ((size_t)nphi * P * 4 + 255) & ~size_t(255);
+  const size_t head = 2 * (tau_d + tau_e + phi_d + phi_e);
+  size_t stage = 0;
+  const size_t per_p = oz_bytes_per_momentum(E, sets, &stage);
+  if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
+  const long long avail = max_scratch_bytes - (long long)head;
+  int pc = avail > 0 ? (int)(avail / (long long)per_p) : 0;
+  if (pc < 1) fail(LC_ERR_NOMEM, "scratch budget %lld B below one momentum (%zu B)", max_scratch_bytes, per_p + head);
+  if (pc > np) pc = np;
+  ctx().scratch.ensure(head + per_p * pc + 4096);
+  char* base = static_cast<char*>(ctx().scratch.p);
+  char* tau_r = base;
+  char* tau_t = tau_r + tau_d;
+  int* tex_r = reinterpret_cast<int*>(tau_t + tau_d);
+  int* tex_t = reinterpret_cast<int*>(tau_t + tau_d + tau_e);
+  char* phi_r = tau_t + tau_d + 2 * tau_e;
+  char* phi_t = phi_r + phi_d;
+  int* pex_r = reinterpret_cast<int*>(phi_t + phi_d);
+  int* pex_t = reinterpret_cast<int*>(phi_t + phi_d + phi_e);
+  char* scratch = base + head;
+  const size_t stage_b = stage;
+  auto call = [&] {
+    const long long b = E.b, m = E.m;
+    launch([=](hipStream_t st) {
+             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, st);
+           },
+           "tau to digits");
+    launch([=](hipStream_t st) {
+             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, st);
+           },
+           "elementals to digits");
+    OzRole R[4];
+    for (int k = 0; k < 4; k++) {
+      const long long m0 = sets ? (long long)k * nt : 0;
+      R[k] = OzRole{phi_r + (size_t)m0 * E.md * 8, phi_t + (size_t)m0 * E.md * 8, pex_r + m0 * P, pex_t + m0 * P,
+                    !sets && (k == 1 || k == 2), (long long)nel * nt};
+    }
+    for (int p0 = 0; p0 < np; p0 += pc) {
+      const int n = (np - p0) < pc ? (np - p0) : pc;
+      run_chunk_oz(E, sets, d_tau, tau_r, tau_t, tex_r, tex_t, R, p0, n, scratch, stage_b, d_raw);
+    }
+    const double* sp[4] = {nullptr, nullptr, nullptr, nullptr};
+    const long long ps = (long long)nel * nt;
+    launch([=](hipStream_t st) {
+             return lc_launch_engine_epilogue(d_raw, d_corr, d_avg, np, nsrc, nt, src0, src_step, nullptr, sp, ps, st);
+           },
+           "engine epilogue");
+    hip_check(hipStreamSynchronize(ctx().stream), "hipStreamSynchronize");
+  };
+  char gkey[320];
+  std::snprintf(gkey, sizeof gkey, "oz %s nt=%d nv=%d np=%d src0=%d step=%d nsrc=%d chunk=%d tau=%p phi=%p raw=%p "
+                "corr=%p avg=%p", sets ? "pipi_sets" : "rho_pipi", nt, nv, np, src0, src_step, nsrc, pc, d_tau, d_phi,
+                d_raw, d_corr, d_avg);
+  run_engine_call(std::string(gkey), call);
+}
+
+}  // namespace lc
 
 extern "C" {
 

@@ -1,0 +1,12 @@
+// Exact form of the batched engines (engine_oz.cpp): entry points called by engine.cpp and
+// engine_sigma.cpp when the mixed form selected by lc_set_mixed_form is the exact one.
+#pragma once
+
+namespace lc {
+
+// rho + pi-pi set over a resident perambulator: back-to-back (d_phi [np][nt][nv][nv]) or explicit
+// momentum sets (sets: d_phi [np][4][nt][nv][nv]); outputs as lc_engine_rho_pipi. Throws lc::Error.
+void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
+                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw, void* d_corr, void* d_avg);
+
+}  // namespace lc

@@ -130,6 +130,9 @@ struct Context {
   // doubleTriangleHor with tau^dag (conjugate transpose) in place of the reference's element-wise
   // conjugate (lc_set_dthor_dagger)
   bool dthor_dagger = false;
+  // form of the mixed-precision engines (lc_set_mixed_form): 0 exact (int8 digits, exact integer
+  // accumulation, gemm_oz.hip), 1 BF16x3 split with FP32 accumulation (gemm_mx.hip)
+  int mixed_form = 0;
   int tile_bpw = 0;
   int active_bpw = 0;
   bool autotune = false;

@@ -32,11 +32,21 @@ register("lc_engine_graph_report", [_VP])
 register("lc_autotune", [_I])
 register("lc_autotune_report", [_VP])
 register("lc_profile_read", [_VP])
+register("lc_set_mixed_form", [_I])
+
+MIXED_EXACT, MIXED_BF16X3 = 0, 1
+
+
+def set_mixed_form(form):
+    """Form of the mixed-precision engines: MIXED_EXACT (default; int8 digits with per-line
+    exponents, exact integer accumulation, FP64 products and traces) or MIXED_BF16X3 (per-slice
+    rescaled FP32 storage, BF16x3 split products with FP32 accumulation)."""
+    check(lib().lc_set_mixed_form(int(form)))
 
 
 def rho_pipi(tau, phi, src0=0, src_step=1, nsrc=None, max_scratch_bytes=0, out=None, mixed=False):
-    """mixed=False: plain FP64 on the FP64 matrix cores; mixed=True: per-slice rescaled FP32
-    storage, BF16x3 split products on the BF16 matrix cores, FP64 traces and sums."""
+    """mixed=False: plain FP64 on the FP64 matrix cores; mixed=True: the mixed form selected by
+    set_mixed_form (default: int8 digits with exact integer accumulation, FP64 traces and sums)."""
     import numpy as np
     from .devmem import DeviceArray
     if not (isinstance(tau, DeviceArray) and isinstance(phi, DeviceArray)):

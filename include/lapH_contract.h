@@ -93,11 +93,19 @@ int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int
                        int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                        void* d_corr, void* d_avg);
 
-/* Mixed-precision form of lc_engine_rho_pipi, same arguments and outputs: tau and Phi are
- * rescaled per time slice to unit Frobenius norm (scales kept in FP64) and stored as FP32;
- * every product runs on the BF16 matrix cores with an exact three-way BF16 split of each FP32
- * operand (six cross products, FP32 accumulation); closing traces, scale factors and source
- * sums are FP64. */
+/* Mixed-precision form of lc_engine_rho_pipi, same arguments and outputs. Two forms
+ * (lc_set_mixed_form):
+ *  LC_MIXED_EXACT (default): every real value of an operand is held as four int8 digits of base
+ *    128 under a power-of-two exponent per matrix line (27 bits below the line's largest
+ *    magnitude); products run on the int8 matrix cores with exact int32 accumulation of the digit
+ *    pairs of total order <= 4, and every product, closing trace and source sum is FP64
+ *    (nv <= 128);
+ *  LC_MIXED_BF16X3: tau and Phi rescaled per time slice to unit Frobenius norm (scales kept in
+ *    FP64) and stored as FP32; products on the BF16 matrix cores with an exact three-way BF16
+ *    split of each FP32 operand (six cross products, FP32 accumulation); FP64 traces. */
+#define LC_MIXED_EXACT 0
+#define LC_MIXED_BF16X3 1
+int lc_set_mixed_form(int form);
 int lc_engine_rho_pipi_mixed(const void* d_tau, const void* d_phi, int nt, int nv, int np, int src0,
                              int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                              void* d_corr, void* d_avg);

@@ -1,0 +1,116 @@
+"""The exact mixed form (int8 digits, exact integer accumulation) of the batched engines against
+the oracle, at the 1e-6 bar of the precision form.
+
+Per source the diagrams carry the digit truncation of each stored intermediate (27 bits below the
+largest magnitude of its line) and the exact products: against the oracle's FP64 rows every
+correlator array is within 1e-6 of its largest value (normwise, measured about 1e-8). The
+source-averaged C(t) is checked normwise and element-wise where |C(t)| >= 1e-3 max_t |C(t)|;
+the fraction of all C(t) within 1e-6 element-wise is printed.
+"""
+import numpy as np
+import pytest
+
+from contractions_code_scattering_amd import inputs as I
+from conftest import relerr
+from test_gpu_engine import _inputs, _oracle
+
+pytestmark = pytest.mark.gpu
+
+BAR = 1e-6
+
+
+def _avg(ref, src, nt):
+    return np.array([[np.mean([ref[k, (t0 + dt) % nt, d] for k, t0 in enumerate(src)]) for dt in range(nt)]
+                     for d in range(8)])
+
+
+def _check_avg(avg_p, ref_avg, bar=BAR):
+    worst = 0.0
+    n_ok = n_all = 0
+    for d in range(8):
+        c, r = avg_p[d], ref_avg[d]
+        assert relerr(c, r) < bar, (d, relerr(c, r))
+        big = np.abs(r) >= 1e-3 * np.abs(r).max()
+        el = np.abs(c - r)[big] / np.abs(r)[big]
+        worst = max(worst, el.max())
+        assert el.max() < bar, (d, el.max())
+        allel = np.abs(c - r) / np.maximum(np.abs(r), 1e-300)
+        n_ok += int((allel <= bar).sum())
+        n_all += allel.size
+    return worst, n_ok / n_all
+
+
+@pytest.mark.parametrize("nt,nv,np_,src0,step", [(4, 8, 2, 0, 1), (4, 16, 1, 1, 2), (3, 25, 2, 0, 1),
+                                                  (3, 33, 1, 0, 1), (4, 64, 2, 0, 1), (2, 100, 1, 0, 1),
+                                                  (2, 128, 1, 0, 1)])
+def test_exact_mixed_vs_oracle(lc, nt, nv, np_, src0, step):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    tau, phi = _inputs(31, nt, nv, np_)
+    src = list(range(src0, nt, step))
+    _, corr, avg = engine.rho_pipi(to_device(tau), to_device(phi), mixed=True, src0=src0, src_step=step)
+    corr, avg = corr.to_numpy(), avg.to_numpy()
+    worst = 0.0
+    for p in range(np_):
+        ref = _oracle(tau, phi, p, src)
+        for d in range(8):
+            e = relerr(corr[p, :, :, d], ref[:, :, d])
+            worst = max(worst, e)
+            assert e < BAR, (p, d, e)
+        we, frac = _check_avg(avg[p], _avg(ref, src, nt))
+        print("nv=%d p=%d per-source normwise %.2e, C(t) element-wise (|C|>=1e-3 max) %.2e, frac<=1e-6 %.4f"
+              % (nv, p, worst, we, frac))
+
+
+def test_exact_mixed_vs_fp64_benchmark_shape(lc):
+    """Nv = 64 (D = 256) over several momenta and seeds, against the FP64 engine."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    for seed in (3, 7):
+        g = I.rng(seed)
+        nt, nv = 6, 64
+        tau = I.perambulator(g, nt, nv, 0.3)
+        phi = np.stack([I.random_elemental(g, nt, nv) for _ in range(3)])
+        t, f = to_device(tau), to_device(phi)
+        _, c64, a64 = engine.rho_pipi(t, f)
+        _, cmx, amx = engine.rho_pipi(t, f, mixed=True)
+        c64, cmx, a64, amx = c64.to_numpy(), cmx.to_numpy(), a64.to_numpy(), amx.to_numpy()
+        for d in range(8):
+            assert relerr(cmx[..., d], c64[..., d]) < BAR, (seed, d)
+        for p in range(3):
+            _check_avg(amx[p], a64[p])
+        print("seed %d:" % seed, engine.rel_errors(amx, a64))
+
+
+@pytest.mark.parametrize("nv", [16, 64])
+def test_exact_sets_vs_fp64(lc, nv):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    g = I.rng(41)
+    nt, nset = 4, 2
+    tau = I.perambulator(g, nt, nv, 0.3)
+    el = np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(4)]) for _ in range(nset)])
+    t, e = to_device(tau), to_device(el)
+    _, c64, a64 = engine.pipi_sets(t, e)
+    _, cmx, amx = engine.pipi_sets(t, e, mixed=True)
+    c64, cmx = c64.to_numpy(), cmx.to_numpy()
+    for d in range(8):
+        assert relerr(cmx[..., d], c64[..., d]) < BAR, d
+    for p in range(nset):
+        _check_avg(amx.to_numpy()[p], a64.to_numpy()[p])
+
+
+def test_exact_chunking_is_bit_identical(lc):
+    """The exact form's results do not depend on how the momenta are chunked."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    nt, nv, np_ = 3, 16, 3
+    tau, phi = _inputs(32, nt, nv, np_)
+    t, f = to_device(tau), to_device(phi)
+    _, c1, _ = engine.rho_pipi(t, f, mixed=True)
+    _, c2, _ = engine.rho_pipi(t, f, mixed=True, max_scratch_bytes=14 << 20)
+    assert np.array_equal(c1.to_numpy(), c2.to_numpy())
