@@ -2,39 +2,40 @@
 // into and out of digit storage.
 //
 // Number format (lc_internal.h). Every real value x of a stored line (a row of an A operand, a
-// column of a B operand) is four balanced base-128 digits D0..D3 in [-64, 64] under the line's
-// power-of-two exponent E: x = 2^(E-27) * v, v = D0*128^3 + D1*128^2 + D2*128 + D3, with E the
-// smallest integer such that every |x| of the line is below 2^E. The representation keeps 27 bits
-// below the largest magnitude of the line (truncation error <= 2^(E-28)).
+// column of a B operand) is four balanced base-255 digits under the line's power-of-two exponent E:
+//   x = 2^(E-30) * v,   v = D0 255^3 + D1 255^2 + D2 255 + D3,   v = rint(x 2^(30-E)),
+// with D1..D3 in [-127, 127], D0 in [-65, 65] and E the smallest integer such that every |x| of the
+// line is below 2^E: 30 bits below the largest magnitude of the line (truncation <= 2^(E-31)).
+// Base 255 instead of 256 keeps every digit's negation inside int8 (the products need negated
+// digits for the complex product and for phases and conjugation).
 //
-// Product. For C = A B every digit pair (a, b) of total order o = a + b <= 4 is one
-// v_mfma_i32_32x32x32_i8 per real product (13 pairs); the pairs of one order share an int32
-// accumulator. Integer accumulation is exact (|acc_o| <= 2 (pairs of order o) K 2^12 < 2^25 for
-// K <= 512), so the only rounding of a product is the digit truncation of its operands and the
-// dropped orders 5 and 6. Order 4 is needed: a typical element sits a few binades below its line's
-// bound, so its leading digit is small while the lower digits span their whole range, and the
-// order-4 cross terms are of the size of the truncation itself (a numerical model of the format at
-// D = 256 gives 7e-8 of max |C| with orders <= 3, 1.5e-8 with orders <= 4). The epilogue forms,
-// per element,
-//   S = 2^28 acc_0 + 2^21 acc_1 + 2^14 acc_2 + 2^7 acc_3 + acc_4    (exact in FP64, < 2^51)
-//   C_ij = alpha * 2^(E_A,i + E_B,j - 40) * S
-// and either stores it in FP64, traces it in FP64 against FP64 closing factors, or writes it
-// back as digits with a fresh exponent per D-row (the largest |C_ij| of the row over all four
-// blocks, which the workgroup strip covers). The complex product takes four real products
-// (re += Ar Br - Ai Bi, im += Ar Bi + Ai Br): 52 MFMAs per 32 x 32 output tile and K step 32.
+// Product. For C = A B every digit pair (a, b) of total order o = a + b <= 3 is one
+// v_mfma_i32_32x32x32_i8 per real product (10 pairs); the pairs of one order share an int32
+// accumulator, |acc_o| <= 2 (o + 1) K 127^2 < 2^27 for K <= 512, so the integer sums are exact and
+// the only rounding of a product is the digit truncation of its operands and the dropped orders
+// 4..6, all near 2^-31 of the largest products (a numerical model of the format at D = 256 gives
+// about 3e-8 of max |C| for random operands). The epilogue forms, per element,
+//   S = ((acc_0 255 + acc_1) 255 + acc_2) 255 + acc_3          (exact in FP64, < 2^48)
+//   C_ij = alpha * 2^(E_A,i + E_B,j - 60) * rn(S 255^3)
+// (the one rounded step, rn, is relative 2^-53)
+// and stores it in FP64, traces it in FP64 against FP64 closing factors, and/or writes it back as
+// digits with a fresh exponent per D-row (the largest |C_ij| of the row over all four blocks,
+// which the workgroup strip covers). The complex product takes four real products (re += Ar Br -
+// Ai Bi, im += Ar Bi + Ai Br): 40 MFMAs per 32 x 32 output tile and K step of 32.
 //
-// Workgroup: NW wave64s own a strip of 32 rows x 32*NW columns of one item; wave w the 32 x 32
-// tile at columns 32 w. Per K step of 32 the operand chunks (32 lines x 32 K x 8 digit planes,
-// one contiguous kilobyte per plane in memory) go global -> LDS by 16-byte direct loads, in the
-// order of the MFMA fragments (lane l: line l % 32, K bytes 16 (l / 32) .. +15), so every
-// fragment read is a linear 16-byte-per-lane LDS read. Two stage buffers, one barrier per step.
-// The A chunk of a dense product is shared by all waves (their rows are the same); each wave has
-// its own B chunk. View phases (powers of i) and conjugation are applied to the fragments as
-// digit negations (exact: the digits are bounded by 64) and a choice of planes.
+// Workgroup: NW wave64s own a strip of 32 rows x 32 NW columns of one item; wave w the 32 x 32
+// tile at columns 32 w. Operands move in chunks of 32 lines x 32 K x 8 digit planes, one
+// contiguous kilobyte per plane in memory, laid out in the order of the MFMA fragments (lane l:
+// line l % 32, K bytes 16 (l / 32) .. +15). The A chunk of a step, shared by the waves, is loaded
+// one step ahead into registers and written to a double-buffered LDS stage; every wave loads its
+// own B fragments straight from global memory into registers (one coalesced 1 KB load per plane),
+// each digit's planes reloaded for the next step right after their last use in the current one.
+// One barrier per step. View phases (powers of i) and conjugation are applied to the fragments as
+// digit negations and a choice of planes.
 //
 // The MFMA is issued with the B fragment first, so lane l holds row l % 32 of C and the 16
-// columns 8 g + 4 (l / 32) + r (g, r < 4): runs of four consecutive columns, stored as one dword
-// per digit plane.
+// columns 8 g + 4 (l / 32) + r (g, r < 4). The epilogue stages the tile in LDS so that FP64
+// stores, closing-factor loads and digit stores are all whole rows / whole kilobytes.
 #include <climits>
 
 #include "lc_device.h"
@@ -43,34 +44,37 @@ namespace lc {
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
-constexpr int NORD = 5;  // digit-pair orders 0..4 accumulated
+typedef double d2v __attribute__((ext_vector_type(2)));
+constexpr int NORD = 4;  // digit-pair orders 0..3 accumulated
 
 __device__ __forceinline__ i32x16 mfma_i8(const i32x4& x, const i32x4& y, const i32x16& c) {
   return __builtin_amdgcn_mfma_i32_32x32x32_i8(x, y, c, 0, 0, 0);
 }
 
-// Negation of four packed int8 digits in [-64, 64]: in the sign-flipped (biased) domain the
-// digits are 64..192, whose negation 256 - b is ~b + 1 with no carry between bytes.
+// Negation of four packed int8 digits in [-127, 127]: in the sign-flipped (biased) domain the
+// digits are 1..255, whose negation 256 - b is ~b + 1 with no carry between bytes.
 __device__ __forceinline__ unsigned neg4(unsigned x) { return ((x ^ 0x7F7F7F7Fu) + 0x01010101u) ^ 0x80808080u; }
-__device__ __forceinline__ i32x4 neg16(const i32x4& v) {
-  return i32x4{(int)neg4((unsigned)v[0]), (int)neg4((unsigned)v[1]), (int)neg4((unsigned)v[2]),
-               (int)neg4((unsigned)v[3])};
+__device__ __forceinline__ void neg16(i32x4& v) {
+  v = i32x4{(int)neg4((unsigned)v[0]), (int)neg4((unsigned)v[1]), (int)neg4((unsigned)v[2]), (int)neg4((unsigned)v[3])};
 }
 
 // Exponent E of a nonzero value: |x| < 2^E (the frexp exponent); INT_MIN for zero, so that zeros
 // never raise a line's exponent.
 __device__ __forceinline__ int exp_of(double x) { return x == 0.0 ? INT_MIN : __builtin_amdgcn_frexp_exp(x); }
 
-// Digits of x under exponent E (|x| < 2^E): v = rint(x 2^(27-E)), |v| <= 2^27; with the bias
-// 64 (1 + 128 + 128^2) the three low digits are plain 7-bit fields minus 64, the top digit the
-// arithmetic remainder (in [-64, 64]). d[0] is the most significant.
+// Balanced base-255 digits of x under exponent E (|x| < 2^E), d[0] the most significant. With the
+// bias 255 * 4210752 (about 2^30) the running value is a 32-bit unsigned number, and floor(u / 255)
+// is the high half of u * 0x80808081 shifted by 7 (exact for every 32-bit u).
 __device__ __forceinline__ void digits_of(double x, int E, int (&d)[4]) {
-  const int v = (int)__builtin_rint(__builtin_ldexp(x, 27 - E));
-  const int w = v + 1056832;
-  d[3] = (w & 127) - 64;
-  d[2] = ((w >> 7) & 127) - 64;
-  d[1] = ((w >> 14) & 127) - 64;
-  d[0] = w >> 21;
+  int v = (int)__builtin_rint(__builtin_ldexp(x, 30 - E));
+#pragma unroll
+  for (int k = 3; k >= 1; k--) {
+    const unsigned u = (unsigned)(v + 127 + 1073741760);
+    const unsigned q = __umulhi(u, 0x80808081u) >> 7;
+    d[k] = (int)(u - 255u * q) - 127;
+    v = (int)q - 4210752;
+  }
+  d[0] = v;
 }
 __device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
   return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
@@ -92,11 +96,22 @@ __device__ __forceinline__ void block_phase(const LcView& v, int x, int y, doubl
   pi = v.ph_r[x][0] * v.ph_c[y][1] + v.ph_r[x][1] * v.ph_c[y][0];
 }
 
+constexpr int TSTR = 32 * 16 + 16;  // LDS row stride of an FP64 epilogue tile (bytes)
+constexpr int DSTR = 48;            // LDS row stride of a staged digit plane (bytes)
+constexpr int MAXA = 4;             // A chunks per step at most (dense x elemental: one per block column)
+
+template <int NW>
+constexpr int lds_bytes() {
+  return (2 * MAXA * 8 * 1024 > NW * 32 * TSTR) ? 2 * MAXA * 8 * 1024 : NW * 32 * TSTR;
+}
+
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, long long total) {
-  constexpr int NSLOT = NW + 1;
-  __shared__ __attribute__((aligned(16))) unsigned char stage[2][NSLOT][8][1024];
+  // stage buffers [2][MAXA][8 planes][1 KB] during the K loop, per-wave epilogue tiles after it
+  __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
+  __shared__ int rowe[NW][32];
+  __shared__ double tred[LC_MAX_FUSED_TRACES][2][NW];
 
   const long long wg = xcd_order(total);
   if (wg < 0) return;
@@ -110,7 +125,9 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   const int rr = (int)(local - item * wpi);
   const int strip = rr / nseg, seg = rr - strip * nseg;
   const int a = (strip * 32) / P, i0 = (strip * 32) % P;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index is uniform: readfirstlane lets the compiler keep everything derived from it
+  // (slots, view blocks, pointers) in scalar registers and the view fields in scalar loads
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wcol0 = seg * NW * 32;          // first D-column of the workgroup
   const int c = (wcol0 + wave * 32) / P;    // this wave's block column
   const int j0 = (wcol0 + wave * 32) % P;   // and first column inside it
@@ -122,51 +139,72 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   const int kind = g.kind;
   const int ksteps = P >> 5;
   const int nstep = (kind == 0 ? 4 : 1) * ksteps;
-  // operand chunks per step: A chunks (shared by the waves of one block column, or by all) then
-  // B chunks (one per wave, or one per 32-column group of an elemental)
+  // A chunks per step: one shared by all waves, or (dense x elemental) one per block column
   const int nA = kind == 2 ? (NW * 32) / P : 1;
-  const int nB = kind == 2 ? ksteps : NW;
   const int sA = kind == 2 ? (wave * 32) / P : 0;
-  const int sB = kind == 2 ? (j0 >> 5) : wave;
   const int cfirst = wcol0 / P;
   const long long offA = voff(g.A.s, co), offB = voff(g.B.s, co);
 
-  // view blocks of slot s at spin index b of the product
+  // view blocks of A slot s, and of this wave's B, at spin index b of the product
   auto a_block = [&](int s, int b, int& x, int& y) {
     x = a;
     y = kind == 2 ? cfirst + s : (kind == 1 ? a : b);
   };
-  auto b_block = [&](int s, int b, int& x, int& y) {
+  auto b_block = [&](int b, int& x, int& y) {
     if (kind == 2) { x = 0; y = 0; return; }
     x = kind == 1 ? a : b;
-    y = (wcol0 + s * 32) / P;
+    y = c;
   };
-  auto b_line0 = [&](int s) { return kind == 2 ? s * 32 : (wcol0 + s * 32) % P; };
 
-  // direct global -> LDS loads of step st into stage buffer bf: one wave-wide 16-byte load per
-  // (chunk, plane), distributed over the waves
-  const int lane_off = 32 * (lane & 31) + 16 * (lane >> 5);
-  auto issue = [&](int st, int bf) {
+  // A: one wave-wide 16-byte load per (chunk, plane), distributed over the waves, held in registers
+  // for one step and then written to LDS. Lane l writes LDS bytes 16 l .. 16 l + 15 of the plane; it
+  // fetches line 8 (l / 16) + (l % 16) / 2, K half (l % 2) ^ ((l / 16) & 1), so that each 16-lane
+  // quarter reads 256 contiguous bytes and a fragment read (lane f: line f % 32, K half f / 32)
+  // touches 64 distinct banks per quarter. (Register staging rather than direct-to-LDS loads: the
+  // compiler cannot tell an LDS read from an LDS load in flight and would wait for all memory.)
+  const int lq = lane >> 4;
+  const int dma_off = (8 * lq + ((lane & 15) >> 1)) * 32 + 16 * ((lane & 1) ^ (lq & 1));
+  const int fL = lane & 31, fk = lane >> 5;
+  const int frag_off = 16 * (16 * (fL >> 3) + 2 * (fL & 7) + (fk ^ ((fL >> 3) & 1)));
+  const int mem_off = 32 * fL + 16 * fk;  // the same fragment in memory order (B loads, digit stores)
+  constexpr int LPW = 8 * MAXA / NW;      // A loads per wave and step at most
+  const int nload = 8 * nA;
+  i32x4 ra[LPW];
+  auto loadA = [&](int st) {
     const int b = st / ksteps, kk = st - b * ksteps;
-    const int nload = 8 * (nA + nB);
-    for (int f = wave; f < nload; f += NW) {
-      const int s = f >> 3, q = f & 7;
-      const char* src;
-      if (s < nA) {
+#pragma unroll
+    for (int i = 0; i < LPW; i++) {
+      const int f = wave + i * NW;
+      if (i == 0 || f < nload) {  // (f < NW <= 8 <= nload for i = 0: every wave loads at least once)
+        const int s = f >> 3, q = f & 7;
         int x, y;
         a_block(s, b, x, y);
-        src = static_cast<const char*>(g.A.base) + 8 * (offA + stored_block(g.A, x, y) * P2) + q * P2 +
-              ((long long)kk * P + i0) * 32;
-      } else {
-        const int sb = s - nA;
-        int x, y;
-        b_block(sb, b, x, y);
-        src = static_cast<const char*>(g.B.base) + 8 * (offB + stored_block(g.B, x, y) * P2) + q * P2 +
-              ((long long)kk * P + b_line0(sb)) * 32;
+        const char* src = static_cast<const char*>(g.A.base) + 8 * (offA + stored_block(g.A, x, y) * P2) +
+                          q * P2 + ((long long)kk * P + i0) * 32;
+        ra[i] = gload<i32x4>(src + dma_off);
       }
-      __builtin_amdgcn_global_load_lds((const void*)(src + lane_off),
-                                       (__attribute__((address_space(3))) void*)&stage[bf][s][q][0], 16, 0, 0);
     }
+  };
+  auto storeA = [&](int bf) {
+#pragma unroll
+    for (int i = 0; i < LPW; i++) {
+      const int f = wave + i * NW;
+      if (f < nload) *reinterpret_cast<i32x4*>(&lds[((bf * MAXA + (f >> 3)) * 8 + (f & 7)) * 1024 + 16 * lane]) = ra[i];
+    }
+  };
+  // LDS-only barrier: the LDS writes before it are visible after it; memory loads in flight stay so
+  auto lds_barrier = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  };
+  // B: this wave's fragments of step st come straight from memory (plane q at + q P^2)
+  auto bsrc = [&](int st) {
+    const int b = st / ksteps, kk = st - b * ksteps;
+    int x, y;
+    b_block(b, x, y);
+    return static_cast<const char*>(g.B.base) + 8 * (offB + stored_block(g.B, x, y) * P2) +
+           ((long long)kk * P + j0) * 32 + mem_off;
   };
 
   i32x16 re[NORD], im[NORD];
@@ -175,24 +213,22 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     re[o] = i32x16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     im[o] = re[o];
   }
-
   const bool conjA = (g.A.flags & LC_CONJ) != 0, conjB = (g.B.flags & LC_CONJ) != 0;
-  issue(0, 0);
-  for (int st = 0; st < nstep; st++) {
-    const int bf = st & 1;
-    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): this wave's loads of step st have landed
-    __syncthreads();                    // and every other wave's; buffer bf^1 is free again
-    if (st + 1 < nstep) issue(st + 1, bf ^ 1);
-    // phase of this wave's product term: B' = phi * conj?(B), A' = conj?(A)
+  // One K step: A from stage buffer bf, B from br. B digit db's two planes are refreshed for the
+  // next step right after their last use here, so every reload has a whole step to land and one
+  // register set suffices (the compiler waits for each plane before its first use).
+  i32x4 br[8];
+  auto compute = [&](int st, int bf, bool more) {
     const int b = st / ksteps;
     int ax, ay, bx, by;
     a_block(sA, b, ax, ay);
-    b_block(sB, b, bx, by);
+    b_block(b, bx, by);
     double p1r, p1i, p2r, p2i;
     block_phase(g.A, ax, ay, p1r, p1i);
     block_phase(g.B, bx, by, p2r, p2i);
     const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
-    // B'r = nr ? -X : X, B'i = ni ? -Y : Y with (X, Y) = (Br, Bi), or (Bi, Br) when swapped
+    // B' = phase * conj?(B): B'r = nr ? -X : X, B'i = ni ? -Y : Y with (X, Y) = (Br, Bi), or
+    // (Bi, Br) when swapped
     bool sw, nr, ni;
     if (phi == 0.0) {
       sw = false;
@@ -207,48 +243,65 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       nr = conjB;
       ni = true;
     }
-    const unsigned char* bsrc = &stage[bf][nA + sB][0][16 * lane];
-    const unsigned char* asrc = &stage[bf][sA][0][16 * lane];
-    i32x4 br[4], bi[4];
+    const char* nsrc = more ? bsrc(st + 1) : nullptr;
+    const unsigned char* asrc = &lds[((bf * MAXA + sA) * 8) * 1024 + frag_off];
 #pragma unroll
-    for (int d = 0; d < 4; d++) {
-      const i32x4 x = *reinterpret_cast<const i32x4*>(bsrc + d * 1024);
-      const i32x4 y = *reinterpret_cast<const i32x4*>(bsrc + (4 + d) * 1024);
-      br[d] = sw ? y : x;
-      bi[d] = sw ? x : y;
-    }
-    if (nr) {
+    for (int db = 0; db < NORD; db++) {
+      // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
+      // each plane then waits only for its own reload, a whole step after it was issued
+      __builtin_amdgcn_sched_barrier(0);
+      i32x4 xr = sw ? br[4 + db] : br[db];
+      i32x4 xi = sw ? br[db] : br[4 + db];
+      if (nr) neg16(xr);
+      if (ni) neg16(xi);
+      // A fragments are read from LDS for every digit pair (cheap, conflict-free) rather than held
+      // for the whole step: the registers go to the accumulators
 #pragma unroll
-      for (int d = 0; d < 4; d++) br[d] = neg16(br[d]);
-    }
-    if (ni) {
-#pragma unroll
-      for (int d = 0; d < 4; d++) bi[d] = neg16(bi[d]);
-    }
-#pragma unroll
-    for (int da = 0; da < 4; da++) {
-      const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
-      const i32x4 ai0 = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
-      const i32x4 ain = neg16(ai0);
-      const i32x4 ai = conjA ? ain : ai0;   // imaginary part of A'
-      const i32x4 nai = conjA ? ai0 : ain;  // and its negation
-#pragma unroll
-      for (int db = 0; db < 4 && db + da < NORD; db++) {
+      for (int da = 0; da + db < NORD; da++) {
         const int o = da + db;
-        re[o] = mfma_i8(br[db], ar, re[o]);
-        im[o] = mfma_i8(bi[db], ar, im[o]);
-        re[o] = mfma_i8(bi[db], nai, re[o]);
-        im[o] = mfma_i8(br[db], ai, im[o]);
+        const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
+        i32x4 ai = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
+        i32x4 nai = ai;
+        if (conjA) neg16(ai);   // imaginary part of A'
+        else neg16(nai);        // and its negation
+        re[o] = mfma_i8(xr, ar, re[o]);
+        im[o] = mfma_i8(xi, ar, im[o]);
+        re[o] = mfma_i8(xi, nai, re[o]);
+        im[o] = mfma_i8(xr, ai, im[o]);
+      }
+      if (more) {
+        br[db] = gload<i32x4>(nsrc + db * P2);
+        br[4 + db] = gload<i32x4>(nsrc + (4 + db) * P2);
       }
     }
+  };
+
+  // The loop runs every step but the last with unconditional loads, so that the compiler's count
+  // of loads in flight is exact on every path and each wait covers only what it needs. The B
+  // planes are issued in the order the reloads use (digit by digit, real then imaginary).
+  loadA(0);
+  {
+    const char* src = bsrc(0);
+#pragma unroll
+    for (int d = 0; d < NORD; d++) {
+      br[d] = gload<i32x4>(src + d * P2);
+      br[4 + d] = gload<i32x4>(src + (4 + d) * P2);
+    }
   }
+  storeA(0);
+  lds_barrier();
+  for (int st = 0; st + 1 < nstep; st++) {
+    // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
+    // one barrier per step suffices (that buffer was last read in step st - 1)
+    loadA(st + 1);
+    compute(st, st & 1, true);
+    storeA((st + 1) & 1);
+    lds_barrier();
+  }
+  compute(nstep - 1, (nstep - 1) & 1, false);
 
   // ---- epilogue
-  // the stage buffer of step nstep (not the one of the last step, which every wave left at the
-  // last barrier) holds the cross-wave reductions
-  int (*rowe)[32] = reinterpret_cast<int (*)[32]>(&stage[nstep & 1][0][0][0]);
-  double (*tred)[2][NW] = reinterpret_cast<double (*)[2][NW]>(&stage[nstep & 1][1][0][0]);
-  // exponents: row i0 + lane % 32 of A; columns b_line0 + 8 g + 4 (lane / 32) + r of B
+  // exponents: row i0 + lane % 32 of A; columns j0 + 8 g + 4 (lane / 32) + r of B
   int ea;
   {
     int x, y;
@@ -259,9 +312,9 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   int eb[16];
   {
     int x, y;
-    b_block(sB, 0, x, y);
+    b_block(0, x, y);
     const int R = stored_block(g.B, x, y) >> 2;
-    const int* pe = g.B.ex + voff(g.B.es, co) + (long long)R * g.B.ebs + b_line0(sB) + 4 * (lane >> 5);
+    const int* pe = g.B.ex + voff(g.B.es, co) + (long long)R * g.B.ebs + j0 + 4 * (lane >> 5);
 #pragma unroll
     for (int gg = 0; gg < 4; gg++) {
       const i32x4 v = gload<i32x4>(pe + 8 * gg);
@@ -273,63 +326,39 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   double vr[16], vi[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) {
-    // 2^21 (128 acc_0 + acc_1) + 2^7 (128 acc_2 + acc_3) + acc_4: the int32 partial combinations
-    // stay below 2^31 for K <= 512
-    const int hr = re[0][q] * 128 + re[1][q], mr = re[2][q] * 128 + re[3][q];
-    const int hi = im[0][q] * 128 + im[1][q], mi = im[2][q] * 128 + im[3][q];
-    const int e = ea + eb[q] - 40;
-    const double xr = __builtin_ldexp((double)hr * 2097152.0 + (double)mr * 128.0 + (double)re[4][q], e);
-    const double xi = __builtin_ldexp((double)hi * 2097152.0 + (double)mi * 128.0 + (double)im[4][q], e);
+    const int e = ea + eb[q] - 60;
+    const double sr = __builtin_fma(__builtin_fma(__builtin_fma((double)re[0][q], 255.0, (double)re[1][q]), 255.0,
+                                                  (double)re[2][q]), 255.0, (double)re[3][q]);
+    const double si = __builtin_fma(__builtin_fma(__builtin_fma((double)im[0][q], 255.0, (double)im[1][q]), 255.0,
+                                                  (double)im[2][q]), 255.0, (double)im[3][q]);
+    const double xr = __builtin_ldexp(sr * 16581375.0, e), xi = __builtin_ldexp(si * 16581375.0, e);  // 255^3
     vr[q] = alr * xr - ali * xi;
     vi[q] = alr * xi + ali * xr;
   }
-  const int row = i0 + (lane & 31);  // row of C inside block (a, c)
-  auto col_of = [&](int q) { return j0 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3); };
+  // this wave's epilogue tile in LDS (the stage buffers are free once every wave is past the loop)
+  __syncthreads();
+  unsigned char* tile = &lds[wave * 32 * TSTR];
+  const int hcol = 4 * (lane >> 5);
 
-  if (g.C) {
-    Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
+  if (g.C || g.ntr > 0) {
+    // the tile as FP64 complex, row-major with a padded row stride
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int col = col_of(q);
-      if (row < nv && col < nv) gstore_c(C + (long long)row * nv + col, Cplx<double>{vr[q], vi[q]});
+      const int col = 8 * (q >> 2) + hcol + (q & 3);
+      *reinterpret_cast<d2v*>(tile + (lane & 31) * TSTR + col * 16) = d2v{vr[q], vi[q]};
     }
-  }
-
-  if (g.Cd) {
-    // exponent of the D-row: largest |value| over every wave of the strip (the whole row)
-    int em = INT_MIN;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int e1 = exp_of(vr[q]), e2 = exp_of(vi[q]);
-      em = e1 > em ? e1 : em;
-      em = e2 > em ? e2 : em;
-    }
-    const int eo = __shfl_xor(em, 32, 64);
-    em = eo > em ? eo : em;
-    if (lane < 32) rowe[wave][lane] = em;
     __syncthreads();
-    int E = INT_MIN;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      const int x = rowe[w][lane & 31];
-      E = x > E ? x : E;
-    }
-    if (E == INT_MIN) E = 0;
-    if (wave == 0 && lane < 32) g.Ce[voff(g.ces, co) + (long long)a * P + i0 + lane] = E;
-    unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, co) + (long long)(a * 4 + c) * P2) +
-                          ((long long)(j0 >> 5) * P + row) * 32 + 4 * (lane >> 5);
-#pragma unroll
-    for (int gg = 0; gg < 4; gg++) {
-      int dr[4][4], di[4][4];  // [column r][digit]
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        digits_of(vr[4 * gg + r], E, dr[r]);
-        digits_of(vi[4 * gg + r], E, di[r]);
-      }
-#pragma unroll
-      for (int d = 0; d < 4; d++) {
-        gstore(base + d * P2 + 8 * gg, pack4(dr[0][d], dr[1][d], dr[2][d], dr[3][d]));
-        gstore(base + (4 + d) * P2 + 8 * gg, pack4(di[0][d], di[1][d], di[2][d], di[3][d]));
+  }
+  if (g.C) {
+    // whole rows: lanes 0..31 one row, 32..63 the next
+    Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
+#pragma unroll 4
+    for (int m = 0; m < 16; m++) {
+      const int lr = 2 * m + (lane >> 5), lc = lane & 31;
+      const int grow = i0 + lr, gcol = j0 + lc;
+      if (grow < nv && gcol < nv) {
+        const d2v v = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
+        gstore_c(C + (long long)grow * nv + gcol, Cplx<double>{v[0], v[1]});
       }
     }
   }
@@ -341,24 +370,32 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       tr_re[t] = 0;
       tr_im[t] = 0;
       if (t >= g.ntr) continue;
-      // block (c, a) of f(Y), element (col, row)
+      // block (c, a) of f(Y), element (col, row): with LC_TV the stored element (row, col), read
+      // along rows; otherwise the stored (col, row), read along its rows (columns of the tile)
       const BlockRef<double> Y = view_block<double>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
       if (Y.zero) continue;
       const bool tv = (Y.flags & LC_TV) != 0;
       const double cj = (Y.flags & LC_CONJ) ? -1.0 : 1.0;
+      double sr = 0, si = 0;
+      // every load of the trace is issued before the first use (one memory round trip)
       Cplx<double> yv[16];
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int col = col_of(q);
-        const long long idx = tv ? (long long)row * nv + col : (long long)col * nv + row;
-        yv[q] = (row < nv && col < nv) ? gload_c(Y.p + idx) : Cplx<double>{0.0, 0.0};
+      for (int m = 0; m < 16; m++) {
+        const int u = 2 * m + (lane >> 5), w = lane & 31;  // u: stored row of the read, w: along it
+        const int grow = i0 + (tv ? u : w), gcol = j0 + (tv ? w : u);
+        const bool in = grow < nv && gcol < nv;
+        const long long idx = !in ? 0 : tv ? (long long)grow * nv + gcol : (long long)gcol * nv + grow;
+        yv[m] = gload_c(Y.p + idx);
+        if (!in) yv[m] = Cplx<double>{0.0, 0.0};
       }
-      double sr = 0, si = 0;
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const double yr = yv[q].x, yi = cj * yv[q].y;
-        sr += vr[q] * yr - vi[q] * yi;
-        si += vr[q] * yi + vi[q] * yr;
+      for (int m = 0; m < 16; m++) {
+        const int u = 2 * m + (lane >> 5), w = lane & 31;
+        const int lr = tv ? u : w, lc = tv ? w : u;  // tile row / column
+        const d2v x = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
+        const double yr = yv[m].x, yi = cj * yv[m].y;
+        sr += x[0] * yr - x[1] * yi;
+        si += x[0] * yi + x[1] * yr;
       }
       const double pr = sr * Y.sr - si * Y.si, pi = sr * Y.si + si * Y.sr;
       const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
@@ -385,8 +422,54 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         sr += tred[tid][0][w];
         si += tred[tid][1][w];
       }
-      typedef double d2v __attribute__((ext_vector_type(2)));
       gstore(g.part[tid] + 2 * (item * wpi + rr), d2v{sr, si});
+    }
+  }
+
+  if (g.Cd) {
+    // exponent of the D-row: largest |value| over every wave of the strip (the whole row)
+    int em = INT_MIN;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int e1 = exp_of(vr[q]), e2 = exp_of(vi[q]);
+      em = e1 > em ? e1 : em;
+      em = e2 > em ? e2 : em;
+    }
+    const int eo = __shfl_xor(em, 32, 64);
+    em = eo > em ? eo : em;
+    if (lane < 32) rowe[wave][lane] = em;
+    __syncthreads();  // also: every wave is done reading its FP64 tile
+    int E = INT_MIN;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      const int x = rowe[w][lane & 31];
+      E = x > E ? x : E;
+    }
+    if (E == INT_MIN) E = 0;
+    if (wave == 0 && lane < 32) g.Ce[voff(g.ces, co) + (long long)a * P + i0 + lane] = E;
+    // digits of the lane's row into the tile ([plane][row][DSTR bytes]), then whole kilobytes out
+#pragma unroll
+    for (int gg = 0; gg < 4; gg++) {
+      int dr[4][4], di[4][4];  // [column r][digit]
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        digits_of(vr[4 * gg + r], E, dr[r]);
+        digits_of(vi[4 * gg + r], E, di[r]);
+      }
+#pragma unroll
+      for (int d = 0; d < 4; d++) {
+        unsigned char* pr = tile + (lane & 31) * DSTR + 8 * gg + hcol;
+        *reinterpret_cast<unsigned*>(pr + d * 32 * DSTR) = pack4(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
+        *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4(di[0][d], di[1][d], di[2][d], di[3][d]);
+      }
+    }
+    __syncthreads();
+    unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, co) + (long long)(a * 4 + c) * P2) +
+                          ((long long)(j0 >> 5) * P + i0) * 32;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const i32x4 v = *reinterpret_cast<const i32x4*>(tile + q * 32 * DSTR + (lane & 31) * DSTR + 16 * (lane >> 5));
+      gstore(base + q * P2 + mem_off, v);
     }
   }
 }
@@ -471,11 +554,11 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
     }
   }
   __syncthreads();
-  // 8 planes x 1 KB each: thread k writes 16 bytes of plane k / 32 ... in two passes
+  // 8 planes x 1 KB each: 512 16-byte pieces, two per thread
   unsigned char* ob = yr + bi * 8 * P2;
 #pragma unroll
   for (int pass = 0; pass < 2; pass++) {
-    const int k = threadIdx.x + 256 * pass;  // 512 16-byte pieces = 8 planes x 64
+    const int k = threadIdx.x + 256 * pass;
     const int q = k >> 6, piece = k & 63;
     const int ln = piece >> 1, half = piece & 1;
     const uint4 v = *reinterpret_cast<const uint4*>(&tr_[q][ln][16 * half]);
@@ -506,8 +589,8 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
   double v[2];
   for (int part = 0; part < 2; part++) {
     long long s = 0;
-    for (int d = 0; d < 4; d++) s = s * 128 + (signed char)b[(part * 4 + d) * P2];
-    v[part] = __builtin_ldexp((double)s, E - 27);
+    for (int d = 0; d < 4; d++) s = s * 255 + (signed char)b[(part * 4 + d) * P2];
+    v[part] = __builtin_ldexp((double)s, E - 30);
   }
   out[gid] = make_double2(v[0], v[1]);
 }

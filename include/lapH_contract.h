@@ -95,11 +95,11 @@ int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int
 
 /* Mixed-precision form of lc_engine_rho_pipi, same arguments and outputs. Two forms
  * (lc_set_mixed_form):
- *  LC_MIXED_EXACT (default): every real value of an operand is held as four int8 digits of base
- *    128 under a power-of-two exponent per matrix line (27 bits below the line's largest
- *    magnitude); products run on the int8 matrix cores with exact int32 accumulation of the digit
- *    pairs of total order <= 4, and every product, closing trace and source sum is FP64
- *    (nv <= 128);
+ *  LC_MIXED_EXACT (default): every real value of an operand is held as four balanced int8
+ *    digits of base 255 under a power-of-two exponent per matrix line (30 bits below the line's
+ *    largest magnitude); products run on the int8 matrix cores with exact int32 accumulation of
+ *    the digit pairs of total order <= 3, and every product, closing trace and source sum is
+ *    FP64 (nv <= 128);
  *  LC_MIXED_BF16X3: tau and Phi rescaled per time slice to unit Frobenius norm (scales kept in
  *    FP64) and stored as FP32; products on the BF16 matrix cores with an exact three-way BF16
  *    split of each FP32 operand (six cross products, FP32 accumulation); FP64 traces. */

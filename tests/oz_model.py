@@ -1,16 +1,20 @@
 """numpy model of the exact form's number format and product (test infrastructure).
 
 A real value x of a stored line with exponent E (every |x| of the line below 2^E) is four
-balanced base-128 digits: v = rint(x 2^(27-E)), v = D0 128^3 + D1 128^2 + D2 128 + D3 with
-D1..D3 in [-64, 63] and D0 in [-64, 64]. A product sums the digit pairs of total order <= 4 in
-integers and scales the result by 2^(E_A + E_B - 40). Integer sums and power-of-two scaling are
-exact, so the HIP kernel must reproduce this model bit for bit.
+balanced base-255 digits: v = rint(x 2^(30-E)), v = D0 255^3 + D1 255^2 + D2 255 + D3 with
+D1..D3 in [-127, 127] and D0 in [-65, 65]. A product sums the digit pairs of total order <= 3 in
+integers, forms S = ((acc_0 255 + acc_1) 255 + acc_2) 255 + acc_3 (exact), multiplies it by
+255^3 in FP64 (one rounding) and scales by 2^(E_A + E_B - 60). Every step is exact or one IEEE
+rounding, so the HIP kernel must reproduce this model bit for bit.
 """
 import numpy as np
 
 NDIG = 4
-MAXO = 4
+MAXO = 3
+BASE = 255
+FRAC = 30  # bits of a line below its exponent
 _MIN = np.iinfo(np.int64).min
+_BIAS = 255 * 4210752  # about 2^30: keeps every intermediate of the digit split non-negative
 
 
 def exps_of(x):
@@ -26,16 +30,22 @@ def line_exps(z, axis):
 
 
 def digits(x, E):
-    """Balanced base-128 digits of x under exponent E (broadcast), most significant first."""
-    v = np.rint(np.ldexp(x, 27 - np.asarray(E))).astype(np.int64)
-    w = v + 1056832  # 64 (1 + 128 + 128^2)
-    return [w >> 21, ((w >> 14) & 127) - 64, ((w >> 7) & 127) - 64, (w & 127) - 64]
+    """Balanced base-255 digits of x under exponent E (broadcast), most significant first."""
+    v = np.rint(np.ldexp(x, FRAC - np.asarray(E))).astype(np.int64)
+    d = [None] * NDIG
+    for k in range(NDIG - 1, 0, -1):
+        u = v + 127 + _BIAS
+        q = u // BASE
+        d[k] = u - BASE * q - 127
+        v = q - 4210752
+    d[0] = v
+    return d
 
 
 def value(d, E):
     """The value a digit set stands for."""
-    v = ((d[0] * 128 + d[1]) * 128 + d[2]) * 128 + d[3]
-    return np.ldexp(v.astype(np.float64), np.asarray(E) - 27)
+    v = ((d[0] * BASE + d[1]) * BASE + d[2]) * BASE + d[3]
+    return np.ldexp(v.astype(np.float64), np.asarray(E) - FRAC)
 
 
 def product(A, B, EA, EB, maxo=MAXO):
@@ -55,11 +65,12 @@ def product(A, B, EA, EB, maxo=MAXO):
 
     def comb(acc):
         acc = [np.asarray(x, dtype=np.int64) for x in acc]
-        hi = acc[0] * 128 + acc[1]
-        mid = acc[2] * 128 + acc[3]
-        return hi.astype(np.float64) * 2097152.0 + mid.astype(np.float64) * 128.0 + acc[4].astype(np.float64)
+        s = acc[0]
+        for o in range(1, MAXO + 1):
+            s = s * BASE + acc[o]
+        return s.astype(np.float64) * float(BASE ** (NDIG - 1))  # s exact (|s| < 2^53), one rounding
 
-    e = EA[:, None] + EB[None, :] - 40
+    e = EA[:, None] + EB[None, :] - 2 * FRAC
     return np.ldexp(comb(re), e) + 1j * np.ldexp(comb(im), e)
 
 
