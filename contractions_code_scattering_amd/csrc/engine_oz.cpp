@@ -100,6 +100,7 @@ struct Staging {
     const void* x;
     long long nmat;
     DArr y;
+    bool cf;
   };
   std::vector<Pending> pending;
 };
@@ -123,7 +124,7 @@ struct Staging {
 // lib/contract_kernels_pipi.cu) reassociated as in the D-space rows (capi.cpp): with P (x) 1
 // commuting with spin matrices, Tr[S1 B] = Tr[X tau^dag] with S1 = P1 tau^dag, and so on; the
 // closing factor of the short diagrams is the FP64 perambulator itself.
-void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* tau_r, const char* tau_t,
+void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* tau_r, const char* tau_t,
                   const int* tex_r, const int* tex_t, const OzRole* R, int p0, int pc, char* scratch,
                   size_t stage_bytes, void* raw) {
   const int nt = E.nt, ns = E.nsrc, P = E.P;
@@ -143,9 +144,7 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
   DArr B4d = sets ? darr(pairs) : DArr{nullptr, nullptr};
   DArr Qat = darr((long long)pc * nt), Qd = darr((long long)pc * nt);
   DArr T1t = darr((long long)pc * ns), T1pt = darr((long long)pc * ns);
-  char* ZhT = bump(cur, (size_t)pairs * b * 16);
-  char* RT = bump(cur, (size_t)pc * ns * b * 16);
-  char* RpT = bump(cur, (size_t)pc * ns * b * 16);
+  DArr ZhTd = darr(pairs), RTd = darr((long long)pc * ns), RpTd = darr((long long)pc * ns);
   Staging stg;
   stg.base = stg.cur = direct ? nullptr : bump(cur, stage_bytes * pc);
 
@@ -156,9 +155,6 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
     LcView v = make_view(a.d, st[0] * bd, st[1] * bd, st[2] * bd, LC_DENSE, flags);
     return with_ex(v, a.e, st[0] * P4i, st[1] * P4i, st[2] * P4i, P);
   };
-  auto fview = [&](const void* p, const long long* st, int flags) {
-    return make_view(p, st[0] * b, st[1] * b, st[2] * b, LC_DENSE, flags);
-  };
   auto cs_of = [&](const long long* st, long long* cs) {
     for (int d = 0; d < 3; d++) cs[d] = st[d] * b;
   };
@@ -166,11 +162,12 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   // digit output of a product into `a` (items laid out with strides st)
   auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
-                   const std::vector<FusedTrace>& tr = {}) {
+                   const std::vector<FusedTrace>& tr = {}, bool cf = false) {
     if (direct) {
       DigitOut d;
       d.d = a.d;
       d.e = a.e;
+      d.cf = cf;
       for (int k = 0; k < 3; k++) {
         d.ds[k] = st[k] * bd;
         d.es[k] = st[k] * P4i;
@@ -183,12 +180,12 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
     long long cs[3];
     cs_of(st, cs);
     pg.gemm_oz(A, Bv, x, cs[0], cs[1], cs[2], nullptr, n0, n1, n2, 1.0, 0.0, tr);
-    stg.pending.push_back(Staging::Pending{x, nmat, a});
+    stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
   };
   // close a level: a barrier, then the digit conversions of its FP64-staged products
   auto close_level = [&] {
     pg.barrier();
-    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e);
+    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf);
     stg.pending.clear();
     stg.cur = stg.base;
   };
@@ -210,9 +207,11 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
   LcView Dt0_TB = make_view(tau_r + (size_t)E.src0 * (nt + 1) * bd * 8, 0, (long long)E.step * (nt + 1) * bd, 0,
                             LC_DENSE, LC_TV | LC_TS);
   Dt0_TB = with_ex(Dt0_TB, tex_r + (long long)E.src0 * (nt + 1) * P, 0, (long long)E.step * (nt + 1) * P, 0, 0);
-  // FP64 tau^dag, the closing factor of the short diagrams
-  const LcView tau_dag_Y = make_view(static_cast<const char*>(tau64) + (size_t)E.src0 * b * 16, nt * b, E.step * b, 0,
-                                     LC_DENSE, LC_DAG);
+  // tau^dag, the closing factor of the short diagrams, in the closing-factor form under the row
+  // exponents: block (c, a) element (j, i) of tau^dag is conj tau_ac(i, j), read along the rows of
+  // tau like the product tile
+  LcView tau_dag_Y = make_view(tau_cf + (size_t)E.src0 * bd * 8, nt * bd, E.step * bd, 0, LC_DENSE, LC_DAG);
+  tau_dag_Y = with_ex(tau_dag_Y, tex_r + (long long)E.src0 * P, (long long)nt * P, (long long)E.step * P, 0, 0);
   // role k (P1..P4) as an operand: X or X^T with X = S or S^dag (adj toggles the adjoint); A
   // reads the operand by rows, B by columns
   auto role = [&](int k, bool sink, bool brole, bool transposed, bool adj = false) {
@@ -278,18 +277,14 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
       pg.gemm_oz(dview(B4d, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
                  rho_traces(false, {0, 1, 2}));
     dgemm(role(2, true, false, false), dview(Qat, SK, LC_TV | LC_TS), Qd, SK, nt, 1, pc);
-    long long cs[3];
-    cs_of(SR, cs);
-    pg.gemm_oz(dview(T1t, SR, 0), role(1, false, true, true), RT, cs[0], cs[1], cs[2], nullptr, 1, ns, pc);
-    pg.gemm_oz(dview(T1pt, SR, 0), role(0, false, true, true), RpT, cs[0], cs[1], cs[2], nullptr, 1, ns, pc);
+    dgemm(dview(T1t, SR, 0), role(1, false, true, true), RTd, SR, 1, ns, pc, {}, true);
+    dgemm(dview(T1pt, SR, 0), role(0, false, true, true), RpTd, SR, 1, ns, pc, {}, true);
   }
   close_level();
   // level 3; U walks the sources before the sinks, so that Q(tf, p) of the momenta in flight is
   // shared by neighbouring workgroups
   {
-    long long cs[3];
-    cs_of(PR, cs);
-    pg.gemm_oz(tau_cj_A, dview(Rrd, PR, LC_TV | LC_TS), ZhT, cs[0], cs[1], cs[2], nullptr, nt, ns, pc);
+    dgemm(tau_cj_A, dview(Rrd, PR, LC_TV | LC_TS), ZhTd, PR, nt, ns, pc, {}, true);
     pg.set_second_axis(1);
     dgemm(right(left(tauv(true, LC_TS), g5), g5), dview(Qd, SK, LC_TV | LC_TS), UTd, PR, nt, ns, pc);
     pg.set_second_axis(0);
@@ -297,24 +292,22 @@ void run_chunk_oz(const OzShape& E, bool sets, const void* tau64, const char* ta
   close_level();
   // level 4
   pg.gemm_oz(tau_dag_A, dview(UTd, PR, LC_TV | LC_TS), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
-             {slot(3, fview(RT, SR, LC_TV | LC_TS)), slot(4, fview(RpT, SR, LC_TV | LC_TS))});
+             {slot(3, dview(RTd, SR, LC_TV | LC_TS)), slot(4, dview(RpTd, SR, LC_TV | LC_TS))});
   pg.gemm_oz(dview(Ld, PR, 0), W_B, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
-             {slot(5, fview(ZhT, PR, LC_TV | LC_TS))});
+             {slot(5, dview(ZhTd, PR, LC_TV | LC_TS))});
   pg.run(ctx().stream);
 }
 
-// Scratch of one momentum (pc = 1): digit pair buffers B, L1, L, Rr, U^T (and B4 for sets), per
-// sink Qa^T, Q, per source T1^T, T1'^T; FP64 Zh^T per pair and R^T, R'^T per source; for P = 128
-// the FP64 staging of the largest level (level 1: three pair products, one per sink, two per
-// source).
+// Scratch of one momentum (pc = 1): digit pair buffers B, L1, L, Rr, Zh^T, U^T (and B4 for sets),
+// per sink Qa^T, Q, per source T1^T, T1'^T, R^T, R'^T; for P = 128 the FP64 staging of the largest
+// level (level 1: three pair products, one per sink, two per source).
 size_t oz_bytes_per_momentum(const OzShape& E, bool sets, size_t* stage) {
   const long long npair = (long long)E.nt * E.nsrc;
   const size_t dm = (size_t)E.bd * 8 + 4 * (size_t)E.P * 4 + 512;  // one digit matrix + exponents
   const size_t fm = (size_t)E.b * 16;
   const size_t st = 4 * E.P <= 256 ? 0 : ((size_t)(sets ? 3 : 2) * npair + E.nt + 2 * E.nsrc) * fm + 16 * 256;
   if (stage) *stage = st;
-  return (size_t)(npair * (sets ? 6 : 5) + 2LL * E.nt + 2LL * E.nsrc) * dm + (size_t)(npair + 2LL * E.nsrc) * fm +
-         st + 16 * 256;
+  return (size_t)(npair * (sets ? 7 : 6) + 2LL * E.nt + 4LL * E.nsrc) * dm + st + 16 * 256;
 }
 
 }  // namespace
@@ -333,7 +326,7 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
   const size_t tau_e = ((size_t)nt * nt * P * 4 + 255) & ~size_t(255);
   const size_t phi_d = ((size_t)nphi * E.md * 8 + 255) & ~size_t(255);
   const size_t phi_e = ((size_t)nphi * P * 4 + 255) & ~size_t(255);
-  const size_t head = 2 * (tau_d + tau_e + phi_d + phi_e);
+  const size_t head = 3 * tau_d + 2 * (tau_e + phi_d + phi_e);  // tau: row, column and closing-factor form
   size_t stage = 0;
   const size_t per_p = oz_bytes_per_momentum(E, sets, &stage);
   if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
@@ -351,16 +344,18 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
   char* phi_t = phi_r + phi_d;
   int* pex_r = reinterpret_cast<int*>(phi_t + phi_d);
   int* pex_t = reinterpret_cast<int*>(phi_t + phi_d + phi_e);
+  char* tau_cf = phi_t + phi_d + 2 * phi_e;
   char* scratch = base + head;
   const size_t stage_b = stage;
   auto call = [&] {
     const long long b = E.b, m = E.m;
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, st);
+             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, tau_cf,
+                                         st);
            },
            "tau to digits");
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, st);
+             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, nullptr, st);
            },
            "elementals to digits");
     OzRole R[4];
@@ -371,7 +366,7 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
     }
     for (int p0 = 0; p0 < np; p0 += pc) {
       const int n = (np - p0) < pc ? (np - p0) : pc;
-      run_chunk_oz(E, sets, d_tau, tau_r, tau_t, tex_r, tex_t, R, p0, n, scratch, stage_b, d_raw);
+      run_chunk_oz(E, sets, tau_cf, tau_r, tau_t, tex_r, tex_t, R, p0, n, scratch, stage_b, d_raw);
     }
     const double* sp[4] = {nullptr, nullptr, nullptr, nullptr};
     const long long ps = (long long)nel * nt;
@@ -411,9 +406,9 @@ int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4,
     hip_check(hipMemcpyAsync(xb.p, h_b, nbb * nv2 * 16, hipMemcpyHostToDevice, s), "copy b");
     int* exa = static_cast<int*>(ea.p);
     int* exb = static_cast<int*>(eb.p);
-    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, s),
+    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, s),
               "convert a");
-    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, s),
+    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, s),
               "convert b");
     // A is read by rows: its row-line planes, or the column-line ones for an eigenvector transpose;
     // B is read by columns: the column-line planes, or the row-line ones for a transpose

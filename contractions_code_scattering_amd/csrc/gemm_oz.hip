@@ -37,6 +37,7 @@
 // columns 8 g + 4 (l / 32) + r (g, r < 4). The epilogue stages the tile in LDS so that FP64
 // stores, closing-factor loads and digit stores are all whole rows / whole kilobytes.
 #include <climits>
+#include <type_traits>
 
 #include "lc_device.h"
 
@@ -76,8 +77,36 @@ __device__ __forceinline__ void digits_of(double x, int E, int (&d)[4]) {
   }
   d[0] = v;
 }
+// The same digits by FP64 arithmetic (the GEMM epilogue's form): v = rint(x 2^(30-E)), then
+// d0 = round(v / 255^3), and each lower digit the rounded quotient of the exact remainder. Every
+// quotient v / 255^k is at least 1 / (2 255^k) away from a half-integer (255 is odd), far beyond the
+// rounding of v * (1 / 255^k), so the digits are the unique balanced ones above.
+__device__ __forceinline__ void digits_f64(double x, int E, int (&d)[4]) {
+  const double v = __builtin_rint(__builtin_ldexp(x, 30 - E));
+  const double q0 = __builtin_rint(v * (1.0 / 16581375.0));
+  double r = __builtin_fma(-q0, 16581375.0, v);
+  const double q1 = __builtin_rint(r * (1.0 / 65025.0));
+  r = __builtin_fma(-q1, 65025.0, r);
+  const double q2 = __builtin_rint(r * (1.0 / 255.0));
+  r = __builtin_fma(-q2, 255.0, r);
+  d[0] = (int)q0;
+  d[1] = (int)q1;
+  d[2] = (int)q2;
+  d[3] = (int)r;
+}
 __device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
   return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
+}
+
+// Closing-factor form of a P x P block: each value as one int32, v = rint(x 2^(30-E)) under the same
+// line exponent E as its digits (the same block size, 8 bytes per complex element). A 32 x 32
+// chunk (rows 32 R.., columns 32 C..) is 8 KB at ((C P / 32 + R) * 8 KB); inside it element
+// (r, j) sits at (j % 16 / 2) KB + (32 (j / 16) + r) * 16 + 8 (j % 2) bytes, so that load k of
+// lane l, 16 bytes at k KB + 16 l, holds columns 16 (l / 32) + 2 k, + 1 of row l % 32.
+__device__ __forceinline__ long long cf_off(int r, int j, int P) {
+  const int jj = j & 31, rr = r & 31;
+  return ((long long)(j >> 5) * (P >> 5) + (r >> 5)) * 8192 + ((jj & 15) >> 1) * 1024 + ((jj >> 4) * 32 + rr) * 16 +
+         8 * (jj & 1);
 }
 
 // Stored block of view block (x, y); its stored block row is the result / 4.
@@ -94,6 +123,13 @@ __device__ __forceinline__ void block_phase(const LcView& v, int x, int y, doubl
   }
   pr = v.ph_r[x][0] * v.ph_c[y][0] - v.ph_r[x][1] * v.ph_c[y][1];
   pi = v.ph_r[x][0] * v.ph_c[y][1] + v.ph_r[x][1] * v.ph_c[y][0];
+}
+
+// A 64-bit value held in lane l of every wave, as a uniform (scalar) value
+__device__ __forceinline__ long long readlane64(long long v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+  return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
 constexpr int TSTR = 32 * 16 + 16;  // LDS row stride of an FP64 epilogue tile (bytes)
@@ -155,6 +191,77 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     x = kind == 1 ? a : b;
     y = c;
   };
+  const bool conjA = (g.A.flags & LC_CONJ) != 0, conjB = (g.B.flags & LC_CONJ) != 0;
+  // Per K block b (kind 0: b < 4; kinds 1 and 2: b = 0) and A chunk (kind 2: slot s < nA, else the
+  // chunk of block b): stored-block byte offsets and this wave's phase flags, computed once in
+  // lanes 0..3 and read in the K loop with v_readlane, so that the loop touches no view fields.
+  // Flags: bit 0 swaps B's planes, bits 1 and 2 negate its real and imaginary part (the phase
+  // times conj?(B), a power of i).
+  long long aoff_l = 0, boff_l = 0;
+  int flg_l = 0;
+  if (lane < 4) {
+    const int idx = lane;
+    int x, y;
+    if (kind == 2) {
+      x = a;
+      y = cfirst + idx;
+    } else {
+      a_block(0, idx, x, y);
+    }
+    if (y < 4) aoff_l = 8 * (offA + stored_block(g.A, x, y) * P2);
+    b_block(idx, x, y);
+    boff_l = 8 * (offB + stored_block(g.B, x, y) * P2);
+    int ax, ay;
+    a_block(sA, idx, ax, ay);
+    double p1r, p1i, p2r, p2i;
+    block_phase(g.A, ax, ay, p1r, p1i);
+    block_phase(g.B, x, y, p2r, p2i);
+    const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
+    bool sw, nr, ni;
+    if (phi == 0.0) {
+      sw = false;
+      nr = phr < 0.0;
+      ni = (phr < 0.0) != conjB;
+    } else if (phi > 0.0) {  // i (Br + i c Bi) = -c Bi + i Br
+      sw = true;
+      nr = !conjB;
+      ni = false;
+    } else {                 // -i (Br + i c Bi) = c Bi - i Br
+      sw = true;
+      nr = conjB;
+      ni = true;
+    }
+    flg_l = (sw ? 1 : 0) | (nr ? 2 : 0) | (ni ? 4 : 0);
+  }
+  // Per fused trace t, in lane 8 + t: the closing factor's block (c, a) (FP64: element pointer;
+  // digits: byte pointer of the stored block and of its row exponents), its read flags (bit 0
+  // LC_TV, bit 1 LC_CONJ) and the trace's factor, alpha_t times the block phase (zero for a zero
+  // block of an elemental view).
+  long long ty_l = 0, te_l = 0;
+  double tfr_l = 0.0, tfi_l = 0.0;
+  int tfl_l = 0;
+  if (lane >= 8 && lane < 8 + g.ntr) {
+    const int t = lane - 8;
+    const LcView& Yv = g.Y[t];
+    double pr, pi;
+    bool zero = false;
+    if (Yv.ex) {
+      const int blk = stored_block(Yv, c, a);
+      ty_l = (long long)(static_cast<const char*>(Yv.base) + 8 * (voff(Yv.s, co) + blk * P2));  // CF block
+      te_l = (long long)(Yv.ex + voff(Yv.es, co) + (long long)(blk >> 2) * Yv.ebs);
+      block_phase(Yv, c, a, pr, pi);
+    } else {
+      const BlockRef<double> Y = view_block<double>(Yv, voff(Yv.s, co), c, a, nv2);
+      ty_l = (long long)Y.p;
+      pr = Y.sr;
+      pi = Y.si;
+      zero = Y.zero;
+    }
+    const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
+    tfr_l = zero ? 0.0 : ar * pr - ai * pi;
+    tfi_l = zero ? 0.0 : ar * pi + ai * pr;
+    tfl_l = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
+  }
 
   // A: one wave-wide 16-byte load per (chunk, plane), distributed over the waves, held in registers
   // for one step and then written to LDS. Lane l writes LDS bytes 16 l .. 16 l + 15 of the plane; it
@@ -177,10 +284,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       const int f = wave + i * NW;
       if (i == 0 || f < nload) {  // (f < NW <= 8 <= nload for i = 0: every wave loads at least once)
         const int s = f >> 3, q = f & 7;
-        int x, y;
-        a_block(s, b, x, y);
-        const char* src = static_cast<const char*>(g.A.base) + 8 * (offA + stored_block(g.A, x, y) * P2) +
-                          q * P2 + ((long long)kk * P + i0) * 32;
+        const char* src = static_cast<const char*>(g.A.base) + readlane64(aoff_l, kind == 2 ? s : b) + q * P2 +
+                          ((long long)kk * P + i0) * 32;
         ra[i] = gload<i32x4>(src + dma_off);
       }
     }
@@ -201,10 +306,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   // B: this wave's fragments of step st come straight from memory (plane q at + q P^2)
   auto bsrc = [&](int st) {
     const int b = st / ksteps, kk = st - b * ksteps;
-    int x, y;
-    b_block(b, x, y);
-    return static_cast<const char*>(g.B.base) + 8 * (offB + stored_block(g.B, x, y) * P2) +
-           ((long long)kk * P + j0) * 32 + mem_off;
+    return static_cast<const char*>(g.B.base) + readlane64(boff_l, b) + ((long long)kk * P + j0) * 32 + mem_off;
   };
 
   i32x16 re[NORD], im[NORD];
@@ -213,36 +315,13 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     re[o] = i32x16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     im[o] = re[o];
   }
-  const bool conjA = (g.A.flags & LC_CONJ) != 0, conjB = (g.B.flags & LC_CONJ) != 0;
   // One K step: A from stage buffer bf, B from br. B digit db's two planes are refreshed for the
   // next step right after their last use here, so every reload has a whole step to land and one
   // register set suffices (the compiler waits for each plane before its first use).
   i32x4 br[8];
   auto compute = [&](int st, int bf, bool more) {
-    const int b = st / ksteps;
-    int ax, ay, bx, by;
-    a_block(sA, b, ax, ay);
-    b_block(b, bx, by);
-    double p1r, p1i, p2r, p2i;
-    block_phase(g.A, ax, ay, p1r, p1i);
-    block_phase(g.B, bx, by, p2r, p2i);
-    const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
-    // B' = phase * conj?(B): B'r = nr ? -X : X, B'i = ni ? -Y : Y with (X, Y) = (Br, Bi), or
-    // (Bi, Br) when swapped
-    bool sw, nr, ni;
-    if (phi == 0.0) {
-      sw = false;
-      nr = phr < 0.0;
-      ni = (phr < 0.0) != conjB;
-    } else if (phi > 0.0) {  // i (Br + i c Bi) = -c Bi + i Br
-      sw = true;
-      nr = !conjB;
-      ni = false;
-    } else {                 // -i (Br + i c Bi) = c Bi - i Br
-      sw = true;
-      nr = conjB;
-      ni = true;
-    }
+    const int fl = __builtin_amdgcn_readlane(flg_l, st / ksteps);
+    const bool sw = (fl & 1) != 0, n1 = (fl & 2) != 0, n2 = (fl & 4) != 0;
     const char* nsrc = more ? bsrc(st + 1) : nullptr;
     const unsigned char* asrc = &lds[((bf * MAXA + sA) * 8) * 1024 + frag_off];
 #pragma unroll
@@ -250,24 +329,28 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
       // each plane then waits only for its own reload, a whole step after it was issued
       __builtin_amdgcn_sched_barrier(0);
-      i32x4 xr = sw ? br[4 + db] : br[db];
-      i32x4 xi = sw ? br[db] : br[4 + db];
-      if (nr) neg16(xr);
-      if (ni) neg16(xi);
+      // B' = X' + i Y' (X' = +-X, Y' = +-Y, (X, Y) = (Br, Bi) or swapped), A' = Ar + i s Ai with
+      // s = -1 for a conjugated A:
+      //   re += X' Ar + (-s Y') Ai,   im += Y' Ar + (s X') Ai
+      // so one digit plane of B is needed with both signs (one negation per digit)
+      i32x4 xp = sw ? br[4 + db] : br[db];
+      i32x4 yp = sw ? br[db] : br[4 + db];
+      if (n1) neg16(xp);
+      if (n2) neg16(yp);
+      i32x4 xa = xp, ya = yp;
+      if (conjA) neg16(xa);
+      else neg16(ya);
       // A fragments are read from LDS for every digit pair (cheap, conflict-free) rather than held
       // for the whole step: the registers go to the accumulators
 #pragma unroll
       for (int da = 0; da + db < NORD; da++) {
         const int o = da + db;
         const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
-        i32x4 ai = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
-        i32x4 nai = ai;
-        if (conjA) neg16(ai);   // imaginary part of A'
-        else neg16(nai);        // and its negation
-        re[o] = mfma_i8(xr, ar, re[o]);
-        im[o] = mfma_i8(xi, ar, im[o]);
-        re[o] = mfma_i8(xi, nai, re[o]);
-        im[o] = mfma_i8(xr, ai, im[o]);
+        const i32x4 ai = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
+        re[o] = mfma_i8(xp, ar, re[o]);
+        im[o] = mfma_i8(yp, ar, im[o]);
+        re[o] = mfma_i8(ya, ai, re[o]);
+        im[o] = mfma_i8(xa, ai, im[o]);
       }
       if (more) {
         br[db] = gload<i32x4>(nsrc + db * P2);
@@ -298,10 +381,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     storeA((st + 1) & 1);
     lds_barrier();
   }
-  compute(nstep - 1, (nstep - 1) & 1, false);
-
-  // ---- epilogue
-  // exponents: row i0 + lane % 32 of A; columns j0 + 8 g + 4 (lane / 32) + r of B
+  // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
+  // j0 + 8 g + 4 (lane / 32) + r of B
   int ea;
   {
     int x, y;
@@ -322,6 +403,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
     }
   }
+  compute(nstep - 1, (nstep - 1) & 1, false);
+
+  // ---- epilogue
+  const bool unit = g.alpha[0] == 1.0 && g.alpha[1] == 0.0;
   const double alr = g.alpha[0], ali = g.alpha[1];
   double vr[16], vi[16];
 #pragma unroll
@@ -332,22 +417,43 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     const double si = __builtin_fma(__builtin_fma(__builtin_fma((double)im[0][q], 255.0, (double)im[1][q]), 255.0,
                                                   (double)im[2][q]), 255.0, (double)im[3][q]);
     const double xr = __builtin_ldexp(sr * 16581375.0, e), xi = __builtin_ldexp(si * 16581375.0, e);  // 255^3
-    vr[q] = alr * xr - ali * xi;
-    vi[q] = alr * xi + ali * xr;
+    vr[q] = unit ? xr : alr * xr - ali * xi;
+    vi[q] = unit ? xi : alr * xi + ali * xr;
   }
-  // this wave's epilogue tile in LDS (the stage buffers are free once every wave is past the loop)
-  __syncthreads();
+  // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
+  // whole row), exchanged through LDS
+  if (g.Cd) {
+    double mx = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
+    int em = exp_of(mx);
+    const int eo = __shfl_xor(em, 32, 64);
+    em = eo > em ? eo : em;
+    if (lane < 32) rowe[wave][lane] = em;
+  }
+  // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
+  // exponents are visible
+  lds_barrier();
+  int E = 0;
+  if (g.Cd) {
+    E = INT_MIN;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+      const int x = rowe[w][lane & 31];
+      E = x > E ? x : E;
+    }
+    if (E == INT_MIN) E = 0;
+    if (wave == 0 && lane < 32) g.Ce[voff(g.ces, co) + (long long)a * P + i0 + lane] = E;
+  }
+  // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
+  // reads it (LDS operations of one wave complete in order: no barrier)
   unsigned char* tile = &lds[wave * 32 * TSTR];
   const int hcol = 4 * (lane >> 5);
-
-  if (g.C || g.ntr > 0) {
-    // the tile as FP64 complex, row-major with a padded row stride
+  const int w_ = lane & 31, h_ = lane >> 5;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int col = 8 * (q >> 2) + hcol + (q & 3);
-      *reinterpret_cast<d2v*>(tile + (lane & 31) * TSTR + col * 16) = d2v{vr[q], vi[q]};
-    }
-    __syncthreads();
+  for (int q = 0; q < 16; q++) {
+    const int col = 8 * (q >> 2) + hcol + (q & 3);
+    *reinterpret_cast<d2v*>(tile + (lane & 31) * TSTR + col * 16) = d2v{vr[q], vi[q]};
   }
   if (g.C) {
     // whole rows: lanes 0..31 one row, 32..63 the next
@@ -364,97 +470,163 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   }
 
   if (g.ntr > 0) {
-    double tr_re[LC_MAX_FUSED_TRACES], tr_im[LC_MAX_FUSED_TRACES];
+    // Every closing-factor load of the product's traces is issued before the first use (one
+    // memory round trip). Trace t reads block (c, a) of f(Y_t), element (col, row): with LC_TV the
+    // stored element (row, col), read along stored rows; otherwise the stored (col, row). Lane l
+    // reads stored row r0 + 2 m + l / 32, column c0 + l % 32.
+    const int w = lane & 31, h = lane >> 5;
+    const bool full = i0 + 32 <= nv && j0 + 32 <= nv;
+    auto traces = [&](auto NT) {
+      constexpr int N = decltype(NT)::value;
+      Cplx<double> yv[N][16];
 #pragma unroll
-    for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
-      tr_re[t] = 0;
-      tr_im[t] = 0;
-      if (t >= g.ntr) continue;
-      // block (c, a) of f(Y), element (col, row): with LC_TV the stored element (row, col), read
-      // along rows; otherwise the stored (col, row), read along its rows (columns of the tile)
-      const BlockRef<double> Y = view_block<double>(g.Y[t], voff(g.Y[t].s, co), c, a, nv2);
-      if (Y.zero) continue;
-      const bool tv = (Y.flags & LC_TV) != 0;
-      const double cj = (Y.flags & LC_CONJ) ? -1.0 : 1.0;
-      double sr = 0, si = 0;
-      // every load of the trace is issued before the first use (one memory round trip)
-      Cplx<double> yv[16];
+      for (int t = 0; t < N; t++) {
+        const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(readlane64(ty_l, 8 + t));
+        const bool tv = (__builtin_amdgcn_readlane(tfl_l, 8 + t) & 1) != 0;
+        const int r0 = tv ? i0 : j0, c0 = tv ? j0 : i0;
+        const Cplx<double>* q = p + (long long)(r0 + h) * nv + c0 + w;
+        const long long rs = 2LL * nv;
+        if (full) {
 #pragma unroll
-      for (int m = 0; m < 16; m++) {
-        const int u = 2 * m + (lane >> 5), w = lane & 31;  // u: stored row of the read, w: along it
-        const int grow = i0 + (tv ? u : w), gcol = j0 + (tv ? w : u);
-        const bool in = grow < nv && gcol < nv;
-        const long long idx = !in ? 0 : tv ? (long long)grow * nv + gcol : (long long)gcol * nv + grow;
-        yv[m] = gload_c(Y.p + idx);
-        if (!in) yv[m] = Cplx<double>{0.0, 0.0};
+          for (int m = 0; m < 16; m++) yv[t][m] = gload_c(q + m * rs);
+        } else {
+#pragma unroll
+          for (int m = 0; m < 16; m++) {
+            const bool in = c0 + w < nv && r0 + 2 * m + h < nv;
+            yv[t][m] = gload_c(in ? q + m * rs : p);
+            if (!in) yv[t][m] = Cplx<double>{0.0, 0.0};
+          }
+        }
       }
+      double tr[N][2];
 #pragma unroll
-      for (int m = 0; m < 16; m++) {
-        const int u = 2 * m + (lane >> 5), w = lane & 31;
-        const int lr = tv ? u : w, lc = tv ? w : u;  // tile row / column
-        const d2v x = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
-        const double yr = yv[m].x, yi = cj * yv[m].y;
-        sr += x[0] * yr - x[1] * yi;
-        si += x[0] * yi + x[1] * yr;
+      for (int t = 0; t < N; t++) {
+        const int fl = __builtin_amdgcn_readlane(tfl_l, 8 + t);
+        const bool tv = (fl & 1) != 0;
+        const double cj = (fl & 2) ? -1.0 : 1.0;
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+          const int u = 2 * m + h;
+          const int lr = tv ? u : w, lc = tv ? w : u;  // tile row / column
+          const d2v x = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
+          const double yr = yv[t][m].x, yi = cj * yv[t][m].y;
+          sr += x[0] * yr - x[1] * yi;
+          si += x[0] * yi + x[1] * yr;
+        }
+        const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
+        const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
+        tr[t][0] = wave_sum(fr * sr - fi * si);
+        tr[t][1] = wave_sum(fr * si + fi * sr);
       }
-      const double pr = sr * Y.sr - si * Y.si, pi = sr * Y.si + si * Y.sr;
-      const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
-      tr_re[t] = ar * pr - ai * pi;
-      tr_im[t] = ar * pi + ai * pr;
-    }
+      if (lane == 0) {
 #pragma unroll
-    for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
-      tr_re[t] = wave_sum(tr_re[t]);
-      tr_im[t] = wave_sum(tr_im[t]);
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int t = 0; t < LC_MAX_FUSED_TRACES; t++) {
-        tred[t][0][wave] = tr_re[t];
-        tred[t][1][wave] = tr_im[t];
+        for (int t = 0; t < N; t++) {
+          tred[t][0][wave] = tr[t][0];
+          tred[t][1][wave] = tr[t][1];
+        }
       }
+    };
+    // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
+    // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its C
+    // tile, and lane l holds row l % 32, columns 16 (l / 32) .. +15 under that row's exponent: each
+    // lane sums FP64 products with exact integers and scales once.
+    auto traces_cf = [&](auto NT) {
+      constexpr int N = decltype(NT)::value;
+      i32x4 yd[N][8];
+      int ey[N];
+#pragma unroll
+      for (int t = 0; t < N; t++) {
+        const unsigned char* yb = reinterpret_cast<const unsigned char*>(readlane64(ty_l, 8 + t)) +
+                                  ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * lane;
+#pragma unroll
+        for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
+        ey[t] = gload<int>(reinterpret_cast<const int*>(readlane64(te_l, 8 + t)) + i0 + w);
+      }
+      double tr[N][2];
+#pragma unroll
+      for (int t = 0; t < N; t++) {
+        const double cj = (__builtin_amdgcn_readlane(tfl_l, 8 + t) & 2) ? -1.0 : 1.0;
+        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+#pragma unroll
+          for (int p2 = 0; p2 < 2; p2++) {
+            const d2v x = *reinterpret_cast<const d2v*>(tile + w * TSTR + (16 * h + 2 * k + p2) * 16);
+            const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
+            s1 += x[0] * yr;
+            s2 += x[1] * yi;
+            s3 += x[0] * yi;
+            s4 += x[1] * yr;
+          }
+        }
+        // x (yr + i cj yi), scaled by the row's 2^(E - 30)
+        const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
+        const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
+        const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
+        const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
+        tr[t][0] = wave_sum(fr * sr - fi * si);
+        tr[t][1] = wave_sum(fr * si + fi * sr);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < N; t++) {
+          tred[t][0][wave] = tr[t][0];
+          tred[t][1][wave] = tr[t][1];
+        }
+      }
+    };
+    if (g.Y[0].ex) {
+      if (g.ntr == 1) traces_cf(std::integral_constant<int, 1>{});
+      else if (g.ntr == 2) traces_cf(std::integral_constant<int, 2>{});
+      else traces_cf(std::integral_constant<int, 3>{});
+    } else {
+      if (g.ntr == 1) traces(std::integral_constant<int, 1>{});
+      else if (g.ntr == 2) traces(std::integral_constant<int, 2>{});
+      else traces(std::integral_constant<int, 3>{});
     }
-    __syncthreads();
+    lds_barrier();
     if (tid < g.ntr) {
       double sr = 0, si = 0;
 #pragma unroll
-      for (int w = 0; w < NW; w++) {
-        sr += tred[tid][0][w];
-        si += tred[tid][1][w];
+      for (int w2 = 0; w2 < NW; w2++) {
+        sr += tred[tid][0][w2];
+        si += tred[tid][1][w2];
       }
       gstore(g.part[tid] + 2 * (item * wpi + rr), d2v{sr, si});
     }
   }
 
-  if (g.Cd) {
-    // exponent of the D-row: largest |value| over every wave of the strip (the whole row)
-    int em = INT_MIN;
+  if (g.Cd && g.cf) {
+    // closing-factor form: lane l converts row l % 32, columns 16 (l / 32) + 2 k, + 1 of its tile
+    // and stores them as one coalesced kilobyte per k
+    unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, co) + (long long)(a * 4 + c) * P2) +
+                          ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * lane;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const d2v x0 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k) * 16);
+      const d2v x1 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k + 1) * 16);
+      const i32x4 v = {(int)__builtin_rint(__builtin_ldexp(x0[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x0[1], 30 - E)),
+                       (int)__builtin_rint(__builtin_ldexp(x1[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x1[1], 30 - E))};
+      gstore(base + 1024 * k, v);
+    }
+  } else if (g.Cd) {
+    // the lane's 16 values back from its tile, their digits into the same tile
+    // ([plane][row][DSTR bytes], overwriting it: this wave's reads come first), then whole
+    // kilobytes out
+    d2v xv[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int e1 = exp_of(vr[q]), e2 = exp_of(vi[q]);
-      em = e1 > em ? e1 : em;
-      em = e2 > em ? e2 : em;
+      const int col = 8 * (q >> 2) + hcol + (q & 3);
+      xv[q] = *reinterpret_cast<const d2v*>(tile + (lane & 31) * TSTR + col * 16);
     }
-    const int eo = __shfl_xor(em, 32, 64);
-    em = eo > em ? eo : em;
-    if (lane < 32) rowe[wave][lane] = em;
-    __syncthreads();  // also: every wave is done reading its FP64 tile
-    int E = INT_MIN;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      const int x = rowe[w][lane & 31];
-      E = x > E ? x : E;
-    }
-    if (E == INT_MIN) E = 0;
-    if (wave == 0 && lane < 32) g.Ce[voff(g.ces, co) + (long long)a * P + i0 + lane] = E;
-    // digits of the lane's row into the tile ([plane][row][DSTR bytes]), then whole kilobytes out
 #pragma unroll
     for (int gg = 0; gg < 4; gg++) {
       int dr[4][4], di[4][4];  // [column r][digit]
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        digits_of(vr[4 * gg + r], E, dr[r]);
-        digits_of(vi[4 * gg + r], E, di[r]);
+        digits_f64(xv[4 * gg + r][0], E, dr[r]);
+        digits_f64(xv[4 * gg + r][1], E, di[r]);
       }
 #pragma unroll
       for (int d = 0; d < 4; d++) {
@@ -463,7 +635,6 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4(di[0][d], di[1][d], di[2][d], di[3][d]);
       }
     }
-    __syncthreads();
     unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, co) + (long long)(a * 4 + c) * P2) +
                           ((long long)(j0 >> 5) * P + i0) * 32;
 #pragma unroll
@@ -516,7 +687,7 @@ __global__ void __launch_bounds__(256) lc_oz_exps_kernel(const double2* __restri
 __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __restrict__ x, long long stride, int nv,
                                                            int P, int nblk, int mode, const int* __restrict__ ex_r,
                                                            const int* __restrict__ ex_t, unsigned char* __restrict__ yr,
-                                                           unsigned char* __restrict__ yt) {
+                                                           unsigned char* __restrict__ yt, int2* __restrict__ ycf) {
   __shared__ __attribute__((aligned(16))) unsigned char tr_[8][32][32], tt_[8][32][32];
   const int tpb = P >> 5;
   const long long t = blockIdx.x;
@@ -535,6 +706,10 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
     if (row < nv && col < nv) v = p[(long long)row * nv + col];
     int dr[4], di[4];
     const int er = ex_r[m * nr + (mode ? (blk >> 2) * P : 0) + row];
+    if (ycf)
+      ycf[(bi * 8 * P2 + cf_off(row, col, P)) / 8] =
+          make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - er)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - er)));
+    if (!yr) continue;
     digits_of(v.x, er, dr);
     digits_of(v.y, er, di);
 #pragma unroll
@@ -553,6 +728,7 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
       }
     }
   }
+  if (!yr) return;  // (uniform: no barrier is skipped by part of the workgroup)
   __syncthreads();
   // 8 planes x 1 KB each: 512 16-byte pieces, two per thread
   unsigned char* ob = yr + bi * 8 * P2;
@@ -612,17 +788,18 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long 
 
 // x: nmat matrices of nblk blocks (nv x nv complex doubles), `stride` complex elements apart.
 // mode 0: y_rows / y_trans (nmat * nblk * 8 P^2 bytes each), ex_r / ex_t (nmat * P ints each).
-// mode 1: y_rows and ex_r (nmat * 4 P ints) only.
+// mode 1: y_rows and ex_r (nmat * 4 P ints) only. y_cf (optional, the size of y_rows): the
+// closing-factor form under the row exponents; y_rows may be null when only that form is wanted.
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           hipStream_t s) {
+                                           void* y_cf, hipStream_t s) {
   if (nmat <= 0) return hipSuccess;
   hipLaunchKernelGGL(lc::lc_oz_exps_kernel, dim3((unsigned)nmat), dim3(256), 0, s, (const double2*)x, stride, nv, P,
                      nblk, mode, ex_r, ex_t);
   const long long tiles = nmat * nblk * (long long)(P / 32) * (P / 32);
   hipLaunchKernelGGL(lc::lc_oz_digits_kernel, dim3((unsigned)tiles), dim3(256), 0, s, (const double2*)x, stride, nv,
                      P, nblk, mode, (const int*)ex_r, (const int*)ex_t, (unsigned char*)y_rows,
-                     (unsigned char*)y_trans);
+                     (unsigned char*)y_trans, (int2*)y_cf);
   return hipGetLastError();
 }
 

@@ -123,6 +123,7 @@ struct LcOzGroup {
   long long cs[3];
   void* Cd;
   int* Ce;
+  int cf;                   // Cd in the closing-factor form instead of digits
   long long cds[3];         // in complex digit elements
   long long ces[3];         // in ints
   double alpha[2];

@@ -427,6 +427,14 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   // one stored block row (A without, B with the spin-block transpose)
   if (kind == 0 && ((A.ebs && (A.flags & LC_TS)) || (B.ebs && !(B.flags & LC_TS))))
     fail(LC_ERR_ARG, "exact-form dense operand lines cross stored block rows");
+  // closing factors: all FP64, or all in the closing-factor form (int32 per value under line
+  // exponents) read in the orientation of the product (LC_TV), dense
+  for (const FusedTrace& f : traces) {
+    if ((f.Y.ex != nullptr) != (traces[0].Y.ex != nullptr))
+      fail(LC_ERR_ARG, "exact-form closing factors of one product mix FP64 and the closing-factor form");
+    if (f.Y.ex && (f.Y.kind != LC_DENSE || !(f.Y.flags & LC_TV)))
+      fail(LC_ERR_ARG, "a closing factor in the closing-factor form must be a dense view read along its stored rows (LC_TV)");
+  }
   const int NW = P == 32 ? 4 : 8;
   const int nseg = (4 * P) / (NW * 32);
   if (dout && nseg != 1) fail(LC_ERR_ARG, "exact-form digit output needs 4 P <= 256 (P = %d)", P);
@@ -440,6 +448,7 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   if (dout) {
     g.Cd = dout->d;
     g.Ce = dout->e;
+    g.cf = dout->cf ? 1 : 0;
     for (int d = 0; d < 3; d++) {
       g.cds[d] = dout->ds[d];
       g.ces[d] = dout->es[d];
@@ -470,13 +479,13 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
 }
 
-void Program::to_digits(const void* x, long long stride, long long nmat, void* y, int* e) {
+void Program::to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf) {
   if (mode_ != OZ) fail(LC_ERR_ARG, "digit conversion outside the exact mode");
   if (nmat <= 0) return;
   Level* l = open_level(false, kOzConvert);
   l->first = converts_.size();
   l->count = 1;
-  converts_.push_back(Convert{x, stride, nmat, y, e});
+  converts_.push_back(Convert{x, stride, nmat, y, e, cf});
   l->closed = true;
 }
 
@@ -580,7 +589,8 @@ void Program::run(hipStream_t s) {
       const Convert cv = converts_[l.first];
       const int P = oz_pad(nv);
       launch([=](hipStream_t st) {
-               return lc_launch_oz_convert(cv.x, cv.stride, cv.nmat, nv, P, 16, 1, cv.e, nullptr, cv.y, nullptr, st);
+               return lc_launch_oz_convert(cv.x, cv.stride, cv.nmat, nv, P, 16, 1, cv.e, nullptr, cv.cf ? nullptr : cv.y,
+                                           nullptr, cv.cf ? cv.y : nullptr, st);
              },
              "digit conversion");
     } else if (variant == kOzGemm) {

@@ -33,7 +33,7 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long 
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           hipStream_t s);
+                                           void* y_cf, hipStream_t s);
 extern "C" hipError_t lc_launch_oz_decode(const void* y, const int* ex, int nv, int P, int nblk, int mode,
                                           long long nmat, void* out, hipStream_t s);
 
@@ -210,6 +210,7 @@ inline int oz_pad(int nv) { return nv <= 32 ? 32 : nv <= 64 ? 64 : 128; }
 struct DigitOut {
   void* d = nullptr;
   int* e = nullptr;
+  bool cf = false;  // the closing-factor form (int32 per value) instead of digits
   long long ds[3] = {0, 0, 0};
   long long es[3] = {0, 0, 0};
 };
@@ -259,7 +260,7 @@ class Program {
                const std::vector<FusedTrace>& traces = {});
   // OZ mode: FP64 D-space matrices (nmat of them, `stride` complex elements apart) to digits with
   // per-D-row exponents (the form gemm_oz writes), as a level of its own.
-  void to_digits(const void* x, long long stride, long long nmat, void* y, int* e);
+  void to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf = false);
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -277,6 +278,7 @@ class Program {
     long long stride, nmat;
     void* y;
     int* e;
+    bool cf;
   };
   struct Level {
     bool is_trace;
