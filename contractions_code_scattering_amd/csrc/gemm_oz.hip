@@ -141,6 +141,13 @@ constexpr int lds_bytes() {
   return (2 * MAXA * 8 * 1024 > NW * 32 * TSTR) ? 2 * MAXA * 8 * 1024 : NW * 32 * TSTR;
 }
 
+// Persistent schedule. The launch holds a few workgroups per XCD (as many as fit at once); XCD x
+// owns the contiguous range of logical workgroups [x per, (x + 1) per), per = ceil(total / 8), and
+// its j-th workgroup walks j, j + nj, j + 2 nj, ... of that range (nj workgroups per XCD). A
+// logical workgroup is one strip of one item. As nj is a multiple of the workgroups per item, a
+// workgroup keeps its strip from item to item: the per-strip tables (view blocks, phases, trace
+// blocks) are built when the product group changes, and the first operand loads of the next item
+// are issued before the epilogue of the current one.
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, long long total) {
@@ -149,119 +156,19 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   __shared__ int rowe[NW][32];
   __shared__ double tred[LC_MAX_FUSED_TRACES][2][NW];
 
-  const long long wg = xcd_order(total);
-  if (wg < 0) return;
-  const int gi = find_group(groups, ng, wg);
-  const LcOzGroup& g = groups[gi];
-  const int nseg = g.nseg;
-  const int nstrip = (4 * P) >> 5;
-  const long long wpi = (long long)nstrip * nseg;  // workgroups per item
-  const long long local = wg - g.wg_begin;
-  const long long item = local / wpi;
-  const int rr = (int)(local - item * wpi);
-  const int strip = rr / nseg, seg = rr - strip * nseg;
-  const int a = (strip * 32) / P, i0 = (strip * 32) % P;
-  // the wave index is uniform: readfirstlane lets the compiler keep everything derived from it
-  // (slots, view blocks, pointers) in scalar registers and the view fields in scalar loads
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wcol0 = seg * NW * 32;          // first D-column of the workgroup
-  const int c = (wcol0 + wave * 32) / P;    // this wave's block column
-  const int j0 = (wcol0 + wave * 32) % P;   // and first column inside it
+  const int xcd = blockIdx.x & 7, nj = gridDim.x >> 3;
+  const long long per = (total + 7) >> 3;
+  const long long wend = (xcd + 1) * per < total ? (xcd + 1) * per : total;
+  long long wg = xcd * per + (blockIdx.x >> 3);
+  if (wg >= wend) return;
 
-  long long co[3];
-  decode_item(item, g.n, co);
+  // the wave index is uniform: readfirstlane lets the compiler keep everything derived from it in
+  // scalar registers
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const long long P2 = (long long)P * P;
   const long long nv2 = (long long)nv * nv;
-  const int kind = g.kind;
   const int ksteps = P >> 5;
-  const int nstep = (kind == 0 ? 4 : 1) * ksteps;
-  // A chunks per step: one shared by all waves, or (dense x elemental) one per block column
-  const int nA = kind == 2 ? (NW * 32) / P : 1;
-  const int sA = kind == 2 ? (wave * 32) / P : 0;
-  const int cfirst = wcol0 / P;
-  const long long offA = voff(g.A.s, co), offB = voff(g.B.s, co);
-
-  // view blocks of A slot s, and of this wave's B, at spin index b of the product
-  auto a_block = [&](int s, int b, int& x, int& y) {
-    x = a;
-    y = kind == 2 ? cfirst + s : (kind == 1 ? a : b);
-  };
-  auto b_block = [&](int b, int& x, int& y) {
-    if (kind == 2) { x = 0; y = 0; return; }
-    x = kind == 1 ? a : b;
-    y = c;
-  };
-  const bool conjA = (g.A.flags & LC_CONJ) != 0, conjB = (g.B.flags & LC_CONJ) != 0;
-  // Per K block b (kind 0: b < 4; kinds 1 and 2: b = 0) and A chunk (kind 2: slot s < nA, else the
-  // chunk of block b): stored-block byte offsets and this wave's phase flags, computed once in
-  // lanes 0..3 and read in the K loop with v_readlane, so that the loop touches no view fields.
-  // Flags: bit 0 swaps B's planes, bits 1 and 2 negate its real and imaginary part (the phase
-  // times conj?(B), a power of i).
-  long long aoff_l = 0, boff_l = 0;
-  int flg_l = 0;
-  if (lane < 4) {
-    const int idx = lane;
-    int x, y;
-    if (kind == 2) {
-      x = a;
-      y = cfirst + idx;
-    } else {
-      a_block(0, idx, x, y);
-    }
-    if (y < 4) aoff_l = 8 * (offA + stored_block(g.A, x, y) * P2);
-    b_block(idx, x, y);
-    boff_l = 8 * (offB + stored_block(g.B, x, y) * P2);
-    int ax, ay;
-    a_block(sA, idx, ax, ay);
-    double p1r, p1i, p2r, p2i;
-    block_phase(g.A, ax, ay, p1r, p1i);
-    block_phase(g.B, x, y, p2r, p2i);
-    const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
-    bool sw, nr, ni;
-    if (phi == 0.0) {
-      sw = false;
-      nr = phr < 0.0;
-      ni = (phr < 0.0) != conjB;
-    } else if (phi > 0.0) {  // i (Br + i c Bi) = -c Bi + i Br
-      sw = true;
-      nr = !conjB;
-      ni = false;
-    } else {                 // -i (Br + i c Bi) = c Bi - i Br
-      sw = true;
-      nr = conjB;
-      ni = true;
-    }
-    flg_l = (sw ? 1 : 0) | (nr ? 2 : 0) | (ni ? 4 : 0);
-  }
-  // Per fused trace t, in lane 8 + t: the closing factor's block (c, a) (FP64: element pointer;
-  // digits: byte pointer of the stored block and of its row exponents), its read flags (bit 0
-  // LC_TV, bit 1 LC_CONJ) and the trace's factor, alpha_t times the block phase (zero for a zero
-  // block of an elemental view).
-  long long ty_l = 0, te_l = 0;
-  double tfr_l = 0.0, tfi_l = 0.0;
-  int tfl_l = 0;
-  if (lane >= 8 && lane < 8 + g.ntr) {
-    const int t = lane - 8;
-    const LcView& Yv = g.Y[t];
-    double pr, pi;
-    bool zero = false;
-    if (Yv.ex) {
-      const int blk = stored_block(Yv, c, a);
-      ty_l = (long long)(static_cast<const char*>(Yv.base) + 8 * (voff(Yv.s, co) + blk * P2));  // CF block
-      te_l = (long long)(Yv.ex + voff(Yv.es, co) + (long long)(blk >> 2) * Yv.ebs);
-      block_phase(Yv, c, a, pr, pi);
-    } else {
-      const BlockRef<double> Y = view_block<double>(Yv, voff(Yv.s, co), c, a, nv2);
-      ty_l = (long long)Y.p;
-      pr = Y.sr;
-      pi = Y.si;
-      zero = Y.zero;
-    }
-    const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
-    tfr_l = zero ? 0.0 : ar * pr - ai * pi;
-    tfi_l = zero ? 0.0 : ar * pi + ai * pr;
-    tfl_l = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
-  }
+  const int nstrip = (4 * P) >> 5;
 
   // A: one wave-wide 16-byte load per (chunk, plane), distributed over the waves, held in registers
   // for one step and then written to LDS. Lane l writes LDS bytes 16 l .. 16 l + 15 of the plane; it
@@ -269,32 +176,185 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   // quarter reads 256 contiguous bytes and a fragment read (lane f: line f % 32, K half f / 32)
   // touches 64 distinct banks per quarter. (Register staging rather than direct-to-LDS loads: the
   // compiler cannot tell an LDS read from an LDS load in flight and would wait for all memory.)
-  const int lq = lane >> 4;
-  const int dma_off = (8 * lq + ((lane & 15) >> 1)) * 32 + 16 * ((lane & 1) ^ (lq & 1));
-  const int fL = lane & 31, fk = lane >> 5;
-  const int frag_off = 16 * (16 * (fL >> 3) + 2 * (fL & 7) + (fk ^ ((fL >> 3) & 1)));
-  const int mem_off = 32 * fL + 16 * fk;  // the same fragment in memory order (B loads, digit stores)
-  constexpr int LPW = 8 * MAXA / NW;      // A loads per wave and step at most
-  const int nload = 8 * nA;
+  // Lane offsets are recomputed at the top of every item from an opaque copy of the lane index, so
+  // that the compiler does not hoist their many derived addresses out of the item loop (they would
+  // stay live across it and spill).
+  int ln = lane, dma_off = 0, frag_off = 0, mem_off = 0, w_ = 0, h_ = 0;
+  auto lane_consts = [&]() {
+    ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int lq = ln >> 4;
+    dma_off = (8 * lq + ((ln & 15) >> 1)) * 32 + 16 * ((ln & 1) ^ (lq & 1));
+    w_ = ln & 31;
+    h_ = ln >> 5;
+    frag_off = 16 * (16 * (w_ >> 3) + 2 * (w_ & 7) + (h_ ^ ((w_ >> 3) & 1)));
+    mem_off = 32 * w_ + 16 * h_;  // the same fragment in memory order (B loads, digit stores)
+  };
+
+  // ---- per-(group, strip) state
+  int gi = find_group(groups, ng, wg);
+  int kind = 0, nseg = 1, wpi_log = 0, rr = 0, a = 0, i0 = 0, c = 0, j0 = 0, nA = 1, sA = 0, nstep = 0, ntr = 0;
+  bool conjA = false;
+  long long ea_blk = 0, eb_blk = 0;  // exponent offsets of this strip's A rows and wave's B columns
+  // Per K block b (kind 0: b < 4; kinds 1 and 2: b = 0) and A chunk (kind 2: slot s < nA, else the
+  // chunk of block b), in lanes 0..3: stored-block byte offsets and this wave's phase flags, read in
+  // the K loop with v_readlane. Flags: bit 0 swaps B's planes, bits 1 and 2 negate its real and
+  // imaginary part (the phase times conj?(B), a power of i). Per fused trace t, in lane 8 + t: the
+  // closing factor's block (c, a) (FP64: byte offset of the block from the view base; closing-factor
+  // form: of the stored block and of its row exponents), its read flags (bit 0 LC_TV, bit 1 LC_CONJ)
+  // and the trace's factor, alpha_t times the block phase (zero for a zero block of an elemental
+  // view). Item offsets are added per item.
+  long long aoff_l = 0, boff_l = 0, ty_l = 0, te_l = 0;
+  int flg_l = 0, tfl_l = 0;
+  double tfr_l = 0.0, tfi_l = 0.0;
+  auto tables = [&]() {
+    const LcOzGroup& g = groups[gi];
+    kind = g.kind;
+    nseg = g.nseg;
+    ntr = g.ntr;
+    wpi_log = 31 - __builtin_clz((unsigned)(nstrip * nseg));
+    rr = (int)((wg - g.wg_begin) & ((1LL << wpi_log) - 1));
+    const int strip = rr / nseg, seg = rr - strip * nseg;
+    a = (strip * 32) / P;
+    i0 = (strip * 32) % P;
+    const int wcol0 = seg * NW * 32;  // first D-column of the workgroup
+    c = (wcol0 + wave * 32) / P;      // this wave's block column
+    j0 = (wcol0 + wave * 32) % P;     // and first column inside it
+    const int cfirst = wcol0 / P;
+    nstep = (kind == 0 ? 4 : 1) * ksteps;
+    nA = kind == 2 ? (NW * 32) / P : 1;  // A chunks per step (dense x elemental: one per block column)
+    sA = kind == 2 ? (wave * 32) / P : 0;
+    conjA = (g.A.flags & LC_CONJ) != 0;
+    const bool conjB = (g.B.flags & LC_CONJ) != 0;
+    // view blocks of A slot s, and of this wave's B, at spin index b of the product
+    auto a_block = [&](int s, int b, int& x, int& y) {
+      x = a;
+      y = kind == 2 ? cfirst + s : (kind == 1 ? a : b);
+    };
+    auto b_block = [&](int b, int& x, int& y) {
+      if (kind == 2) {
+        x = 0;
+        y = 0;
+        return;
+      }
+      x = kind == 1 ? a : b;
+      y = c;
+    };
+    {
+      int x, y;
+      a_block(sA, 0, x, y);
+      ea_blk = (long long)(stored_block(g.A, x, y) >> 2) * g.A.ebs + i0;
+      b_block(0, x, y);
+      eb_blk = (long long)(stored_block(g.B, x, y) >> 2) * g.B.ebs + j0 + 4 * (lane >> 5);
+    }
+    aoff_l = 0;
+    boff_l = 0;
+    flg_l = 0;
+    if (lane < 4) {
+      const int idx = lane;
+      int x, y;
+      if (kind == 2) {
+        x = a;
+        y = cfirst + idx;
+      } else {
+        a_block(0, idx, x, y);
+      }
+      if (y < 4) aoff_l = 8 * stored_block(g.A, x, y) * P2;
+      b_block(idx, x, y);
+      boff_l = 8 * stored_block(g.B, x, y) * P2;
+      int ax, ay;
+      a_block(sA, idx, ax, ay);
+      double p1r, p1i, p2r, p2i;
+      block_phase(g.A, ax, ay, p1r, p1i);
+      block_phase(g.B, x, y, p2r, p2i);
+      const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
+      bool sw, nr, ni;
+      if (phi == 0.0) {
+        sw = false;
+        nr = phr < 0.0;
+        ni = (phr < 0.0) != conjB;
+      } else if (phi > 0.0) {  // i (Br + i c Bi) = -c Bi + i Br
+        sw = true;
+        nr = !conjB;
+        ni = false;
+      } else {                 // -i (Br + i c Bi) = c Bi - i Br
+        sw = true;
+        nr = conjB;
+        ni = true;
+      }
+      flg_l = (sw ? 1 : 0) | (nr ? 2 : 0) | (ni ? 4 : 0);
+    }
+    ty_l = 0;
+    te_l = 0;
+    tfr_l = 0.0;
+    tfi_l = 0.0;
+    tfl_l = 0;
+    if (lane >= 8 && lane < 8 + ntr) {
+      const int t = lane - 8;
+      const LcView& Yv = g.Y[t];
+      double pr, pi;
+      bool zero = false;
+      if (Yv.ex) {
+        const int blk = stored_block(Yv, c, a);
+        ty_l = 8 * blk * P2;
+        te_l = 4 * ((long long)(blk >> 2) * Yv.ebs);
+        block_phase(Yv, c, a, pr, pi);
+      } else {
+        const BlockRef<double> Y = view_block<double>(Yv, 0, c, a, nv2);
+        ty_l = (long long)(reinterpret_cast<const char*>(Y.p) - static_cast<const char*>(Yv.base));
+        pr = Y.sr;
+        pi = Y.si;
+        zero = Y.zero;
+      }
+      const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
+      tfr_l = zero ? 0.0 : ar * pr - ai * pi;
+      tfi_l = zero ? 0.0 : ar * pi + ai * pr;
+      tfl_l = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
+    }
+  };
+
+  // ---- per-item state: batch coordinates and the operand byte offsets of the item
+  struct ItemPos {
+    long long item, co[3], offA8, offB8;
+  };  // (the coordinates are 64-bit for voff; items count below 2^31)
+  auto item_pos = [&](long long w) {
+    const LcOzGroup& g = groups[gi];
+    ItemPos q;
+    q.item = (w - g.wg_begin) >> wpi_log;
+    const unsigned it = (unsigned)q.item, n0 = (unsigned)g.n[0], n1 = (unsigned)g.n[1];
+    const unsigned q0 = it / n0;
+    q.co[0] = it - q0 * n0;
+    q.co[1] = q0 % n1;
+    q.co[2] = q0 / n1;
+    q.offA8 = 8 * voff(g.A.s, q.co);
+    q.offB8 = 8 * voff(g.B.s, q.co);
+    return q;
+  };
+
+  constexpr int LPW = 8 * MAXA / NW;  // A loads per wave and step at most
   i32x4 ra[LPW];
-  auto loadA = [&](int st) {
+  i32x4 br[8];
+  auto loadA = [&](int st, long long offA8) {
+    const LcOzGroup& g = groups[gi];
     const int b = st / ksteps, kk = st - b * ksteps;
+    const int nload = 8 * nA;
 #pragma unroll
     for (int i = 0; i < LPW; i++) {
       const int f = wave + i * NW;
       if (i == 0 || f < nload) {  // (f < NW <= 8 <= nload for i = 0: every wave loads at least once)
         const int s = f >> 3, q = f & 7;
-        const char* src = static_cast<const char*>(g.A.base) + readlane64(aoff_l, kind == 2 ? s : b) + q * P2 +
-                          ((long long)kk * P + i0) * 32;
+        const char* src = static_cast<const char*>(g.A.base) + offA8 + readlane64(aoff_l, kind == 2 ? s : b) +
+                          q * P2 + ((long long)kk * P + i0) * 32;
         ra[i] = gload<i32x4>(src + dma_off);
       }
     }
   };
   auto storeA = [&](int bf) {
+    const int nload = 8 * nA;
 #pragma unroll
     for (int i = 0; i < LPW; i++) {
       const int f = wave + i * NW;
-      if (f < nload) *reinterpret_cast<i32x4*>(&lds[((bf * MAXA + (f >> 3)) * 8 + (f & 7)) * 1024 + 16 * lane]) = ra[i];
+      if (f < nload) *reinterpret_cast<i32x4*>(&lds[((bf * MAXA + (f >> 3)) * 8 + (f & 7)) * 1024 + 16 * ln]) = ra[i];
     }
   };
   // LDS-only barrier: the LDS writes before it are visible after it; memory loads in flight stay so
@@ -304,343 +364,360 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   };
   // B: this wave's fragments of step st come straight from memory (plane q at + q P^2)
-  auto bsrc = [&](int st) {
+  auto bsrc = [&](int st, long long offB8) {
+    const LcOzGroup& g = groups[gi];
     const int b = st / ksteps, kk = st - b * ksteps;
-    return static_cast<const char*>(g.B.base) + readlane64(boff_l, b) + ((long long)kk * P + j0) * 32 + mem_off;
+    return static_cast<const char*>(g.B.base) + offB8 + readlane64(boff_l, b) + ((long long)kk * P + j0) * 32 + mem_off;
   };
-
-  i32x16 re[NORD], im[NORD];
-#pragma unroll
-  for (int o = 0; o < NORD; o++) {
-    re[o] = i32x16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    im[o] = re[o];
-  }
-  // One K step: A from stage buffer bf, B from br. B digit db's two planes are refreshed for the
-  // next step right after their last use here, so every reload has a whole step to land and one
-  // register set suffices (the compiler waits for each plane before its first use).
-  i32x4 br[8];
-  auto compute = [&](int st, int bf, bool more) {
-    const int fl = __builtin_amdgcn_readlane(flg_l, st / ksteps);
-    const bool sw = (fl & 1) != 0, n1 = (fl & 2) != 0, n2 = (fl & 4) != 0;
-    const char* nsrc = more ? bsrc(st + 1) : nullptr;
-    const unsigned char* asrc = &lds[((bf * MAXA + sA) * 8) * 1024 + frag_off];
-#pragma unroll
-    for (int db = 0; db < NORD; db++) {
-      // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
-      // each plane then waits only for its own reload, a whole step after it was issued
-      __builtin_amdgcn_sched_barrier(0);
-      // B' = X' + i Y' (X' = +-X, Y' = +-Y, (X, Y) = (Br, Bi) or swapped), A' = Ar + i s Ai with
-      // s = -1 for a conjugated A:
-      //   re += X' Ar + (-s Y') Ai,   im += Y' Ar + (s X') Ai
-      // so one digit plane of B is needed with both signs (one negation per digit)
-      i32x4 xp = sw ? br[4 + db] : br[db];
-      i32x4 yp = sw ? br[db] : br[4 + db];
-      if (n1) neg16(xp);
-      if (n2) neg16(yp);
-      i32x4 xa = xp, ya = yp;
-      if (conjA) neg16(xa);
-      else neg16(ya);
-      // A fragments are read from LDS for every digit pair (cheap, conflict-free) rather than held
-      // for the whole step: the registers go to the accumulators
-#pragma unroll
-      for (int da = 0; da + db < NORD; da++) {
-        const int o = da + db;
-        const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
-        const i32x4 ai = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
-        re[o] = mfma_i8(xp, ar, re[o]);
-        im[o] = mfma_i8(yp, ar, im[o]);
-        re[o] = mfma_i8(ya, ai, re[o]);
-        im[o] = mfma_i8(xa, ai, im[o]);
-      }
-      if (more) {
-        br[db] = gload<i32x4>(nsrc + db * P2);
-        br[4 + db] = gload<i32x4>(nsrc + (4 + db) * P2);
-      }
-    }
-  };
-
-  // The loop runs every step but the last with unconditional loads, so that the compiler's count
-  // of loads in flight is exact on every path and each wait covers only what it needs. The B
-  // planes are issued in the order the reloads use (digit by digit, real then imaginary).
-  loadA(0);
-  {
-    const char* src = bsrc(0);
+  // the first loads of an item: A of step 0 and B of step 0, issued in the order the reloads use
+  // (digit by digit, real then imaginary)
+  auto prologue = [&](const ItemPos& q) {
+    loadA(0, q.offA8);
+    const char* src = bsrc(0, q.offB8);
 #pragma unroll
     for (int d = 0; d < NORD; d++) {
       br[d] = gload<i32x4>(src + d * P2);
       br[4 + d] = gload<i32x4>(src + (4 + d) * P2);
     }
-  }
-  storeA(0);
-  lds_barrier();
-  for (int st = 0; st + 1 < nstep; st++) {
-    // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
-    // one barrier per step suffices (that buffer was last read in step st - 1)
-    loadA(st + 1);
-    compute(st, st & 1, true);
-    storeA((st + 1) & 1);
-    lds_barrier();
-  }
-  // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
-  // j0 + 8 g + 4 (lane / 32) + r of B
-  int ea;
-  {
-    int x, y;
-    a_block(sA, 0, x, y);
-    const int R = stored_block(g.A, x, y) >> 2;
-    ea = g.A.ex[voff(g.A.es, co) + (long long)R * g.A.ebs + i0 + (lane & 31)];
-  }
-  int eb[16];
-  {
-    int x, y;
-    b_block(0, x, y);
-    const int R = stored_block(g.B, x, y) >> 2;
-    const int* pe = g.B.ex + voff(g.B.es, co) + (long long)R * g.B.ebs + j0 + 4 * (lane >> 5);
-#pragma unroll
-    for (int gg = 0; gg < 4; gg++) {
-      const i32x4 v = gload<i32x4>(pe + 8 * gg);
-#pragma unroll
-      for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
-    }
-  }
-  compute(nstep - 1, (nstep - 1) & 1, false);
+  };
 
-  // ---- epilogue
-  const bool unit = g.alpha[0] == 1.0 && g.alpha[1] == 0.0;
-  const double alr = g.alpha[0], ali = g.alpha[1];
-  double vr[16], vi[16];
+  lane_consts();
+  tables();
+  ItemPos cur = item_pos(wg);
+  prologue(cur);
+
+  for (;;) {
+    lane_consts();
+    const LcOzGroup& g = groups[gi];
+    i32x16 re[NORD], im[NORD];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int e = ea + eb[q] - 60;
-    const double sr = __builtin_fma(__builtin_fma(__builtin_fma((double)re[0][q], 255.0, (double)re[1][q]), 255.0,
-                                                  (double)re[2][q]), 255.0, (double)re[3][q]);
-    const double si = __builtin_fma(__builtin_fma(__builtin_fma((double)im[0][q], 255.0, (double)im[1][q]), 255.0,
-                                                  (double)im[2][q]), 255.0, (double)im[3][q]);
-    const double xr = __builtin_ldexp(sr * 16581375.0, e), xi = __builtin_ldexp(si * 16581375.0, e);  // 255^3
-    vr[q] = unit ? xr : alr * xr - ali * xi;
-    vi[q] = unit ? xi : alr * xi + ali * xr;
-  }
-  // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
-  // whole row), exchanged through LDS
-  if (g.Cd) {
-    double mx = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
-    int em = exp_of(mx);
-    const int eo = __shfl_xor(em, 32, 64);
-    em = eo > em ? eo : em;
-    if (lane < 32) rowe[wave][lane] = em;
-  }
-  // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
-  // exponents are visible
-  lds_barrier();
-  int E = 0;
-  if (g.Cd) {
-    E = INT_MIN;
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-      const int x = rowe[w][lane & 31];
-      E = x > E ? x : E;
+    for (int o = 0; o < NORD; o++) {
+      re[o] = i32x16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      im[o] = re[o];
     }
-    if (E == INT_MIN) E = 0;
-    if (wave == 0 && lane < 32) g.Ce[voff(g.ces, co) + (long long)a * P + i0 + lane] = E;
-  }
-  // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
-  // reads it (LDS operations of one wave complete in order: no barrier)
-  unsigned char* tile = &lds[wave * 32 * TSTR];
-  const int hcol = 4 * (lane >> 5);
-  const int w_ = lane & 31, h_ = lane >> 5;
+    // One K step: A from stage buffer bf, B from br. B digit db's two planes are refreshed for the
+    // next step right after their last use here, so every reload has a whole step to land and one
+    // register set suffices (the compiler waits for each plane before its first use).
+    auto compute = [&](int st, int bf, bool more) {
+      const int fl = __builtin_amdgcn_readlane(flg_l, st / ksteps);
+      const bool sw = (fl & 1) != 0, n1 = (fl & 2) != 0, n2 = (fl & 4) != 0;
+      const char* nsrc = more ? bsrc(st + 1, cur.offB8) : nullptr;
+      const unsigned char* asrc = &lds[((bf * MAXA + sA) * 8) * 1024 + frag_off];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int col = 8 * (q >> 2) + hcol + (q & 3);
-    *reinterpret_cast<d2v*>(tile + (lane & 31) * TSTR + col * 16) = d2v{vr[q], vi[q]};
-  }
-  if (g.C) {
-    // whole rows: lanes 0..31 one row, 32..63 the next
-    Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2;
-#pragma unroll 4
-    for (int m = 0; m < 16; m++) {
-      const int lr = 2 * m + (lane >> 5), lc = lane & 31;
-      const int grow = i0 + lr, gcol = j0 + lc;
-      if (grow < nv && gcol < nv) {
-        const d2v v = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
-        gstore_c(C + (long long)grow * nv + gcol, Cplx<double>{v[0], v[1]});
+      for (int db = 0; db < NORD; db++) {
+        // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
+        // each plane then waits only for its own reload, a whole step after it was issued
+        __builtin_amdgcn_sched_barrier(0);
+        // B' = X' + i Y' (X' = +-X, Y' = +-Y, (X, Y) = (Br, Bi) or swapped), A' = Ar + i s Ai with
+        // s = -1 for a conjugated A:
+        //   re += X' Ar + (-s Y') Ai,   im += Y' Ar + (s X') Ai
+        // so one digit plane of B is needed with both signs (one negation per digit)
+        i32x4 xp = sw ? br[4 + db] : br[db];
+        i32x4 yp = sw ? br[db] : br[4 + db];
+        if (n1) neg16(xp);
+        if (n2) neg16(yp);
+        i32x4 xa = xp, ya = yp;
+        if (conjA) neg16(xa);
+        else neg16(ya);
+        // A fragments are read from LDS for every digit pair (cheap, conflict-free) rather than
+        // held for the whole step: the registers go to the accumulators
+#pragma unroll
+        for (int da = 0; da + db < NORD; da++) {
+          const int o = da + db;
+          const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
+          const i32x4 ai = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
+          re[o] = mfma_i8(xp, ar, re[o]);
+          im[o] = mfma_i8(yp, ar, im[o]);
+          re[o] = mfma_i8(ya, ai, re[o]);
+          im[o] = mfma_i8(xa, ai, im[o]);
+        }
+        if (more) {
+          br[db] = gload<i32x4>(nsrc + db * P2);
+          br[4 + db] = gload<i32x4>(nsrc + (4 + db) * P2);
+        }
+      }
+    };
+
+    // The loop runs every step but the last with unconditional loads, so that the compiler's count
+    // of loads in flight is exact on every path and each wait covers only what it needs.
+    storeA(0);
+    lds_barrier();
+    for (int st = 0; st + 1 < nstep; st++) {
+      // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
+      // one barrier per step suffices (that buffer was last read in step st - 1)
+      loadA(st + 1, cur.offA8);
+      compute(st, st & 1, true);
+      storeA((st + 1) & 1);
+      lds_barrier();
+    }
+    // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
+    // j0 + 8 g + 4 (lane / 32) + r of B
+    const int ea = gload<int>(g.A.ex + voff(g.A.es, cur.co) + ea_blk + w_);
+    int eb[16];
+    {
+      const int* pe = g.B.ex + voff(g.B.es, cur.co) + eb_blk;
+#pragma unroll
+      for (int gg = 0; gg < 4; gg++) {
+        const i32x4 v = gload<i32x4>(pe + 8 * gg);
+#pragma unroll
+        for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
       }
     }
-  }
+    compute(nstep - 1, (nstep - 1) & 1, false);
 
-  if (g.ntr > 0) {
-    // Every closing-factor load of the product's traces is issued before the first use (one
-    // memory round trip). Trace t reads block (c, a) of f(Y_t), element (col, row): with LC_TV the
-    // stored element (row, col), read along stored rows; otherwise the stored (col, row). Lane l
-    // reads stored row r0 + 2 m + l / 32, column c0 + l % 32.
-    const int w = lane & 31, h = lane >> 5;
-    const bool full = i0 + 32 <= nv && j0 + 32 <= nv;
-    auto traces = [&](auto NT) {
-      constexpr int N = decltype(NT)::value;
-      Cplx<double> yv[N][16];
+    // ---- epilogue
+    const bool unit = g.alpha[0] == 1.0 && g.alpha[1] == 0.0;
+    const double alr = g.alpha[0], ali = g.alpha[1];
+    double vr[16], vi[16];
 #pragma unroll
-      for (int t = 0; t < N; t++) {
-        const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(readlane64(ty_l, 8 + t));
-        const bool tv = (__builtin_amdgcn_readlane(tfl_l, 8 + t) & 1) != 0;
+    for (int q = 0; q < 16; q++) {
+      const int e = ea + eb[q] - 60;
+      const double sr = __builtin_fma(__builtin_fma(__builtin_fma((double)re[0][q], 255.0, (double)re[1][q]), 255.0,
+                                                    (double)re[2][q]), 255.0, (double)re[3][q]);
+      const double si = __builtin_fma(__builtin_fma(__builtin_fma((double)im[0][q], 255.0, (double)im[1][q]), 255.0,
+                                                    (double)im[2][q]), 255.0, (double)im[3][q]);
+      const double xr = __builtin_ldexp(sr * 16581375.0, e), xi = __builtin_ldexp(si * 16581375.0, e);  // 255^3
+      vr[q] = unit ? xr : alr * xr - ali * xi;
+      vi[q] = unit ? xi : alr * xi + ali * xr;
+    }
+    // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
+    // whole row), exchanged through LDS
+    if (g.Cd) {
+      double mx = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
+      int em = exp_of(mx);
+      const int eo = __shfl_xor(em, 32, 64);
+      em = eo > em ? eo : em;
+      if (lane < 32) rowe[wave][lane] = em;
+    }
+    // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
+    // exponents are visible
+    lds_barrier();
+    int E = 0;
+    if (g.Cd) {
+      E = INT_MIN;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const int x = rowe[w][w_];
+        E = x > E ? x : E;
+      }
+      if (E == INT_MIN) E = 0;
+      if (wave == 0 && lane < 32) gstore(g.Ce + voff(g.ces, cur.co) + (long long)a * P + i0 + lane, E);
+    }
+    // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
+    // reads it (LDS operations of one wave complete in order: no barrier)
+    unsigned char* tile = &lds[wave * 32 * TSTR];
+    const int hcol = 4 * h_;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int col = 8 * (q >> 2) + hcol + (q & 3);
+      *reinterpret_cast<d2v*>(tile + w_ * TSTR + col * 16) = d2v{vr[q], vi[q]};
+    }
+    if (g.C) {
+      // whole rows: lanes 0..31 one row, 32..63 the next
+      Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + voff(g.cs, cur.co) + (long long)(a * 4 + c) * nv2;
+#pragma unroll 4
+      for (int m = 0; m < 16; m++) {
+        const int lr = 2 * m + h_, lc = w_;
+        const int grow = i0 + lr, gcol = j0 + lc;
+        if (grow < nv && gcol < nv) {
+          const d2v v = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
+          gstore_c(C + (long long)grow * nv + gcol, Cplx<double>{v[0], v[1]});
+        }
+      }
+    }
+
+    if (ntr > 0) {
+      // Every closing-factor load of the product's traces is issued before the first use (one
+      // memory round trip).
+      const int w = w_, h = h_;
+      // FP64 closing factors (one trace at a time: 64 registers of loads each): trace t reads block
+      // (c, a) of f(Y_t), element (col, row): with LC_TV the stored element (row, col), read along
+      // stored rows; otherwise the stored (col, row). Lane l reads stored row r0 + 2 m + l / 32,
+      // column c0 + l % 32.
+      const bool full = i0 + 32 <= nv && j0 + 32 <= nv;
+      auto trace_fp64 = [&](int t) {
+        Cplx<double> yv[16];
+        const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(
+            static_cast<const char*>(g.Y[t].base) + 16 * voff(g.Y[t].s, cur.co) + readlane64(ty_l, 8 + t));
+        const int fl = __builtin_amdgcn_readlane(tfl_l, 8 + t);
+        const bool tv = (fl & 1) != 0;
+        const double cj = (fl & 2) ? -1.0 : 1.0;
         const int r0 = tv ? i0 : j0, c0 = tv ? j0 : i0;
         const Cplx<double>* q = p + (long long)(r0 + h) * nv + c0 + w;
         const long long rs = 2LL * nv;
         if (full) {
 #pragma unroll
-          for (int m = 0; m < 16; m++) yv[t][m] = gload_c(q + m * rs);
+          for (int m = 0; m < 16; m++) yv[m] = gload_c(q + m * rs);
         } else {
 #pragma unroll
           for (int m = 0; m < 16; m++) {
             const bool in = c0 + w < nv && r0 + 2 * m + h < nv;
-            yv[t][m] = gload_c(in ? q + m * rs : p);
-            if (!in) yv[t][m] = Cplx<double>{0.0, 0.0};
+            yv[m] = gload_c(in ? q + m * rs : p);
+            if (!in) yv[m] = Cplx<double>{0.0, 0.0};
           }
         }
-      }
-      double tr[N][2];
-#pragma unroll
-      for (int t = 0; t < N; t++) {
-        const int fl = __builtin_amdgcn_readlane(tfl_l, 8 + t);
-        const bool tv = (fl & 1) != 0;
-        const double cj = (fl & 2) ? -1.0 : 1.0;
         double sr = 0, si = 0;
 #pragma unroll
         for (int m = 0; m < 16; m++) {
           const int u = 2 * m + h;
           const int lr = tv ? u : w, lc = tv ? w : u;  // tile row / column
           const d2v x = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
-          const double yr = yv[t][m].x, yi = cj * yv[t][m].y;
+          const double yr = yv[m].x, yi = cj * yv[m].y;
           sr += x[0] * yr - x[1] * yi;
           si += x[0] * yi + x[1] * yr;
         }
         const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
         const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
-        tr[t][0] = wave_sum(fr * sr - fi * si);
-        tr[t][1] = wave_sum(fr * si + fi * sr);
-      }
-      if (lane == 0) {
+        const double tre = wave_sum(fr * sr - fi * si), tim = wave_sum(fr * si + fi * sr);
+        if (lane == 0) {
+          tred[t][0][wave] = tre;
+          tred[t][1][wave] = tim;
+        }
+      };
+      // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
+      // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its
+      // C tile, and lane l holds row l % 32, columns 16 (l / 32) .. +15 under that row's exponent:
+      // each lane sums FP64 products with exact integers and scales once.
+      auto traces_cf = [&](auto NT) {
+        constexpr int N = decltype(NT)::value;
+        i32x4 yd[N][8];
+        int ey[N];
 #pragma unroll
         for (int t = 0; t < N; t++) {
-          tred[t][0][wave] = tr[t][0];
-          tred[t][1][wave] = tr[t][1];
+          const unsigned char* yb = static_cast<const unsigned char*>(g.Y[t].base) + 8 * voff(g.Y[t].s, cur.co) +
+                                    readlane64(ty_l, 8 + t) +
+                                    ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
+#pragma unroll
+          for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
+          const char* eyb = reinterpret_cast<const char*>(g.Y[t].ex + voff(g.Y[t].es, cur.co)) + readlane64(te_l, 8 + t);
+          ey[t] = gload<int>(reinterpret_cast<const int*>(eyb) + i0 + w);
         }
-      }
-    };
-    // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
-    // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its C
-    // tile, and lane l holds row l % 32, columns 16 (l / 32) .. +15 under that row's exponent: each
-    // lane sums FP64 products with exact integers and scales once.
-    auto traces_cf = [&](auto NT) {
-      constexpr int N = decltype(NT)::value;
-      i32x4 yd[N][8];
-      int ey[N];
+        double tr[N][2];
 #pragma unroll
-      for (int t = 0; t < N; t++) {
-        const unsigned char* yb = reinterpret_cast<const unsigned char*>(readlane64(ty_l, 8 + t)) +
-                                  ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * lane;
+        for (int t = 0; t < N; t++) {
+          const double cj = (__builtin_amdgcn_readlane(tfl_l, 8 + t) & 2) ? -1.0 : 1.0;
+          double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
-        ey[t] = gload<int>(reinterpret_cast<const int*>(readlane64(te_l, 8 + t)) + i0 + w);
-      }
-      double tr[N][2];
+          for (int k = 0; k < 8; k++) {
 #pragma unroll
-      for (int t = 0; t < N; t++) {
-        const double cj = (__builtin_amdgcn_readlane(tfl_l, 8 + t) & 2) ? -1.0 : 1.0;
-        double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+            for (int p2 = 0; p2 < 2; p2++) {
+              const d2v x = *reinterpret_cast<const d2v*>(tile + w * TSTR + (16 * h + 2 * k + p2) * 16);
+              const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
+              s1 += x[0] * yr;
+              s2 += x[1] * yi;
+              s3 += x[0] * yi;
+              s4 += x[1] * yr;
+            }
+          }
+          // x (yr + i cj yi), scaled by the row's 2^(E - 30)
+          const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
+          const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
+          const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
+          const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
+          tr[t][0] = wave_sum(fr * sr - fi * si);
+          tr[t][1] = wave_sum(fr * si + fi * sr);
+        }
+        if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-#pragma unroll
-          for (int p2 = 0; p2 < 2; p2++) {
-            const d2v x = *reinterpret_cast<const d2v*>(tile + w * TSTR + (16 * h + 2 * k + p2) * 16);
-            const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
-            s1 += x[0] * yr;
-            s2 += x[1] * yi;
-            s3 += x[0] * yi;
-            s4 += x[1] * yr;
+          for (int t = 0; t < N; t++) {
+            tred[t][0][wave] = tr[t][0];
+            tred[t][1][wave] = tr[t][1];
           }
         }
-        // x (yr + i cj yi), scaled by the row's 2^(E - 30)
-        const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
-        const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
-        const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
-        const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
-        tr[t][0] = wave_sum(fr * sr - fi * si);
-        tr[t][1] = wave_sum(fr * si + fi * sr);
+      };
+      if (g.Y[0].ex) {
+        if (ntr == 1) traces_cf(std::integral_constant<int, 1>{});
+        else if (ntr == 2) traces_cf(std::integral_constant<int, 2>{});
+        else traces_cf(std::integral_constant<int, 3>{});
+      } else {
+        for (int t = 0; t < ntr; t++) trace_fp64(t);
       }
-      if (lane == 0) {
+      lds_barrier();
+      if (tid < ntr) {
+        double sr = 0, si = 0;
 #pragma unroll
-        for (int t = 0; t < N; t++) {
-          tred[t][0][wave] = tr[t][0];
-          tred[t][1][wave] = tr[t][1];
+        for (int w2 = 0; w2 < NW; w2++) {
+          sr += tred[tid][0][w2];
+          si += tred[tid][1][w2];
+        }
+        gstore(g.part[tid] + 2 * ((cur.item << wpi_log) + rr), d2v{sr, si});
+      }
+    }
+
+    // the next item of this workgroup: with the same tables (same group and strip), its first loads
+    // go out now and land while this item's output is written
+    const long long nwg = wg + nj;
+    const bool more_items = nwg < wend;
+    int ngi = gi;
+    if (more_items)
+      while (ngi + 1 < ng && groups[ngi + 1].wg_begin <= nwg) ngi++;
+    const bool pre = more_items && ngi == gi && (nj & ((1 << wpi_log) - 1)) == 0;
+    ItemPos nxt = cur;
+    if (pre) {
+      nxt = item_pos(nwg);
+      prologue(nxt);
+    }
+
+    if (g.Cd && g.cf) {
+      // closing-factor form: lane l converts row l % 32, columns 16 (l / 32) + 2 k, + 1 of its tile
+      // and stores them as one coalesced kilobyte per k
+      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+                            ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const d2v x0 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k) * 16);
+        const d2v x1 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k + 1) * 16);
+        const i32x4 v = {(int)__builtin_rint(__builtin_ldexp(x0[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x0[1], 30 - E)),
+                         (int)__builtin_rint(__builtin_ldexp(x1[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x1[1], 30 - E))};
+        gstore(base + 1024 * k, v);
+      }
+    } else if (g.Cd) {
+      // the lane's 16 values back from its tile, their digits into the same tile
+      // ([plane][row][DSTR bytes], overwriting it: this wave's reads come first), then whole
+      // kilobytes out
+      d2v xv[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int col = 8 * (q >> 2) + hcol + (q & 3);
+        xv[q] = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + col * 16);
+      }
+#pragma unroll
+      for (int gg = 0; gg < 4; gg++) {
+        int dr[4][4], di[4][4];  // [column r][digit]
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          digits_f64(xv[4 * gg + r][0], E, dr[r]);
+          digits_f64(xv[4 * gg + r][1], E, di[r]);
+        }
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+          unsigned char* pr = tile + w_ * DSTR + 8 * gg + hcol;
+          *reinterpret_cast<unsigned*>(pr + d * 32 * DSTR) = pack4(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
+          *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4(di[0][d], di[1][d], di[2][d], di[3][d]);
         }
       }
-    };
-    if (g.Y[0].ex) {
-      if (g.ntr == 1) traces_cf(std::integral_constant<int, 1>{});
-      else if (g.ntr == 2) traces_cf(std::integral_constant<int, 2>{});
-      else traces_cf(std::integral_constant<int, 3>{});
-    } else {
-      if (g.ntr == 1) traces(std::integral_constant<int, 1>{});
-      else if (g.ntr == 2) traces(std::integral_constant<int, 2>{});
-      else traces(std::integral_constant<int, 3>{});
-    }
-    lds_barrier();
-    if (tid < g.ntr) {
-      double sr = 0, si = 0;
+      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+                            ((long long)(j0 >> 5) * P + i0) * 32;
 #pragma unroll
-      for (int w2 = 0; w2 < NW; w2++) {
-        sr += tred[tid][0][w2];
-        si += tred[tid][1][w2];
+      for (int q = 0; q < 8; q++) {
+        const i32x4 v = *reinterpret_cast<const i32x4*>(tile + q * 32 * DSTR + w_ * DSTR + 16 * h_);
+        gstore(base + q * P2 + mem_off, v);
       }
-      gstore(g.part[tid] + 2 * (item * wpi + rr), d2v{sr, si});
     }
-  }
 
-  if (g.Cd && g.cf) {
-    // closing-factor form: lane l converts row l % 32, columns 16 (l / 32) + 2 k, + 1 of its tile
-    // and stores them as one coalesced kilobyte per k
-    unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, co) + (long long)(a * 4 + c) * P2) +
-                          ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * lane;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const d2v x0 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k) * 16);
-      const d2v x1 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k + 1) * 16);
-      const i32x4 v = {(int)__builtin_rint(__builtin_ldexp(x0[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x0[1], 30 - E)),
-                       (int)__builtin_rint(__builtin_ldexp(x1[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x1[1], 30 - E))};
-      gstore(base + 1024 * k, v);
-    }
-  } else if (g.Cd) {
-    // the lane's 16 values back from its tile, their digits into the same tile
-    // ([plane][row][DSTR bytes], overwriting it: this wave's reads come first), then whole
-    // kilobytes out
-    d2v xv[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int col = 8 * (q >> 2) + hcol + (q & 3);
-      xv[q] = *reinterpret_cast<const d2v*>(tile + (lane & 31) * TSTR + col * 16);
-    }
-#pragma unroll
-    for (int gg = 0; gg < 4; gg++) {
-      int dr[4][4], di[4][4];  // [column r][digit]
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        digits_f64(xv[4 * gg + r][0], E, dr[r]);
-        digits_f64(xv[4 * gg + r][1], E, di[r]);
-      }
-#pragma unroll
-      for (int d = 0; d < 4; d++) {
-        unsigned char* pr = tile + (lane & 31) * DSTR + 8 * gg + hcol;
-        *reinterpret_cast<unsigned*>(pr + d * 32 * DSTR) = pack4(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
-        *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4(di[0][d], di[1][d], di[2][d], di[3][d]);
-      }
-    }
-    unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, co) + (long long)(a * 4 + c) * P2) +
-                          ((long long)(j0 >> 5) * P + i0) * 32;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const i32x4 v = *reinterpret_cast<const i32x4*>(tile + q * 32 * DSTR + (lane & 31) * DSTR + 16 * (lane >> 5));
-      gstore(base + q * P2 + mem_off, v);
+    if (!more_items) break;
+    // every wave is done with its tile and the trace sums before the stage buffers are reused
+    lds_barrier();
+    wg = nwg;
+    if (pre) {
+      cur = nxt;
+    } else {
+      gi = ngi;
+      tables();
+      cur = item_pos(wg);
+      prologue(cur);
     }
   }
 }
@@ -777,8 +854,18 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long long total_wg, int nv, int P,
                                         hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
+  // persistent: as many workgroups per XCD as are resident at once (one per CU with 8 waves and
+  // their LDS, two with 4 waves), at most one per logical workgroup
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 8)
+      ncu = 256;
+  }
+  const long long slots = (long long)(ncu / 8) * (P == 32 ? 2 : 1);
   const long long per = (total_wg + 7) / 8;
-  const dim3 grid((unsigned)(8 * per));
+  const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
   if (P == 32)
     hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4>), grid, dim3(256), 0, s, d_groups, ng, nv, P, total_wg);
   else
