@@ -421,15 +421,22 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         else neg16(ya);
         // A fragments are read from LDS for every digit pair (cheap, conflict-free) rather than
         // held for the whole step: the registers go to the accumulators
+        // the products with Ar of every pair first, then those with Ai: an accumulator is reused
+        // 2 (NORD - db) MFMAs later rather than 2
+        i32x4 ai[NORD];
 #pragma unroll
         for (int da = 0; da + db < NORD; da++) {
           const int o = da + db;
           const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
-          const i32x4 ai = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
+          ai[da] = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
           re[o] = mfma_i8(xp, ar, re[o]);
           im[o] = mfma_i8(yp, ar, im[o]);
-          re[o] = mfma_i8(ya, ai, re[o]);
-          im[o] = mfma_i8(xa, ai, im[o]);
+        }
+#pragma unroll
+        for (int da = 0; da + db < NORD; da++) {
+          const int o = da + db;
+          re[o] = mfma_i8(ya, ai[da], re[o]);
+          im[o] = mfma_i8(xa, ai[da], im[o]);
         }
         if (more) {
           br[db] = gload<i32x4>(nsrc + db * P2);
