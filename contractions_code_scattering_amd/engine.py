@@ -282,7 +282,9 @@ def build_elementals_types(evec, dvec, phase, out=None):
 
 def errors_per_t(c_test, c_ref, corr_ref=None, src0=0, src_step=1):
     """Per source-time separation t: max over (momentum, diagram) of |dC(t)|/|C(t)| and of
-    |dC(t)|/max_t|C(t)|, and the fraction of all C(t) within 1e-6 relative. With corr_ref (the
+    |dC(t)|/max_t|C(t)|, and the fraction of all C(t) within 1e-6 relative; the same element-wise
+    bound over the significant C(t) (|C(t)| >= 1e-3 max_t |C(t)| of their correlator), with the
+    fraction of C(t) that are significant and of those within 1e-6. With corr_ref (the
     reference per-source correlators [np][nsrc][nt][ncorr]) also |dC(t)| / S(t), where S(t) is the
     source average of |c(t0 + t, t0)|: the scale of the summed terms, which a C(t) that happens to
     cancel between sources does not have."""
@@ -291,9 +293,14 @@ def errors_per_t(c_test, c_ref, corr_ref=None, src0=0, src_step=1):
     a = np.abs(c_ref)
     rel = d / np.where(a > 0, a, np.inf)
     norm = d / a.max(axis=2, keepdims=True)
+    sig = a >= 1e-3 * a.max(axis=2, keepdims=True)  # the C(t) of at least 1e-3 of their correlator's max
+    rs = np.where(sig, rel, 0.0)
     out = dict(max_rel=[float('%.3g' % v) for v in rel.max(axis=(0, 1))],
                max_normwise=[float('%.3g' % v) for v in norm.max(axis=(0, 1))],
-               frac_rel_le_1e6=float((rel <= 1e-6).mean()))
+               frac_rel_le_1e6=float((rel <= 1e-6).mean()),
+               max_rel_significant=[float('%.3g' % v) for v in rs.max(axis=(0, 1))],
+               frac_significant_rel_le_1e6=float((rel[sig] <= 1e-6).mean()) if sig.any() else 1.0,
+               significant_fraction=float(sig.mean()))
     if corr_ref is not None:
         np_, nsrc, nt, nc = corr_ref.shape
         scale = np.zeros((np_, nc, nt))
@@ -309,11 +316,15 @@ def errors_per_t(c_test, c_ref, corr_ref=None, src0=0, src_step=1):
 
 
 def rel_errors(c_test, c_ref):
-    """Per diagram: max over (momentum, t) of |dC|/|C| and of |dC|/max_t|C| (normwise)."""
+    """Per diagram: max over (momentum, t) of |dC|/|C|, of |dC|/max_t|C| (normwise) and of |dC|/|C|
+    over the significant C(t) (|C(t)| >= 1e-3 max_t |C(t)|)."""
     import numpy as np
     d = np.abs(c_test - c_ref)          # [np][ncorr][nt]
     a = np.abs(c_ref)
     rel = d / np.where(a > 0, a, np.inf)
     norm = d / a.max(axis=2, keepdims=True)
-    return {name: dict(max_rel=float(rel[:, k].max()), max_normwise=float(norm[:, k].max()))
+    sig = a >= 1e-3 * a.max(axis=2, keepdims=True)
+    rs = np.where(sig, rel, 0.0)
+    return {name: dict(max_rel=float(rel[:, k].max()), max_normwise=float(norm[:, k].max()),
+                       max_rel_significant=float(rs[:, k].max()))
             for k, name in enumerate(CORR_NAMES)}
