@@ -76,6 +76,9 @@ def parse(argv=None):
     ap.add_argument("--momname", default="")
     ap.add_argument("--elemname", default="")
     ap.add_argument("--linkname", default="", help="smeared gauge links (derivative elemental types)")
+    ap.add_argument("--covderivNorm", type=float, default=0.5,
+                    help="normalisation c of the covariant derivative c [U v(x+j) - U^dag v(x-j)] (the drivers' "
+                         "external covderiv; not in the reference): type 1 scales as c^2, types 2-4 as c")
     ap.add_argument("--outname", default="out.dat")
     ap.add_argument("--binary", action="store_true", help="also write <outname>.bin (FP64 values)")
     a = ap.parse_args(argv)
@@ -198,7 +201,7 @@ def main(argv=None):
                         raise SystemExit("%s holds %d complex values, expected Nt*3*Nx*Ny*Nz*9 = %d"
                                          % (a.linkname, lk.size, nlink))
                     links = to_device(lk.reshape(nt, 3, nsite // 3, 3, 3))
-                    dvec = covariant_derivative(links, evec, a.Nx, a.Ny, a.Nz)
+                    dvec = covariant_derivative(links, evec, a.Nx, a.Ny, a.Nz, c=a.covderivNorm)
                     del links
                     for m in moms:
                         records.append(driver_elementals(evec, dvec, m, a))

@@ -344,7 +344,7 @@ extern "C" hipError_t lc_launch_phi_bilinear(const void* X, const void* Y, const
                                              long long nsite, int np, int nks, void* part, double alpha,
                                              int accumulate, void* phi, long long pstride, hipStream_t s);
 extern "C" hipError_t lc_launch_covderiv(const void* U, const void* V, int nt, int nv, int nx, int ny, int nz,
-                                         void* dV, hipStream_t s);
+                                         double c, void* dV, hipStream_t s);
 
 extern "C" {
 
@@ -617,6 +617,11 @@ int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv,
 
 int lc_covariant_derivative(const void* d_links, const void* d_evec, int nt, int nv, int nx, int ny, int nz,
                             void* d_dvec) {
+  return lc_covariant_derivative_scaled(d_links, d_evec, nt, nv, nx, ny, nz, 0.5, d_dvec);
+}
+
+int lc_covariant_derivative_scaled(const void* d_links, const void* d_evec, int nt, int nv, int nx, int ny, int nz,
+                                   double c, void* d_dvec) {
   return guarded([&] {
     require_init();
     if (nt < 1 || nv < 1 || nv > 1024 || nx < 1 || ny < 1 || nz < 1 ||
@@ -624,7 +629,7 @@ int lc_covariant_derivative(const void* d_links, const void* d_evec, int nt, int
       fail(LC_ERR_ARG, "bad sizes nt=%d nv=%d lattice %dx%dx%d", nt, nv, nx, ny, nz);
     check_ptr(d_links, "d_links"); check_ptr(d_evec, "d_evec"); check_ptr(d_dvec, "d_dvec");
     hipStream_t s = ctx().stream;
-    hip_check(lc_launch_covderiv(d_links, d_evec, nt, nv, nx, ny, nz, d_dvec, s), "covariant derivative launch");
+    hip_check(lc_launch_covderiv(d_links, d_evec, nt, nv, nx, ny, nz, c, d_dvec, s), "covariant derivative launch");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   });
 }

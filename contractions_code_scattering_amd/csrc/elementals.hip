@@ -166,14 +166,15 @@ __global__ void __launch_bounds__(256) lc_phi_reduce_kernel(const double2* __res
   }
 }
 
-// dV[j][t][c][3 * site + a] = (nabla_j V[t][c])(site)_a; U [nt][3][nsite][3][3] row-major SU(3)
-// links U_j(x) (colour index a, b), V [nt][nv][3 * nsite], sites x + nx (y + ny z).
+// dV[j][t][c][3 * site + a] = (nabla_j V[t][c])(site)_a = cn [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)]_a
+// (cn = 1/2: the central difference); U [nt][3][nsite][3][3] row-major SU(3) links U_j(x) (colour
+// index a, b), V [nt][nv][3 * nsite], sites x + nx (y + ny z).
 // One thread per (t, j, site) keeps U_j(x) and U_j(x - j) in registers and walks the nv vectors,
 // so the links are read once; the three directions of a site chunk are neighbouring workgroups,
 // so the vector reads of one direction find the others' in L2.
 __global__ void __launch_bounds__(256) lc_covderiv_kernel(const double2* __restrict__ U,
                                                           const double2* __restrict__ V, int nt, int nv, int nx,
-                                                          int ny, int nz, double2* __restrict__ dV) {
+                                                          int ny, int nz, double cn, double2* __restrict__ dV) {
   const long long nsite = (long long)nx * ny * nz;
   const long long nchunk = (nsite + 255) / 256;
   const long long b = blockIdx.x;
@@ -219,7 +220,7 @@ __global__ void __launch_bounds__(256) lc_covderiv_kernel(const double2* __restr
         re -= w.x * vb[q].x + w.y * vb[q].y;  // conj(w) * vb
         im -= w.x * vb[q].y - w.y * vb[q].x;
       }
-      out[(long long)c * vstride + a] = double2{0.5 * re, 0.5 * im};
+      out[(long long)c * vstride + a] = double2{cn * re, cn * im};
     }
   }
 }
@@ -255,9 +256,9 @@ extern "C" hipError_t lc_launch_phi(const void* V, const void* ph, int nt, int n
 }
 
 extern "C" hipError_t lc_launch_covderiv(const void* U, const void* V, int nt, int nv, int nx, int ny, int nz,
-                                         void* dV, hipStream_t s) {
+                                         double c, void* dV, hipStream_t s) {
   const long long nchunk = ((long long)nx * ny * nz + 255) / 256;
   hipLaunchKernelGGL(lc::lc_covderiv_kernel, dim3((unsigned)(3LL * nt * nchunk)), dim3(256), 0, s,
-                     (const double2*)U, (const double2*)V, nt, nv, nx, ny, nz, (double2*)dV);
+                     (const double2*)U, (const double2*)V, nt, nv, nx, ny, nz, c, (double2*)dV);
   return hipGetLastError();
 }

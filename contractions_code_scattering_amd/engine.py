@@ -23,6 +23,7 @@ register("lc_engine_sigma_set_mixed", [_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _LL
 register("lc_selftest_mfma_bf16", [_VP, _VP, _VP])
 register("lc_build_elementals", [_VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_covariant_derivative", [_VP, _VP, _I, _I, _I, _I, _I, _VP])
+register("lc_covariant_derivative_scaled", [_VP, _VP, _I, _I, _I, _I, _I, ctypes.c_double, _VP])
 register("lc_build_elementals_types", [_VP, _VP, _VP, _I, _I, _LL, _I, _VP])
 register("lc_profile", [_I])
 register("lc_set_tile_params", [_I])
@@ -230,10 +231,12 @@ def build_elementals(evec, phase, out=None):
     return out
 
 
-def covariant_derivative(links, evec, nx, ny, nz, out=None):
-    """Covariant derivatives of the eigenvectors on the GPU (lc_covariant_derivative):
+def covariant_derivative(links, evec, nx, ny, nz, out=None, c=0.5):
+    """Covariant derivatives of the eigenvectors on the GPU (lc_covariant_derivative_scaled):
     links [nt][3][nx*ny*nz][3][3], evec [nt][nv][3*nx*ny*nz] DeviceArrays (complex128) ->
-    DeviceArray [3][nt][nv][3*nx*ny*nz], (nabla_j v)(x) = 1/2 [U_j(x) v(x+j) - U_j(x-j)^dag v(x-j)]."""
+    DeviceArray [3][nt][nv][3*nx*ny*nz], (nabla_j v)(x) = c [U_j(x) v(x+j) - U_j(x-j)^dag v(x-j)]; c is
+    the stencil normalisation of the drivers' external covderiv (not in the reference; 1/2 here by
+    default, the central difference)."""
     import numpy as np
     from .devmem import DeviceArray
     if not (isinstance(links, DeviceArray) and isinstance(evec, DeviceArray)):
@@ -250,7 +253,7 @@ def covariant_derivative(links, evec, nx, ny, nz, out=None):
         out = DeviceArray((3, nt, nv, nk), np.complex128)
     elif tuple(out.shape) != (3, nt, nv, nk):
         raise ValueError("out must be [3][nt][nv][3*nx*ny*nz]")
-    check(lib().lc_covariant_derivative(links.ptr, evec.ptr, nt, nv, nx, ny, nz, out.ptr))
+    check(lib().lc_covariant_derivative_scaled(links.ptr, evec.ptr, nt, nv, nx, ny, nz, float(c), out.ptr))
     return out
 
 

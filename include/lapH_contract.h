@@ -206,14 +206,17 @@ int lc_build_elementals(const void* d_evec, const void* d_phase, int nt, int nv,
 
 /* Covariant derivatives of the Laplacian eigenvectors, the drivers' lm.covderiv(j, evec, dl[j])
  * (tests/compute_rho.cpp:140-141, from the external gwu-qcd library), on the GPU:
- *   d_dvec[j][t][c][3x + a] = 1/2 [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)]_a,  v = d_evec[t][c],
+ *   d_dvec[j][t][c][3x + a] = c [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)]_a,  v = d_evec[t][c],
  * periodic in the spatial directions j = 0, 1, 2 (x, y, z). Sites x = ix + nx (iy + ny iz), colour
  * fastest: d_evec [nt][nv][3 nx ny nz], d_links [nt][3][nx ny nz][3][3] (row-major SU(3) U_j(x)),
- * d_dvec [3][nt][nv][3 nx ny nz], complex double. The stencil normalisation of gwu-qcd's covderiv is
- * not known here: another overall factor c scales type 1 of lc_build_elementals_types by c^2 and
- * types 2-4 by c. */
+ * d_dvec [3][nt][nv][3 nx ny nz], complex double. The normalisation c of gwu-qcd's covderiv stencil
+ * is not in the reference (an external library): lc_covariant_derivative uses the central
+ * difference, c = 1/2; lc_covariant_derivative_scaled takes c. Type 1 of lc_build_elementals_types
+ * scales as c^2, types 2-4 as c. */
 int lc_covariant_derivative(const void* d_links, const void* d_evec, int nt, int nv, int nx, int ny, int nz,
                             void* d_dvec);
+int lc_covariant_derivative_scaled(const void* d_links, const void* d_evec, int nt, int nv, int nx, int ny, int nz,
+                                   double c, void* d_dvec);
 
 /* The five elemental types of the rho / rho-pi drivers (tests/compute_rho.cpp:116-229, the tmpMom
  * layout) for np momenta, from eigenvectors and their covariant derivatives:

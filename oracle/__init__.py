@@ -289,12 +289,13 @@ def read_prop_file(path, nt, nv):
 
 
 # ------------------------------------------------------------------ elemental types
-def covariant_derivative(U, V, nx, ny, nz):
-    """(nabla_j v)(x) = 1/2 [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)], periodic, j = x, y, z: the
+def covariant_derivative(U, V, nx, ny, nz, c=0.5):
+    """(nabla_j v)(x) = c [U_j(x) v(x + j) - U_j(x - j)^dag v(x - j)], periodic, j = x, y, z: the
     symmetric lattice covariant derivative standing in for gwu-qcd's lapmult::covderiv, which the
     drivers call (tests/compute_rho.cpp:140-141) and which is not part of the reference; its
-    normalisation is therefore not pinned to gwu-qcd. U [nt][3][nsite][3][3], V [nt][nv][3 nsite]
-    with sites x + nx (y + ny z), colour fastest; returns [3][nt][nv][3 nsite]."""
+    normalisation c (1/2: the central difference) is therefore not pinned to gwu-qcd.
+    U [nt][3][nsite][3][3], V [nt][nv][3 nsite] with sites x + nx (y + ny z), colour fastest;
+    returns [3][nt][nv][3 nsite]."""
     nt, nv = V.shape[:2]
     v = V.reshape(nt, nv, nz, ny, nx, 3)
     u = U.reshape(nt, 3, nz, ny, nx, 3, 3)
@@ -304,7 +305,7 @@ def covariant_derivative(U, V, nx, ny, nz):
         ub = np.roll(uj, 1, axis=ax - 1)        # U_j(x - j)
         fwd = np.einsum("tzyxab,tczyxb->tczyxa", uj, np.roll(v, -1, axis=ax))
         bwd = np.einsum("tzyxba,tczyxb->tczyxa", ub.conj(), np.roll(v, 1, axis=ax))
-        out[j] = 0.5 * (fwd - bwd)
+        out[j] = c * (fwd - bwd)
     return out.reshape(3, nt, nv, -1)
 
 

@@ -116,3 +116,27 @@ def test_zero_momentum_symmetries():
             np.testing.assert_allclose(c, 2 * np.conj(V[t]) @ dV[j, t].T, atol=1e-10)
             np.testing.assert_allclose(c, -np.conj(c.T), atol=1e-10)
             assert np.abs(c).max() > 1e-3
+
+
+def test_normalisation_scaling_laws():
+    """The covariant derivative's normalisation c (gwu-qcd's covderiv is not in the reference, so c is
+    a parameter): dV scales as c, type 0 does not depend on c, type 1 scales as c^2 and types 2-4 as c;
+    power-of-two ratios are exact."""
+    nt, nv, nx, ny, nz = 2, 3, 3, 3, 4
+    U, V, ph = _lattice(5, nt, nv, nx, ny, nz, 2)
+    d5 = O.covariant_derivative(U, V, nx, ny, nz)  # c = 1/2
+    for c in (1.0, 0.25, 0.3):
+        dc = O.covariant_derivative(U, V, nx, ny, nz, c=c)
+        r = c / 0.5
+        if r in (2.0, 0.5):
+            assert np.array_equal(dc, r * d5)
+        else:
+            np.testing.assert_allclose(dc, r * d5, rtol=1e-14, atol=0)
+        T5, Tc = O.elemental_types(V, d5, ph), O.elemental_types(V, dc, ph)
+        np.testing.assert_array_equal(Tc[:, 0], T5[:, 0])
+        scale = np.abs(T5).max()
+        assert np.abs(Tc[:, 1] - r * r * T5[:, 1]).max() <= 1e-13 * scale
+        assert np.abs(Tc[:, 2:] - r * T5[:, 2:]).max() <= 1e-13 * scale
+        # the laws have teeth: type 1 does not scale as c, types 2-4 not as c^2
+        assert np.abs(Tc[:, 1] - r * T5[:, 1]).max() > 1e-3 * scale
+        assert np.abs(Tc[:, 2:] - r * r * T5[:, 2:]).max() > 1e-3 * scale

@@ -92,3 +92,31 @@ def test_types_gauge_invariant_full_lattice(lc):
         assert relerr(T[1, t], np.conj(T[1, t].T)) < 1e-12
         for j in range(3):
             assert relerr(T[2 + j, t], -np.conj(T[2 + j, t].T)) < 1e-12
+
+
+@pytest.mark.parametrize("c", [1.0, 0.25, 0.3])
+def test_normalisation_scaling_laws(lc, c):
+    """lc_covariant_derivative_scaled: dV(c) = (c / (1/2)) dV(1/2), bit-exact for power-of-two ratios
+    and against the oracle with the same c; type 0 independent of c, type 1 ~ c^2, types 2-4 ~ c."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    nt, nv, nx, ny, nz = 2, 8, 4, 4, 4
+    U, V, ph = _lattice(40, nt, nv, nx, ny, nz, 2)
+    dU, dV_, dph = to_device(U), to_device(V), to_device(ph)
+    print("covariant derivative normalisation c=%g" % c, flush=True)
+    d5 = engine.covariant_derivative(dU, dV_, nx, ny, nz)
+    dc = engine.covariant_derivative(dU, dV_, nx, ny, nz, c=c)
+    a5, ac = d5.to_numpy(), dc.to_numpy()
+    r = c / 0.5
+    if r in (2.0, 0.5):
+        assert np.array_equal(ac, r * a5)
+    assert relerr(ac, O.covariant_derivative(U, V, nx, ny, nz, c=c)) < 1e-13
+    T5 = engine.build_elementals_types(dV_, d5, dph).to_numpy()
+    Tc = engine.build_elementals_types(dV_, dc, dph).to_numpy()
+    assert np.array_equal(Tc[:, 0], T5[:, 0])
+    if r in (2.0, 0.5):
+        assert np.array_equal(Tc[:, 1], r * r * T5[:, 1])
+        assert np.array_equal(Tc[:, 2:], r * T5[:, 2:])
+    else:
+        assert relerr(Tc[:, 1], r * r * T5[:, 1]) < 1e-13
+        assert relerr(Tc[:, 2:], r * T5[:, 2:]) < 1e-13

@@ -68,7 +68,7 @@ def test_exact_mixed_vs_fp64_benchmark_shape(lc):
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
     engine.set_mixed_form(engine.MIXED_EXACT)
-    for seed in (3, 7):
+    for seed in (3, 7, 11, 13):
         g = I.rng(seed)
         nt, nv = 6, 64
         tau = I.perambulator(g, nt, nv, 0.3)
