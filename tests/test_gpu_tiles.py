@@ -1,5 +1,5 @@
-"""The mixed GEMM's tile shape (output tiles per workgroup), the in-project autotuner, and momentum
-chunking of the mixed engines.
+"""The BF16x3 mixed GEMM's tile shape (output tiles per workgroup), the in-project autotuner, and
+momentum chunking of the mixed engines.
 
 The tile shape only regroups the same products, so every setting must give the results of the
 built-in rule, to FP64 rounding of the regrouped trace partial sums (1e-13 normwise; the mixed
@@ -18,12 +18,16 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def tiles(lc):
+    """The engine with the BF16x3 mixed form selected (the form whose tile shape these tests set;
+    the exact form has a fixed tile) and the default form restored afterwards."""
     from contractions_code_scattering_amd import engine
+    engine.set_mixed_form(engine.MIXED_BF16X3)
     engine.set_tile_params(0)
     engine.autotune(False, forget=True)
     yield engine
     engine.set_tile_params(0)
     engine.autotune(False, forget=True)
+    engine.set_mixed_form(engine.MIXED_EXACT)
 
 
 def _rho_pipi_inputs(seed, nt, nv, np_):
