@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcontract_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["kernels.hip", "gemm_mx.hip", "gemm_oz.hip", "elementals.hip"]
+HIP_SOURCES = ["kernels.hip", "gemm_mx.hip", "gemm_oz.hip", "elementals.hip", "noalloc.hip"]
 CXX_SOURCES = ["runtime.cpp", "capi.cpp", "engine.cpp", "engine_sigma.cpp", "engine_oz.cpp", "io.cpp", "contract.cpp", "interface.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

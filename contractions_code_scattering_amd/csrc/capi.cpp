@@ -345,6 +345,11 @@ extern "C" hipError_t lc_launch_phi_bilinear(const void* X, const void* Y, const
                                              int accumulate, void* phi, long long pstride, hipStream_t s);
 extern "C" hipError_t lc_launch_covderiv(const void* U, const void* V, int nt, int nv, int nx, int ny, int nz,
                                          double c, void* dV, hipStream_t s);
+extern "C" hipError_t lc_noalloc_setup(int device);
+extern "C" int lc_noalloc_loop_chunks(int nv);
+extern "C" long long lc_noalloc_capacity(void);
+extern "C" hipError_t lc_launch_loop_unity_noalloc(int dbl, const void* diag, const void* mom, int nt, int nv,
+                                                   int device, void* out, hipStream_t s);
 
 extern "C" {
 
@@ -370,6 +375,7 @@ int lc_init(int device) {
     if (device < 0 || device >= n) fail(LC_ERR_NODEV, "device id %d is not allowed (%d devices)", device, n);
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreate(&c.stream), "hipStreamCreate");
+    hip_check(lc_noalloc_setup(device), "address of the allocation-free scratch array");
     c.device = device;
     c.on = true;
   });
@@ -498,6 +504,21 @@ int lc_loop_unity(int prec, const void* d_diag, const void* d_mom, int nt, int n
     check_sizes(prec, nt, nv);
     check_ptr(d_diag, "d_diag"); check_ptr(d_mom, "d_mom"); check_ptr(h_loop, "h_loop");
     loop_unity(Shape(prec, nt, nv), d_diag, d_mom, h_loop);
+  });
+}
+
+int lc_loop_unity_noalloc(int prec, const void* d_diag, const void* d_mom, int nt, int nv, void* d_loop) {
+  return guarded([&] {
+    require_init();
+    check_sizes(prec, nt, nv);
+    check_ptr(d_diag, "d_diag"); check_ptr(d_mom, "d_mom"); check_ptr(d_loop, "d_loop");
+    const long long parts = (long long)nt * lc_noalloc_loop_chunks(nv);
+    if (parts > lc_noalloc_capacity())
+      fail(LC_ERR_ARG, "lc_loop_unity_noalloc: Nt=%d, NeV=%d need %lld partial sums, above the fixed %lld", nt, nv,
+           parts, lc_noalloc_capacity());
+    hip_check(lc_launch_loop_unity_noalloc(prec == LC_DOUBLE, d_diag, d_mom, nt, nv, ctx().device, d_loop,
+                                           ctx().stream),
+              "allocation-free loop_unity launch");
   });
 }
 

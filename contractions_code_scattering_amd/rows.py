@@ -8,7 +8,10 @@ import ctypes
 
 import numpy as np
 
-from ._capi import DOUBLE, SINGLE, check, lib
+from ._capi import DOUBLE, SINGLE, check, lib, register
+
+register("lc_loop_unity_noalloc", [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p])
 
 
 def init(device=0):
@@ -88,6 +91,17 @@ def loop_unity(diag, mom):
     out, po = _out((nt,), npdt)
     check(lib().lc_loop_unity(prec, _p(diag, diag.shape, diag.dtype), _p(mom, (nt, nv, nv), diag.dtype),
                               nt, nv, po))
+    return out
+
+
+def loop_unity_noalloc(diag, mom, out):
+    """lc_loop_unity_noalloc: the loop_unity trace into the DeviceArray out [nt] with no allocation
+    (partial sums in the library's fixed device array); ordered on the library stream, not
+    synchronised."""
+    nt, nv = _dims(diag)
+    prec, npdt = _prec(diag)
+    check(lib().lc_loop_unity_noalloc(prec, _p(diag, diag.shape, diag.dtype), _p(mom, (nt, nv, nv), diag.dtype),
+                                      nt, nv, _p(out, (nt,), diag.dtype)))
     return out
 
 

@@ -66,6 +66,15 @@ int lc_pipi_row(int prec, int iso, const void* d_row, const void* d_diag, const 
 /* run_ContractLoopUnity (lib/contract_kernels_loop_unity.cu:95-107): h_loop [nt] complex. */
 int lc_loop_unity(int prec, const void* d_diag, const void* d_mom, int nt, int nv, void* h_loop);
 
+/* lc_loop_unity without any allocation, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops); no counterpart in the reference. Same inputs; the result
+ * d_loop [nt] complex (prec) is written on the device, ordered on the library stream (no
+ * synchronisation: lc_synchronize or a later library call). Its partial sums use one fixed device
+ * array of 65536 complex doubles (1 MiB, held on each initialised device until the process ends):
+ * Nt * max(1, min(64, ceil(4 NeV^2 / 4096))) must not exceed 65536, else LC_ERR_ARG. Calls that
+ * overlap on one device share that array and race. */
+int lc_loop_unity_noalloc(int prec, const void* d_diag, const void* d_mom, int nt, int nv, void* d_loop);
+
 /* run_ContractSigma (lib/contract_kernels_sigma.cu:134-147): h_corr [nt][4] complex. */
 int lc_sigma_row(int prec, const void* d_row, const void* d_x2, int tf, int nt, int nv,
                  void* h_corr);
