@@ -138,7 +138,7 @@ constexpr int MAXA = 4;             // A chunks per step at most (dense x elemen
 
 template <int NW>
 constexpr int lds_bytes() {
-  return (2 * MAXA * 8 * 1024 > NW * 32 * TSTR) ? 2 * MAXA * 8 * 1024 : NW * 32 * TSTR;
+  return (2 * MAXA * 16 * 1024 > NW * 32 * TSTR) ? 2 * MAXA * 16 * 1024 : NW * 32 * TSTR;
 }
 
 // Persistent schedule. The launch holds a few workgroups per XCD (as many as fit at once); XCD x
@@ -151,10 +151,19 @@ constexpr int lds_bytes() {
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, long long total) {
-  // stage buffers [2][MAXA][8 planes][1 KB] during the K loop, per-wave epilogue tiles after it
+  // stage buffers [2][MAXA][16 planes][1 KB] during the K loop (the 8 digit planes of an A chunk
+  // and their negations), per-wave epilogue tiles after it
   __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
   __shared__ int rowe[NW][32];
   __shared__ double tred[LC_MAX_FUSED_TRACES][2][NW];
+  // per-wave tables of the current product group and strip (filled at a group change, read with
+  // uniform LDS loads; in registers they would stay live through the K loop)
+  struct WaveTab {
+    long long aoff[4], boff[4], ty[LC_MAX_FUSED_TRACES], te[LC_MAX_FUSED_TRACES];
+    double tfr[LC_MAX_FUSED_TRACES], tfi[LC_MAX_FUSED_TRACES];
+    int flg[4], tfl[LC_MAX_FUSED_TRACES];
+  };
+  __shared__ WaveTab wtab[NW];
 
   const int xcd = blockIdx.x & 7, nj = gridDim.x >> 3;
   const long long per = (total + 7) >> 3;
@@ -194,19 +203,28 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   // ---- per-(group, strip) state
   int gi = find_group(groups, ng, wg);
   int kind = 0, nseg = 1, wpi_log = 0, rr = 0, a = 0, i0 = 0, c = 0, j0 = 0, nA = 1, sA = 0, nstep = 0, ntr = 0;
-  bool conjA = false;
-  long long ea_blk = 0, eb_blk = 0;  // exponent offsets of this strip's A rows and wave's B columns
+  bool conjB = false;
+  int ea_blk = 0, eb_blk = 0;  // exponent offsets of this strip's A rows and wave's B columns
   // Per K block b (kind 0: b < 4; kinds 1 and 2: b = 0) and A chunk (kind 2: slot s < nA, else the
-  // chunk of block b), in lanes 0..3: stored-block byte offsets and this wave's phase flags, read in
-  // the K loop with v_readlane. Flags: bit 0 swaps B's planes, bits 1 and 2 negate its real and
-  // imaginary part (the phase times conj?(B), a power of i). Per fused trace t, in lane 8 + t: the
+  // chunk of block b), in wtab[wave] (written by lanes 0..3): stored-block byte offsets and this
+  // wave's A plane selectors. The product of a block pair is ph a b with ph = i^k the
+  // pair's phase, a = Ar + i sA Ai (sA = -1: conjugated A) and b = Br + i sB Bi; with w = ph a and
+  // B's imaginary part taken as sB Bi,
+  //   re += Br re(w) - (sB Bi) im(w),   im += Br im(w) + (sB Bi) re(w),
+  // where re(w) and im(w) are each one of Ar, Ai, -Ar, -Ai: the stage buffer holds all four (plane
+  // group 0..3), and the selectors (bits 0-1: re(w), bits 2-3: im(w)) pick them, so the K loop
+  // moves the B fragments to the matrix cores untouched (a conjugated B: one negation per digit).
+  // Per fused trace t (written by lane 8 + t): the
   // closing factor's block (c, a) (FP64: byte offset of the block from the view base; closing-factor
   // form: of the stored block and of its row exponents), its read flags (bit 0 LC_TV, bit 1 LC_CONJ)
   // and the trace's factor, alpha_t times the block phase (zero for a zero block of an elemental
   // view). Item offsets are added per item.
-  long long aoff_l = 0, boff_l = 0, ty_l = 0, te_l = 0;
-  int flg_l = 0, tfl_l = 0;
-  double tfr_l = 0.0, tfi_l = 0.0;
+  auto rfl = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  auto rfl64 = [](long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+  };
   auto tables = [&]() {
     const LcOzGroup& g = groups[gi];
     kind = g.kind;
@@ -224,8 +242,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     nstep = (kind == 0 ? 4 : 1) * ksteps;
     nA = kind == 2 ? (NW * 32) / P : 1;  // A chunks per step (dense x elemental: one per block column)
     sA = kind == 2 ? (wave * 32) / P : 0;
-    conjA = (g.A.flags & LC_CONJ) != 0;
-    const bool conjB = (g.B.flags & LC_CONJ) != 0;
+    const bool conjA = (g.A.flags & LC_CONJ) != 0;
+    conjB = (g.B.flags & LC_CONJ) != 0;
     // view blocks of A slot s, and of this wave's B, at spin index b of the product
     auto a_block = [&](int s, int b, int& x, int& y) {
       x = a;
@@ -243,14 +261,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     {
       int x, y;
       a_block(sA, 0, x, y);
-      ea_blk = (long long)(stored_block(g.A, x, y) >> 2) * g.A.ebs + i0;
+      ea_blk = (stored_block(g.A, x, y) >> 2) * g.A.ebs + i0;
       b_block(0, x, y);
-      eb_blk = (long long)(stored_block(g.B, x, y) >> 2) * g.B.ebs + j0 + 4 * (lane >> 5);
+      eb_blk = (stored_block(g.B, x, y) >> 2) * g.B.ebs + j0;
     }
-    aoff_l = 0;
-    boff_l = 0;
-    flg_l = 0;
     if (lane < 4) {
+      long long aoff_l = 0;
       const int idx = lane;
       int x, y;
       if (kind == 2) {
@@ -260,36 +276,36 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         a_block(0, idx, x, y);
       }
       if (y < 4) aoff_l = 8 * stored_block(g.A, x, y) * P2;
+      wtab[wave].aoff[idx] = aoff_l;
       b_block(idx, x, y);
-      boff_l = 8 * stored_block(g.B, x, y) * P2;
+      wtab[wave].boff[idx] = 8 * stored_block(g.B, x, y) * P2;
       int ax, ay;
       a_block(sA, idx, ax, ay);
       double p1r, p1i, p2r, p2i;
       block_phase(g.A, ax, ay, p1r, p1i);
       block_phase(g.B, x, y, p2r, p2i);
       const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
-      bool sw, nr, ni;
-      if (phi == 0.0) {
-        sw = false;
-        nr = phr < 0.0;
-        ni = (phr < 0.0) != conjB;
-      } else if (phi > 0.0) {  // i (Br + i c Bi) = -c Bi + i Br
-        sw = true;
-        nr = !conjB;
-        ni = false;
-      } else {                 // -i (Br + i c Bi) = c Bi - i Br
-        sw = true;
-        nr = conjB;
-        ni = true;
+      const int k = phi == 0.0 ? (phr > 0.0 ? 0 : 2) : (phi > 0.0 ? 1 : 3);  // ph = i^k
+      // plane groups: 0 Ar, 1 Ai, 2 -Ar, 3 -Ai; sA Ai is group 1 (sA = 1) or 3
+      const int gai = conjA ? 3 : 1;
+      int sre, sim;
+      if (k == 0) {
+        sre = 0;
+        sim = gai;
+      } else if (k == 1) {  // i a = -sA Ai + i Ar
+        sre = gai ^ 2;
+        sim = 0;
+      } else if (k == 2) {  // -a
+        sre = 2;
+        sim = gai ^ 2;
+      } else {              // -i a = sA Ai - i Ar
+        sre = gai;
+        sim = 2;
       }
-      flg_l = (sw ? 1 : 0) | (nr ? 2 : 0) | (ni ? 4 : 0);
+      wtab[wave].flg[idx] = sre | (sim << 2);
     }
-    ty_l = 0;
-    te_l = 0;
-    tfr_l = 0.0;
-    tfi_l = 0.0;
-    tfl_l = 0;
     if (lane >= 8 && lane < 8 + ntr) {
+      long long ty_l = 0, te_l = 0;
       const int t = lane - 8;
       const LcView& Yv = g.Y[t];
       double pr, pi;
@@ -307,27 +323,33 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         zero = Y.zero;
       }
       const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
-      tfr_l = zero ? 0.0 : ar * pr - ai * pi;
-      tfi_l = zero ? 0.0 : ar * pi + ai * pr;
-      tfl_l = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
+      wtab[wave].ty[t] = ty_l;
+      wtab[wave].te[t] = te_l;
+      wtab[wave].tfr[t] = zero ? 0.0 : ar * pr - ai * pi;
+      wtab[wave].tfi[t] = zero ? 0.0 : ar * pi + ai * pr;
+      wtab[wave].tfl[t] = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
     }
   };
 
   // ---- per-item state: batch coordinates and the operand byte offsets of the item
   struct ItemPos {
-    long long item, co[3], offA8, offB8;
-  };  // (the coordinates are 64-bit for voff; items count below 2^31)
+    int item, co[3];  // (items of a group count below 2^31)
+    long long offA8, offB8;
+  };
+  auto vo = [](const long long* st, const int* co) {
+    return st[0] * co[0] + st[1] * co[1] + st[2] * co[2];
+  };
   auto item_pos = [&](long long w) {
     const LcOzGroup& g = groups[gi];
     ItemPos q;
-    q.item = (w - g.wg_begin) >> wpi_log;
+    q.item = (int)((w - g.wg_begin) >> wpi_log);
     const unsigned it = (unsigned)q.item, n0 = (unsigned)g.n[0], n1 = (unsigned)g.n[1];
     const unsigned q0 = it / n0;
-    q.co[0] = it - q0 * n0;
-    q.co[1] = q0 % n1;
-    q.co[2] = q0 / n1;
-    q.offA8 = 8 * voff(g.A.s, q.co);
-    q.offB8 = 8 * voff(g.B.s, q.co);
+    q.co[0] = (int)(it - q0 * n0);
+    q.co[1] = (int)(q0 % n1);
+    q.co[2] = (int)(q0 / n1);
+    q.offA8 = 8 * vo(g.A.s, q.co);
+    q.offB8 = 8 * vo(g.B.s, q.co);
     return q;
   };
 
@@ -343,18 +365,26 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       const int f = wave + i * NW;
       if (i == 0 || f < nload) {  // (f < NW <= 8 <= nload for i = 0: every wave loads at least once)
         const int s = f >> 3, q = f & 7;
-        const char* src = static_cast<const char*>(g.A.base) + offA8 + readlane64(aoff_l, kind == 2 ? s : b) +
+        const char* src = static_cast<const char*>(g.A.base) + offA8 + rfl64(wtab[wave].aoff[kind == 2 ? s : b]) +
                           q * P2 + ((long long)kk * P + i0) * 32;
         ra[i] = gload<i32x4>(src + dma_off);
       }
     }
   };
+  // plane q of chunk s (q < 4: Ar digit q, else Ai digit q - 4) into plane group q / 4, and its
+  // negation into group q / 4 + 2
   auto storeA = [&](int bf) {
     const int nload = 8 * nA;
 #pragma unroll
     for (int i = 0; i < LPW; i++) {
       const int f = wave + i * NW;
-      if (f < nload) *reinterpret_cast<i32x4*>(&lds[((bf * MAXA + (f >> 3)) * 8 + (f & 7)) * 1024 + 16 * ln]) = ra[i];
+      if (f < nload) {
+        unsigned char* d = &lds[((bf * MAXA + (f >> 3)) * 16 + (f & 7)) * 1024 + 16 * ln];
+        *reinterpret_cast<i32x4*>(d) = ra[i];
+        i32x4 n = ra[i];
+        neg16(n);
+        *reinterpret_cast<i32x4*>(d + 8 * 1024) = n;
+      }
     }
   };
   // LDS-only barrier: the LDS writes before it are visible after it; memory loads in flight stay so
@@ -367,7 +397,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   auto bsrc = [&](int st, long long offB8) {
     const LcOzGroup& g = groups[gi];
     const int b = st / ksteps, kk = st - b * ksteps;
-    return static_cast<const char*>(g.B.base) + offB8 + readlane64(boff_l, b) + ((long long)kk * P + j0) * 32 + mem_off;
+    return static_cast<const char*>(g.B.base) + offB8 + rfl64(wtab[wave].boff[b]) + ((long long)kk * P + j0) * 32 +
+           mem_off;
   };
   // the first loads of an item: A of step 0 and B of step 0, issued in the order the reloads use
   // (digit by digit, real then imaginary)
@@ -399,44 +430,51 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     // next step right after their last use here, so every reload has a whole step to land and one
     // register set suffices (the compiler waits for each plane before its first use).
     auto compute = [&](int st, int bf, bool more) {
-      const int fl = __builtin_amdgcn_readlane(flg_l, st / ksteps);
-      const bool sw = (fl & 1) != 0, n1 = (fl & 2) != 0, n2 = (fl & 4) != 0;
+      const int fl = rfl(wtab[wave].flg[st / ksteps]);
+      const int ore = (fl & 3) * 4096, oim = ((fl >> 2) & 3) * 4096;  // plane groups of re(w), im(w)
+      const int onim = oim ^ 8192;                                     // and of -im(w)
       const char* nsrc = more ? bsrc(st + 1, cur.offB8) : nullptr;
-      const unsigned char* asrc = &lds[((bf * MAXA + sA) * 8) * 1024 + frag_off];
+      const unsigned char* asrc = &lds[((bf * MAXA + sA) * 16) * 1024 + frag_off];
+      auto rd = [&](int off) { return *reinterpret_cast<const i32x4*>(asrc + off); };
+      // one digit pair: re += Br re(w) - (sB Bi) im(w), im += Br im(w) + (sB Bi) re(w); re is reused
+      // two MFMAs later
+      auto pair = [&](int o, const i32x4& xr, const i32x4& xi, const i32x4& w, const i32x4& wi, const i32x4& nw) {
+        re[o] = mfma_i8(xr, w, re[o]);
+        im[o] = mfma_i8(xr, wi, im[o]);
+        re[o] = mfma_i8(xi, nw, re[o]);
+        im[o] = mfma_i8(xi, w, im[o]);
+      };
+      // A digit 0 meets every B digit: its three fragments stay in registers for the step; the
+      // fragments of the other pairs are read one pair ahead of their products
+      const i32x4 z0 = rd(ore), z1 = rd(oim), z2 = rd(onim);
 #pragma unroll
       for (int db = 0; db < NORD; db++) {
         // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
-        // each plane then waits only for its own reload, a whole step after it was issued
-        __builtin_amdgcn_sched_barrier(0);
-        // B' = X' + i Y' (X' = +-X, Y' = +-Y, (X, Y) = (Br, Bi) or swapped), A' = Ar + i s Ai with
-        // s = -1 for a conjugated A:
-        //   re += X' Ar + (-s Y') Ai,   im += Y' Ar + (s X') Ai
-        // so one digit plane of B is needed with both signs (one negation per digit)
-        i32x4 xp = sw ? br[4 + db] : br[db];
-        i32x4 yp = sw ? br[db] : br[4 + db];
-        if (n1) neg16(xp);
-        if (n2) neg16(yp);
-        i32x4 xa = xp, ya = yp;
-        if (conjA) neg16(xa);
-        else neg16(ya);
-        // A fragments are read from LDS for every digit pair (cheap, conflict-free) rather than
-        // held for the whole step: the registers go to the accumulators
-        // the products with Ar of every pair first, then those with Ai: an accumulator is reused
-        // 2 (NORD - db) MFMAs later rather than 2
-        i32x4 ai[NORD];
-#pragma unroll
-        for (int da = 0; da + db < NORD; da++) {
-          const int o = da + db;
-          const i32x4 ar = *reinterpret_cast<const i32x4*>(asrc + da * 1024);
-          ai[da] = *reinterpret_cast<const i32x4*>(asrc + (4 + da) * 1024);
-          re[o] = mfma_i8(xp, ar, re[o]);
-          im[o] = mfma_i8(yp, ar, im[o]);
+        // each plane then waits only for its own reload, a whole step after it was issued (LDS
+        // reads may still move across)
+        __builtin_amdgcn_sched_barrier(0x0100);
+        const i32x4 xr = br[db];
+        i32x4 xi = br[4 + db];
+        if (conjB) neg16(xi);
+        i32x4 c0, c1, c2;
+        if (db + 1 < NORD) {
+          c0 = rd(ore + 1024);
+          c1 = rd(oim + 1024);
+          c2 = rd(onim + 1024);
         }
+        pair(db, xr, xi, z0, z1, z2);
 #pragma unroll
-        for (int da = 0; da + db < NORD; da++) {
-          const int o = da + db;
-          re[o] = mfma_i8(ya, ai[da], re[o]);
-          im[o] = mfma_i8(xa, ai[da], im[o]);
+        for (int da = 1; da + db < NORD; da++) {
+          i32x4 n0, n1, n2;
+          if (da + 1 + db < NORD) {
+            n0 = rd(ore + (da + 1) * 1024);
+            n1 = rd(oim + (da + 1) * 1024);
+            n2 = rd(onim + (da + 1) * 1024);
+          }
+          pair(da + db, xr, xi, c0, c1, c2);
+          c0 = n0;
+          c1 = n1;
+          c2 = n2;
         }
         if (more) {
           br[db] = gload<i32x4>(nsrc + db * P2);
@@ -459,10 +497,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     }
     // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
     // j0 + 8 g + 4 (lane / 32) + r of B
-    const int ea = gload<int>(g.A.ex + voff(g.A.es, cur.co) + ea_blk + w_);
+    const int ea = gload<int>(g.A.ex + vo(g.A.es, cur.co) + ea_blk + w_);
     int eb[16];
     {
-      const int* pe = g.B.ex + voff(g.B.es, cur.co) + eb_blk;
+      const int* pe = g.B.ex + vo(g.B.es, cur.co) + eb_blk + 4 * h_;
 #pragma unroll
       for (int gg = 0; gg < 4; gg++) {
         const i32x4 v = gload<i32x4>(pe + 8 * gg);
@@ -510,7 +548,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         E = x > E ? x : E;
       }
       if (E == INT_MIN) E = 0;
-      if (wave == 0 && lane < 32) gstore(g.Ce + voff(g.ces, cur.co) + (long long)a * P + i0 + lane, E);
+      if (wave == 0 && lane < 32) gstore(g.Ce + vo(g.ces, cur.co) + (long long)a * P + i0 + lane, E);
     }
     // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
     // reads it (LDS operations of one wave complete in order: no barrier)
@@ -523,7 +561,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     }
     if (g.C) {
       // whole rows: lanes 0..31 one row, 32..63 the next
-      Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + voff(g.cs, cur.co) + (long long)(a * 4 + c) * nv2;
+      Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + vo(g.cs, cur.co) + (long long)(a * 4 + c) * nv2;
 #pragma unroll 4
       for (int m = 0; m < 16; m++) {
         const int lr = 2 * m + h_, lc = w_;
@@ -547,8 +585,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       auto trace_fp64 = [&](int t) {
         Cplx<double> yv[16];
         const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(
-            static_cast<const char*>(g.Y[t].base) + 16 * voff(g.Y[t].s, cur.co) + readlane64(ty_l, 8 + t));
-        const int fl = __builtin_amdgcn_readlane(tfl_l, 8 + t);
+            static_cast<const char*>(g.Y[t].base) + 16 * vo(g.Y[t].s, cur.co) + rfl64(wtab[wave].ty[t]));
+        const int fl = rfl(wtab[wave].tfl[t]);
         const bool tv = (fl & 1) != 0;
         const double cj = (fl & 2) ? -1.0 : 1.0;
         const int r0 = tv ? i0 : j0, c0 = tv ? j0 : i0;
@@ -575,8 +613,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           sr += x[0] * yr - x[1] * yi;
           si += x[0] * yi + x[1] * yr;
         }
-        const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
-        const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
+        const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfr[t])));
+        const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfi[t])));
         const double tre = wave_sum(fr * sr - fi * si), tim = wave_sum(fr * si + fi * sr);
         if (lane == 0) {
           tred[t][0][wave] = tre;
@@ -593,18 +631,18 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         int ey[N];
 #pragma unroll
         for (int t = 0; t < N; t++) {
-          const unsigned char* yb = static_cast<const unsigned char*>(g.Y[t].base) + 8 * voff(g.Y[t].s, cur.co) +
-                                    readlane64(ty_l, 8 + t) +
+          const unsigned char* yb = static_cast<const unsigned char*>(g.Y[t].base) + 8 * vo(g.Y[t].s, cur.co) +
+                                    rfl64(wtab[wave].ty[t]) +
                                     ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
 #pragma unroll
           for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
-          const char* eyb = reinterpret_cast<const char*>(g.Y[t].ex + voff(g.Y[t].es, cur.co)) + readlane64(te_l, 8 + t);
+          const char* eyb = reinterpret_cast<const char*>(g.Y[t].ex + vo(g.Y[t].es, cur.co)) + rfl64(wtab[wave].te[t]);
           ey[t] = gload<int>(reinterpret_cast<const int*>(eyb) + i0 + w);
         }
         double tr[N][2];
 #pragma unroll
         for (int t = 0; t < N; t++) {
-          const double cj = (__builtin_amdgcn_readlane(tfl_l, 8 + t) & 2) ? -1.0 : 1.0;
+          const double cj = (rfl(wtab[wave].tfl[t]) & 2) ? -1.0 : 1.0;
           double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
 #pragma unroll
           for (int k = 0; k < 8; k++) {
@@ -621,8 +659,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           // x (yr + i cj yi), scaled by the row's 2^(E - 30)
           const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
           const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
-          const double fr = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfr_l), 8 + t));
-          const double fi = __builtin_bit_cast(double, readlane64(__builtin_bit_cast(long long, tfi_l), 8 + t));
+          const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfr[t])));
+          const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfi[t])));
           tr[t][0] = wave_sum(fr * sr - fi * si);
           tr[t][1] = wave_sum(fr * si + fi * sr);
         }
@@ -649,7 +687,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           sr += tred[tid][0][w2];
           si += tred[tid][1][w2];
         }
-        gstore(g.part[tid] + 2 * ((cur.item << wpi_log) + rr), d2v{sr, si});
+        gstore(g.part[tid] + 2 * (((long long)cur.item << wpi_log) + rr), d2v{sr, si});
       }
     }
 
@@ -670,7 +708,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     if (g.Cd && g.cf) {
       // closing-factor form: lane l converts row l % 32, columns 16 (l / 32) + 2 k, + 1 of its tile
       // and stores them as one coalesced kilobyte per k
-      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
                             ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
 #pragma unroll
       for (int k = 0; k < 8; k++) {
@@ -705,7 +743,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4(di[0][d], di[1][d], di[2][d], di[3][d]);
         }
       }
-      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (voff(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
                             ((long long)(j0 >> 5) * P + i0) * 32;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
@@ -861,8 +899,8 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long long total_wg, int nv, int P,
                                         hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
-  // persistent: as many workgroups per XCD as are resident at once (one per CU with 8 waves and
-  // their LDS, two with 4 waves), at most one per logical workgroup
+  // persistent: as many workgroups per XCD as are resident at once (one per CU), at most one per
+  // logical workgroup
   static int ncu = 0;
   if (!ncu) {
     int dev = 0;
@@ -870,7 +908,7 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long 
         ncu < 8)
       ncu = 256;
   }
-  const long long slots = (long long)(ncu / 8) * (P == 32 ? 2 : 1);
+  const long long slots = ncu / 8;  // (the stage buffers take 128 KB of LDS: one workgroup per CU)
   const long long per = (total_wg + 7) / 8;
   const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
   if (P == 32)
