@@ -77,25 +77,39 @@ __device__ __forceinline__ void digits_of(double x, int E, int (&d)[4]) {
   }
   d[0] = v;
 }
-// The same digits by FP64 arithmetic (the GEMM epilogue's form): v = rint(x 2^(30-E)), then
-// d0 = round(v / 255^3), and each lower digit the rounded quotient of the exact remainder. Every
-// quotient v / 255^k is at least 1 / (2 255^k) away from a half-integer (255 is odd), far beyond the
-// rounding of v * (1 / 255^k), so the digits are the unique balanced ones above.
-__device__ __forceinline__ void digits_f64(double x, int E, int (&d)[4]) {
-  const double v = __builtin_rint(__builtin_ldexp(x, 30 - E));
-  const double q0 = __builtin_rint(v * (1.0 / 16581375.0));
-  double r = __builtin_fma(-q0, 16581375.0, v);
-  const double q1 = __builtin_rint(r * (1.0 / 65025.0));
-  r = __builtin_fma(-q1, 65025.0, r);
-  const double q2 = __builtin_rint(r * (1.0 / 255.0));
-  r = __builtin_fma(-q2, 255.0, r);
-  d[0] = (int)q0;
-  d[1] = (int)q1;
-  d[2] = (int)q2;
-  d[3] = (int)r;
+// The low bytes of a, b, c, d as one word (three byte permutes)
+__device__ __forceinline__ unsigned pack4p(int a, int b, int c, int d) {
+  const unsigned lo = __builtin_amdgcn_perm((unsigned)b, (unsigned)a, 0x0c0c0400u);
+  const unsigned hi = __builtin_amdgcn_perm((unsigned)d, (unsigned)c, 0x0c0c0400u);
+  return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
-__device__ __forceinline__ unsigned pack4(int a, int b, int c, int d) {
-  return (unsigned)(a & 255) | ((unsigned)(b & 255) << 8) | ((unsigned)(c & 255) << 16) | ((unsigned)d << 24);
+
+// Round-to-integer constant 1.5 2^52: fma(y, c, RND) rounds the exact y c to an integer Q (|Q| < 2^51)
+// and holds Q mod 2^32 in its low word.
+constexpr double RND = 6755399441055744.0;
+__device__ __forceinline__ int lo32(double t) {
+  return (int)(unsigned)(__builtin_bit_cast(unsigned long long, t) & 0xffffffffull);
+}
+// v = rint(x s) as an integer, for s = 2^(30 - E) and |x| < 2^E (one fma)
+__device__ __forceinline__ int scaled_int(double x, double s) { return lo32(__builtin_fma(x, s, RND)); }
+// The digits of digits_of by FP64 arithmetic in 8 operations (the GEMM epilogue's form), for
+// s = 2^(30 - E) and |x| < 2^E: with v = rint(x s) and Q_k = round(v / 255^k),
+//   d = (Q_3, Q_2 - 255 Q_3, Q_1 - 255 Q_2, v - 255 Q_1).
+// v / 255^(k+1) = Q_k / 255 + (v / 255^k - Q_k) / 255; the second term is below 1/510 in magnitude and
+// Q_k / 255 is at least 1/510 away from a half-integer (255 is odd), so Q_(k+1) = round(Q_k / 255):
+// each difference is a balanced digit, and these are the unique digits. v / 255^k itself is at least
+// 1/(2 255^k) from a half-integer, far beyond the error of the product with the rounded 1/255^k
+// (below 2^-44); |Q_1| < 2^23, so the multiples of 255 are 24-bit products.
+__device__ __forceinline__ void digits_fast(double x, double s, int (&d)[4]) {
+  const double t0 = __builtin_fma(x, s, RND);
+  const double v = t0 - RND;
+  const int q1 = lo32(__builtin_fma(v, 1.0 / 255.0, RND));
+  const int q2 = lo32(__builtin_fma(v, 1.0 / 65025.0, RND));
+  const int q3 = lo32(__builtin_fma(v, 1.0 / 16581375.0, RND));
+  d[0] = q3;
+  d[1] = q2 + __mul24(q3, -255);
+  d[2] = q1 + __mul24(q2, -255);
+  d[3] = lo32(t0) + __mul24(q1, -255);
 }
 
 // Closing-factor form of a P x P block: each value as one int32, v = rint(x 2^(30-E)) under the same
@@ -155,7 +169,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   // and their negations), per-wave epilogue tiles after it
   __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
   __shared__ int rowe[NW][32];
-  __shared__ double tred[LC_MAX_FUSED_TRACES][2][NW];
+  // per-lane partial sums of the fused traces (re, im), reduced by one wave per value
+  __shared__ d2v tsum[LC_MAX_FUSED_TRACES][NW * 64];
   // per-wave tables of the current product group and strip (filled at a group change, read with
   // uniform LDS loads; in registers they would stay live through the K loop)
   struct WaveTab {
@@ -511,19 +526,27 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     compute(nstep - 1, (nstep - 1) & 1, false);
 
     // ---- epilogue
-    const bool unit = g.alpha[0] == 1.0 && g.alpha[1] == 0.0;
-    const double alr = g.alpha[0], ali = g.alpha[1];
+    // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact: |acc0| <= 2 K 65^2 < 2^23 (a 24-bit product)
+    // and |acc0 255 + acc1| < 2^31 for K <= 512; the value is 255^3 2^(E_A + E_B - 60) S, rounded once
     double vr[16], vi[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      const int e = ea + eb[q] - 60;
-      const double sr = __builtin_fma(__builtin_fma(__builtin_fma((double)re[0][q], 255.0, (double)re[1][q]), 255.0,
-                                                    (double)re[2][q]), 255.0, (double)re[3][q]);
-      const double si = __builtin_fma(__builtin_fma(__builtin_fma((double)im[0][q], 255.0, (double)im[1][q]), 255.0,
-                                                    (double)im[2][q]), 255.0, (double)im[3][q]);
-      const double xr = __builtin_ldexp(sr * 16581375.0, e), xi = __builtin_ldexp(si * 16581375.0, e);  // 255^3
-      vr[q] = unit ? xr : alr * xr - ali * xi;
-      vi[q] = unit ? xi : alr * xi + ali * xr;
+      const double sc = __builtin_ldexp(16581375.0, ea + eb[q] - 60);
+      const double sr = __builtin_fma(__builtin_fma((double)(__mul24(re[0][q], 255) + re[1][q]), 255.0, (double)re[2][q]),
+                                      255.0, (double)re[3][q]);
+      const double si = __builtin_fma(__builtin_fma((double)(__mul24(im[0][q], 255) + im[1][q]), 255.0, (double)im[2][q]),
+                                      255.0, (double)im[3][q]);
+      vr[q] = sr * sc;
+      vi[q] = si * sc;
+    }
+    if (!(g.alpha[0] == 1.0 && g.alpha[1] == 0.0)) {
+      const double alr = g.alpha[0], ali = g.alpha[1];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const double xr = vr[q], xi = vi[q];
+        vr[q] = alr * xr - ali * xi;
+        vi[q] = alr * xi + ali * xr;
+      }
     }
     // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
     // whole row), exchanged through LDS
@@ -550,6 +573,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       if (E == INT_MIN) E = 0;
       if (wave == 0 && lane < 32) gstore(g.Ce + vo(g.ces, cur.co) + (long long)a * P + i0 + lane, E);
     }
+    const double sE = __builtin_ldexp(1.0, 30 - E);
     // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
     // reads it (LDS operations of one wave complete in order: no barrier)
     unsigned char* tile = &lds[wave * 32 * TSTR];
@@ -615,11 +639,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         }
         const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfr[t])));
         const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfi[t])));
-        const double tre = wave_sum(fr * sr - fi * si), tim = wave_sum(fr * si + fi * sr);
-        if (lane == 0) {
-          tred[t][0][wave] = tre;
-          tred[t][1][wave] = tim;
-        }
+        tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
       };
       // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
       // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its
@@ -639,7 +659,6 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           const char* eyb = reinterpret_cast<const char*>(g.Y[t].ex + vo(g.Y[t].es, cur.co)) + rfl64(wtab[wave].te[t]);
           ey[t] = gload<int>(reinterpret_cast<const int*>(eyb) + i0 + w);
         }
-        double tr[N][2];
 #pragma unroll
         for (int t = 0; t < N; t++) {
           const double cj = (rfl(wtab[wave].tfl[t]) & 2) ? -1.0 : 1.0;
@@ -661,15 +680,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
           const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfr[t])));
           const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfi[t])));
-          tr[t][0] = wave_sum(fr * sr - fi * si);
-          tr[t][1] = wave_sum(fr * si + fi * sr);
-        }
-        if (lane == 0) {
-#pragma unroll
-          for (int t = 0; t < N; t++) {
-            tred[t][0][wave] = tr[t][0];
-            tred[t][1][wave] = tr[t][1];
-          }
+          tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
         }
       };
       if (g.Y[0].ex) {
@@ -680,14 +691,15 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         for (int t = 0; t < ntr; t++) trace_fp64(t);
       }
       lds_barrier();
-      if (tid < ntr) {
-        double sr = 0, si = 0;
+      // wave v sums component k = v % 2 of trace t = v / 2 over the workgroup's 64 NW lanes
+      for (int v = wave; v < 2 * ntr; v += NW) {
+        const int t = v >> 1, k = v & 1;
+        const double* ts = reinterpret_cast<const double*>(&tsum[t][0]) + k;
+        double sm = 0;
 #pragma unroll
-        for (int w2 = 0; w2 < NW; w2++) {
-          sr += tred[tid][0][w2];
-          si += tred[tid][1][w2];
-        }
-        gstore(g.part[tid] + 2 * (((long long)cur.item << wpi_log) + rr), d2v{sr, si});
+        for (int w2 = 0; w2 < NW; w2++) sm += ts[2 * (w2 * 64 + lane)];
+        sm = wave_sum(sm);
+        if (lane == 0) gstore(g.part[t] + 2 * (((long long)cur.item << wpi_log) + rr) + k, sm);
       }
     }
 
@@ -714,8 +726,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       for (int k = 0; k < 8; k++) {
         const d2v x0 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k) * 16);
         const d2v x1 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k + 1) * 16);
-        const i32x4 v = {(int)__builtin_rint(__builtin_ldexp(x0[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x0[1], 30 - E)),
-                         (int)__builtin_rint(__builtin_ldexp(x1[0], 30 - E)), (int)__builtin_rint(__builtin_ldexp(x1[1], 30 - E))};
+        const i32x4 v = {scaled_int(x0[0], sE), scaled_int(x0[1], sE), scaled_int(x1[0], sE), scaled_int(x1[1], sE)};
         gstore(base + 1024 * k, v);
       }
     } else if (g.Cd) {
@@ -733,14 +744,14 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         int dr[4][4], di[4][4];  // [column r][digit]
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          digits_f64(xv[4 * gg + r][0], E, dr[r]);
-          digits_f64(xv[4 * gg + r][1], E, di[r]);
+          digits_fast(xv[4 * gg + r][0], sE, dr[r]);
+          digits_fast(xv[4 * gg + r][1], sE, di[r]);
         }
 #pragma unroll
         for (int d = 0; d < 4; d++) {
           unsigned char* pr = tile + w_ * DSTR + 8 * gg + hcol;
-          *reinterpret_cast<unsigned*>(pr + d * 32 * DSTR) = pack4(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
-          *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4(di[0][d], di[1][d], di[2][d], di[3][d]);
+          *reinterpret_cast<unsigned*>(pr + d * 32 * DSTR) = pack4p(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
+          *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4p(di[0][d], di[1][d], di[2][d], di[3][d]);
         }
       }
       unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
