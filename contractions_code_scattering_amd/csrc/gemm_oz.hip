@@ -114,13 +114,15 @@ __device__ __forceinline__ void digits_fast(double x, double s, int (&d)[4]) {
 
 // Closing-factor form of a P x P block: each value as one int32, v = rint(x 2^(30-E)) under the same
 // line exponent E as its digits (the same block size, 8 bytes per complex element). A 32 x 32
-// chunk (rows 32 R.., columns 32 C..) is 8 KB at ((C P / 32 + R) * 8 KB); inside it element
-// (r, j) sits at (j % 16 / 2) KB + (32 (j / 16) + r) * 16 + 8 (j % 2) bytes, so that load k of
-// lane l, 16 bytes at k KB + 16 l, holds columns 16 (l / 32) + 2 k, + 1 of row l % 32.
+// chunk (rows 32 R.., columns 32 C..) is 8 KB at ((C P / 32 + R) * 8 KB); inside it element (r, j),
+// with j = 8 g + 4 h + 2 p + e (g < 4, h, p, e < 2), sits at (2 g + p) KB + (32 h + r) * 16 + 8 e
+// bytes, so that load k of lane l, 16 bytes at k KB + 16 l, holds columns 8 (k / 2) + 4 (l / 32) +
+// 2 (k % 2), + 1 of row l % 32: the entries q = 2 k, 2 k + 1 of the lane's accumulator tile (row
+// l % 32, columns 8 (q / 4) + 4 (l / 32) + q % 4).
 __device__ __forceinline__ long long cf_off(int r, int j, int P) {
   const int jj = j & 31, rr = r & 31;
-  return ((long long)(j >> 5) * (P >> 5) + (r >> 5)) * 8192 + ((jj & 15) >> 1) * 1024 + ((jj >> 4) * 32 + rr) * 16 +
-         8 * (jj & 1);
+  return ((long long)(j >> 5) * (P >> 5) + (r >> 5)) * 8192 + (2 * (jj >> 3) + ((jj >> 1) & 1)) * 1024 +
+         (32 * ((jj >> 2) & 1) + rr) * 16 + 8 * (jj & 1);
 }
 
 // Stored block of view block (x, y); its stored block row is the result / 4.
@@ -559,9 +561,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       em = eo > em ? eo : em;
       if (lane < 32) rowe[wave][lane] = em;
     }
+    // The FP64 tile in LDS serves only FP64 outputs and FP64 closing factors; traces in the
+    // closing-factor form, closing-factor outputs and digit outputs work on the lane's registers.
+    const bool use_tile = g.C || (ntr > 0 && !g.Y[0].ex);
     // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
     // exponents are visible
-    lds_barrier();
+    if (g.Cd || use_tile) lds_barrier();
     int E = 0;
     if (g.Cd) {
       E = INT_MIN;
@@ -578,10 +583,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     // reads it (LDS operations of one wave complete in order: no barrier)
     unsigned char* tile = &lds[wave * 32 * TSTR];
     const int hcol = 4 * h_;
+    if (use_tile) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int col = 8 * (q >> 2) + hcol + (q & 3);
-      *reinterpret_cast<d2v*>(tile + w_ * TSTR + col * 16) = d2v{vr[q], vi[q]};
+      for (int q = 0; q < 16; q++) {
+        const int col = 8 * (q >> 2) + hcol + (q & 3);
+        *reinterpret_cast<d2v*>(tile + w_ * TSTR + col * 16) = d2v{vr[q], vi[q]};
+      }
     }
     if (g.C) {
       // whole rows: lanes 0..31 one row, 32..63 the next
@@ -643,8 +650,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       };
       // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
       // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its
-      // C tile, and lane l holds row l % 32, columns 16 (l / 32) .. +15 under that row's exponent:
-      // each lane sums FP64 products with exact integers and scales once.
+      // C tile, laid out like the accumulators: load k of lane l holds the values of its own tile
+      // entries q = 2 k and 2 k + 1 (row l % 32, columns 8 (k / 2) + 4 (l / 32) + 2 (k % 2), + 1),
+      // under that row's exponent, so each lane sums FP64 products of its registers with exact
+      // integers and scales once.
       auto traces_cf = [&](auto NT) {
         constexpr int N = decltype(NT)::value;
         i32x4 yd[N][8];
@@ -667,12 +676,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
           for (int k = 0; k < 8; k++) {
 #pragma unroll
             for (int p2 = 0; p2 < 2; p2++) {
-              const d2v x = *reinterpret_cast<const d2v*>(tile + w * TSTR + (16 * h + 2 * k + p2) * 16);
+              const double xr = vr[2 * k + p2], xi = vi[2 * k + p2];
               const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
-              s1 += x[0] * yr;
-              s2 += x[1] * yi;
-              s3 += x[0] * yi;
-              s4 += x[1] * yr;
+              s1 += xr * yr;
+              s2 += xi * yi;
+              s3 += xr * yi;
+              s4 += xi * yr;
             }
           }
           // x (yr + i cj yi), scaled by the row's 2^(E - 30)
@@ -718,47 +727,46 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     }
 
     if (g.Cd && g.cf) {
-      // closing-factor form: lane l converts row l % 32, columns 16 (l / 32) + 2 k, + 1 of its tile
-      // and stores them as one coalesced kilobyte per k
+      // closing-factor form: load k of the chunk holds the lane's own entries q = 2 k, 2 k + 1 (see
+      // traces_cf); one coalesced kilobyte per k
       unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
                             ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const d2v x0 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k) * 16);
-        const d2v x1 = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + (16 * h_ + 2 * k + 1) * 16);
-        const i32x4 v = {scaled_int(x0[0], sE), scaled_int(x0[1], sE), scaled_int(x1[0], sE), scaled_int(x1[1], sE)};
+        const i32x4 v = {scaled_int(vr[2 * k], sE), scaled_int(vi[2 * k], sE), scaled_int(vr[2 * k + 1], sE),
+                         scaled_int(vi[2 * k + 1], sE)};
         gstore(base + 1024 * k, v);
       }
     } else if (g.Cd) {
-      // the lane's 16 values back from its tile, their digits into the same tile
-      // ([plane][row][DSTR bytes], overwriting it: this wave's reads come first), then whole
-      // kilobytes out
-      d2v xv[16];
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int col = 8 * (q >> 2) + hcol + (q & 3);
-        xv[q] = *reinterpret_cast<const d2v*>(tile + w_ * TSTR + col * 16);
-      }
+      // digits of the lane's 16 values: word W[d][gg] packs digit plane d (re digits 0..3, then im)
+      // of columns 8 gg + 4 (l / 32) .. + 3 of row l % 32. A stored plane row of a lane is 16
+      // adjacent columns (16 (l / 32) ..): lane l keeps W[.][0..1] (l < 32) or W[.][2..3] and takes
+      // the other two from lane l ^ 32, one v_permlane32_swap per pair (gg, gg + 2): the swap of
+      // the upper lanes of x with the lower lanes of y leaves x = (W_gg of lanes 0..31, W_gg+2 of
+      // lanes 0..31) and y = (W_gg of lanes 32..63, W_gg+2 of lanes 32..63), which are the words
+      // 2 gg' and 2 gg' + 1 of both halves.
+      unsigned wd[8][4];
 #pragma unroll
       for (int gg = 0; gg < 4; gg++) {
         int dr[4][4], di[4][4];  // [column r][digit]
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          digits_fast(xv[4 * gg + r][0], sE, dr[r]);
-          digits_fast(xv[4 * gg + r][1], sE, di[r]);
+          digits_fast(vr[4 * gg + r], sE, dr[r]);
+          digits_fast(vi[4 * gg + r], sE, di[r]);
         }
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-          unsigned char* pr = tile + w_ * DSTR + 8 * gg + hcol;
-          *reinterpret_cast<unsigned*>(pr + d * 32 * DSTR) = pack4p(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
-          *reinterpret_cast<unsigned*>(pr + (4 + d) * 32 * DSTR) = pack4p(di[0][d], di[1][d], di[2][d], di[3][d]);
+          wd[d][gg] = pack4p(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
+          wd[4 + d][gg] = pack4p(di[0][d], di[1][d], di[2][d], di[3][d]);
         }
       }
       unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
                             ((long long)(j0 >> 5) * P + i0) * 32;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        const i32x4 v = *reinterpret_cast<const i32x4*>(tile + q * 32 * DSTR + w_ * DSTR + 16 * h_);
+        const auto s0 = __builtin_amdgcn_permlane32_swap(wd[q][0], wd[q][2], false, false);
+        const auto s1 = __builtin_amdgcn_permlane32_swap(wd[q][1], wd[q][3], false, false);
+        const i32x4 v = {(int)s0[0], (int)s0[1], (int)s1[0], (int)s1[1]};
         gstore(base + q * P2 + mem_off, v);
       }
     }
