@@ -160,9 +160,12 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   };
   Program pg(E.nv, Program::OZ);
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
-  // digit output of a product into `a` (items laid out with strides st)
+  // digit output of a product into `a` (items laid out with strides st); with twin_B, the twin
+  // product A twin_B traced by twin_tr (formed by the same kernel from A on the chip when the
+  // digits are written directly, else as a product of its own)
   auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
-                   const std::vector<FusedTrace>& tr = {}, bool cf = false) {
+                   const std::vector<FusedTrace>& tr = {}, bool cf = false, const LcView* twin_B = nullptr,
+                   const std::vector<FusedTrace>* twin_tr = nullptr) {
     if (direct) {
       DigitOut d;
       d.d = a.d;
@@ -172,7 +175,7 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
         d.ds[k] = st[k] * bd;
         d.es[k] = st[k] * P4i;
       }
-      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr);
+      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr, twin_B, twin_tr);
       return;
     }
     const long long nmat = (long long)n0 * n1 * n2;
@@ -181,6 +184,7 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     cs_of(st, cs);
     pg.gemm_oz(A, Bv, x, cs[0], cs[1], cs[2], nullptr, n0, n1, n2, 1.0, 0.0, tr);
     stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
+    if (twin_B) pg.gemm_oz(A, *twin_B, nullptr, 0, 0, 0, nullptr, n0, n1, n2, 1.0, 0.0, *twin_tr);
   };
   // close a level: a barrier, then the digit conversions of its FP64-staged products
   auto close_level = [&] {
@@ -260,19 +264,19 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   close_level();
   // level 2
   {
+    // L = L1 P1 with its twin Y = L1 P2 (star2), Rr = B P2 with its twin X = B P1 (star1): each
+    // pair shares its A operand
     std::vector<FusedTrace> tl = sets ? std::vector<FusedTrace>{} : rho_traces(false, {0, 1});
     tl.push_back(slot(8, tau_dag_Y));
-    dgemm(dview(L1d, PR, 0), role(0, false, true, false), Ld, PR, nt, ns, pc, tl);
+    const LcView P1B = role(0, false, true, false), P2B = role(1, false, true, false);
+    const std::vector<FusedTrace> ty = {slot(7, tau_dag_Y)}, tx = {slot(6, tau_dag_Y)};
+    dgemm(dview(L1d, PR, 0), P1B, Ld, PR, nt, ns, pc, tl, false, &P2B, &ty);
     std::vector<FusedTrace> tr = {slot(9, tau_dag_Y)};
     if (!sets) {
       const std::vector<FusedTrace> r2 = rho_traces(true, {2});
       tr.push_back(r2[0]);
     }
-    dgemm(dview(Bd, PR, 0), role(1, false, true, false), Rrd, PR, nt, ns, pc, tr);
-    pg.gemm_oz(dview(L1d, PR, 0), role(1, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
-               {slot(7, tau_dag_Y)});
-    pg.gemm_oz(dview(Bd, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
-               {slot(6, tau_dag_Y)});
+    dgemm(dview(Bd, PR, 0), P2B, Rrd, PR, nt, ns, pc, tr, false, &P1B, &tx);
     if (sets)
       pg.gemm_oz(dview(B4d, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
                  rho_traces(false, {0, 1, 2}));

@@ -166,7 +166,8 @@ constexpr int lds_bytes() {
 // are issued before the epilogue of the current one.
 template <int NW>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
-lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, long long total) {
+lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup* __restrict__ twins, int nv, int P,
+                  long long total) {
   // stage buffers [2][MAXA][16 planes][1 KB] during the K loop (the 8 digit planes of an A chunk
   // and their negations), per-wave epilogue tiles after it
   __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
@@ -174,13 +175,15 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   // per-lane partial sums of the fused traces (re, im), reduced by one wave per value
   __shared__ d2v tsum[LC_MAX_FUSED_TRACES][NW * 64];
   // per-wave tables of the current product group and strip (filled at a group change, read with
-  // uniform LDS loads; in registers they would stay live through the K loop)
+  // uniform LDS loads; in registers they would stay live through the K loop): A offsets, and per
+  // sub-product (0: the group's product, 1: its twin) the B offsets, plane selectors and traces
   struct WaveTab {
-    long long aoff[4], boff[4], ty[LC_MAX_FUSED_TRACES], te[LC_MAX_FUSED_TRACES];
+    long long boff[4], ty[LC_MAX_FUSED_TRACES], te[LC_MAX_FUSED_TRACES];
     double tfr[LC_MAX_FUSED_TRACES], tfi[LC_MAX_FUSED_TRACES];
     int flg[4], tfl[LC_MAX_FUSED_TRACES];
   };
-  __shared__ WaveTab wtab[NW];
+  __shared__ WaveTab wtab[2][NW];
+  __shared__ long long aofft[NW][4];
 
   const int xcd = blockIdx.x & 7, nj = gridDim.x >> 3;
   const long long per = (total + 7) >> 3;
@@ -219,14 +222,16 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
 
   // ---- per-(group, strip) state
   int gi = find_group(groups, ng, wg);
-  int kind = 0, nseg = 1, wpi_log = 0, rr = 0, a = 0, i0 = 0, c = 0, j0 = 0, nA = 1, sA = 0, nstep = 0, ntr = 0;
-  bool conjB = false;
-  int ea_blk = 0, eb_blk = 0;  // exponent offsets of this strip's A rows and wave's B columns
+  int kind = 0, nseg = 1, wpi_log = 0, rr = 0, a = 0, i0 = 0, c = 0, j0 = 0, nA = 1, sA = 0, nstep = 0;
+  int twi = -1;                     // the group's twin in `twins`, or -1
+  int ntr_s[2] = {0, 0};            // fused traces per sub-product
+  bool conjB_s[2] = {false, false};
+  int ea_blk = 0, eb_blk_s[2] = {0, 0};  // exponent offsets of this strip's A rows and wave's B columns
   // Per K block b (kind 0: b < 4; kinds 1 and 2: b = 0) and A chunk (kind 2: slot s < nA, else the
-  // chunk of block b), in wtab[wave] (written by lanes 0..3): stored-block byte offsets and this
-  // wave's A plane selectors. The product of a block pair is ph a b with ph = i^k the
-  // pair's phase, a = Ar + i sA Ai (sA = -1: conjugated A) and b = Br + i sB Bi; with w = ph a and
-  // B's imaginary part taken as sB Bi,
+  // chunk of block b), in aofft[wave] and wtab[sub][wave] (written by lanes 0..3): stored-block
+  // byte offsets and this wave's A plane selectors. The product of a block pair is ph a b with
+  // ph = i^k the pair's phase, a = Ar + i sA Ai (sA = -1: conjugated A) and b = Br + i sB Bi; with
+  // w = ph a and B's imaginary part taken as sB Bi,
   //   re += Br re(w) - (sB Bi) im(w),   im += Br im(w) + (sB Bi) re(w),
   // where re(w) and im(w) are each one of Ar, Ai, -Ar, -Ai: the stage buffer holds all four (plane
   // group 0..3), and the selectors (bits 0-1: re(w), bits 2-3: im(w)) pick them, so the K loop
@@ -235,7 +240,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   // closing factor's block (c, a) (FP64: byte offset of the block from the view base; closing-factor
   // form: of the stored block and of its row exponents), its read flags (bit 0 LC_TV, bit 1 LC_CONJ)
   // and the trace's factor, alpha_t times the block phase (zero for a zero block of an elemental
-  // view). Item offsets are added per item.
+  // view). Item offsets are added per item. A twin product shares the group's A, kind and items.
   auto rfl = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
   auto rfl64 = [](long long v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)v);
@@ -246,7 +251,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     const LcOzGroup& g = groups[gi];
     kind = g.kind;
     nseg = g.nseg;
-    ntr = g.ntr;
+    twi = g.twin;
     wpi_log = 31 - __builtin_clz((unsigned)(nstrip * nseg));
     rr = (int)((wg - g.wg_begin) & ((1LL << wpi_log) - 1));
     const int strip = rr / nseg, seg = rr - strip * nseg;
@@ -260,7 +265,6 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     nA = kind == 2 ? (NW * 32) / P : 1;  // A chunks per step (dense x elemental: one per block column)
     sA = kind == 2 ? (wave * 32) / P : 0;
     const bool conjA = (g.A.flags & LC_CONJ) != 0;
-    conjB = (g.B.flags & LC_CONJ) != 0;
     // view blocks of A slot s, and of this wave's B, at spin index b of the product
     auto a_block = [&](int s, int b, int& x, int& y) {
       x = a;
@@ -279,8 +283,6 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       int x, y;
       a_block(sA, 0, x, y);
       ea_blk = (stored_block(g.A, x, y) >> 2) * g.A.ebs + i0;
-      b_block(0, x, y);
-      eb_blk = (stored_block(g.B, x, y) >> 2) * g.B.ebs + j0;
     }
     if (lane < 4) {
       long long aoff_l = 0;
@@ -293,58 +295,74 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
         a_block(0, idx, x, y);
       }
       if (y < 4) aoff_l = 8 * stored_block(g.A, x, y) * P2;
-      wtab[wave].aoff[idx] = aoff_l;
-      b_block(idx, x, y);
-      wtab[wave].boff[idx] = 8 * stored_block(g.B, x, y) * P2;
-      int ax, ay;
-      a_block(sA, idx, ax, ay);
-      double p1r, p1i, p2r, p2i;
-      block_phase(g.A, ax, ay, p1r, p1i);
-      block_phase(g.B, x, y, p2r, p2i);
-      const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
-      const int k = phi == 0.0 ? (phr > 0.0 ? 0 : 2) : (phi > 0.0 ? 1 : 3);  // ph = i^k
-      // plane groups: 0 Ar, 1 Ai, 2 -Ar, 3 -Ai; sA Ai is group 1 (sA = 1) or 3
-      const int gai = conjA ? 3 : 1;
-      int sre, sim;
-      if (k == 0) {
-        sre = 0;
-        sim = gai;
-      } else if (k == 1) {  // i a = -sA Ai + i Ar
-        sre = gai ^ 2;
-        sim = 0;
-      } else if (k == 2) {  // -a
-        sre = 2;
-        sim = gai ^ 2;
-      } else {              // -i a = sA Ai - i Ar
-        sre = gai;
-        sim = 2;
-      }
-      wtab[wave].flg[idx] = sre | (sim << 2);
+      aofft[wave][idx] = aoff_l;
     }
-    if (lane >= 8 && lane < 8 + ntr) {
-      long long ty_l = 0, te_l = 0;
-      const int t = lane - 8;
-      const LcView& Yv = g.Y[t];
-      double pr, pi;
-      bool zero = false;
-      if (Yv.ex) {
-        const int blk = stored_block(Yv, c, a);
-        ty_l = 8 * blk * P2;
-        te_l = 4 * ((long long)(blk >> 2) * Yv.ebs);
-        block_phase(Yv, c, a, pr, pi);
-      } else {
-        const BlockRef<double> Y = view_block<double>(Yv, 0, c, a, nv2);
-        ty_l = (long long)(reinterpret_cast<const char*>(Y.p) - static_cast<const char*>(Yv.base));
-        pr = Y.sr;
-        pi = Y.si;
-        zero = Y.zero;
+    for (int sub = 0; sub < (twi >= 0 ? 2 : 1); sub++) {
+      const LcOzGroup& gs = sub ? twins[twi] : g;
+      const int ntr = gs.ntr;
+      ntr_s[sub] = ntr;
+      conjB_s[sub] = (gs.B.flags & LC_CONJ) != 0;
+      {
+        int x, y;
+        b_block(0, x, y);
+        eb_blk_s[sub] = (stored_block(gs.B, x, y) >> 2) * gs.B.ebs + j0;
       }
-      const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
-      wtab[wave].ty[t] = ty_l;
-      wtab[wave].te[t] = te_l;
-      wtab[wave].tfr[t] = zero ? 0.0 : ar * pr - ai * pi;
-      wtab[wave].tfi[t] = zero ? 0.0 : ar * pi + ai * pr;
-      wtab[wave].tfl[t] = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
+      WaveTab& wt = wtab[sub][wave];
+      if (lane < 4) {
+        const int idx = lane;
+        int x, y;
+        b_block(idx, x, y);
+        wt.boff[idx] = 8 * stored_block(gs.B, x, y) * P2;
+        int ax, ay;
+        a_block(sA, idx, ax, ay);
+        double p1r, p1i, p2r, p2i;
+        block_phase(g.A, ax, ay, p1r, p1i);
+        block_phase(gs.B, x, y, p2r, p2i);
+        const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
+        const int k = phi == 0.0 ? (phr > 0.0 ? 0 : 2) : (phi > 0.0 ? 1 : 3);  // ph = i^k
+        // plane groups: 0 Ar, 1 Ai, 2 -Ar, 3 -Ai; sA Ai is group 1 (sA = 1) or 3
+        const int gai = conjA ? 3 : 1;
+        int sre, sim;
+        if (k == 0) {
+          sre = 0;
+          sim = gai;
+        } else if (k == 1) {  // i a = -sA Ai + i Ar
+          sre = gai ^ 2;
+          sim = 0;
+        } else if (k == 2) {  // -a
+          sre = 2;
+          sim = gai ^ 2;
+        } else {              // -i a = sA Ai - i Ar
+          sre = gai;
+          sim = 2;
+        }
+        wt.flg[idx] = sre | (sim << 2);
+      }
+      if (lane >= 8 && lane < 8 + ntr) {
+        long long ty_l = 0, te_l = 0;
+        const int t = lane - 8;
+        const LcView& Yv = gs.Y[t];
+        double pr, pi;
+        bool zero = false;
+        if (Yv.ex) {
+          const int blk = stored_block(Yv, c, a);
+          ty_l = 8 * blk * P2;
+          te_l = 4 * ((long long)(blk >> 2) * Yv.ebs);
+          block_phase(Yv, c, a, pr, pi);
+        } else {
+          const BlockRef<double> Y = view_block<double>(Yv, 0, c, a, nv2);
+          ty_l = (long long)(reinterpret_cast<const char*>(Y.p) - static_cast<const char*>(Yv.base));
+          pr = Y.sr;
+          pi = Y.si;
+          zero = Y.zero;
+        }
+        const double ar = gs.tr_alpha[t][0], ai = gs.tr_alpha[t][1];
+        wt.ty[t] = ty_l;
+        wt.te[t] = te_l;
+        wt.tfr[t] = zero ? 0.0 : ar * pr - ai * pi;
+        wt.tfi[t] = zero ? 0.0 : ar * pi + ai * pr;
+        wt.tfl[t] = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
+      }
     }
   };
 
@@ -382,7 +400,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
       const int f = wave + i * NW;
       if (i == 0 || f < nload) {  // (f < NW <= 8 <= nload for i = 0: every wave loads at least once)
         const int s = f >> 3, q = f & 7;
-        const char* src = static_cast<const char*>(g.A.base) + offA8 + rfl64(wtab[wave].aoff[kind == 2 ? s : b]) +
+        const char* src = static_cast<const char*>(g.A.base) + offA8 + rfl64(aofft[wave][kind == 2 ? s : b]) +
                           q * P2 + ((long long)kk * P + i0) * 32;
         ra[i] = gload<i32x4>(src + dma_off);
       }
@@ -410,23 +428,27 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   };
-  // B: this wave's fragments of step st come straight from memory (plane q at + q P^2)
-  auto bsrc = [&](int st, long long offB8) {
-    const LcOzGroup& g = groups[gi];
+  // B of sub-product sub: this wave's fragments of step st come straight from memory (plane q at
+  // + q P^2)
+  auto bsrc = [&](int st, long long offB8, int sub) {
+    const LcOzGroup& gs = sub ? twins[twi] : groups[gi];
     const int b = st / ksteps, kk = st - b * ksteps;
-    return static_cast<const char*>(g.B.base) + offB8 + rfl64(wtab[wave].boff[b]) + ((long long)kk * P + j0) * 32 +
-           mem_off;
+    return static_cast<const char*>(gs.B.base) + offB8 + rfl64(wtab[sub][wave].boff[b]) +
+           ((long long)kk * P + j0) * 32 + mem_off;
   };
-  // the first loads of an item: A of step 0 and B of step 0, issued in the order the reloads use
-  // (digit by digit, real then imaginary)
-  auto prologue = [&](const ItemPos& q) {
-    loadA(0, q.offA8);
-    const char* src = bsrc(0, q.offB8);
+  // B of step 0, issued in the order the reloads use (digit by digit, real then imaginary)
+  auto loadB0 = [&](long long offB8, int sub) {
+    const char* src = bsrc(0, offB8, sub);
 #pragma unroll
     for (int d = 0; d < NORD; d++) {
       br[d] = gload<i32x4>(src + d * P2);
       br[4 + d] = gload<i32x4>(src + (4 + d) * P2);
     }
+  };
+  // the first loads of an item: A of step 0 and B of step 0
+  auto prologue = [&](const ItemPos& q) {
+    loadA(0, q.offA8);
+    loadB0(q.offB8, 0);
   };
 
   lane_consts();
@@ -437,338 +459,368 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, int nv, int P, l
   for (;;) {
     lane_consts();
     const LcOzGroup& g = groups[gi];
-    i32x16 re[NORD], im[NORD];
+    // next item of this workgroup (decided in the item's last sub-product)
+    long long nwg = 0;
+    bool more_items = false, pre = false;
+    int ngi = gi;
+    ItemPos nxt;  // (set when pre)
+    // sub-product 0 is the group's product; 1, its twin, reuses the A operand staged for 0 (the
+    // twin needs every step of A in the stage buffers: nstep <= 2). One code path per sub-product.
+    auto run_sub = [&](auto SUB) {
+      constexpr int sub = decltype(SUB)::value;
+      const LcOzGroup& gs = sub ? twins[twi] : g;
+      const int ntr = ntr_s[sub];
+      const bool conjB = conjB_s[sub];
+      const long long offB8 = sub ? 8 * vo(gs.B.s, cur.co) : cur.offB8;
+      i32x16 re[NORD], im[NORD];
 #pragma unroll
-    for (int o = 0; o < NORD; o++) {
-      re[o] = i32x16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      im[o] = re[o];
-    }
-    // One K step: A from stage buffer bf, B from br. B digit db's two planes are refreshed for the
-    // next step right after their last use here, so every reload has a whole step to land and one
-    // register set suffices (the compiler waits for each plane before its first use).
-    auto compute = [&](int st, int bf, bool more) {
-      const int fl = rfl(wtab[wave].flg[st / ksteps]);
-      const int ore = (fl & 3) * 4096, oim = ((fl >> 2) & 3) * 4096;  // plane groups of re(w), im(w)
-      const int onim = oim ^ 8192;                                     // and of -im(w)
-      const char* nsrc = more ? bsrc(st + 1, cur.offB8) : nullptr;
-      const unsigned char* asrc = &lds[((bf * MAXA + sA) * 16) * 1024 + frag_off];
-      auto rd = [&](int off) { return *reinterpret_cast<const i32x4*>(asrc + off); };
-      // one digit pair: re += Br re(w) - (sB Bi) im(w), im += Br im(w) + (sB Bi) re(w); re is reused
-      // two MFMAs later
-      auto pair = [&](int o, const i32x4& xr, const i32x4& xi, const i32x4& w, const i32x4& wi, const i32x4& nw) {
-        re[o] = mfma_i8(xr, w, re[o]);
-        im[o] = mfma_i8(xr, wi, im[o]);
-        re[o] = mfma_i8(xi, nw, re[o]);
-        im[o] = mfma_i8(xi, w, im[o]);
-      };
-      // A digit 0 meets every B digit: its three fragments stay in registers for the step; the
-      // fragments of the other pairs are read one pair ahead of their products
-      const i32x4 z0 = rd(ore), z1 = rd(oim), z2 = rd(onim);
+      for (int o = 0; o < NORD; o++) {
+        re[o] = i32x16{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        im[o] = re[o];
+      }
+      // One K step: A from stage buffer bf, B from br. B digit db's two planes are refreshed for the
+      // next step right after their last use here, so every reload has a whole step to land and one
+      // register set suffices (the compiler waits for each plane before its first use).
+      auto compute = [&](int st, int bf, bool more) {
+        const int fl = rfl(wtab[sub][wave].flg[st / ksteps]);
+        const int ore = (fl & 3) * 4096, oim = ((fl >> 2) & 3) * 4096;  // plane groups of re(w), im(w)
+        const int onim = oim ^ 8192;                                     // and of -im(w)
+        const char* nsrc = more ? bsrc(st + 1, offB8, sub) : nullptr;
+        const unsigned char* asrc = &lds[((bf * MAXA + sA) * 16) * 1024 + frag_off];
+        auto rd = [&](int off) { return *reinterpret_cast<const i32x4*>(asrc + off); };
+        // one digit pair: re += Br re(w) - (sB Bi) im(w), im += Br im(w) + (sB Bi) re(w); re is reused
+        // two MFMAs later
+        auto pair = [&](int o, const i32x4& xr, const i32x4& xi, const i32x4& w, const i32x4& wi, const i32x4& nw) {
+          re[o] = mfma_i8(xr, w, re[o]);
+          im[o] = mfma_i8(xr, wi, im[o]);
+          re[o] = mfma_i8(xi, nw, re[o]);
+          im[o] = mfma_i8(xi, w, im[o]);
+        };
+        // A digit 0 meets every B digit: its three fragments stay in registers for the step; the
+        // fragments of the other pairs are read one pair ahead of their products
+        const i32x4 z0 = rd(ore), z1 = rd(oim), z2 = rd(onim);
 #pragma unroll
-      for (int db = 0; db < NORD; db++) {
-        // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
-        // each plane then waits only for its own reload, a whole step after it was issued (LDS
-        // reads may still move across)
-        __builtin_amdgcn_sched_barrier(0x0100);
-        const i32x4 xr = br[db];
-        i32x4 xi = br[4 + db];
-        if (conjB) neg16(xi);
-        i32x4 c0, c1, c2;
-        if (db + 1 < NORD) {
-          c0 = rd(ore + 1024);
-          c1 = rd(oim + 1024);
-          c2 = rd(onim + 1024);
-        }
-        pair(db, xr, xi, z0, z1, z2);
-#pragma unroll
-        for (int da = 1; da + db < NORD; da++) {
-          i32x4 n0, n1, n2;
-          if (da + 1 + db < NORD) {
-            n0 = rd(ore + (da + 1) * 1024);
-            n1 = rd(oim + (da + 1) * 1024);
-            n2 = rd(onim + (da + 1) * 1024);
+        for (int db = 0; db < NORD; db++) {
+          // keeps the scheduler from hoisting digit db's use of B above the previous digits' products:
+          // each plane then waits only for its own reload, a whole step after it was issued (LDS
+          // reads may still move across)
+          __builtin_amdgcn_sched_barrier(0x0100);
+          const i32x4 xr = br[db];
+          i32x4 xi = br[4 + db];
+          if (conjB) neg16(xi);
+          i32x4 c0, c1, c2;
+          if (db + 1 < NORD) {
+            c0 = rd(ore + 1024);
+            c1 = rd(oim + 1024);
+            c2 = rd(onim + 1024);
           }
-          pair(da + db, xr, xi, c0, c1, c2);
-          c0 = n0;
-          c1 = n1;
-          c2 = n2;
+          pair(db, xr, xi, z0, z1, z2);
+#pragma unroll
+          for (int da = 1; da + db < NORD; da++) {
+            i32x4 n0, n1, n2;
+            if (da + 1 + db < NORD) {
+              n0 = rd(ore + (da + 1) * 1024);
+              n1 = rd(oim + (da + 1) * 1024);
+              n2 = rd(onim + (da + 1) * 1024);
+            }
+            pair(da + db, xr, xi, c0, c1, c2);
+            c0 = n0;
+            c1 = n1;
+            c2 = n2;
+          }
+          if (more) {
+            br[db] = gload<i32x4>(nsrc + db * P2);
+            br[4 + db] = gload<i32x4>(nsrc + (4 + db) * P2);
+          }
         }
-        if (more) {
-          br[db] = gload<i32x4>(nsrc + db * P2);
-          br[4 + db] = gload<i32x4>(nsrc + (4 + db) * P2);
+      };
+
+      // The loop runs every step but the last with unconditional loads, so that the compiler's count
+      // of loads in flight is exact on every path and each wait covers only what it needs.
+      if (sub == 0) {
+        storeA(0);
+        lds_barrier();
+        for (int st = 0; st + 1 < nstep; st++) {
+          // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
+          // one barrier per step suffices (that buffer was last read in step st - 1)
+          loadA(st + 1, cur.offA8);
+          compute(st, st & 1, true);
+          storeA((st + 1) & 1);
+          lds_barrier();
+        }
+      } else {
+        // the twin: every step of A is still in the stage buffers
+        for (int st = 0; st + 1 < nstep; st++) compute(st, st & 1, true);
+      }
+      // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
+      // j0 + 8 g + 4 (lane / 32) + r of B
+      const int ea = gload<int>(g.A.ex + vo(g.A.es, cur.co) + ea_blk + w_);
+      int eb[16];
+      {
+        const int* pe = gs.B.ex + vo(gs.B.es, cur.co) + eb_blk_s[sub] + 4 * h_;
+#pragma unroll
+        for (int gg = 0; gg < 4; gg++) {
+          const i32x4 v = gload<i32x4>(pe + 8 * gg);
+#pragma unroll
+          for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
         }
       }
-    };
+      compute(nstep - 1, (nstep - 1) & 1, false);
 
-    // The loop runs every step but the last with unconditional loads, so that the compiler's count
-    // of loads in flight is exact on every path and each wait covers only what it needs.
-    storeA(0);
-    lds_barrier();
-    for (int st = 0; st + 1 < nstep; st++) {
-      // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
-      // one barrier per step suffices (that buffer was last read in step st - 1)
-      loadA(st + 1, cur.offA8);
-      compute(st, st & 1, true);
-      storeA((st + 1) & 1);
-      lds_barrier();
-    }
-    // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
-    // j0 + 8 g + 4 (lane / 32) + r of B
-    const int ea = gload<int>(g.A.ex + vo(g.A.es, cur.co) + ea_blk + w_);
-    int eb[16];
-    {
-      const int* pe = g.B.ex + vo(g.B.es, cur.co) + eb_blk + 4 * h_;
-#pragma unroll
-      for (int gg = 0; gg < 4; gg++) {
-        const i32x4 v = gload<i32x4>(pe + 8 * gg);
-#pragma unroll
-        for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
-      }
-    }
-    compute(nstep - 1, (nstep - 1) & 1, false);
-
-    // ---- epilogue
-    // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact: |acc0| <= 2 K 65^2 < 2^23 (a 24-bit product)
-    // and |acc0 255 + acc1| < 2^31 for K <= 512; the value is 255^3 2^(E_A + E_B - 60) S, rounded once
-    double vr[16], vi[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const double sc = __builtin_ldexp(16581375.0, ea + eb[q] - 60);
-      const double sr = __builtin_fma(__builtin_fma((double)(__mul24(re[0][q], 255) + re[1][q]), 255.0, (double)re[2][q]),
-                                      255.0, (double)re[3][q]);
-      const double si = __builtin_fma(__builtin_fma((double)(__mul24(im[0][q], 255) + im[1][q]), 255.0, (double)im[2][q]),
-                                      255.0, (double)im[3][q]);
-      vr[q] = sr * sc;
-      vi[q] = si * sc;
-    }
-    if (!(g.alpha[0] == 1.0 && g.alpha[1] == 0.0)) {
-      const double alr = g.alpha[0], ali = g.alpha[1];
+      // ---- epilogue
+      // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact: |acc0| <= 2 K 65^2 < 2^23 (a 24-bit product)
+      // and |acc0 255 + acc1| < 2^31 for K <= 512; the value is 255^3 2^(E_A + E_B - 60) S, rounded once
+      double vr[16], vi[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
-        const double xr = vr[q], xi = vi[q];
-        vr[q] = alr * xr - ali * xi;
-        vi[q] = alr * xi + ali * xr;
+        const double sc = __builtin_ldexp(16581375.0, ea + eb[q] - 60);
+        const double sr = __builtin_fma(__builtin_fma((double)(__mul24(re[0][q], 255) + re[1][q]), 255.0, (double)re[2][q]),
+                                        255.0, (double)re[3][q]);
+        const double si = __builtin_fma(__builtin_fma((double)(__mul24(im[0][q], 255) + im[1][q]), 255.0, (double)im[2][q]),
+                                        255.0, (double)im[3][q]);
+        vr[q] = sr * sc;
+        vi[q] = si * sc;
       }
-    }
-    // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
-    // whole row), exchanged through LDS
-    if (g.Cd) {
-      double mx = 0.0;
+      if (!(gs.alpha[0] == 1.0 && gs.alpha[1] == 0.0)) {
+        const double alr = gs.alpha[0], ali = gs.alpha[1];
 #pragma unroll
-      for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
-      int em = exp_of(mx);
-      const int eo = __shfl_xor(em, 32, 64);
-      em = eo > em ? eo : em;
-      if (lane < 32) rowe[wave][lane] = em;
-    }
-    // The FP64 tile in LDS serves only FP64 outputs and FP64 closing factors; traces in the
-    // closing-factor form, closing-factor outputs and digit outputs work on the lane's registers.
-    const bool use_tile = g.C || (ntr > 0 && !g.Y[0].ex);
-    // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
-    // exponents are visible
-    if (g.Cd || use_tile) lds_barrier();
-    int E = 0;
-    if (g.Cd) {
-      E = INT_MIN;
-#pragma unroll
-      for (int w = 0; w < NW; w++) {
-        const int x = rowe[w][w_];
-        E = x > E ? x : E;
+        for (int q = 0; q < 16; q++) {
+          const double xr = vr[q], xi = vi[q];
+          vr[q] = alr * xr - ali * xi;
+          vi[q] = alr * xi + ali * xr;
+        }
       }
-      if (E == INT_MIN) E = 0;
-      if (wave == 0 && lane < 32) gstore(g.Ce + vo(g.ces, cur.co) + (long long)a * P + i0 + lane, E);
-    }
-    const double sE = __builtin_ldexp(1.0, 30 - E);
-    // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
-    // reads it (LDS operations of one wave complete in order: no barrier)
-    unsigned char* tile = &lds[wave * 32 * TSTR];
-    const int hcol = 4 * h_;
-    if (use_tile) {
+      // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
+      // whole row), exchanged through LDS
+      if (gs.Cd) {
+        double mx = 0.0;
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int col = 8 * (q >> 2) + hcol + (q & 3);
-        *reinterpret_cast<d2v*>(tile + w_ * TSTR + col * 16) = d2v{vr[q], vi[q]};
+        for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
+        int em = exp_of(mx);
+        const int eo = __shfl_xor(em, 32, 64);
+        em = eo > em ? eo : em;
+        if (lane < 32) rowe[wave][lane] = em;
       }
-    }
-    if (g.C) {
-      // whole rows: lanes 0..31 one row, 32..63 the next
-      Cplx<double>* C = reinterpret_cast<Cplx<double>*>(g.C) + vo(g.cs, cur.co) + (long long)(a * 4 + c) * nv2;
+      // The FP64 tile in LDS serves only FP64 outputs and FP64 closing factors; traces in the
+      // closing-factor form, closing-factor outputs and digit outputs work on the lane's registers.
+      const bool use_tile = gs.C || (ntr > 0 && !gs.Y[0].ex);
+      // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
+      // exponents are visible
+      if (gs.Cd || use_tile) lds_barrier();
+      int E = 0;
+      if (gs.Cd) {
+        E = INT_MIN;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          const int x = rowe[w][w_];
+          E = x > E ? x : E;
+        }
+        if (E == INT_MIN) E = 0;
+        if (wave == 0 && lane < 32) gstore(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, E);
+      }
+      const double sE = __builtin_ldexp(1.0, 30 - E);
+      // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
+      // reads it (LDS operations of one wave complete in order: no barrier)
+      unsigned char* tile = &lds[wave * 32 * TSTR];
+      const int hcol = 4 * h_;
+      if (use_tile) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int col = 8 * (q >> 2) + hcol + (q & 3);
+          *reinterpret_cast<d2v*>(tile + w_ * TSTR + col * 16) = d2v{vr[q], vi[q]};
+        }
+      }
+      if (gs.C) {
+        // whole rows: lanes 0..31 one row, 32..63 the next
+        Cplx<double>* C = reinterpret_cast<Cplx<double>*>(gs.C) + vo(gs.cs, cur.co) + (long long)(a * 4 + c) * nv2;
 #pragma unroll 4
-      for (int m = 0; m < 16; m++) {
-        const int lr = 2 * m + h_, lc = w_;
-        const int grow = i0 + lr, gcol = j0 + lc;
-        if (grow < nv && gcol < nv) {
-          const d2v v = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
-          gstore_c(C + (long long)grow * nv + gcol, Cplx<double>{v[0], v[1]});
-        }
-      }
-    }
-
-    if (ntr > 0) {
-      // Every closing-factor load of the product's traces is issued before the first use (one
-      // memory round trip).
-      const int w = w_, h = h_;
-      // FP64 closing factors (one trace at a time: 64 registers of loads each): trace t reads block
-      // (c, a) of f(Y_t), element (col, row): with LC_TV the stored element (row, col), read along
-      // stored rows; otherwise the stored (col, row). Lane l reads stored row r0 + 2 m + l / 32,
-      // column c0 + l % 32.
-      const bool full = i0 + 32 <= nv && j0 + 32 <= nv;
-      auto trace_fp64 = [&](int t) {
-        Cplx<double> yv[16];
-        const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(
-            static_cast<const char*>(g.Y[t].base) + 16 * vo(g.Y[t].s, cur.co) + rfl64(wtab[wave].ty[t]));
-        const int fl = rfl(wtab[wave].tfl[t]);
-        const bool tv = (fl & 1) != 0;
-        const double cj = (fl & 2) ? -1.0 : 1.0;
-        const int r0 = tv ? i0 : j0, c0 = tv ? j0 : i0;
-        const Cplx<double>* q = p + (long long)(r0 + h) * nv + c0 + w;
-        const long long rs = 2LL * nv;
-        if (full) {
-#pragma unroll
-          for (int m = 0; m < 16; m++) yv[m] = gload_c(q + m * rs);
-        } else {
-#pragma unroll
-          for (int m = 0; m < 16; m++) {
-            const bool in = c0 + w < nv && r0 + 2 * m + h < nv;
-            yv[m] = gload_c(in ? q + m * rs : p);
-            if (!in) yv[m] = Cplx<double>{0.0, 0.0};
+        for (int m = 0; m < 16; m++) {
+          const int lr = 2 * m + h_, lc = w_;
+          const int grow = i0 + lr, gcol = j0 + lc;
+          if (grow < nv && gcol < nv) {
+            const d2v v = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
+            gstore_c(C + (long long)grow * nv + gcol, Cplx<double>{v[0], v[1]});
           }
         }
-        double sr = 0, si = 0;
+      }
+
+      if (ntr > 0) {
+        // Every closing-factor load of the product's traces is issued before the first use (one
+        // memory round trip).
+        const int w = w_, h = h_;
+        // FP64 closing factors (one trace at a time: 64 registers of loads each): trace t reads block
+        // (c, a) of f(Y_t), element (col, row): with LC_TV the stored element (row, col), read along
+        // stored rows; otherwise the stored (col, row). Lane l reads stored row r0 + 2 m + l / 32,
+        // column c0 + l % 32.
+        const bool full = i0 + 32 <= nv && j0 + 32 <= nv;
+        auto trace_fp64 = [&](int t) {
+          Cplx<double> yv[16];
+          const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(
+              static_cast<const char*>(gs.Y[t].base) + 16 * vo(gs.Y[t].s, cur.co) + rfl64(wtab[sub][wave].ty[t]));
+          const int fl = rfl(wtab[sub][wave].tfl[t]);
+          const bool tv = (fl & 1) != 0;
+          const double cj = (fl & 2) ? -1.0 : 1.0;
+          const int r0 = tv ? i0 : j0, c0 = tv ? j0 : i0;
+          const Cplx<double>* q = p + (long long)(r0 + h) * nv + c0 + w;
+          const long long rs = 2LL * nv;
+          if (full) {
 #pragma unroll
-        for (int m = 0; m < 16; m++) {
-          const int u = 2 * m + h;
-          const int lr = tv ? u : w, lc = tv ? w : u;  // tile row / column
-          const d2v x = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
-          const double yr = yv[m].x, yi = cj * yv[m].y;
-          sr += x[0] * yr - x[1] * yi;
-          si += x[0] * yi + x[1] * yr;
-        }
-        const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfr[t])));
-        const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfi[t])));
-        tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
-      };
-      // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
-      // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its
-      // C tile, laid out like the accumulators: load k of lane l holds the values of its own tile
-      // entries q = 2 k and 2 k + 1 (row l % 32, columns 8 (k / 2) + 4 (l / 32) + 2 (k % 2), + 1),
-      // under that row's exponent, so each lane sums FP64 products of its registers with exact
-      // integers and scales once.
-      auto traces_cf = [&](auto NT) {
-        constexpr int N = decltype(NT)::value;
-        i32x4 yd[N][8];
-        int ey[N];
+            for (int m = 0; m < 16; m++) yv[m] = gload_c(q + m * rs);
+          } else {
 #pragma unroll
-        for (int t = 0; t < N; t++) {
-          const unsigned char* yb = static_cast<const unsigned char*>(g.Y[t].base) + 8 * vo(g.Y[t].s, cur.co) +
-                                    rfl64(wtab[wave].ty[t]) +
-                                    ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
-#pragma unroll
-          for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
-          const char* eyb = reinterpret_cast<const char*>(g.Y[t].ex + vo(g.Y[t].es, cur.co)) + rfl64(wtab[wave].te[t]);
-          ey[t] = gload<int>(reinterpret_cast<const int*>(eyb) + i0 + w);
-        }
-#pragma unroll
-        for (int t = 0; t < N; t++) {
-          const double cj = (rfl(wtab[wave].tfl[t]) & 2) ? -1.0 : 1.0;
-          double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
-#pragma unroll
-          for (int k = 0; k < 8; k++) {
-#pragma unroll
-            for (int p2 = 0; p2 < 2; p2++) {
-              const double xr = vr[2 * k + p2], xi = vi[2 * k + p2];
-              const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
-              s1 += xr * yr;
-              s2 += xi * yi;
-              s3 += xr * yi;
-              s4 += xi * yr;
+            for (int m = 0; m < 16; m++) {
+              const bool in = c0 + w < nv && r0 + 2 * m + h < nv;
+              yv[m] = gload_c(in ? q + m * rs : p);
+              if (!in) yv[m] = Cplx<double>{0.0, 0.0};
             }
           }
-          // x (yr + i cj yi), scaled by the row's 2^(E - 30)
-          const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
-          const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
-          const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfr[t])));
-          const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[wave].tfi[t])));
+          double sr = 0, si = 0;
+#pragma unroll
+          for (int m = 0; m < 16; m++) {
+            const int u = 2 * m + h;
+            const int lr = tv ? u : w, lc = tv ? w : u;  // tile row / column
+            const d2v x = *reinterpret_cast<const d2v*>(tile + lr * TSTR + lc * 16);
+            const double yr = yv[m].x, yi = cj * yv[m].y;
+            sr += x[0] * yr - x[1] * yi;
+            si += x[0] * yi + x[1] * yr;
+          }
+          const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfr[t])));
+          const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfi[t])));
           tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
+        };
+        // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
+        // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its
+        // C tile, laid out like the accumulators: load k of lane l holds the values of its own tile
+        // entries q = 2 k and 2 k + 1 (row l % 32, columns 8 (k / 2) + 4 (l / 32) + 2 (k % 2), + 1),
+        // under that row's exponent, so each lane sums FP64 products of its registers with exact
+        // integers and scales once.
+        auto traces_cf = [&](auto NT) {
+          constexpr int N = decltype(NT)::value;
+          i32x4 yd[N][8];
+          int ey[N];
+#pragma unroll
+          for (int t = 0; t < N; t++) {
+            const unsigned char* yb = static_cast<const unsigned char*>(gs.Y[t].base) + 8 * vo(gs.Y[t].s, cur.co) +
+                                      rfl64(wtab[sub][wave].ty[t]) +
+                                      ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
+#pragma unroll
+            for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
+            const char* eyb = reinterpret_cast<const char*>(gs.Y[t].ex + vo(gs.Y[t].es, cur.co)) + rfl64(wtab[sub][wave].te[t]);
+            ey[t] = gload<int>(reinterpret_cast<const int*>(eyb) + i0 + w);
+          }
+#pragma unroll
+          for (int t = 0; t < N; t++) {
+            const double cj = (rfl(wtab[sub][wave].tfl[t]) & 2) ? -1.0 : 1.0;
+            double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+#pragma unroll
+              for (int p2 = 0; p2 < 2; p2++) {
+                const double xr = vr[2 * k + p2], xi = vi[2 * k + p2];
+                const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
+                s1 += xr * yr;
+                s2 += xi * yi;
+                s3 += xr * yi;
+                s4 += xi * yr;
+              }
+            }
+            // x (yr + i cj yi), scaled by the row's 2^(E - 30)
+            const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
+            const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
+            const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfr[t])));
+            const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfi[t])));
+            tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
+          }
+        };
+        if (gs.Y[0].ex) {
+          if (ntr == 1) traces_cf(std::integral_constant<int, 1>{});
+          else if (ntr == 2) traces_cf(std::integral_constant<int, 2>{});
+          else traces_cf(std::integral_constant<int, 3>{});
+        } else {
+          for (int t = 0; t < ntr; t++) trace_fp64(t);
         }
-      };
-      if (g.Y[0].ex) {
-        if (ntr == 1) traces_cf(std::integral_constant<int, 1>{});
-        else if (ntr == 2) traces_cf(std::integral_constant<int, 2>{});
-        else traces_cf(std::integral_constant<int, 3>{});
+        lds_barrier();
+        // wave v sums component k = v % 2 of trace t = v / 2 over the workgroup's 64 NW lanes
+        for (int v = wave; v < 2 * ntr; v += NW) {
+          const int t = v >> 1, k = v & 1;
+          const double* ts = reinterpret_cast<const double*>(&tsum[t][0]) + k;
+          double sm = 0;
+#pragma unroll
+          for (int w2 = 0; w2 < NW; w2++) sm += ts[2 * (w2 * 64 + lane)];
+          sm = wave_sum(sm);
+          if (lane == 0) gstore(gs.part[t] + 2 * (((long long)cur.item << wpi_log) + rr) + k, sm);
+        }
+      }
+
+      const bool last_sub = sub == 1 || twi < 0;  // (uniform)
+      if (last_sub) {
+        // the next item of this workgroup: with the same tables (same group and strip), its first
+        // loads go out now and land while this item's output is written
+        nwg = wg + nj;
+        more_items = nwg < wend;
+        if (more_items)
+          while (ngi + 1 < ng && groups[ngi + 1].wg_begin <= nwg) ngi++;
+        pre = more_items && ngi == gi && (nj & ((1 << wpi_log) - 1)) == 0;
+        if (pre) {
+          nxt = item_pos(nwg);
+          prologue(nxt);
+        }
       } else {
-        for (int t = 0; t < ntr; t++) trace_fp64(t);
+        // the twin's first B fragments
+        loadB0(8 * vo(twins[twi].B.s, cur.co), 1);
       }
+
+      if (gs.Cd && gs.cf) {
+        // closing-factor form: load k of the chunk holds the lane's own entries q = 2 k, 2 k + 1 (see
+        // traces_cf); one coalesced kilobyte per k
+        unsigned char* base = static_cast<unsigned char*>(gs.Cd) + 8 * (vo(gs.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+                              ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const i32x4 v = {scaled_int(vr[2 * k], sE), scaled_int(vi[2 * k], sE), scaled_int(vr[2 * k + 1], sE),
+                           scaled_int(vi[2 * k + 1], sE)};
+          gstore(base + 1024 * k, v);
+        }
+      } else if (gs.Cd) {
+        // digits of the lane's 16 values: word W[d][gg] packs digit plane d (re digits 0..3, then im)
+        // of columns 8 gg + 4 (l / 32) .. + 3 of row l % 32. A stored plane row of a lane is 16
+        // adjacent columns (16 (l / 32) ..): lane l keeps W[.][0..1] (l < 32) or W[.][2..3] and takes
+        // the other two from lane l ^ 32, one v_permlane32_swap per pair (gg, gg + 2): the swap of
+        // the upper lanes of x with the lower lanes of y leaves x = (W_gg of lanes 0..31, W_gg+2 of
+        // lanes 0..31) and y = (W_gg of lanes 32..63, W_gg+2 of lanes 32..63), which are the words
+        // 2 gg' and 2 gg' + 1 of both halves.
+        unsigned wd[8][4];
+#pragma unroll
+        for (int gg = 0; gg < 4; gg++) {
+          int dr[4][4], di[4][4];  // [column r][digit]
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            digits_fast(vr[4 * gg + r], sE, dr[r]);
+            digits_fast(vi[4 * gg + r], sE, di[r]);
+          }
+#pragma unroll
+          for (int d = 0; d < 4; d++) {
+            wd[d][gg] = pack4p(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
+            wd[4 + d][gg] = pack4p(di[0][d], di[1][d], di[2][d], di[3][d]);
+          }
+        }
+        unsigned char* base = static_cast<unsigned char*>(gs.Cd) + 8 * (vo(gs.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+                              ((long long)(j0 >> 5) * P + i0) * 32;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const auto s0 = __builtin_amdgcn_permlane32_swap(wd[q][0], wd[q][2], false, false);
+          const auto s1 = __builtin_amdgcn_permlane32_swap(wd[q][1], wd[q][3], false, false);
+          const i32x4 v = {(int)s0[0], (int)s0[1], (int)s1[0], (int)s1[1]};
+          gstore(base + q * P2 + mem_off, v);
+        }
+      }
+
+    };
+    run_sub(std::integral_constant<int, 0>{});
+    if (twi >= 0) {
+      // the twin writes the trace sums that slower waves may still be reading
       lds_barrier();
-      // wave v sums component k = v % 2 of trace t = v / 2 over the workgroup's 64 NW lanes
-      for (int v = wave; v < 2 * ntr; v += NW) {
-        const int t = v >> 1, k = v & 1;
-        const double* ts = reinterpret_cast<const double*>(&tsum[t][0]) + k;
-        double sm = 0;
-#pragma unroll
-        for (int w2 = 0; w2 < NW; w2++) sm += ts[2 * (w2 * 64 + lane)];
-        sm = wave_sum(sm);
-        if (lane == 0) gstore(g.part[t] + 2 * (((long long)cur.item << wpi_log) + rr) + k, sm);
-      }
-    }
-
-    // the next item of this workgroup: with the same tables (same group and strip), its first loads
-    // go out now and land while this item's output is written
-    const long long nwg = wg + nj;
-    const bool more_items = nwg < wend;
-    int ngi = gi;
-    if (more_items)
-      while (ngi + 1 < ng && groups[ngi + 1].wg_begin <= nwg) ngi++;
-    const bool pre = more_items && ngi == gi && (nj & ((1 << wpi_log) - 1)) == 0;
-    ItemPos nxt = cur;
-    if (pre) {
-      nxt = item_pos(nwg);
-      prologue(nxt);
-    }
-
-    if (g.Cd && g.cf) {
-      // closing-factor form: load k of the chunk holds the lane's own entries q = 2 k, 2 k + 1 (see
-      // traces_cf); one coalesced kilobyte per k
-      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
-                            ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const i32x4 v = {scaled_int(vr[2 * k], sE), scaled_int(vi[2 * k], sE), scaled_int(vr[2 * k + 1], sE),
-                         scaled_int(vi[2 * k + 1], sE)};
-        gstore(base + 1024 * k, v);
-      }
-    } else if (g.Cd) {
-      // digits of the lane's 16 values: word W[d][gg] packs digit plane d (re digits 0..3, then im)
-      // of columns 8 gg + 4 (l / 32) .. + 3 of row l % 32. A stored plane row of a lane is 16
-      // adjacent columns (16 (l / 32) ..): lane l keeps W[.][0..1] (l < 32) or W[.][2..3] and takes
-      // the other two from lane l ^ 32, one v_permlane32_swap per pair (gg, gg + 2): the swap of
-      // the upper lanes of x with the lower lanes of y leaves x = (W_gg of lanes 0..31, W_gg+2 of
-      // lanes 0..31) and y = (W_gg of lanes 32..63, W_gg+2 of lanes 32..63), which are the words
-      // 2 gg' and 2 gg' + 1 of both halves.
-      unsigned wd[8][4];
-#pragma unroll
-      for (int gg = 0; gg < 4; gg++) {
-        int dr[4][4], di[4][4];  // [column r][digit]
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          digits_fast(vr[4 * gg + r], sE, dr[r]);
-          digits_fast(vi[4 * gg + r], sE, di[r]);
-        }
-#pragma unroll
-        for (int d = 0; d < 4; d++) {
-          wd[d][gg] = pack4p(dr[0][d], dr[1][d], dr[2][d], dr[3][d]);
-          wd[4 + d][gg] = pack4p(di[0][d], di[1][d], di[2][d], di[3][d]);
-        }
-      }
-      unsigned char* base = static_cast<unsigned char*>(g.Cd) + 8 * (vo(g.cds, cur.co) + (long long)(a * 4 + c) * P2) +
-                            ((long long)(j0 >> 5) * P + i0) * 32;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const auto s0 = __builtin_amdgcn_permlane32_swap(wd[q][0], wd[q][2], false, false);
-        const auto s1 = __builtin_amdgcn_permlane32_swap(wd[q][1], wd[q][3], false, false);
-        const i32x4 v = {(int)s0[0], (int)s0[1], (int)s1[0], (int)s1[1]};
-        gstore(base + q * P2 + mem_off, v);
-      }
+      run_sub(std::integral_constant<int, 1>{});
     }
 
     if (!more_items) break;
@@ -915,8 +967,8 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
 }  // namespace lc
 
 // NW = 4 for P = 32 (strips of 128 columns), 8 otherwise (256 columns).
-extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long long total_wg, int nv, int P,
-                                        hipStream_t s) {
+extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
+                                        long long total_wg, int nv, int P, hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
   // persistent: as many workgroups per XCD as are resident at once (one per CU), at most one per
   // logical workgroup
@@ -931,9 +983,9 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long 
   const long long per = (total_wg + 7) / 8;
   const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
   if (P == 32)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4>), grid, dim3(256), 0, s, d_groups, ng, nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
   else
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8>), grid, dim3(512), 0, s, d_groups, ng, nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
   return hipGetLastError();
 }
 

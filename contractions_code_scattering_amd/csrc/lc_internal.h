@@ -131,6 +131,10 @@ struct LcOzGroup {
   int kind;                 // 0 dense x dense, 1 elemental x dense, 2 dense x elemental
   int ntr;
   int nseg;                 // column segments of the workgroup strip per D-row
+  // index of the group's twin in the launch's twin array, or -1. A twin is a second product with
+  // the same A operand, kind and items and its own B and fused traces (no C or Cd), formed per item
+  // right after this one from the A operand already staged on the chip (K <= 2 * 32 per block)
+  int twin;
   long long wg_begin;
   LcView Y[LC_MAX_FUSED_TRACES];
   double tr_alpha[LC_MAX_FUSED_TRACES][2];

@@ -371,11 +371,12 @@ void Program::add_reduce(Level* l, const FusedTrace& f, int n0, int n1, int n2, 
 
 void Program::gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
                       const DigitOut* dout, int n0, int n1, int n2, double ar, double ai,
-                      const std::vector<FusedTrace>& traces) {
+                      const std::vector<FusedTrace>& traces, const LcView* twin_B,
+                      const std::vector<FusedTrace>* twin_traces) {
   if (mode_ != OZ) fail(LC_ERR_ARG, "exact-form product outside the exact mode");
   const long long cs[3] = {cs0, cs1, cs2};
   if (!last_axis_fastest_ || n2 <= 1) {
-    oz_items(A, B, C, cs, dout, n0, n1, n2, ar, ai, traces);
+    oz_items(A, B, C, cs, dout, n0, n1, n2, ar, ai, traces, twin_B, twin_traces);
     return;
   }
   // the same batch with its axes relabelled as in batch(): axis 2 fastest, then the second axis
@@ -389,11 +390,14 @@ void Program::gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, 
     rot3(v.es, v.es);
     return v;
   };
-  std::vector<FusedTrace> tr = traces;
-  for (FusedTrace& f : tr) {
-    f.Y = rot(f.Y);
-    rot3(f.os, f.os);
-  }
+  std::vector<FusedTrace> tr = traces, tr2 = twin_traces ? *twin_traces : std::vector<FusedTrace>{};
+  for (std::vector<FusedTrace>* v : {&tr, &tr2})
+    for (FusedTrace& f : *v) {
+      f.Y = rot(f.Y);
+      rot3(f.os, f.os);
+    }
+  LcView tb;
+  if (twin_B) tb = rot(*twin_B);
   long long rc[3];
   rot3(cs, rc);
   DigitOut d;
@@ -403,11 +407,13 @@ void Program::gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, 
     rot3(dout->es, d.es);
   }
   const int nn[3] = {n0, n1, n2};
-  oz_items(rot(A), rot(B), C, rc, dout ? &d : nullptr, nn[2], nn[o1], nn[o2], ar, ai, tr);
+  oz_items(rot(A), rot(B), C, rc, dout ? &d : nullptr, nn[2], nn[o1], nn[o2], ar, ai, tr, twin_B ? &tb : nullptr,
+           twin_B ? &tr2 : nullptr);
 }
 
 void Program::oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
-                       int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces) {
+                       int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
+                       const LcView* twin_B, const std::vector<FusedTrace>* twin_traces) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   const int P = oz_pad(nv_);
   if (nv_ > 128) fail(LC_ERR_ARG, "exact form supports nv <= 128 (got %d)", nv_);
@@ -429,15 +435,37 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
     fail(LC_ERR_ARG, "exact-form dense operand lines cross stored block rows");
   // closing factors: all FP64, or all in the closing-factor form (int32 per value under line
   // exponents) read in the orientation of the product (LC_TV), dense
-  for (const FusedTrace& f : traces) {
-    if ((f.Y.ex != nullptr) != (traces[0].Y.ex != nullptr))
-      fail(LC_ERR_ARG, "exact-form closing factors of one product mix FP64 and the closing-factor form");
-    if (f.Y.ex && (f.Y.kind != LC_DENSE || !(f.Y.flags & LC_TV)))
-      fail(LC_ERR_ARG, "a closing factor in the closing-factor form must be a dense view read along its stored rows (LC_TV)");
-  }
+  auto check_traces = [](const std::vector<FusedTrace>& trs) {
+    for (const FusedTrace& f : trs) {
+      if ((f.Y.ex != nullptr) != (trs[0].Y.ex != nullptr))
+        fail(LC_ERR_ARG, "exact-form closing factors of one product mix FP64 and the closing-factor form");
+      if (f.Y.ex && (f.Y.kind != LC_DENSE || !(f.Y.flags & LC_TV)))
+        fail(LC_ERR_ARG, "a closing factor in the closing-factor form must be a dense view read along its stored rows (LC_TV)");
+    }
+  };
+  check_traces(traces);
   const int NW = P == 32 ? 4 : 8;
   const int nseg = (4 * P) / (NW * 32);
   if (dout && nseg != 1) fail(LC_ERR_ARG, "exact-form digit output needs 4 P <= 256 (P = %d)", P);
+  // a twin reuses the A operand the kernel staged for this product: every K step of it must still
+  // be in the two stage buffers (one block of K <= 64), and nothing of the product may use the
+  // FP64 tile that overlaps them (no FP64 output, closing factors in the closing-factor form)
+  if (twin_B) {
+    if (!twin_traces || twin_traces->empty() || (int)twin_traces->size() > LC_MAX_FUSED_TRACES)
+      fail(LC_ERR_ARG, "a twin product needs 1..%d fused traces", LC_MAX_FUSED_TRACES);
+    if (kind == 0 || P > 64 || nseg != 1) fail(LC_ERR_ARG, "twin products need an elemental factor and P <= 64");
+    if ((twin_B->kind == LC_KRON) != (B.kind == LC_KRON) || !(twin_B->flags & LC_TV) || !twin_B->ex)
+      fail(LC_ERR_ARG, "a twin's B must be read like B (same kind, by columns, with exponents)");
+    for (int a = 0; a < 4; a++)
+      for (const double* ph : {twin_B->ph_r[a], twin_B->ph_c[a]})
+        if (!((ph[1] == 0 && (ph[0] == 1 || ph[0] == -1)) || (ph[0] == 0 && (ph[1] == 1 || ph[1] == -1))))
+          fail(LC_ERR_ARG, "exact-form operand phase (%g, %g) is not a power of i", ph[0], ph[1]);
+    if (C) fail(LC_ERR_ARG, "a product with a twin cannot have an FP64 output");
+    if (!traces.empty() && !traces[0].Y.ex)
+      fail(LC_ERR_ARG, "a product with a twin needs its closing factors in the closing-factor form");
+    check_traces(*twin_traces);
+    if (!(*twin_traces)[0].Y.ex) fail(LC_ERR_ARG, "a twin's closing factors must be in the closing-factor form");
+  }
   Level* l = open_level(false, kOzGemm);
   LcOzGroup g;
   std::memset(&g, 0, sizeof g);
@@ -459,6 +487,7 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   g.n[0] = n0; g.n[1] = n1; g.n[2] = n2;
   g.kind = kind;
   g.nseg = nseg;
+  g.twin = -1;
   g.wg_begin = l->total_wg;
   const long long items = (long long)n0 * n1 * n2;
   const long long per_item = (long long)(4 * P / 32) * nseg;
@@ -474,9 +503,32 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
     patches_.back().group = ozs_.size();
     patches_.back().oz = true;
   }
-  ozs_.push_back(g);
   const int nterm = kind == 0 ? 4 : 1;
   gemm_flops_ += (double)items * 16 * nterm * 8.0 * nv_ * (double)nv_ * nv_;
+  if (twin_B) {
+    LcOzGroup t = g;
+    t.B = *twin_B;
+    t.Cd = nullptr;
+    t.Ce = nullptr;
+    t.cf = 0;
+    t.alpha[0] = 1.0;
+    t.alpha[1] = 0.0;
+    t.ntr = (int)twin_traces->size();
+    for (int k = 0; k < t.ntr; k++) {
+      const FusedTrace& f = (*twin_traces)[k];
+      t.Y[k] = f.Y;
+      t.tr_alpha[k][0] = f.ar;
+      t.tr_alpha[k][1] = f.ai;
+      add_reduce(l, f, n0, n1, n2, per_item, k);
+      patches_.back().group = oztw_.size();
+      patches_.back().oz = true;
+      patches_.back().twin = true;
+    }
+    g.twin = (int)oztw_.size();
+    oztw_.push_back(t);
+    gemm_flops_ += (double)items * 16 * 8.0 * nv_ * (double)nv_ * nv_;
+  }
+  ozs_.push_back(g);
 }
 
 void Program::to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf) {
@@ -519,7 +571,7 @@ void Program::run(hipStream_t s) {
   char* pbase = static_cast<char*>(c.partials.p);
   for (const Patch& p : patches_) {
     double* q = reinterpret_cast<double*>(pbase + p.part_off);
-    if (p.oz) ozs_[p.group].part[p.slot] = q;
+    if (p.oz) (p.twin ? oztw_ : ozs_)[p.group].part[p.slot] = q;
     else if (p.slot < LC_MAX_FUSED_TRACES) gemms_[p.group].part[p.slot] = q;
     else gemms_[p.group].partx = q;
   }
@@ -543,6 +595,7 @@ void Program::run(hipStream_t s) {
   const size_t jb = jobs.size() * sizeof(LcReduceJob);
   const size_t bb = begins.size() * sizeof(long long);
   const size_t ob = ozs_.size() * sizeof(LcOzGroup);
+  const size_t wb = oztw_.size() * sizeof(LcOzGroup);
   const size_t toff = (gb + 255) & ~size_t(255);
   const size_t joff = (toff + tb + 255) & ~size_t(255);
   const size_t boff = (joff + jb + 255) & ~size_t(255);
@@ -554,14 +607,17 @@ void Program::run(hipStream_t s) {
     c.log->bufs.emplace_back();
     dbuf = &c.log->bufs.back();
   }
-  dbuf->ensure(ooff + ob + 256);
+  const size_t woff = (ooff + ob + 255) & ~size_t(255);
+  dbuf->ensure(woff + wb + 256);
   char* base = static_cast<char*>(dbuf->p);
   if (gb) hip_check(hipMemcpyAsync(base, gemms_.data(), gb, hipMemcpyHostToDevice, s), "upload gemm groups");
   if (tb) hip_check(hipMemcpyAsync(base + toff, traces_.data(), tb, hipMemcpyHostToDevice, s), "upload trace groups");
   if (jb) hip_check(hipMemcpyAsync(base + joff, jobs.data(), jb, hipMemcpyHostToDevice, s), "upload reduce jobs");
   if (bb) hip_check(hipMemcpyAsync(base + boff, begins.data(), bb, hipMemcpyHostToDevice, s), "upload reduce offsets");
   if (ob) hip_check(hipMemcpyAsync(base + ooff, ozs_.data(), ob, hipMemcpyHostToDevice, s), "upload exact-form groups");
+  if (wb) hip_check(hipMemcpyAsync(base + woff, oztw_.data(), wb, hipMemcpyHostToDevice, s), "upload exact-form twins");
   const LcOzGroup* dozs = reinterpret_cast<const LcOzGroup*>(base + ooff);
+  const LcOzGroup* dtw = reinterpret_cast<const LcOzGroup*>(base + woff);
   const LcReduceJob* dj = reinterpret_cast<const LcReduceJob*>(base + joff);
   const long long* db = reinterpret_cast<const long long*>(base + boff);
   // host vectors may be pageable: make sure the copies have landed before they can change
@@ -596,7 +652,8 @@ void Program::run(hipStream_t s) {
     } else if (variant == kOzGemm) {
       const LcOzGroup* og = dozs + l.first;
       const int P = oz_pad(nv);
-      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, total_wg, nv, P, st); }, "exact gemm launch");
+      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, st); },
+             "exact gemm launch");
     } else {
       const LcTraceGroup* tg = dt + l.first;
       const LcGemmGroup* gg = dg + l.first;

@@ -29,7 +29,8 @@ extern "C" hipError_t lc_launch_split_rescale(const void* x, void* y_rows, void*
 extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, const void* b,
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
-extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, long long total_wg, int nv, int P,
+extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
+                                        long long total_wg, int nv, int P,
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
@@ -255,9 +256,12 @@ class Program {
   // OZ mode: C = alpha * A * B with A, B digit views (A read by rows, B by columns: B carries
   // LC_TV), phases powers of i. Outputs: FP64 C (optional, strides cs in complex elements of nv^2
   // blocks), digits with per-D-row exponents (optional; needs 4 P <= 256), fused traces (FP64 Y).
+  // With twin_B (P <= 64, no C): the twin product A twin_B over the same batch, formed per item
+  // right after A B from the A operand already on the chip, delivering only twin_traces.
   void gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
                const DigitOut* dout, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
-               const std::vector<FusedTrace>& traces = {});
+               const std::vector<FusedTrace>& traces = {}, const LcView* twin_B = nullptr,
+               const std::vector<FusedTrace>* twin_traces = nullptr);
   // OZ mode: FP64 D-space matrices (nmat of them, `stride` complex elements apart) to digits with
   // per-D-row exponents (the form gemm_oz writes), as a level of its own.
   void to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf = false);
@@ -294,12 +298,14 @@ class Program {
     int slot;
     size_t part_off;  // bytes into the partials buffer
     bool oz = false;
+    bool twin = false;  // (oz) the group is in the twin array
   };
   static constexpr int kOzGemm = 10, kOzConvert = 11;
-  std::vector<LcOzGroup> ozs_;
+  std::vector<LcOzGroup> ozs_, oztw_;
   std::vector<Convert> converts_;
   void oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
-                int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces);
+                int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
+                const LcView* twin_B, const std::vector<FusedTrace>* twin_traces);
   int nv_;
   bool dbl_;
   Mode mode_;
