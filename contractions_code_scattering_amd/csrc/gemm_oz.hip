@@ -2,21 +2,24 @@
 // into and out of digit storage.
 //
 // Number format (lc_internal.h). Every real value x of a stored line (a row of an A operand, a
-// column of a B operand) is four balanced base-255 digits under the line's power-of-two exponent E:
-//   x = 2^(E-30) * v,   v = D0 255^3 + D1 255^2 + D2 255 + D3,   v = rint(x 2^(30-E)),
-// with D1..D3 in [-127, 127], D0 in [-65, 65] and E the smallest integer such that every |x| of the
-// line is below 2^E: 30 bits below the largest magnitude of the line (truncation <= 2^(E-31)).
-// Base 255 instead of 256 keeps every digit's negation inside int8 (the products need negated
-// digits for the complex product and for phases and conjugation).
+// column of a B operand) is four balanced base-255 digits under the line's power-of-two exponent E
+// and the fixed factor SIGMA = 63/32:
+//   x = v / (SIGMA 2^(30-E)),   v = D0 255^3 + D1 255^2 + D2 255 + D3,   v = rint(x SIGMA 2^(30-E)),
+// with D0..D3 in [-127, 127] (|v| < SIGMA 2^30 <= (255^4 - 1) / 2) and E the smallest integer such
+// that every |x| of the line is below 2^E: truncation <= 2^(E-31) / SIGMA. SIGMA lets the leading
+// digit use the whole int8 range: the largest products of a line then dominate the dropped digit
+// orders by (127 / 65)^2 more than under a plain power of two. Base 255 instead of 256 keeps every
+// digit's negation inside int8 (the products need negated digits for the complex product and for
+// phases and conjugation).
 //
 // Product. For C = A B every digit pair (a, b) of total order o = a + b <= 3 is one
 // v_mfma_i32_32x32x32_i8 per real product (10 pairs); the pairs of one order share an int32
 // accumulator, |acc_o| <= 2 (o + 1) K 127^2 < 2^27 for K <= 512, so the integer sums are exact and
 // the only rounding of a product is the digit truncation of its operands and the dropped orders
-// 4..6, all near 2^-31 of the largest products (a numerical model of the format at D = 256 gives
-// about 3e-8 of max |C| for random operands). The epilogue forms, per element,
-//   S = ((acc_0 255 + acc_1) 255 + acc_2) 255 + acc_3          (exact in FP64, < 2^48)
-//   C_ij = alpha * 2^(E_A,i + E_B,j - 60) * rn(S 255^3)
+// 4..6, all near 2^-32 of the largest products (a numerical model of the format at D = 256 gives
+// about 6e-9 of max |C| for random operands). The epilogue forms, per element,
+//   S = ((acc_0 255 + acc_1) 255 + acc_2) 255 + acc_3          (exact in FP64, < 2^51)
+//   C_ij = alpha * 2^(E_A,i + E_B,j - 60) * rn(S UNSCALE),   UNSCALE = rn(255^3 / SIGMA^2)
 // (the one rounded step, rn, is relative 2^-53)
 // and stores it in FP64, traces it in FP64 against FP64 closing factors, and/or writes it back as
 // digits with a fresh exponent per D-row (the largest |C_ij| of the row over all four blocks,
@@ -34,8 +37,9 @@
 // digit negations and a choice of planes.
 //
 // The MFMA is issued with the B fragment first, so lane l holds row l % 32 of C and the 16
-// columns 8 g + 4 (l / 32) + r (g, r < 4). The epilogue stages the tile in LDS so that FP64
-// stores, closing-factor loads and digit stores are all whole rows / whole kilobytes.
+// columns 8 g + 4 (l / 32) + r (g, r < 4). The epilogue traces and converts from these registers
+// (closing factors are laid out like them; digit rows are exchanged between the lane halves) and
+// stages the tile in LDS only for FP64 outputs and FP64 closing factors, stored as whole rows.
 #include <climits>
 #include <type_traits>
 
@@ -63,17 +67,30 @@ __device__ __forceinline__ void neg16(i32x4& v) {
 // never raise a line's exponent.
 __device__ __forceinline__ int exp_of(double x) { return x == 0.0 ? INT_MIN : __builtin_amdgcn_frexp_exp(x); }
 
+// Round-to-integer constant 1.5 2^52: fma(y, c, RND) rounds the exact y c to an integer Q (|Q| < 2^51)
+// and holds Q mod 2^32 in its low word.
+constexpr double RND = 6755399441055744.0;
+__device__ __forceinline__ int lo32(double t) {
+  return (int)(unsigned)(__builtin_bit_cast(unsigned long long, t) & 0xffffffffull);
+}
+
+// Digit scale of a line: v = rint(x SIGMA 2^(30 - E)), the exact product rounded once; a product
+// of two lines comes back with UNSCALE = 255^3 / SIGMA^2 (rounded to FP64 once, the same constant
+// as the format's model).
+constexpr double SIGMA = 1.96875;
+constexpr double UNSCALE = 16581375.0 * 1024.0 / 3969.0;
+
 // Balanced base-255 digits of x under exponent E (|x| < 2^E), d[0] the most significant. With the
-// bias 255 * 4210752 (about 2^30) the running value is a 32-bit unsigned number, and floor(u / 255)
-// is the high half of u * 0x80808081 shifted by 7 (exact for every 32-bit u).
+// bias 255 * 8290000 (about 2^31) the running value is a 32-bit unsigned number (|v| < 2^31), and
+// floor(u / 255) is the high half of u * 0x80808081 shifted by 7 (exact for every 32-bit u).
 __device__ __forceinline__ void digits_of(double x, int E, int (&d)[4]) {
-  int v = (int)__builtin_rint(__builtin_ldexp(x, 30 - E));
+  int v = lo32(__builtin_fma(x, __builtin_ldexp(SIGMA, 30 - E), RND));
 #pragma unroll
   for (int k = 3; k >= 1; k--) {
-    const unsigned u = (unsigned)(v + 127 + 1073741760);
+    const unsigned u = (unsigned)v + 127u + 2113950000u;
     const unsigned q = __umulhi(u, 0x80808081u) >> 7;
     d[k] = (int)(u - 255u * q) - 127;
-    v = (int)q - 4210752;
+    v = (int)q - 8290000;
   }
   d[0] = v;
 }
@@ -84,22 +101,16 @@ __device__ __forceinline__ unsigned pack4p(int a, int b, int c, int d) {
   return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
 }
 
-// Round-to-integer constant 1.5 2^52: fma(y, c, RND) rounds the exact y c to an integer Q (|Q| < 2^51)
-// and holds Q mod 2^32 in its low word.
-constexpr double RND = 6755399441055744.0;
-__device__ __forceinline__ int lo32(double t) {
-  return (int)(unsigned)(__builtin_bit_cast(unsigned long long, t) & 0xffffffffull);
-}
 // v = rint(x s) as an integer, for s = 2^(30 - E) and |x| < 2^E (one fma)
 __device__ __forceinline__ int scaled_int(double x, double s) { return lo32(__builtin_fma(x, s, RND)); }
 // The digits of digits_of by FP64 arithmetic in 8 operations (the GEMM epilogue's form), for
-// s = 2^(30 - E) and |x| < 2^E: with v = rint(x s) and Q_k = round(v / 255^k),
+// s = SIGMA 2^(30 - E) and |x| < 2^E: with v = rint(x s) and Q_k = round(v / 255^k),
 //   d = (Q_3, Q_2 - 255 Q_3, Q_1 - 255 Q_2, v - 255 Q_1).
 // v / 255^(k+1) = Q_k / 255 + (v / 255^k - Q_k) / 255; the second term is below 1/510 in magnitude and
 // Q_k / 255 is at least 1/510 away from a half-integer (255 is odd), so Q_(k+1) = round(Q_k / 255):
 // each difference is a balanced digit, and these are the unique digits. v / 255^k itself is at least
 // 1/(2 255^k) from a half-integer, far beyond the error of the product with the rounded 1/255^k
-// (below 2^-44); |Q_1| < 2^23, so the multiples of 255 are 24-bit products.
+// (below 2^-44); |Q_1| <= 2^31 / 255 < 2^23, so the multiples of 255 are 24-bit products.
 __device__ __forceinline__ void digits_fast(double x, double s, int (&d)[4]) {
   const double t0 = __builtin_fma(x, s, RND);
   const double v = t0 - RND;
@@ -164,7 +175,9 @@ constexpr int lds_bytes() {
 // workgroup keeps its strip from item to item: the per-strip tables (view blocks, phases, trace
 // blocks) are built when the product group changes, and the first operand loads of the next item
 // are issued before the epilogue of the current one.
-template <int NW>
+// K256: every K sum of the launch has at most 256 terms (P <= 64), which lets the epilogue combine
+// the two leading accumulators in 24-bit integer arithmetic.
+template <int NW, bool K256>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup* __restrict__ twins, int nv, int P,
                   long long total) {
@@ -568,18 +581,19 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       compute(nstep - 1, (nstep - 1) & 1, false);
 
       // ---- epilogue
-      // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact: |acc0| <= 2 K 65^2 < 2^23 (a 24-bit product)
-      // and |acc0 255 + acc1| < 2^31 for K <= 512; the value is 255^3 2^(E_A + E_B - 60) S, rounded once
+      // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact in FP64 (|acc_o| < 2^27, |S| < 2^51);
+      // the value is UNSCALE 2^(E_A + E_B - 60) S, rounded once. For K <= 256 (P <= 64) the first
+      // step is integer: |acc0| <= 2 K 127^2 < 2^23 (a 24-bit product) and |acc0 255 + acc1| < 2^31.
+      auto comb = [](int a0, int a1, int a2, int a3) {
+        const double h = K256 ? (double)(__mul24(a0, 255) + a1) : __builtin_fma((double)a0, 255.0, (double)a1);
+        return __builtin_fma(__builtin_fma(h, 255.0, (double)a2), 255.0, (double)a3);
+      };
       double vr[16], vi[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
-        const double sc = __builtin_ldexp(16581375.0, ea + eb[q] - 60);
-        const double sr = __builtin_fma(__builtin_fma((double)(__mul24(re[0][q], 255) + re[1][q]), 255.0, (double)re[2][q]),
-                                        255.0, (double)re[3][q]);
-        const double si = __builtin_fma(__builtin_fma((double)(__mul24(im[0][q], 255) + im[1][q]), 255.0, (double)im[2][q]),
-                                        255.0, (double)im[3][q]);
-        vr[q] = sr * sc;
-        vi[q] = si * sc;
+        const double sc = __builtin_ldexp(UNSCALE, ea + eb[q] - 60);
+        vr[q] = comb(re[0][q], re[1][q], re[2][q], re[3][q]) * sc;
+        vi[q] = comb(im[0][q], im[1][q], im[2][q], im[3][q]) * sc;
       }
       if (!(gs.alpha[0] == 1.0 && gs.alpha[1] == 0.0)) {
         const double alr = gs.alpha[0], ali = gs.alpha[1];
@@ -790,13 +804,14 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         // lanes 0..31) and y = (W_gg of lanes 32..63, W_gg+2 of lanes 32..63), which are the words
         // 2 gg' and 2 gg' + 1 of both halves.
         unsigned wd[8][4];
+        const double sD = sE * SIGMA;  // (exact)
 #pragma unroll
         for (int gg = 0; gg < 4; gg++) {
           int dr[4][4], di[4][4];  // [column r][digit]
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            digits_fast(vr[4 * gg + r], sE, dr[r]);
-            digits_fast(vi[4 * gg + r], sE, di[r]);
+            digits_fast(vr[4 * gg + r], sD, dr[r]);
+            digits_fast(vi[4 * gg + r], sD, di[r]);
           }
 #pragma unroll
           for (int d = 0; d < 4; d++) {
@@ -959,14 +974,14 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
   for (int part = 0; part < 2; part++) {
     long long s = 0;
     for (int d = 0; d < 4; d++) s = s * 255 + (signed char)b[(part * 4 + d) * P2];
-    v[part] = __builtin_ldexp((double)s, E - 30);
+    v[part] = __builtin_ldexp((double)s, E - 30) / SIGMA;
   }
   out[gid] = make_double2(v[0], v[1]);
 }
 
 }  // namespace lc
 
-// NW = 4 for P = 32 (strips of 128 columns), 8 otherwise (256 columns).
+// NW = 4 for P = 32 (strips of 128 columns), 8 otherwise (256 columns); K256 for P <= 64.
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
                                         long long total_wg, int nv, int P, hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
@@ -983,9 +998,12 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
   const long long per = (total_wg + 7) / 8;
   const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
   if (P == 32)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
+  else if (P == 64)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
   else
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
+                       total_wg);
   return hipGetLastError();
 }
 

@@ -101,22 +101,23 @@ struct LcTraceGroup {
 };
 
 // Exact form (gemm_oz.hip). A digit matrix holds every real value x of a stored line as four
-// balanced base-255 digits (int8; D1..D3 in [-127, 127], D0 in [-65, 65]) under the line's
-// exponent E:
-//   x = 2^(E - 30) * (D0 * 255^3 + D1 * 255^2 + D2 * 255 + D3),   |x| < 2^E,
-// i.e. 30 bits below the largest magnitude of the line. Blocks are P x P (P = Nv rounded up to a
-// multiple of 32, zero padded); block (a,b) of a D-space matrix sits at (a*4+b) * 8 * P^2 bytes,
-// plane q = part * 4 + digit (part 0 real, 1 imaginary) at q * P^2 bytes of the block, and inside
-// a plane element (line, k) at ((k / 32) * P + line) * 32 + k % 32: each 32-line x 32-K chunk is
-// one contiguous kilobyte. Offsets and strides count 8-byte "complex digit elements" (P^2 per
-// block), so a digit view is addressed like a complex-float view of P x P blocks.
+// balanced base-255 digits (int8, each in [-127, 127]) under the line's exponent E:
+//   x = (D0 * 255^3 + D1 * 255^2 + D2 * 255 + D3) / (SIGMA * 2^(30 - E)),   |x| < 2^E,
+// with SIGMA = 63/32, i.e. about 31 bits below the largest magnitude of the line. Blocks are
+// P x P (P = Nv rounded up to a multiple of 32, zero padded); block (a,b) of a D-space matrix sits
+// at (a*4+b) * 8 * P^2 bytes, plane q = part * 4 + digit (part 0 real, 1 imaginary) at q * P^2
+// bytes of the block, and inside a plane element (line, k) at ((k / 32) * P + line) * 32 + k % 32:
+// each 32-line x 32-K chunk is one contiguous kilobyte. Offsets and strides count 8-byte "complex
+// digit elements" (P^2 per block), so a digit view is addressed like a complex-float view of P x P
+// blocks.
 //
 // C = alpha * A * B over a batch with A, B digit views (A read by rows, B by columns, as in the
 // mixed form). Digit products of total order <= 3 accumulate exactly in int32 (one accumulator
 // per order), and the FP64 epilogue combines them, so the only rounding is the digit truncation
-// of the operands and the dropped orders 4..6 (about 2^-31 of the largest products). Outputs: FP64 complex C (blocks of nv x nv), and/or digits Cd with one exponent
-// per D-row written to Ce ([4][P] ints per matrix; needs the workgroup strip to cover whole rows,
-// i.e. 4 P <= 256), and fused closing traces as in the other kernels.
+// of the operands and the dropped orders 4..6 (about 2^-32 of the largest products). Outputs:
+// FP64 complex C (blocks of nv x nv), and/or digits Cd with one exponent per D-row written to Ce
+// ([4][P] ints per matrix; needs the workgroup strip to cover whole rows, i.e. 4 P <= 256), and
+// fused closing traces as in the other kernels.
 struct LcOzGroup {
   LcView A, B;
   void* C;

@@ -1,20 +1,24 @@
 """numpy model of the exact form's number format and product (test infrastructure).
 
 A real value x of a stored line with exponent E (every |x| of the line below 2^E) is four
-balanced base-255 digits: v = rint(x 2^(30-E)), v = D0 255^3 + D1 255^2 + D2 255 + D3 with
-D1..D3 in [-127, 127] and D0 in [-65, 65]. A product sums the digit pairs of total order <= 3 in
-integers, forms S = ((acc_0 255 + acc_1) 255 + acc_2) 255 + acc_3 (exact), multiplies it by
-255^3 in FP64 (one rounding) and scales by 2^(E_A + E_B - 60). Every step is exact or one IEEE
-rounding, so the HIP kernel must reproduce this model bit for bit.
+balanced base-255 digits: v = rint(2^(30-E) x SIGMA), v = D0 255^3 + D1 255^2 + D2 255 + D3
+with every digit in [-127, 127] (SIGMA = 63/32, so |v| <= SIGMA 2^30 < (255^4 - 1) / 2). A product
+sums the digit pairs of total order <= 3 in integers, forms S = ((acc_0 255 + acc_1) 255 + acc_2)
+255 + acc_3 (exact), multiplies it by UNSCALE = rn(255^3 / SIGMA^2) in FP64 (one rounding) and
+scales by 2^(E_A + E_B - 60). Every step is exact or one IEEE rounding, so the HIP kernel must
+reproduce this model bit for bit.
 """
 import numpy as np
 
 NDIG = 4
 MAXO = 3
 BASE = 255
-FRAC = 30  # bits of a line below its exponent
+FRAC = 30  # bits of a line below its exponent, before the factor SIGMA
+SIGMA = 1.96875  # 63/32: the leading digit spans [-127, 127]
+UNSCALE = 16581375.0 * 1024.0 / 3969.0  # rn(255^3 / SIGMA^2)
 _MIN = np.iinfo(np.int64).min
-_BIAS = 255 * 4210752  # about 2^30: keeps every intermediate of the digit split non-negative
+_MB = 8290000
+_BIAS = 255 * _MB  # about 2^31: keeps every intermediate of the digit split non-negative
 
 
 def exps_of(x):
@@ -31,13 +35,16 @@ def line_exps(z, axis):
 
 def digits(x, E):
     """Balanced base-255 digits of x under exponent E (broadcast), most significant first."""
-    v = np.rint(np.ldexp(x, FRAC - np.asarray(E))).astype(np.int64)
+    # the exact product x SIGMA 2^(30-E) (at most 59 significant bits: exact in the x86 extended
+    # format) rounded once to the nearest integer, ties to even
+    xl = np.asarray(x, dtype=np.longdouble)
+    v = np.rint(np.ldexp(xl, FRAC - np.asarray(E)) * np.longdouble(SIGMA)).astype(np.int64)
     d = [None] * NDIG
     for k in range(NDIG - 1, 0, -1):
         u = v + 127 + _BIAS
         q = u // BASE
         d[k] = u - BASE * q - 127
-        v = q - 4210752
+        v = q - _MB
     d[0] = v
     return d
 
@@ -45,7 +52,7 @@ def digits(x, E):
 def value(d, E):
     """The value a digit set stands for."""
     v = ((d[0] * BASE + d[1]) * BASE + d[2]) * BASE + d[3]
-    return np.ldexp(v.astype(np.float64), np.asarray(E) - FRAC)
+    return np.ldexp(v.astype(np.float64), np.asarray(E) - FRAC) / SIGMA
 
 
 def product(A, B, EA, EB, maxo=MAXO):
@@ -68,7 +75,7 @@ def product(A, B, EA, EB, maxo=MAXO):
         s = acc[0]
         for o in range(1, MAXO + 1):
             s = s * BASE + acc[o]
-        return s.astype(np.float64) * float(BASE ** (NDIG - 1))  # s exact (|s| < 2^53), one rounding
+        return s.astype(np.float64) * UNSCALE  # s exact (|s| < 2^53), one rounding
 
     e = EA[:, None] + EB[None, :] - 2 * FRAC
     return np.ldexp(comb(re), e) + 1j * np.ldexp(comb(im), e)
