@@ -484,7 +484,7 @@ def main():
     max_rel = max(e["max_rel"] for e in errs.values())
     max_norm = max(e["max_normwise"] for e in errs.values())
     max_sig = max(e["max_rel_significant"] for e in errs.values())
-    kname = "lc_gemm_oz_kernel<8>"
+    kname = "lc_gemm_oz_kernel<8, true>"
     roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",
                 frac=round(smx["frac"], 4), traffic=traffic_from_profiles(kname, nmom),
                 traffic_source="profiles/traffic_rho_pipi_D256_Lt48_%dmom.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
