@@ -371,6 +371,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=2, help="sink rows for the CPU baseline (0: skip)")
     ap.add_argument("--nmom", type=int, default=19)
+    ap.add_argument("--seed", type=int, default=3, help="seed of the synthetic inputs")
     ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling aid: run only one form (the JSON line needs both)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -406,7 +407,7 @@ def main():
     from contractions_code_scattering_amd.devmem import DeviceArray, synchronize, to_device
 
     nv, nt, nmom = 64, 48, args.nmom
-    tau_h, phi_h = workload(nv, nt, nmom)
+    tau_h, phi_h = workload(nv, nt, nmom, seed=args.seed)
     rows.init(0)
     # the mixed form: exact int8 digits (default) or BF16x3; the BF16x3 form picks its tile shape on
     # its first (untimed) call per shape
@@ -501,7 +502,7 @@ def main():
                 data="synthetic",
                 config=dict(workload="rho(3 pol)+pipi(5 diagrams) Nv=64 D=256 Lt=48 all 48 sources "
                                      "%d momenta" % nmom,
-                            nv=nv, D=4 * nv, lt=nt, sources=nsrc, momenta=nmom,
+                            nv=nv, D=4 * nv, lt=nt, sources=nsrc, momenta=nmom, seed=args.seed,
                             mode="mixed, exact form: 4 balanced base-255 int8 digits per value under per-line "
                                  "exponents, 10 digit pairs on int8 MFMA with exact int32 accumulation, FP64 "
                                  "epilogue, closing traces and sums",

@@ -1,9 +1,9 @@
 """The exact mixed form (int8 digits, exact integer accumulation) of the batched engines against
 the oracle, at the 1e-6 bar of the precision form.
 
-Per source the diagrams carry the digit truncation of each stored intermediate (30 bits below the
-largest magnitude of its line) and the exact products: against the oracle's FP64 rows every
-correlator array is within 1e-6 of its largest value (normwise, measured about 1e-8). The
+Per source the diagrams carry the digit truncation of each stored intermediate (about 31 bits
+below the largest magnitude of its line) and the exact products: against the oracle's FP64 rows
+every correlator array is within 1e-6 of its largest value (normwise, measured about 1e-9). The
 source-averaged C(t) is checked normwise and element-wise where |C(t)| >= 1e-3 max_t |C(t)|;
 the fraction of all C(t) within 1e-6 element-wise is printed.
 """
