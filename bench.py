@@ -477,7 +477,15 @@ def main():
                     flops_per_launch=p["gemm_flops"] / max(p["gemm_launches"], 1),
                     effective_tflops=p["gemm_flops"] / args.steps / (ms * 1e-3) / 1e12)
 
-    peak_mx = PEAK_TFLOPS["int8"]
+    # the roofline of the mixed form's GEMM: the exact form on the int8 matrix cores (10 int8 products
+    # per algorithmic multiply-add), or the BF16x3 form on the BF16 ones (6 piece products)
+    if args.bf16x3:
+        mx_unit, mx_ops = "bf16", 6.0
+        kname, kdesc = "lc_gemm_mx_kernel", "BF16x3 pieces on v_mfma_f32_16x16x32_bf16, FP32 accumulation"
+    else:
+        mx_unit, mx_ops = "int8", EXACT_OPS_PER_FLOP
+        kname, kdesc = "lc_gemm_oz_kernel<8, true>", "int8 digits on v_mfma_i32_32x32x32_i8, exact int32 accumulation"
+    peak_mx = PEAK_TFLOPS[mx_unit]
     s64 = stats(r64, PEAK_TFLOPS["f64"])
     smx = stats(rmx, peak_mx)
     errs = engine.rel_errors(rmx["avg"], r64["avg"])
@@ -485,27 +493,30 @@ def main():
     max_rel = max(e["max_rel"] for e in errs.values())
     max_norm = max(e["max_normwise"] for e in errs.values())
     max_sig = max(e["max_rel_significant"] for e in errs.values())
-    kname = "lc_gemm_oz_kernel<8, true>"
     roof = dict(bound="mfma", achieved=round(smx["achieved"], 3), peak=round(peak_mx, 1), unit="TFLOP/s",
                 frac=round(smx["frac"], 4), traffic=traffic_from_profiles(kname, nmom),
                 traffic_source="profiles/traffic_rho_pipi_D256_Lt48_%dmom.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                "scaled by a 1 GiB copy; not measured in this run)" % nmom,
-                kernel=kname + " (int8 digits on v_mfma_i32_32x32x32_i8, exact int32 accumulation)",
-                peak_basis="published MI355X int8 dense peak, 5.0 POP/s; achieved counts algorithmic flops "
-                           "(8 per complex multiply-add), not the 10 int8 digit products each costs",
+                kernel=kname + " (" + kdesc + ")",
+                peak_basis="published MI355X %s dense peak, %.1f %s; achieved counts algorithmic flops (8 per "
+                           "complex multiply-add), not the %d %s products each costs"
+                           % (mx_unit, peak_mx / 1000, "POP/s" if mx_unit == "int8" else "PFLOP/s", mx_ops, mx_unit),
                 flops_per_launch=smx["flops_per_launch"], avg_launch_ms=round(smx["avg_launch_ms"], 4),
-                int8_issue_utilization=round(smx["achieved"] * EXACT_OPS_PER_FLOP / PEAK_TFLOPS["int8"], 4),
                 rate_over_published_peaks={k: round(smx["achieved"] / v, 4) for k, v in PEAK_TFLOPS.items()})
+    roof[mx_unit + "_issue_utilization"] = round(smx["achieved"] * mx_ops / peak_mx, 4)
     line = dict(metric=METRIC, value=round(smx["value"], 1), unit="correlators/s", n_gpus=1,
                 steps=args.steps, warmup=args.warmup, ms_per_step=round(smx["ms_per_step"], 3),
-                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="i8-digits/f64",
+                higher_is_better=True, scaling="weak", vs_baseline=None,
+                dtype="bf16x3/f32/f64" if args.bf16x3 else "i8-digits/f64",
                 data="synthetic",
                 config=dict(workload="rho(3 pol)+pipi(5 diagrams) Nv=64 D=256 Lt=48 all 48 sources "
                                      "%d momenta" % nmom,
                             nv=nv, D=4 * nv, lt=nt, sources=nsrc, momenta=nmom, seed=args.seed,
-                            mode="mixed, exact form: 4 balanced base-255 int8 digits per value under per-line "
-                                 "exponents, 10 digit pairs on int8 MFMA with exact int32 accumulation, FP64 "
-                                 "epilogue, closing traces and sums",
+                            mode=("mixed, BF16x3 form: FP32 storage split into three BF16 pieces, 6 piece "
+                                  "products on BF16 MFMA with FP32 accumulation, FP64 traces and sums") if args.bf16x3
+                            else ("mixed, exact form: 4 balanced base-255 int8 digits per value under per-line "
+                                  "exponents (scale 63/32 2^(30-E)), 10 digit pairs on int8 MFMA with exact int32 "
+                                  "accumulation, FP64 epilogue, closing traces and sums"),
                             correlators_per_step=ncorr),
                 roofline=roof,
                 mixed_vs_fp64=round(smx["value"] / s64["value"], 3),
