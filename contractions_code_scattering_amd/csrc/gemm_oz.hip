@@ -176,13 +176,15 @@ constexpr int lds_bytes() {
 // blocks) are built when the product group changes, and the first operand loads of the next item
 // are issued before the epilogue of the current one.
 // K256: every K sum of the launch has at most 256 terms (P <= 64), which lets the epilogue combine
-// the two leading accumulators in 24-bit integer arithmetic.
-template <int NW, bool K256>
+// the two leading accumulators in 24-bit integer arithmetic. RUN: the launch's groups are runs of
+// items sharing A (LcOzGroup::arun); without it the run logic is compiled out.
+template <int NW, bool K256, bool RUN>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup* __restrict__ twins, int nv, int P,
                   long long total) {
-  // stage buffers [2][MAXA][16 planes][1 KB] during the K loop (the 8 digit planes of an A chunk
-  // and their negations), per-wave epilogue tiles after it
+  // stage buffers during the K loop, [2][MAXA] chunks of 16 planes x 1 KB (the 8 digit planes of
+  // an A chunk and their negations), or [nstep <= 8][1] for a run of dense products; the per-wave
+  // FP64 epilogue tiles overlap them
   __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
   __shared__ int rowe[NW][32];
   // per-lane partial sums of the fused traces (re, im), reduced by one wave per value
@@ -237,6 +239,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   int gi = find_group(groups, ng, wg);
   int kind = 0, nseg = 1, wpi_log = 0, rr = 0, a = 0, i0 = 0, c = 0, j0 = 0, nA = 1, sA = 0, nstep = 0;
   int twi = -1;                     // the group's twin in `twins`, or -1
+  int arun = 1;                     // items per logical workgroup (a run sharing its staged A)
+  int mi = 0;                       // item of the run in progress
   int ntr_s[2] = {0, 0};            // fused traces per sub-product
   bool conjB_s[2] = {false, false};
   int ea_blk = 0, eb_blk_s[2] = {0, 0};  // exponent offsets of this strip's A rows and wave's B columns
@@ -265,6 +269,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     kind = g.kind;
     nseg = g.nseg;
     twi = g.twin;
+    arun = RUN ? g.arun : 1;
     wpi_log = 31 - __builtin_clz((unsigned)(nstrip * nseg));
     rr = (int)((wg - g.wg_begin) & ((1LL << wpi_log) - 1));
     const int strip = rr / nseg, seg = rr - strip * nseg;
@@ -390,7 +395,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   auto item_pos = [&](long long w) {
     const LcOzGroup& g = groups[gi];
     ItemPos q;
-    q.item = (int)((w - g.wg_begin) >> wpi_log);
+    q.item = (int)((w - g.wg_begin) >> wpi_log) * arun;
     const unsigned it = (unsigned)q.item, n0 = (unsigned)g.n[0], n1 = (unsigned)g.n[1];
     const unsigned q0 = it / n0;
     q.co[0] = (int)(it - q0 * n0);
@@ -427,7 +432,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     for (int i = 0; i < LPW; i++) {
       const int f = wave + i * NW;
       if (f < nload) {
-        unsigned char* d = &lds[((bf * MAXA + (f >> 3)) * 16 + (f & 7)) * 1024 + 16 * ln];
+        unsigned char* d = &lds[((bf * (RUN ? 1 : MAXA) + (f >> 3)) * 16 + (f & 7)) * 1024 + 16 * ln];
         *reinterpret_cast<i32x4*>(d) = ra[i];
         i32x4 n = ra[i];
         neg16(n);
@@ -499,7 +504,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         const int ore = (fl & 3) * 4096, oim = ((fl >> 2) & 3) * 4096;  // plane groups of re(w), im(w)
         const int onim = oim ^ 8192;                                     // and of -im(w)
         const char* nsrc = more ? bsrc(st + 1, offB8, sub) : nullptr;
-        const unsigned char* asrc = &lds[((bf * MAXA + sA) * 16) * 1024 + frag_off];
+        const unsigned char* asrc = &lds[((bf * (RUN ? 1 : MAXA) + sA) * 16) * 1024 + frag_off];
         auto rd = [&](int off) { return *reinterpret_cast<const i32x4*>(asrc + off); };
         // one digit pair: re += Br re(w) - (sB Bi) im(w), im += Br im(w) + (sB Bi) re(w); re is reused
         // two MFMAs later
@@ -549,21 +554,23 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       };
 
       // The loop runs every step but the last with unconditional loads, so that the compiler's count
-      // of loads in flight is exact on every path and each wait covers only what it needs.
-      if (sub == 0) {
+      // of loads in flight is exact on every path and each wait covers only what it needs. Stage
+      // buffer of step st: st in a run (every step kept for the run's later items), else st % 2.
+      auto sbuf = [&](int st) { return RUN ? st : (st & 1); };
+      if (sub == 0 && (!RUN || mi == 0)) {
         storeA(0);
         lds_barrier();
         for (int st = 0; st + 1 < nstep; st++) {
           // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
           // one barrier per step suffices (that buffer was last read in step st - 1)
           loadA(st + 1, cur.offA8);
-          compute(st, st & 1, true);
-          storeA((st + 1) & 1);
+          compute(st, sbuf(st), true);
+          storeA(sbuf(st + 1));
           lds_barrier();
         }
       } else {
-        // the twin: every step of A is still in the stage buffers
-        for (int st = 0; st + 1 < nstep; st++) compute(st, st & 1, true);
+        // the twin, or a later item of a run: every step of A is still in the stage buffers
+        for (int st = 0; st + 1 < nstep; st++) compute(st, sbuf(st), true);
       }
       // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
       // j0 + 8 g + 4 (lane / 32) + r of B
@@ -578,7 +585,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
         }
       }
-      compute(nstep - 1, (nstep - 1) & 1, false);
+      compute(nstep - 1, sbuf(nstep - 1), false);
 
       // ---- epilogue
       // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact in FP64 (|acc_o| < 2^27, |S| < 2^51);
@@ -767,7 +774,11 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       }
 
       const bool last_sub = sub == 1 || twi < 0;  // (uniform)
-      if (last_sub) {
+      if (RUN && last_sub && mi + 1 < arun) {
+        // the next item of the run: its first B fragments (A is on the chip)
+        const LcOzGroup& g0 = groups[gi];
+        loadB0(cur.offB8 + 8 * g0.B.s[0], 0);
+      } else if (last_sub) {
         // the next item of this workgroup: with the same tables (same group and strip), its first
         // loads go out now and land while this item's output is written
         nwg = wg + nj;
@@ -837,6 +848,17 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       lds_barrier();
       run_sub(std::integral_constant<int, 1>{});
     }
+    if (RUN && mi + 1 < arun) {
+      // the next item of the run (A staged once, with the first): it writes the row exponents and
+      // trace sums that slower waves may still be reading
+      lds_barrier();
+      mi++;
+      cur.item++;
+      cur.co[0]++;
+      cur.offB8 += 8 * g.B.s[0];
+      continue;
+    }
+    mi = 0;
 
     if (!more_items) break;
     // every wave is done with its tile and the trace sums before the stage buffers are reused
@@ -982,9 +1004,11 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
 }  // namespace lc
 
 // NW = 4 for P = 32 (strips of 128 columns), 8 otherwise (256 columns); K256 for P <= 64.
+// run: every group of the launch is a run (LcOzGroup::arun > 1; P <= 64).
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
-                                        long long total_wg, int nv, int P, hipStream_t s) {
+                                        long long total_wg, int nv, int P, int run, hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
+  if (run && P > 64) return hipErrorInvalidValue;
   // persistent: as many workgroups per XCD as are resident at once (one per CU), at most one per
   // logical workgroup
   static int ncu = 0;
@@ -997,12 +1021,20 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
   const long long slots = ncu / 8;  // (the stage buffers take 128 KB of LDS: one workgroup per CU)
   const long long per = (total_wg + 7) / 8;
   const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
-  if (P == 32)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
+  if (P == 32 && run)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, true>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P,
+                       total_wg);
+  else if (P == 32)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, false>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P,
+                       total_wg);
+  else if (P == 64 && run)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, true>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
+                       total_wg);
   else if (P == 64)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, false>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
+                       total_wg);
   else
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false, false>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
                        total_wg);
   return hipGetLastError();
 }

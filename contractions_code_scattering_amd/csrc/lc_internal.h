@@ -136,6 +136,10 @@ struct LcOzGroup {
   // the same A operand, kind and items and its own B and fused traces (no C or Cd), formed per item
   // right after this one from the A operand already staged on the chip (K <= 2 * 32 per block)
   int twin;
+  // items per logical workgroup (1, or n[0] when A does not depend on axis 0): a workgroup stages
+  // its strip of A once (every K step in its own stage buffer, K <= 8 * 32) and forms the products
+  // of the n[0] items along axis 0 with it
+  int arun;
   long long wg_begin;
   LcView Y[LC_MAX_FUSED_TRACES];
   double tr_alpha[LC_MAX_FUSED_TRACES][2];

@@ -181,7 +181,7 @@ Program::Level* Program::open_level(bool is_trace, int variant) {
     l.is_trace = is_trace;
     l.closed = false;
     l.variant = variant;
-    l.first = is_trace ? traces_.size() : variant == kOzGemm ? ozs_.size() : gemms_.size();
+    l.first = is_trace ? traces_.size() : (variant == kOzGemm || variant == kOzGemmRun) ? ozs_.size() : gemms_.size();
     l.count = 0;
     l.total_wg = 0;
     levels_.push_back(l);
@@ -466,7 +466,14 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
     check_traces(*twin_traces);
     if (!(*twin_traces)[0].Y.ex) fail(LC_ERR_ARG, "a twin's closing factors must be in the closing-factor form");
   }
-  Level* l = open_level(false, kOzGemm);
+  // runs of items along axis 0 that share A: a dense x dense product (every K step kept on the
+  // chip, 4 P / 32 <= 8 stage buffers of 16 KB), A independent of axis 0, nothing that uses the
+  // FP64 tile (it overlaps the stage buffers)
+  const bool run_ok = kind == 0 && n0 > 1 && P <= 64 && A.s[0] == 0 && A.es[0] == 0 && !C && !twin_B &&
+                      (traces.empty() || traces[0].Y.ex);
+  // a launch holds runs only or single items only: the persistent workgroups split the launch's
+  // logical workgroups evenly, which balances the work only when they are alike
+  Level* l = open_level(false, run_ok ? kOzGemmRun : kOzGemm);
   LcOzGroup g;
   std::memset(&g, 0, sizeof g);
   g.A = A;
@@ -488,10 +495,11 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   g.kind = kind;
   g.nseg = nseg;
   g.twin = -1;
+  g.arun = run_ok ? n0 : 1;
   g.wg_begin = l->total_wg;
   const long long items = (long long)n0 * n1 * n2;
   const long long per_item = (long long)(4 * P / 32) * nseg;
-  l->total_wg += items * per_item;
+  l->total_wg += items / g.arun * per_item;
   l->count++;
   g.ntr = (int)traces.size();
   for (int t = 0; t < g.ntr; t++) {
@@ -649,10 +657,11 @@ void Program::run(hipStream_t s) {
                                            nullptr, cv.cf ? cv.y : nullptr, st);
              },
              "digit conversion");
-    } else if (variant == kOzGemm) {
+    } else if (variant == kOzGemm || variant == kOzGemmRun) {
       const LcOzGroup* og = dozs + l.first;
       const int P = oz_pad(nv);
-      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, st); },
+      const int run = variant == kOzGemmRun ? 1 : 0;
+      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, run, st); },
              "exact gemm launch");
     } else {
       const LcTraceGroup* tg = dt + l.first;

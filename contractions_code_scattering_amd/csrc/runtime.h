@@ -30,7 +30,7 @@ extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, cons
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
-                                        long long total_wg, int nv, int P,
+                                        long long total_wg, int nv, int P, int run,
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
@@ -287,7 +287,7 @@ class Program {
   struct Level {
     bool is_trace;
     bool closed;
-    int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzConvert
+    int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzGemmRun, kOzConvert
     size_t first;
     int count;
     long long total_wg;
@@ -300,7 +300,7 @@ class Program {
     bool oz = false;
     bool twin = false;  // (oz) the group is in the twin array
   };
-  static constexpr int kOzGemm = 10, kOzConvert = 11;
+  static constexpr int kOzGemm = 10, kOzConvert = 11, kOzGemmRun = 12;
   std::vector<LcOzGroup> ozs_, oztw_;
   std::vector<Convert> converts_;
   void oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
