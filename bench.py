@@ -484,7 +484,7 @@ def main():
         kname, kdesc = "lc_gemm_mx_kernel", "BF16x3 pieces on v_mfma_f32_16x16x32_bf16, FP32 accumulation"
     else:
         mx_unit, mx_ops = "int8", EXACT_OPS_PER_FLOP
-        kname, kdesc = "lc_gemm_oz_kernel<8, true>", "int8 digits on v_mfma_i32_32x32x32_i8, exact int32 accumulation"
+        kname, kdesc = "lc_gemm_oz_kernel<8, true, *>", "int8 digits on v_mfma_i32_32x32x32_i8, exact int32 accumulation"
     peak_mx = PEAK_TFLOPS[mx_unit]
     s64 = stats(r64, PEAK_TFLOPS["f64"])
     smx = stats(rmx, peak_mx)
