@@ -259,10 +259,11 @@ def elementals_measurement(L=24, nv=64, nt=48, nmom=19, seed=6):
     return res
 
 
-def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
-    """The batched sigma / sigma-pi / rho-pi set on its own workload, both forms, one timed pass
-    each after one warm-up pass: Nv = 64 (D = 256), Lt = 64, sources every 4th time slice (16),
-    27 momentum sets of 11 elemental types each, rho-pi polarisation idir = 2."""
+def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4, bf16x3=False):
+    """The batched sigma / sigma-pi / rho-pi set on its own workload, one timed pass per form after
+    one warm-up pass: Nv = 64 (D = 256), Lt = 64, sources every 4th time slice (16), 27 momentum
+    sets of 11 elemental types each, rho-pi polarisation idir = 2. Forms: FP64, the selected mixed
+    form ("mixed"), and with the exact form selected also BF16x3 for comparison."""
     import numpy as np
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd import inputs as I
@@ -274,8 +275,13 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
     ncorr = nmom * nsrc * nt * 10
     res = {}
     outs = {}
-    for mixed in (False, True):
-        engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)  # warm-up (mixed: tunes the tile shape)
+    sel = engine.MIXED_BF16X3 if bf16x3 else engine.MIXED_EXACT
+    forms = [("fp64", False, sel), ("mixed", True, sel)]
+    if not bf16x3:
+        forms.append(("bf16x3", True, engine.MIXED_BF16X3))
+    for key, mixed, form in forms:
+        engine.set_mixed_form(form)
+        engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)  # warm-up (BF16x3: tunes the tile shape)
         synchronize()
         tune = engine.autotune_report()
         engine.profile(True)
@@ -285,18 +291,26 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4):
         dt = time.perf_counter() - t
         p = engine.profile_read()
         engine.profile(False)
-        outs[mixed] = raw.to_numpy()
-        key = "mixed" if mixed else "fp64"
+        outs[key] = raw.to_numpy()
         res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
                         gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
         if mixed:
+            res[key]["form"] = "bf16x3" if form == engine.MIXED_BF16X3 else "exact (int8 digits)"
+        if mixed and form == engine.MIXED_BF16X3 and tune.get("how") in ("tuned", "cached"):
             res[key]["tiles_per_wg"] = tune["tiles_per_wg"]
             res[key]["tile_candidates_ms"] = {k: round(v, 2) for k, v in tune["candidate_ms"].items()}
         log("sigma set %s: %.1f ms" % (key, dt * 1e3))
-    d = np.abs(outs[True] - outs[False]).reshape(-1, 10)
-    a = np.abs(outs[False]).reshape(-1, 10)
+    engine.set_mixed_form(sel)
+    a = np.abs(outs["fp64"]).reshape(-1, 10)
+
+    def err(k):
+        d = np.abs(outs[k] - outs["fp64"]).reshape(-1, 10)
+        return float((d / a.max(axis=0)).max())
     res["mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
-    res["max_normwise_err"] = float((d.max(axis=0) / a.max(axis=0)).max())
+    res["max_normwise_err"] = err("mixed")
+    if "bf16x3" in res:
+        res["bf16x3"]["vs_fp64"] = round(res["bf16x3"]["value"] / res["fp64"]["value"], 3)
+        res["bf16x3"]["max_normwise_err"] = err("bf16x3")
     res["unit"] = "correlators/s"
     res["config"] = dict(workload="sigma one_fish (4) + sigma-pi triangle (2) + rho-pi (4) per (tf, t0, p)",
                          nv=nv, lt=nt, sources=nsrc, momentum_sets=nmom, correlators_per_pass=ncorr)
@@ -536,7 +550,7 @@ def main():
                                               avg_launch_ms=round(s64["avg_launch_ms"], 4))),
                 effective_tflops=round(smx["effective_tflops"], 3))
     if args.sigma_set:
-        line["sigma_set"] = sigma_set_measurement()
+        line["sigma_set"] = sigma_set_measurement(bf16x3=args.bf16x3)
     if args.d512 > 0:
         line["d512"] = d512_measurement(args.d512)
     if args.elementals:

@@ -314,6 +314,188 @@ size_t oz_bytes_per_momentum(const OzShape& E, bool sets, size_t* stage) {
   return (size_t)(npair * (sets ? 7 : 6) + 2LL * E.nt + 4LL * E.nsrc) * dm + st + 16 * 256;
 }
 
+// ------------------------------------------------------------------------------------------
+// sigma / sigma-pi / rho-pi set. Elementals el [np][nt][11][nv][nv] by type (engine_sigma.cpp):
+// 0, 1 X_l (sigma), 2, 3 X_ip (triangle), 4..8 P12 (rho-pi), 9, 10 P1, P2; raw slots as there.
+constexpr int SG_NEL = 11;
+enum { SG_SIG = 0, SG_TRI = 2, SG_P12 = 4, SG_P1 = 9, SG_P2 = 10 };
+
+// Per (tf, t0, p), with tau = tau(tf, t0), X_l = X_l(tf) at the sink, X_r = X_r(t0), P1, P2 at the
+// source and D(t) = tau(t, t), the traces of lib/contract_kernels_sigma.cu (one_fish),
+// lib/contract_kernels_sigmapi.cu (triangle) and lib/contract_kernels_rhopi.cu in the row forms of
+// engine_sigma.cpp:
+//   1  B_l = X_l tau (closing-factor form, l = 0, 1); per source T1^T = P1^T D(t0)^T (digits)
+//   2  A_r = tau X_r (r = 0, 1: twins sharing tau) -> sigma(l, r) = Tr[A_r (g5 B_l^dag g5)];
+//      per source R = P2 T1 (digits)
+//   3  Y^T = conj(tau) R^T g5, the transpose of Y = (g5 R) tau^dag (closing-factor form; a run
+//      over the momenta of a (sink, source) pair: A is the perambulator)
+//   4  Z = X_ip(tf) tau -> triangle(ip) = Tr[Z (Y g5)];
+//      Z' = P12_ip(tf) tau -> rhopi(iop) = Tr[Z' (Y G_iop)], G_iop = g5 rho_{iop+1}(idir)
+struct OzSigmaIn {
+  const char *tau_r, *tau_t, *el_r, *el_t;
+  const int *tex_r, *tex_t, *eex_r, *eex_t;
+};
+
+void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0, int pc, char* scratch,
+                        size_t stage_bytes, void* raw) {
+  const int nt = E.nt, ns = E.nsrc, P = E.P;
+  const long long b = E.b, bd = E.bd, md = E.md;
+  const long long npair = (long long)nt * ns;
+  const long long P4i = 4LL * P;
+  const bool direct = 4 * P <= 256;
+  char* cur = scratch;
+  auto darr = [&](long long nmat) {
+    DArr a;
+    a.d = bump(cur, (size_t)nmat * bd * 8);
+    a.e = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 4));
+    return a;
+  };
+  const long long pairs = (long long)pc * npair;
+  const DArr Bl[2] = {darr(pairs), darr(pairs)};
+  const DArr YTd = darr(pairs);
+  const DArr T1t = darr((long long)pc * ns), Rd = darr((long long)pc * ns);
+  Staging stg;
+  stg.base = stg.cur = direct ? nullptr : bump(cur, stage_bytes * pc);
+
+  const long long PR[3] = {1, nt, npair}, SR[3] = {0, 1, ns};
+  auto dview = [&](const DArr& a, const long long* st, int flags) {
+    LcView v = make_view(a.d, st[0] * bd, st[1] * bd, st[2] * bd, LC_DENSE, flags);
+    return with_ex(v, a.e, st[0] * P4i, st[1] * P4i, st[2] * P4i, P);
+  };
+  Program pg(E.nv, Program::OZ);
+  pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
+  auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
+                   bool cf) {
+    if (direct) {
+      DigitOut d;
+      d.d = a.d;
+      d.e = a.e;
+      d.cf = cf;
+      for (int k = 0; k < 3; k++) {
+        d.ds[k] = st[k] * bd;
+        d.es[k] = st[k] * P4i;
+      }
+      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2);
+      return;
+    }
+    const long long nmat = (long long)n0 * n1 * n2;
+    char* x = bump(stg.cur, (size_t)nmat * b * 16);
+    pg.gemm_oz(A, Bv, x, st[0] * b, st[1] * b, st[2] * b, nullptr, n0, n1, n2);
+    stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
+  };
+  // trace-only pair products, with a twin on the same A where the kernel forms it (P <= 64)
+  auto tgemm = [&](const LcView& A, const LcView& Bv, const std::vector<FusedTrace>& tr, const LcView* twin_B = nullptr,
+                   const std::vector<FusedTrace>* twin_tr = nullptr) {
+    if (twin_B && !direct) {
+      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0, tr);
+      pg.gemm_oz(A, *twin_B, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0, *twin_tr);
+      return;
+    }
+    pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0, tr, twin_B, twin_tr);
+  };
+  auto close_level = [&] {
+    pg.barrier();
+    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf);
+    stg.pending.clear();
+    stg.cur = stg.base;
+  };
+
+  // perambulator tau(tf, t0_k): row-line planes (A role) or column-line planes (B role)
+  auto tauv = [&](bool tplanes, int flags) {
+    LcView v = make_view((tplanes ? in.tau_t : in.tau_r) + (size_t)E.src0 * bd * 8, nt * bd, E.step * bd, 0,
+                         LC_DENSE, flags);
+    return with_ex(v, (tplanes ? in.tex_t : in.tex_r) + (long long)E.src0 * P, (long long)nt * P,
+                   (long long)E.step * P, 0, 0);
+  };
+  const LcView tau_B = tauv(true, LC_TV);
+  LcView Dt0_TB = make_view(in.tau_r + (size_t)E.src0 * (nt + 1) * bd * 8, 0, (long long)E.step * (nt + 1) * bd, 0,
+                            LC_DENSE, LC_TV | LC_TS);
+  Dt0_TB = with_ex(Dt0_TB, in.tex_r + (long long)E.src0 * (nt + 1) * P, 0, (long long)E.step * (nt + 1) * P, 0, 0);
+  // elemental type j as an operand (A reads rows, B columns), X or X^T, at the sink (per tf) or
+  // the source (per k); items of momentum p0 + axis-2 index
+  const long long eP = (long long)nt * SG_NEL;  // matrices per momentum
+  auto elv = [&](int j, bool sink, bool brole, bool transposed) {
+    const bool tl = transposed != brole;  // the column-line planes
+    const long long m0 = (long long)p0 * eP + (sink ? 0 : (long long)E.src0 * SG_NEL) + j;
+    const char* base = (tl ? in.el_t : in.el_r) + (size_t)m0 * md * 8;
+    const int* ex = (tl ? in.eex_t : in.eex_r) + m0 * P;
+    const int fl = brole ? LC_TV : 0;
+    if (sink) return with_ex(make_view(base, SG_NEL * md, 0, eP * md, LC_KRON, fl), ex, SG_NEL * P, 0, eP * P, 0);
+    const long long ks = (long long)E.step * SG_NEL;
+    return with_ex(make_view(base, 0, ks * md, eP * md, LC_KRON, fl), ex, 0, ks * P, eP * P, 0);
+  };
+  char* rw = static_cast<char*>(raw) + (size_t)p0 * ns * nt * 10 * 16;
+  const long long O0 = 10, O1 = nt * 10LL, O2 = (long long)ns * nt * 10;
+  auto slot = [&](int k, const LcView& Y) {
+    FusedTrace f = lc::fused(Y, rw + (size_t)k * 16, O0, O1, O2);
+    f.out_double = 1;
+    return f;
+  };
+  const Spin g5 = gamma(5);
+
+  // level 1
+  for (int l = 0; l < 2; l++) dgemm(elv(SG_SIG + l, true, false, false), tau_B, Bl[l], PR, nt, ns, pc, true);
+  dgemm(elv(SG_P1, false, false, true), Dt0_TB, T1t, SR, 1, ns, pc, false);
+  close_level();
+  // level 2
+  {
+    std::vector<FusedTrace> ts[2];
+    for (int r = 0; r < 2; r++)
+      for (int l = 0; l < 2; l++) ts[r].push_back(slot(l * 2 + r, right(left(dview(Bl[l], PR, LC_DAG), g5), g5)));
+    const LcView X1 = elv(SG_SIG + 1, false, true, false);
+    tgemm(tauv(false, 0), elv(SG_SIG + 0, false, true, false), ts[0], &X1, &ts[1]);
+    dgemm(elv(SG_P2, false, false, false), dview(T1t, SR, LC_TV | LC_TS), Rd, SR, 1, ns, pc, false);
+  }
+  close_level();
+  // level 3
+  dgemm(tauv(false, LC_CONJ), right(dview(Rd, SR, LC_TV | LC_TS), g5), YTd, PR, nt, ns, pc, true);
+  close_level();
+  // level 4
+  const LcView Y = dview(YTd, PR, LC_TV | LC_TS);
+  for (int ip = 0; ip < 2; ip++) tgemm(elv(SG_TRI + ip, true, false, false), tau_B, {slot(4 + ip, right(Y, g5))});
+  const int ips[3] = {0, 1, 2 + idir};
+  for (int j = 0; j < 3; j++) {
+    std::vector<FusedTrace> t4;
+    for (int iop = 0; iop < 4; iop++)
+      if (rhopi_ip(idir, iop) == ips[j]) t4.push_back(slot(6 + iop, right(Y, g5 * rhoj(iop + 1, idir, false))));
+    tgemm(elv(SG_P12 + ips[j], true, false, false), tau_B, t4);
+  }
+  pg.run(ctx().stream);
+}
+
+// Scratch of one momentum set: pair buffers B_0, B_1, Y^T, per source T1^T, R; for P = 128 the
+// FP64 staging of the largest level (level 1: two pair products and one per source).
+size_t oz_sigma_bytes_per_momentum_(const OzShape& E, size_t* stage) {
+  const long long npair = (long long)E.nt * E.nsrc;
+  const size_t dm = (size_t)E.bd * 8 + 4 * (size_t)E.P * 4 + 512;
+  const size_t fm = (size_t)E.b * 16;
+  const size_t st = 4 * E.P <= 256 ? 0 : ((size_t)2 * npair + E.nsrc) * fm + 8 * 256;
+  if (stage) *stage = st;
+  return (size_t)(3 * npair + 2LL * E.nsrc) * dm + st + 8 * 256;
+}
+
+OzShape oz_shape(int nt, int nv, int src0, int src_step, int nsrc) {
+  const int P = oz_pad(nv);
+  return OzShape{nt, nv, P, src0, src_step, nsrc, (long long)nv * nv, 16LL * nv * nv, (long long)P * P,
+                 16LL * P * P};
+}
+
+// Inputs of the sigma chain in digits, ahead of the scratch area: tau (row-line and column-line
+// planes, exponents per line index) and the elementals (likewise).
+struct OzSigmaHead {
+  size_t tau_d, tau_e, el_d, el_e;
+  size_t total() const { return 2 * (tau_d + tau_e + el_d + el_e); }
+};
+OzSigmaHead oz_sigma_head(const OzShape& E, int np) {
+  const long long nel = (long long)np * E.nt * SG_NEL;
+  OzSigmaHead h;
+  h.tau_d = ((size_t)E.nt * E.nt * E.bd * 8 + 255) & ~size_t(255);
+  h.tau_e = ((size_t)E.nt * E.nt * E.P * 4 + 255) & ~size_t(255);
+  h.el_d = ((size_t)nel * E.md * 8 + 255) & ~size_t(255);
+  h.el_e = ((size_t)nel * E.P * 4 + 255) & ~size_t(255);
+  return h;
+}
+
 }  // namespace
 
 namespace lc {
@@ -385,6 +567,47 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
                 "corr=%p avg=%p", sets ? "pipi_sets" : "rho_pipi", nt, nv, np, src0, src_step, nsrc, pc, d_tau, d_phi,
                 d_raw, d_corr, d_avg);
   run_engine_call(std::string(gkey), call);
+}
+
+size_t oz_sigma_head_bytes(int nt, int nv, int np) {
+  if (nv > 128) fail(LC_ERR_ARG, "the exact mixed form supports nv <= 128 (got %d)", nv);
+  return oz_sigma_head(oz_shape(nt, nv, 0, 1, 1), np).total();
+}
+
+size_t oz_sigma_bytes_per_momentum(int nt, int nv, int nsrc) {
+  return oz_sigma_bytes_per_momentum_(oz_shape(nt, nv, 0, 1, nsrc), nullptr);
+}
+
+void oz_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, int idir, int src0, int src_step,
+                  int nsrc, char* head, char* scratch, int pc, void* d_raw) {
+  const OzShape E = oz_shape(nt, nv, src0, src_step, nsrc);
+  const OzSigmaHead h = oz_sigma_head(E, np);
+  size_t stage = 0;
+  (void)oz_sigma_bytes_per_momentum_(E, &stage);
+  const int P = E.P;
+  char* tau_r = head;
+  char* tau_t = tau_r + h.tau_d;
+  int* tex_r = reinterpret_cast<int*>(tau_t + h.tau_d);
+  int* tex_t = reinterpret_cast<int*>(tau_t + h.tau_d + h.tau_e);
+  char* el_r = tau_t + h.tau_d + 2 * h.tau_e;
+  char* el_t = el_r + h.el_d;
+  int* eex_r = reinterpret_cast<int*>(el_t + h.el_d);
+  int* eex_t = reinterpret_cast<int*>(el_t + h.el_d + h.el_e);
+  const long long b = E.b, m = E.m, nel = (long long)np * nt * SG_NEL;
+  launch([=](hipStream_t st) {
+           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, nullptr,
+                                       st);
+         },
+         "tau to digits");
+  launch([=](hipStream_t st) {
+           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, 0, eex_r, eex_t, el_r, el_t, nullptr, st);
+         },
+         "elementals to digits");
+  const OzSigmaIn in{tau_r, tau_t, el_r, el_t, tex_r, tex_t, eex_r, eex_t};
+  for (int p0 = 0; p0 < np; p0 += pc) {
+    const int n = (np - p0) < pc ? (np - p0) : pc;
+    run_chunk_oz_sigma(E, idir, in, p0, n, scratch, stage, d_raw);
+  }
 }
 
 }  // namespace lc

@@ -28,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include "engine_oz.h"
 #include "runtime.h"
 #include "spin.h"
 
@@ -291,6 +292,9 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     if (!d_tau || !d_el || !d_raw || !d_loops) fail(LC_ERR_ARG, "null pointer");
     Shape E{nt, nv, np, src0, src_step, nsrc, idir, (long long)nv * nv, 16LL * nv * nv};
     hipStream_t s = ctx().stream;
+    // the exact form (mixed, LC_MIXED_EXACT): tau and the elementals in digits ahead of the scratch
+    const bool exact = mixed && ctx().mixed_form == 0;
+    if (exact) mixed = false;  // (the split-storage form below is the BF16x3 one)
     // mixed form: tau and the elementals split twice + FP64 per-slice scales, ahead of the scratch
     // the fused-sandwich form (Nv = 64) also keeps tau in FP32 for its closing traces
     const bool sw = mixed && nv == 64;
@@ -298,9 +302,11 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     const size_t tau_f32 = sw ? (((size_t)nt * nt * E.blk * 8 + 255) & ~size_t(255)) : 0;
     const size_t el_split = mixed ? (((size_t)np * nt * NT_EL * E.m * 12 + 255) & ~size_t(255)) : 0;
     const size_t scale_bytes = mixed ? (((size_t)nt * nt + (size_t)np * nt * NT_EL) * 8 + 255) & ~size_t(255) : 0;
-    const size_t head = 2 * tau_split + tau_f32 + 2 * el_split + scale_bytes;
+    const size_t head = exact ? oz_sigma_head_bytes(nt, nv, np) : 2 * tau_split + tau_f32 + 2 * el_split + scale_bytes;
     const size_t loop_per_p = 2 * ((((size_t)nt * E.blk * 16) + 255) & ~size_t(255));
-    const size_t per_p = sw ? sw_bytes_per_momentum(E) : bytes_per_momentum(E, mixed);
+    const size_t per_p = exact ? oz_sigma_bytes_per_momentum(nt, nv, nsrc)
+                         : sw  ? sw_bytes_per_momentum(E)
+                               : bytes_per_momentum(E, mixed);
     const size_t need1 = per_p > loop_per_p ? per_p : loop_per_p;
     if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
     const long long avail = max_scratch_bytes - (long long)head;
@@ -335,10 +341,14 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
         tau = Store{tr, tt};
         el = Store{er, et};
       }
-      for (int p0 = 0; p0 < np; p0 += pc) {
-        const int n = (np - p0) < pc ? (np - p0) : pc;
-        if (sw) run_chunk_sw(E, tau, tau32, el, p0, n, scratch, d_raw);
-        else run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
+      if (exact) {
+        oz_sigma_set(d_tau, d_el, nt, nv, np, idir, src0, src_step, nsrc, base, scratch, pc, d_raw);
+      } else {
+        for (int p0 = 0; p0 < np; p0 += pc) {
+          const int n = (np - p0) < pc ? (np - p0) : pc;
+          if (sw) run_chunk_sw(E, tau, tau32, el, p0, n, scratch, d_raw);
+          else run_chunk(E, mixed, tau, el, p0, n, scratch, d_raw);
+        }
       }
       if (mixed) {
         double* sc = scales;
@@ -356,7 +366,7 @@ static int sigma_impl(bool mixed, const void* d_tau, const void* d_el, int nt, i
     char key[160], gkey[320];
     std::snprintf(key, sizeof key, "sigma_set nt=%d nv=%d np=%d idir=%d src0=%d step=%d nsrc=%d chunk=%d", nt, nv, np,
                   idir, src0, src_step, nsrc, pc);
-    std::snprintf(gkey, sizeof gkey, "%s mx=%d tau=%p el=%p raw=%p loops=%p", key, mixed ? 1 : 0, d_tau, d_el, d_raw,
+    std::snprintf(gkey, sizeof gkey, "%s mx=%d tau=%p el=%p raw=%p loops=%p", key, exact ? 2 : mixed ? 1 : 0, d_tau, d_el, d_raw,
                   d_loops);
     const std::string graph_key(gkey);
     run_with_tiles(key, mixed, [&] { run_engine_call(graph_key, call); });

@@ -3,7 +3,7 @@ restatement of run_ContractSigma, run_ContractSigmaPi_triangle, run_ContractRhoP
 run_ContractLoopUnity and run_ContractSigmaPi_fish_hor.
 
 FP64 bar: 1e-12 relative per output array (normwise) and per element (floor 1e-3 of the array max).
-Mixed form: 2e-6 normwise against the FP64 engine.
+Mixed forms against the FP64 engine: the exact form within 1e-6 normwise, BF16x3 within 2e-6.
 """
 import numpy as np
 import pytest
@@ -67,18 +67,24 @@ def test_sigma_set_vs_oracle(lc, nt, nv, np_, idir, src0, step):
         assert relerr(loops[..., d], ref_loops[..., d]) <= TOL, d
 
 
+@pytest.mark.parametrize("form", ["exact", "bf16x3"])
 @pytest.mark.parametrize("nt,nv,np_,src0,step", [(3, 16, 2, 0, 1), (3, 64, 1, 0, 1), (4, 64, 3, 0, 1),
                                                   (5, 64, 2, 1, 2), (3, 33, 1, 0, 1), (2, 128, 1, 0, 1)])
-def test_sigma_set_mixed_vs_fp64(lc, nt, nv, np_, src0, step):
-    """Nv = 64 runs the fused-sandwich chain, other Nv the four-level chain."""
+def test_sigma_set_mixed_vs_fp64(lc, form, nt, nv, np_, src0, step):
+    """Both mixed forms: the exact one within 1e-6, BF16x3 (at Nv = 64 the fused-sandwich chain,
+    other Nv the four-level chain) within 2e-6."""
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
     tau, el = _inputs(22, nt, nv, np_)
     t, e = to_device(tau), to_device(el)
     r64, l64 = engine.sigma_set(t, e, idir=1, src0=src0, src_step=step)
-    rmx, lmx = engine.sigma_set(t, e, idir=1, mixed=True, src0=src0, src_step=step)
+    engine.set_mixed_form(engine.MIXED_EXACT if form == "exact" else engine.MIXED_BF16X3)
+    try:
+        rmx, lmx = engine.sigma_set(t, e, idir=1, mixed=True, src0=src0, src_step=step)
+    finally:
+        engine.set_mixed_form(engine.MIXED_EXACT)
     r64, rmx = r64.to_numpy(), rmx.to_numpy()
     errs = {engine.SIGMA_RAW_NAMES[d]: relerr(rmx[..., d], r64[..., d]) for d in range(10)}
-    print("sigma set, mixed vs fp64 (normwise):", errs)
-    assert max(errs.values()) < 2e-6, errs
+    print("sigma set, mixed (%s) vs fp64 (normwise):" % form, errs)
+    assert max(errs.values()) < (1e-6 if form == "exact" else 2e-6), errs
     assert relerr(lmx.to_numpy(), l64.to_numpy()) < 1e-12  # loops stay FP64

@@ -74,8 +74,11 @@ def test_rho_pipi_replays(eng, nv, mixed):
     assert eng.graph_report() == "eager" and _same(_host(out), ref_a)
 
 
-def test_rerecorded_after_scratch_grows(eng):
+def test_rerecorded_after_scratch_grows(eng, lc):
     from contractions_code_scattering_amd.devmem import to_device
+    # a fresh context, so that the scratch holds only what this test's calls need
+    lc.finalize()
+    lc.init(0)
     nt, nv, np_ = 3, 64, 1
     g = I.rng(52)
     tau = to_device(I.perambulator(g, nt, nv, 0.3))
@@ -104,17 +107,20 @@ def test_sigma_set_replays_and_autotune(eng):
     el = to_device(np.ascontiguousarray(np.stack([np.stack([I.random_elemental(g, nt, nv) for _ in range(11)], axis=1)
                                                   for _ in range(np_)])))
     out = (DeviceArray((np_, nt, nt, 10), np.complex128), DeviceArray((np_, nt, 3), np.complex128))
-    for mixed in (False, True):
-        eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
-        ref = _host(out)
-        eng.graph(True)
-        eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
-        assert eng.graph_report() == "recorded" and _same(_host(out), ref)
-        eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
-        assert eng.graph_report() == "replayed" and _same(_host(out), ref)
-        eng.graph(False)
-    # with the autotuner on, the candidate runs are not recorded; the chosen shape is
+    forms = [(False, eng.MIXED_EXACT), (True, eng.MIXED_EXACT), (True, eng.MIXED_BF16X3)]
     try:
+        for mixed, form in forms:
+            eng.set_mixed_form(form)
+            eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
+            ref = _host(out)
+            eng.graph(True)
+            eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
+            assert eng.graph_report() == "recorded" and _same(_host(out), ref)
+            eng.sigma_set(tau, el, idir=1, mixed=mixed, out=out)
+            assert eng.graph_report() == "replayed" and _same(_host(out), ref)
+            eng.graph(False)
+        # with the autotuner on (BF16x3 tile shape), the candidate runs are not recorded; the chosen
+        # shape is (ref: the BF16x3 result of the built-in shape)
         eng.autotune(True, forget=True)
         eng.graph(True)
         eng.sigma_set(tau, el, idir=1, mixed=True, out=out)
@@ -128,6 +134,7 @@ def test_sigma_set_replays_and_autotune(eng):
         assert relerr_ok(_host(out), ref)
     finally:
         eng.autotune(False, forget=True)
+        eng.set_mixed_form(eng.MIXED_EXACT)
 
 
 def relerr_ok(a, b):

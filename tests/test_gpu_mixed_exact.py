@@ -114,3 +114,64 @@ def test_exact_chunking_is_bit_identical(lc):
     _, c1, _ = engine.rho_pipi(t, f, mixed=True)
     _, c2, _ = engine.rho_pipi(t, f, mixed=True, max_scratch_bytes=14 << 20)
     assert np.array_equal(c1.to_numpy(), c2.to_numpy())
+
+
+# ---- the sigma / sigma-pi / rho-pi set in the exact form
+@pytest.mark.parametrize("nt,nv,np_,idir,src0,step", [(4, 8, 2, 0, 0, 1), (5, 16, 1, 2, 1, 2), (3, 33, 2, 1, 0, 1),
+                                                       (4, 64, 2, 2, 0, 1), (2, 100, 1, 0, 0, 1),
+                                                       (2, 128, 1, 1, 0, 1)])
+def test_exact_sigma_set_vs_oracle(lc, nt, nv, np_, idir, src0, step):
+    """Every raw trace array (sigma one_fish, sigma-pi triangle, rho-pi) within 1e-6 of its
+    largest value against the oracle's rows, and element-wise where |x| >= 1e-3 of it; the loops
+    are FP64 (1e-12)."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    from test_gpu_engine_sigma import _inputs as _sg_inputs, _oracle as _sg_oracle
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    tau, el = _sg_inputs(33, nt, nv, np_)
+    src = list(range(src0, nt, step))
+    raw, loops = engine.sigma_set(to_device(tau), to_device(el), idir=idir, mixed=True, src0=src0, src_step=step)
+    raw, loops = raw.to_numpy(), loops.to_numpy()
+    ref_raw, ref_loops = _sg_oracle(tau, el, idir, src)
+    errs = {}
+    for d in range(10):
+        c, r = raw[..., d], ref_raw[..., d]
+        errs[engine.SIGMA_RAW_NAMES[d]] = relerr(c, r)
+        assert relerr(c, r) < BAR, (engine.SIGMA_RAW_NAMES[d], relerr(c, r))
+        big = np.abs(r) >= 1e-3 * np.abs(r).max()
+        assert (np.abs(c - r)[big] / np.abs(r)[big]).max() < BAR, engine.SIGMA_RAW_NAMES[d]
+    for d in range(3):
+        assert relerr(loops[..., d], ref_loops[..., d]) < 1e-12, d
+    print("nv=%d sigma set, exact vs oracle (normwise): max %.2e" % (nv, max(errs.values())))
+
+
+def test_exact_sigma_set_vs_fp64_benchmark_shape(lc):
+    """Nv = 64 (D = 256), sources every 4th slice, several momentum sets and seeds, against the FP64
+    engine."""
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    from test_gpu_engine_sigma import _inputs as _sg_inputs
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    for seed in (5, 9):
+        tau, el = _sg_inputs(seed, 8, 64, 3)
+        t, e = to_device(tau), to_device(el)
+        r64, _ = engine.sigma_set(t, e, idir=2, src_step=4)
+        rmx, _ = engine.sigma_set(t, e, idir=2, src_step=4, mixed=True)
+        r64, rmx = r64.to_numpy(), rmx.to_numpy()
+        errs = [relerr(rmx[..., d], r64[..., d]) for d in range(10)]
+        print("seed %d: sigma set exact vs fp64, max normwise %.2e" % (seed, max(errs)))
+        assert max(errs) < BAR, errs
+
+
+def test_exact_sigma_chunking_is_bit_identical(lc):
+    from contractions_code_scattering_amd import engine
+    from contractions_code_scattering_amd.devmem import to_device
+    from test_gpu_engine_sigma import _inputs as _sg_inputs
+    from test_gpu_tiles import _one_momentum_budget
+    engine.set_mixed_form(engine.MIXED_EXACT)
+    tau, el = _sg_inputs(34, 3, 16, 3)
+    t, e = to_device(tau), to_device(el)
+    raw0, loops0 = (x.to_numpy() for x in engine.sigma_set(t, e, idir=0, mixed=True, src0=1))
+    budget = _one_momentum_budget(lambda b: engine.sigma_set(t, e, idir=0, mixed=True, src0=1, max_scratch_bytes=b))
+    raw1, loops1 = (x.to_numpy() for x in engine.sigma_set(t, e, idir=0, mixed=True, src0=1, max_scratch_bytes=budget))
+    assert np.array_equal(raw0, raw1) and np.array_equal(loops0, loops1)
