@@ -327,12 +327,15 @@ enum { SG_SIG = 0, SG_TRI = 2, SG_P12 = 4, SG_P1 = 9, SG_P2 = 10 };
 //   1  B_l = X_l tau (closing-factor form, l = 0, 1); per source T1^T = P1^T D(t0)^T (digits)
 //   2  A_r = tau X_r (r = 0, 1: twins sharing tau) -> sigma(l, r) = Tr[A_r (g5 B_l^dag g5)];
 //      per source R = P2 T1 (digits)
-//   3  Y^T = conj(tau) R^T g5, the transpose of Y = (g5 R) tau^dag (closing-factor form; a run
-//      over the momenta of a (sink, source) pair: A is the perambulator)
-//   4  Z = X_ip(tf) tau -> triangle(ip) = Tr[Z (Y g5)];
-//      Z' = P12_ip(tf) tau -> rhopi(iop) = Tr[Z' (Y G_iop)], G_iop = g5 rho_{iop+1}(idir)
+//   3  Y^T = conj(tau) R^T g5, the transpose of Y = (g5 R) tau^dag (digits; a run over the momenta
+//      of a (sink, source) pair: A is the perambulator)
+//   4  V = tau Y (a run likewise) -> triangle(ip) = Tr[V (X_ip(tf) (x) g5)],
+//      rhopi(iop) = Tr[V (P12_ip(tf) (x) G_iop)], G_iop = g5 rho_{iop+1}(idir)
+// Level 4 is the reference's Tr[(g5 X tau)(g5 R tau^dag)] and Tr[(G P12 tau)(g5 R tau^dag)] with
+// X (x) 1 commuting with the spin matrices: one product per item, six traces against the elementals
+// (x) spin in the closing-factor form (transposed planes), instead of one product per elemental.
 struct OzSigmaIn {
-  const char *tau_r, *tau_t, *el_r, *el_t;
+  const char *tau_r, *tau_t, *el_r, *el_t, *el_cft;
   const int *tex_r, *tex_t, *eex_r, *eex_t;
 };
 
@@ -448,18 +451,20 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
   }
   close_level();
   // level 3
-  dgemm(tauv(false, LC_CONJ), right(dview(Rd, SR, LC_TV | LC_TS), g5), YTd, PR, nt, ns, pc, true);
+  dgemm(tauv(false, LC_CONJ), right(dview(Rd, SR, LC_TV | LC_TS), g5), YTd, PR, nt, ns, pc, false);
   close_level();
-  // level 4
-  const LcView Y = dview(YTd, PR, LC_TV | LC_TS);
-  for (int ip = 0; ip < 2; ip++) tgemm(elv(SG_TRI + ip, true, false, false), tau_B, {slot(4 + ip, right(Y, g5))});
-  const int ips[3] = {0, 1, 2 + idir};
-  for (int j = 0; j < 3; j++) {
-    std::vector<FusedTrace> t4;
-    for (int iop = 0; iop < 4; iop++)
-      if (rhopi_ip(idir, iop) == ips[j]) t4.push_back(slot(6 + iop, right(Y, g5 * rhoj(iop + 1, idir, false))));
-    tgemm(elv(SG_P12 + ips[j], true, false, false), tau_B, t4);
-  }
+  // level 4: the elementals X(tf) as closing factors X (x) S, read through the closing-factor form
+  // of X^T (its rows are the columns of X, under the column exponents)
+  auto elcf = [&](int j) {
+    const long long m0 = (long long)p0 * eP + j;
+    const LcView v = make_view(in.el_cft + (size_t)m0 * md * 8, SG_NEL * md, 0, eP * md, LC_KRON, LC_TV);
+    return with_ex(v, in.eex_t + m0 * P, SG_NEL * P, 0, eP * P, 0);
+  };
+  std::vector<FusedTrace> t6;
+  for (int ip = 0; ip < 2; ip++) t6.push_back(slot(4 + ip, kron_right(elcf(SG_TRI + ip), g5)));
+  for (int iop = 0; iop < 4; iop++)
+    t6.push_back(slot(6 + iop, kron_right(elcf(SG_P12 + rhopi_ip(idir, iop)), g5 * rhoj(iop + 1, idir, false))));
+  tgemm(tauv(false, 0), dview(YTd, PR, LC_TV | LC_TS), t6);
   pg.run(ctx().stream);
 }
 
@@ -481,10 +486,11 @@ OzShape oz_shape(int nt, int nv, int src0, int src_step, int nsrc) {
 }
 
 // Inputs of the sigma chain in digits, ahead of the scratch area: tau (row-line and column-line
-// planes, exponents per line index) and the elementals (likewise).
+// planes, exponents per line index) and the elementals (likewise, and the closing-factor form of
+// their transposes).
 struct OzSigmaHead {
   size_t tau_d, tau_e, el_d, el_e;
-  size_t total() const { return 2 * (tau_d + tau_e + el_d + el_e); }
+  size_t total() const { return 2 * (tau_d + tau_e + el_d + el_e) + el_d; }
 };
 OzSigmaHead oz_sigma_head(const OzShape& E, int np) {
   const long long nel = (long long)np * E.nt * SG_NEL;
@@ -536,12 +542,12 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
   auto call = [&] {
     const long long b = E.b, m = E.m;
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, tau_cf,
+             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, tau_cf, nullptr,
                                          st);
            },
            "tau to digits");
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, nullptr, st);
+             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, nullptr, nullptr, st);
            },
            "elementals to digits");
     OzRole R[4];
@@ -593,17 +599,18 @@ void oz_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, i
   char* el_t = el_r + h.el_d;
   int* eex_r = reinterpret_cast<int*>(el_t + h.el_d);
   int* eex_t = reinterpret_cast<int*>(el_t + h.el_d + h.el_e);
+  char* el_cft = el_t + h.el_d + 2 * h.el_e;
   const long long b = E.b, m = E.m, nel = (long long)np * nt * SG_NEL;
   launch([=](hipStream_t st) {
-           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, nullptr,
+           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, nullptr, nullptr,
                                        st);
          },
          "tau to digits");
   launch([=](hipStream_t st) {
-           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, 0, eex_r, eex_t, el_r, el_t, nullptr, st);
+           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, 0, eex_r, eex_t, el_r, el_t, nullptr, el_cft, st);
          },
          "elementals to digits");
-  const OzSigmaIn in{tau_r, tau_t, el_r, el_t, tex_r, tex_t, eex_r, eex_t};
+  const OzSigmaIn in{tau_r, tau_t, el_r, el_t, el_cft, tex_r, tex_t, eex_r, eex_t};
   for (int p0 = 0; p0 < np; p0 += pc) {
     const int n = (np - p0) < pc ? (np - p0) : pc;
     run_chunk_oz_sigma(E, idir, in, p0, n, scratch, stage, d_raw);
@@ -633,9 +640,9 @@ int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4,
     hip_check(hipMemcpyAsync(xb.p, h_b, nbb * nv2 * 16, hipMemcpyHostToDevice, s), "copy b");
     int* exa = static_cast<int*>(ea.p);
     int* exb = static_cast<int*>(eb.p);
-    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, s),
+    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, nullptr, s),
               "convert a");
-    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, s),
+    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, nullptr, s),
               "convert b");
     // A is read by rows: its row-line planes, or the column-line ones for an eigenvector transpose;
     // B is read by columns: the column-line planes, or the row-line ones for a transpose

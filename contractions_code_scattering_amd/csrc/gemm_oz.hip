@@ -152,6 +152,23 @@ __device__ __forceinline__ void block_phase(const LcView& v, int x, int y, doubl
   pi = v.ph_r[x][0] * v.ph_c[y][1] + v.ph_r[x][1] * v.ph_c[y][0];
 }
 
+// x of lane l plus x of lane l ^ 32, then of l ^ 16: every lane holds the sum over its quad
+// {l, l ^ 16, l ^ 32, l ^ 48}. A swap of a register with itself leaves the pair (x_l, x_partner) in
+// the two results (their order depends on the lane half; the sum does not).
+__device__ __forceinline__ double quad_sum(double x) {
+  auto both = [](double v, auto swp) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    const auto l2 = swp(lo);
+    const auto h2 = swp(hi);
+    const double a = __builtin_bit_cast(double, ((unsigned long long)h2[0] << 32) | l2[0]);
+    const double b = __builtin_bit_cast(double, ((unsigned long long)h2[1] << 32) | l2[1]);
+    return a + b;
+  };
+  x = both(x, [](unsigned w) { return __builtin_amdgcn_permlane32_swap(w, w, false, false); });
+  return both(x, [](unsigned w) { return __builtin_amdgcn_permlane16_swap(w, w, false, false); });
+}
+
 // A 64-bit value held in lane l of every wave, as a uniform (scalar) value
 __device__ __forceinline__ long long readlane64(long long v, int l) {
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, l);
@@ -177,8 +194,10 @@ constexpr int lds_bytes() {
 // are issued before the epilogue of the current one.
 // K256: every K sum of the launch has at most 256 terms (P <= 64), which lets the epilogue combine
 // the two leading accumulators in 24-bit integer arithmetic. RUN: the launch's groups are runs of
-// items sharing A (LcOzGroup::arun); without it the run logic is compiled out.
-template <int NW, bool K256, bool RUN>
+// items sharing A (LcOzGroup::arun); without it the run logic is compiled out. XTR: the extended
+// trace path, up to LC_OZ_MAX_TRACES traces per product and elemental closing factors (X (x) S);
+// without it at most LC_MAX_FUSED_TRACES dense ones, with smaller tables and per-lane partials.
+template <int NW, bool K256, bool RUN, bool XTR>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup* __restrict__ twins, int nv, int P,
                   long long total) {
@@ -187,15 +206,19 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   // FP64 epilogue tiles overlap them
   __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
   __shared__ int rowe[NW][32];
-  // per-lane partial sums of the fused traces (re, im), reduced by one wave per value
-  __shared__ d2v tsum[LC_MAX_FUSED_TRACES][NW * 64];
+  // partial sums of the fused traces (re, im), reduced by one wave per value: one per lane, or with
+  // XTR one per lane quad {l, l + 16, l + 32, l + 48} (summed with cross-lane moves first)
+  constexpr int MAXT = XTR ? LC_OZ_MAX_TRACES : LC_MAX_FUSED_TRACES;
+  constexpr int TSL = XTR ? 16 : 64;  // partial sums per wave
+  __shared__ d2v tsum[MAXT][NW * TSL];
   // per-wave tables of the current product group and strip (filled at a group change, read with
   // uniform LDS loads; in registers they would stay live through the K loop): A offsets, and per
   // sub-product (0: the group's product, 1: its twin) the B offsets, plane selectors and traces
   struct WaveTab {
-    long long boff[4], ty[LC_MAX_FUSED_TRACES], te[LC_MAX_FUSED_TRACES];
-    double tfr[LC_MAX_FUSED_TRACES], tfi[LC_MAX_FUSED_TRACES];
-    int flg[4], tfl[LC_MAX_FUSED_TRACES];
+    long long boff[4];
+    double tfr[MAXT], tfi[MAXT];
+    int ty[MAXT], te[MAXT];
+    int flg[4], tfl[MAXT];
   };
   __shared__ WaveTab wtab[2][NW];
   __shared__ long long aofft[NW][4];
@@ -242,6 +265,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   int arun = 1;                     // items per logical workgroup (a run sharing its staged A)
   int mi = 0;                       // item of the run in progress
   int ntr_s[2] = {0, 0};            // fused traces per sub-product
+  bool ykron_s[2] = {false, false};  // and whether any of their closing factors is elemental
   bool conjB_s[2] = {false, false};
   int ea_blk = 0, eb_blk_s[2] = {0, 0};  // exponent offsets of this strip's A rows and wave's B columns
   // Per K block b (kind 0: b < 4; kinds 1 and 2: b = 0) and A chunk (kind 2: slot s < nA, else the
@@ -255,9 +279,9 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   // moves the B fragments to the matrix cores untouched (a conjugated B: one negation per digit).
   // Per fused trace t (written by lane 8 + t): the
   // closing factor's block (c, a) (FP64: byte offset of the block from the view base; closing-factor
-  // form: of the stored block and of its row exponents), its read flags (bit 0 LC_TV, bit 1 LC_CONJ)
-  // and the trace's factor, alpha_t times the block phase (zero for a zero block of an elemental
-  // view). Item offsets are added per item. A twin product shares the group's A, kind and items.
+  // form: of the stored block and of its row exponents), its read flags (bit 0 LC_TV, bit 1 LC_CONJ,
+  // bit 2 a zero block) and the trace's factor, alpha_t times the block phase (zero for a zero
+  // block). Item offsets are added per item. A twin product shares the group's A, kind and items.
   auto rfl = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
   auto rfl64 = [](long long v) {
     const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)v);
@@ -319,6 +343,9 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       const LcOzGroup& gs = sub ? twins[twi] : g;
       const int ntr = gs.ntr;
       ntr_s[sub] = ntr;
+      bool yk = false;
+      for (int t = 0; t < ntr; t++) yk = yk || gs.Y[t].kind == LC_KRON;
+      ykron_s[sub] = yk;
       conjB_s[sub] = (gs.B.flags & LC_CONJ) != 0;
       {
         int x, y;
@@ -362,7 +389,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         const LcView& Yv = gs.Y[t];
         double pr, pi;
         bool zero = false;
-        if (Yv.ex) {
+        if (Yv.ex && Yv.kind == LC_KRON) {
+          // X (x) S: block (c, a) is the phase times the one stored block where S has its non-zero
+          zero = Yv.perm_r[c] != Yv.perm_c[a];
+          pr = Yv.ph_r[c][0] * Yv.ph_c[a][0] - Yv.ph_r[c][1] * Yv.ph_c[a][1];
+          pi = Yv.ph_r[c][0] * Yv.ph_c[a][1] + Yv.ph_r[c][1] * Yv.ph_c[a][0];
+        } else if (Yv.ex) {
           const int blk = stored_block(Yv, c, a);
           ty_l = 8 * blk * P2;
           te_l = 4 * ((long long)(blk >> 2) * Yv.ebs);
@@ -375,11 +407,11 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           zero = Y.zero;
         }
         const double ar = gs.tr_alpha[t][0], ai = gs.tr_alpha[t][1];
-        wt.ty[t] = ty_l;
-        wt.te[t] = te_l;
+        wt.ty[t] = (int)ty_l;
+        wt.te[t] = (int)te_l;
         wt.tfr[t] = zero ? 0.0 : ar * pr - ai * pi;
         wt.tfi[t] = zero ? 0.0 : ar * pi + ai * pr;
-        wt.tfl[t] = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0);
+        wt.tfl[t] = ((Yv.flags & LC_TV) ? 1 : 0) | ((Yv.flags & LC_CONJ) ? 2 : 0) | (zero ? 4 : 0);
       }
     }
   };
@@ -666,9 +698,18 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       }
 
       if (ntr > 0) {
-        // Every closing-factor load of the product's traces is issued before the first use (one
-        // memory round trip).
         const int w = w_, h = h_;
+        // the trace value of this lane into the partial sums: summed over its lane quad first, with
+        // the lane swaps of v_permlane32_swap (lanes l, l ^ 32) and v_permlane16_swap (l, l ^ 16)
+        auto put_tsum = [&](int t, double xr, double xi) {
+          if constexpr (XTR) {
+            xr = quad_sum(xr);
+            xi = quad_sum(xi);
+            if (lane < 16) tsum[t][wave * 16 + lane] = d2v{xr, xi};
+          } else {
+            tsum[t][tid] = d2v{xr, xi};
+          }
+        };
         // FP64 closing factors (one trace at a time: 64 registers of loads each): trace t reads block
         // (c, a) of f(Y_t), element (col, row): with LC_TV the stored element (row, col), read along
         // stored rows; otherwise the stored (col, row). Lane l reads stored row r0 + 2 m + l / 32,
@@ -677,7 +718,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         auto trace_fp64 = [&](int t) {
           Cplx<double> yv[16];
           const Cplx<double>* p = reinterpret_cast<const Cplx<double>*>(
-              static_cast<const char*>(gs.Y[t].base) + 16 * vo(gs.Y[t].s, cur.co) + rfl64(wtab[sub][wave].ty[t]));
+              static_cast<const char*>(gs.Y[t].base) + 16 * vo(gs.Y[t].s, cur.co) + rfl(wtab[sub][wave].ty[t]));
           const int fl = rfl(wtab[sub][wave].tfl[t]);
           const bool tv = (fl & 1) != 0;
           const double cj = (fl & 2) ? -1.0 : 1.0;
@@ -707,67 +748,100 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           }
           const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfr[t])));
           const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfi[t])));
-          tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
+          put_tsum(t, fr * sr - fi * si, fr * si + fi * sr);
         };
         // Closing factors in the closing-factor form (every trace of the product, LC_TV): block (c, a)
         // of f(Y) element (col, row) is the stored (row, col), so the wave's part is the chunk of its
         // C tile, laid out like the accumulators: load k of lane l holds the values of its own tile
         // entries q = 2 k and 2 k + 1 (row l % 32, columns 8 (k / 2) + 4 (l / 32) + 2 (k % 2), + 1),
         // under that row's exponent, so each lane sums FP64 products of its registers with exact
-        // integers and scales once.
-        auto traces_cf = [&](auto NT) {
-          constexpr int N = decltype(NT)::value;
+        // integers and scales once. Traces T0 .. T0 + N - 1, every load of them issued before the
+        // first use (one memory round trip); with SKIP (elemental closing factors) a zero block of
+        // the wave is skipped.
+        auto traces_cf = [&](auto T0c, auto NT, auto SKIPc) {
+          constexpr int T0 = decltype(T0c)::value, N = decltype(NT)::value;
+          constexpr bool SKIP = decltype(SKIPc)::value;
           i32x4 yd[N][8];
-          int ey[N];
+          int ey[N], fl[N];
 #pragma unroll
           for (int t = 0; t < N; t++) {
-            const unsigned char* yb = static_cast<const unsigned char*>(gs.Y[t].base) + 8 * vo(gs.Y[t].s, cur.co) +
-                                      rfl64(wtab[sub][wave].ty[t]) +
+            fl[t] = rfl(wtab[sub][wave].tfl[T0 + t]);
+            if (SKIP && (fl[t] & 4)) continue;
+            const LcView& Yv = gs.Y[T0 + t];
+            const unsigned char* yb = static_cast<const unsigned char*>(Yv.base) + 8 * vo(Yv.s, cur.co) +
+                                      rfl(wtab[sub][wave].ty[T0 + t]) +
                                       ((long long)(j0 >> 5) * (P >> 5) + (i0 >> 5)) * 8192 + 16 * ln;
 #pragma unroll
             for (int k = 0; k < 8; k++) yd[t][k] = gload<i32x4>(yb + 1024 * k);
-            const char* eyb = reinterpret_cast<const char*>(gs.Y[t].ex + vo(gs.Y[t].es, cur.co)) + rfl64(wtab[sub][wave].te[t]);
+            const char* eyb = reinterpret_cast<const char*>(Yv.ex + vo(Yv.es, cur.co)) + rfl(wtab[sub][wave].te[T0 + t]);
             ey[t] = gload<int>(reinterpret_cast<const int*>(eyb) + i0 + w);
           }
 #pragma unroll
           for (int t = 0; t < N; t++) {
-            const double cj = (rfl(wtab[sub][wave].tfl[t]) & 2) ? -1.0 : 1.0;
-            double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+            double sr = 0.0, si = 0.0;
+            if (!SKIP || !(fl[t] & 4)) {
+              const double cj = (fl[t] & 2) ? -1.0 : 1.0;
+              double s1 = 0, s2 = 0, s3 = 0, s4 = 0;
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
+              for (int k = 0; k < 8; k++) {
 #pragma unroll
-              for (int p2 = 0; p2 < 2; p2++) {
-                const double xr = vr[2 * k + p2], xi = vi[2 * k + p2];
-                const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
-                s1 += xr * yr;
-                s2 += xi * yi;
-                s3 += xr * yi;
-                s4 += xi * yr;
+                for (int p2 = 0; p2 < 2; p2++) {
+                  const double xr = vr[2 * k + p2], xi = vi[2 * k + p2];
+                  const double yr = (double)yd[t][k][2 * p2], yi = (double)yd[t][k][2 * p2 + 1];
+                  s1 += xr * yr;
+                  s2 += xi * yi;
+                  s3 += xr * yi;
+                  s4 += xi * yr;
+                }
               }
+              // x (yr + i cj yi), scaled by the row's 2^(E - 30)
+              const double ur = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
+              const double ui = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
+              const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfr[T0 + t])));
+              const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfi[T0 + t])));
+              sr = fr * ur - fi * ui;
+              si = fr * ui + fi * ur;
             }
-            // x (yr + i cj yi), scaled by the row's 2^(E - 30)
-            const double sr = __builtin_ldexp(s1 - cj * s2, ey[t] - 30);
-            const double si = __builtin_ldexp(cj * s3 + s4, ey[t] - 30);
-            const double fr = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfr[t])));
-            const double fi = __builtin_bit_cast(double, rfl64(__builtin_bit_cast(long long, wtab[sub][wave].tfi[t])));
-            tsum[t][tid] = d2v{fr * sr - fi * si, fr * si + fi * sr};
+            put_tsum(T0 + t, sr, si);
           }
         };
-        if (gs.Y[0].ex) {
-          if (ntr == 1) traces_cf(std::integral_constant<int, 1>{});
-          else if (ntr == 2) traces_cf(std::integral_constant<int, 2>{});
-          else traces_cf(std::integral_constant<int, 3>{});
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        using NO = std::false_type;
+        using YES = std::true_type;
+        if (gs.Y[0].ex && (!XTR || !ykron_s[sub])) {
+          if (ntr == 1) traces_cf(I0{}, I1{}, NO{});
+          else if (ntr == 2) traces_cf(I0{}, I2{}, NO{});
+          else traces_cf(I0{}, I3{}, NO{});
+          // more than three traces: the product's own (a twin has at most three)
+          if constexpr (XTR && sub == 0) {
+            if (ntr == 4) traces_cf(I3{}, I1{}, NO{});
+            else if (ntr == 5) traces_cf(I3{}, I2{}, NO{});
+            else if (ntr == 6) traces_cf(I3{}, I3{}, NO{});
+          }
+        } else if (XTR && gs.Y[0].ex) {
+          // elemental closing factors (X (x) S): most blocks of a wave are zero
+          if (ntr == 1) traces_cf(I0{}, I1{}, YES{});
+          else if (ntr == 2) traces_cf(I0{}, I2{}, YES{});
+          else traces_cf(I0{}, I3{}, YES{});
+          if constexpr (sub == 0) {
+            if (ntr == 4) traces_cf(I3{}, I1{}, YES{});
+            else if (ntr == 5) traces_cf(I3{}, I2{}, YES{});
+            else if (ntr == 6) traces_cf(I3{}, I3{}, YES{});
+          }
         } else {
           for (int t = 0; t < ntr; t++) trace_fp64(t);
         }
         lds_barrier();
-        // wave v sums component k = v % 2 of trace t = v / 2 over the workgroup's 64 NW lanes
+        // wave v sums component k = v % 2 of trace t = v / 2 over the workgroup's TSL NW partial sums
         for (int v = wave; v < 2 * ntr; v += NW) {
           const int t = v >> 1, k = v & 1;
           const double* ts = reinterpret_cast<const double*>(&tsum[t][0]) + k;
           double sm = 0;
 #pragma unroll
-          for (int w2 = 0; w2 < NW; w2++) sm += ts[2 * (w2 * 64 + lane)];
+          for (int j = 0; j < NW * TSL / 64; j++) sm += ts[2 * (lane + 64 * j)];
           sm = wave_sum(sm);
           if (lane == 0) gstore(gs.part[t] + 2 * (((long long)cur.item << wpi_log) + rr) + k, sm);
         }
@@ -914,10 +988,13 @@ __global__ void __launch_bounds__(256) lc_oz_exps_kernel(const double2* __restri
 // Digits. One workgroup per 32 x 32 tile of a P x P block (zero outside nv x nv): the row-line
 // planes (line = row, K = column) always, the column-line planes (line = column, K = row) in mode
 // 0. Both are one contiguous kilobyte per plane and tile; the bytes are gathered in LDS first.
+// Optionally the closing-factor form under the row exponents (ycf), and in mode 0 that of the
+// transposed block under the column exponents (ycft: element (j, i) of X^T at its place).
 __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __restrict__ x, long long stride, int nv,
                                                            int P, int nblk, int mode, const int* __restrict__ ex_r,
                                                            const int* __restrict__ ex_t, unsigned char* __restrict__ yr,
-                                                           unsigned char* __restrict__ yt, int2* __restrict__ ycf) {
+                                                           unsigned char* __restrict__ yt, int2* __restrict__ ycf,
+                                                           int2* __restrict__ ycft) {
   __shared__ __attribute__((aligned(16))) unsigned char tr_[8][32][32], tt_[8][32][32];
   const int tpb = P >> 5;
   const long long t = blockIdx.x;
@@ -939,6 +1016,11 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
     if (ycf)
       ycf[(bi * 8 * P2 + cf_off(row, col, P)) / 8] =
           make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - er)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - er)));
+    if (ycft) {
+      const int ec = ex_t[m * P + col];
+      ycft[(bi * 8 * P2 + cf_off(col, row, P)) / 8] =
+          make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - ec)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - ec)));
+    }
     if (!yr) continue;
     digits_of(v.x, er, dr);
     digits_of(v.y, er, di);
@@ -1004,9 +1086,30 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
 }  // namespace lc
 
 // NW = 4 for P = 32 (strips of 128 columns), 8 otherwise (256 columns); K256 for P <= 64.
-// run: every group of the launch is a run (LcOzGroup::arun > 1; P <= 64).
+// run: every group of the launch is a run (LcOzGroup::arun > 1; P <= 64). xtr: a group of the
+// launch has more than LC_MAX_FUSED_TRACES traces or elemental closing factors.
+template <bool XTR>
+static void launch_oz(dim3 grid, const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins, long long total_wg,
+                      int nv, int P, int run, hipStream_t s) {
+  if (P == 32 && run)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, true, XTR>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv,
+                       P, total_wg);
+  else if (P == 32)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, false, XTR>), grid, dim3(256), 0, s, d_groups, ng, d_twins,
+                       nv, P, total_wg);
+  else if (P == 64 && run)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, true, XTR>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv,
+                       P, total_wg);
+  else if (P == 64)
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, false, XTR>), grid, dim3(512), 0, s, d_groups, ng, d_twins,
+                       nv, P, total_wg);
+  else
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false, false, XTR>), grid, dim3(512), 0, s, d_groups, ng, d_twins,
+                       nv, P, total_wg);
+}
+
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
-                                        long long total_wg, int nv, int P, int run, hipStream_t s) {
+                                        long long total_wg, int nv, int P, int run, int xtr, hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
   if (run && P > 64) return hipErrorInvalidValue;
   // persistent: as many workgroups per XCD as are resident at once (one per CU), at most one per
@@ -1021,21 +1124,8 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
   const long long slots = ncu / 8;  // (the stage buffers take 128 KB of LDS: one workgroup per CU)
   const long long per = (total_wg + 7) / 8;
   const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
-  if (P == 32 && run)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, true>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P,
-                       total_wg);
-  else if (P == 32)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, false>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv, P,
-                       total_wg);
-  else if (P == 64 && run)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, true>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
-                       total_wg);
-  else if (P == 64)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, false>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
-                       total_wg);
-  else
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false, false>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv, P,
-                       total_wg);
+  if (xtr) launch_oz<true>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
+  else launch_oz<false>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
   return hipGetLastError();
 }
 
@@ -1043,16 +1133,19 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
 // mode 0: y_rows / y_trans (nmat * nblk * 8 P^2 bytes each), ex_r / ex_t (nmat * P ints each).
 // mode 1: y_rows and ex_r (nmat * 4 P ints) only. y_cf (optional, the size of y_rows): the
 // closing-factor form under the row exponents; y_rows may be null when only that form is wanted.
+// y_cft (optional, mode 0): the closing-factor form of every transposed block under the column
+// exponents.
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           void* y_cf, hipStream_t s) {
+                                           void* y_cf, void* y_cft, hipStream_t s) {
   if (nmat <= 0) return hipSuccess;
+  if (y_cft && mode != 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(lc::lc_oz_exps_kernel, dim3((unsigned)nmat), dim3(256), 0, s, (const double2*)x, stride, nv, P,
                      nblk, mode, ex_r, ex_t);
   const long long tiles = nmat * nblk * (long long)(P / 32) * (P / 32);
   hipLaunchKernelGGL(lc::lc_oz_digits_kernel, dim3((unsigned)tiles), dim3(256), 0, s, (const double2*)x, stride, nv,
                      P, nblk, mode, (const int*)ex_r, (const int*)ex_t, (unsigned char*)y_rows,
-                     (unsigned char*)y_trans, (int2*)y_cf);
+                     (unsigned char*)y_trans, (int2*)y_cf, (int2*)y_cft);
   return hipGetLastError();
 }
 

@@ -38,6 +38,9 @@ struct LcView {
 };
 
 #define LC_MAX_FUSED_TRACES 3
+// fused traces of one product of the exact kernel (gemm_oz.hip), whose closing factors may also be
+// elemental views with a spin matrix (see LcOzGroup)
+#define LC_OZ_MAX_TRACES 6
 
 // C = alpha * A * B over a batch (i0 < n0, i1 < n1, i2 < n2). C is a plain D-space matrix
 // (not stored when C == nullptr). Optionally fused closing traces: for t < ntr,
@@ -117,7 +120,10 @@ struct LcTraceGroup {
 // of the operands and the dropped orders 4..6 (about 2^-32 of the largest products). Outputs:
 // FP64 complex C (blocks of nv x nv), and/or digits Cd with one exponent per D-row written to Ce
 // ([4][P] ints per matrix; needs the workgroup strip to cover whole rows, i.e. 4 P <= 256), and
-// fused closing traces as in the other kernels.
+// fused closing traces as in the other kernels. A closing factor in the closing-factor form may be
+// an elemental view with a spin matrix S (X (x) S, block (a, b) = ph_r[a] ph_c[b] X where
+// perm_r[a] == perm_c[b], else zero); every other reader of an elemental view takes it as X (x) 1
+// and requires the identity perms and phases.
 struct LcOzGroup {
   LcView A, B;
   void* C;
@@ -141,7 +147,7 @@ struct LcOzGroup {
   // of the n[0] items along axis 0 with it
   int arun;
   long long wg_begin;
-  LcView Y[LC_MAX_FUSED_TRACES];
-  double tr_alpha[LC_MAX_FUSED_TRACES][2];
-  double* part[LC_MAX_FUSED_TRACES];
+  LcView Y[LC_OZ_MAX_TRACES];
+  double tr_alpha[LC_OZ_MAX_TRACES][2];
+  double* part[LC_OZ_MAX_TRACES];
 };

@@ -4,6 +4,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "spin.h"
+
 namespace lc {
 
 Context& ctx() {
@@ -258,6 +260,12 @@ void Program::gemm_items(const LcView& A, const LcView& B, void* C, long long cs
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
   if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
   if (!C && !Cs && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
+  {
+    bool plain = kron_plain(A) && kron_plain(B) && (!B2 || kron_plain(*B2)) && (!B3 || kron_plain(*B3)) &&
+                 (!trx || kron_plain(trx->Y));
+    for (const FusedTrace& f : traces) plain = plain && kron_plain(f.Y);
+    if (!plain) fail(LC_ERR_ARG, "an elemental view with a spin matrix outside the exact kernel's closing factors");
+  }
   if (Cs && mode_ != MX) fail(LC_ERR_ARG, "split output outside the mixed-precision mode");
   int variant = 0;
   if (mode_ == MX) {
@@ -418,8 +426,10 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   const int P = oz_pad(nv_);
   if (nv_ > 128) fail(LC_ERR_ARG, "exact form supports nv <= 128 (got %d)", nv_);
   if (A.kind == LC_KRON && B.kind == LC_KRON) fail(LC_ERR_ARG, "elemental x elemental product");
-  if ((int)traces.size() > LC_MAX_FUSED_TRACES) fail(LC_ERR_ARG, "too many fused traces");
+  if ((int)traces.size() > LC_OZ_MAX_TRACES) fail(LC_ERR_ARG, "too many fused traces");
   if (!C && !dout && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
+  if (!kron_plain(A) || !kron_plain(B) || (twin_B && !kron_plain(*twin_B)))
+    fail(LC_ERR_ARG, "an elemental operand with a spin matrix");
   if ((A.flags & LC_TV) || !(B.flags & LC_TV))
     fail(LC_ERR_ARG, "exact-form operands must be read along contiguous K (A by rows, B by columns)");
   if (!A.ex || !B.ex) fail(LC_ERR_ARG, "exact-form operand without exponents");
@@ -434,13 +444,16 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   if (kind == 0 && ((A.ebs && (A.flags & LC_TS)) || (B.ebs && !(B.flags & LC_TS))))
     fail(LC_ERR_ARG, "exact-form dense operand lines cross stored block rows");
   // closing factors: all FP64, or all in the closing-factor form (int32 per value under line
-  // exponents) read in the orientation of the product (LC_TV), dense
+  // exponents) read in the orientation of the product (LC_TV), dense or an elemental with a spin
+  // matrix (one exponent per line index)
   auto check_traces = [](const std::vector<FusedTrace>& trs) {
     for (const FusedTrace& f : trs) {
       if ((f.Y.ex != nullptr) != (trs[0].Y.ex != nullptr))
         fail(LC_ERR_ARG, "exact-form closing factors of one product mix FP64 and the closing-factor form");
-      if (f.Y.ex && (f.Y.kind != LC_DENSE || !(f.Y.flags & LC_TV)))
-        fail(LC_ERR_ARG, "a closing factor in the closing-factor form must be a dense view read along its stored rows (LC_TV)");
+      if (f.Y.ex && !(f.Y.flags & LC_TV))
+        fail(LC_ERR_ARG, "a closing factor in the closing-factor form must be read along its stored rows (LC_TV)");
+      if (f.Y.kind == LC_KRON && (f.Y.ebs || (!f.Y.ex && !kron_plain(f.Y))))
+        fail(LC_ERR_ARG, "an elemental closing factor with per-block exponents or an FP64 one with a spin matrix");
     }
   };
   check_traces(traces);
@@ -452,7 +465,7 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   // FP64 tile that overlaps them (no FP64 output, closing factors in the closing-factor form)
   if (twin_B) {
     if (!twin_traces || twin_traces->empty() || (int)twin_traces->size() > LC_MAX_FUSED_TRACES)
-      fail(LC_ERR_ARG, "a twin product needs 1..%d fused traces", LC_MAX_FUSED_TRACES);
+      fail(LC_ERR_ARG, "a twin product needs 1..%d fused traces", LC_MAX_FUSED_TRACES);  // (the kernel's twin path)
     if (kind == 0 || P > 64 || nseg != 1) fail(LC_ERR_ARG, "twin products need an elemental factor and P <= 64");
     if ((twin_B->kind == LC_KRON) != (B.kind == LC_KRON) || !(twin_B->flags & LC_TV) || !twin_B->ex)
       fail(LC_ERR_ARG, "a twin's B must be read like B (same kind, by columns, with exponents)");
@@ -474,6 +487,12 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   // a launch holds runs only or single items only: the persistent workgroups split the launch's
   // logical workgroups evenly, which balances the work only when they are alike
   Level* l = open_level(false, run_ok ? kOzGemmRun : kOzGemm);
+  // the kernel's extended trace path: more than LC_MAX_FUSED_TRACES traces, or elemental closing factors
+  if ((int)traces.size() > LC_MAX_FUSED_TRACES) l->xtr = true;
+  for (const std::vector<FusedTrace>* v : {&traces, twin_traces})
+    if (v)
+      for (const FusedTrace& f : *v)
+        if (f.Y.kind == LC_KRON) l->xtr = true;
   LcOzGroup g;
   std::memset(&g, 0, sizeof g);
   g.A = A;
@@ -552,6 +571,7 @@ void Program::to_digits(const void* x, long long stride, long long nmat, void* y
 void Program::trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
                     long long os2, int n0, int n1, int n2, double ar, double ai, bool accumulate) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+  if (!kron_plain(X) || !kron_plain(Y)) fail(LC_ERR_ARG, "an elemental view with a spin matrix in a trace");
   Level* l = open_level(true);
   LcTraceGroup g;
   std::memset(&g, 0, sizeof g);
@@ -654,14 +674,14 @@ void Program::run(hipStream_t s) {
       const int P = oz_pad(nv);
       launch([=](hipStream_t st) {
                return lc_launch_oz_convert(cv.x, cv.stride, cv.nmat, nv, P, 16, 1, cv.e, nullptr, cv.cf ? nullptr : cv.y,
-                                           nullptr, cv.cf ? cv.y : nullptr, st);
+                                           nullptr, cv.cf ? cv.y : nullptr, nullptr, st);
              },
              "digit conversion");
     } else if (variant == kOzGemm || variant == kOzGemmRun) {
       const LcOzGroup* og = dozs + l.first;
       const int P = oz_pad(nv);
-      const int run = variant == kOzGemmRun ? 1 : 0;
-      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, run, st); },
+      const int run = variant == kOzGemmRun ? 1 : 0, xtr = l.xtr ? 1 : 0;
+      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, run, xtr, st); },
              "exact gemm launch");
     } else {
       const LcTraceGroup* tg = dt + l.first;

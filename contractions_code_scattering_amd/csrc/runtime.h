@@ -30,11 +30,11 @@ extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, cons
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
-                                        long long total_wg, int nv, int P, int run,
+                                        long long total_wg, int nv, int P, int run, int xtr,
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           void* y_cf, hipStream_t s);
+                                           void* y_cf, void* y_cft, hipStream_t s);
 extern "C" hipError_t lc_launch_oz_decode(const void* y, const int* ex, int nv, int P, int nblk, int mode,
                                           long long nmat, void* out, hipStream_t s);
 
@@ -288,6 +288,7 @@ class Program {
     bool is_trace;
     bool closed;
     int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzGemmRun, kOzConvert
+    bool xtr = false;  // OZ: a group with more than LC_MAX_FUSED_TRACES traces or elemental closing factors
     size_t first;
     int count;
     long long total_wg;

@@ -151,4 +151,28 @@ inline LcView right(LcView v, const Spin& G) {
   return r;
 }
 
+// (X (x) 1) Gamma for an elemental view X: the elemental view X (x) Gamma, block (a, b) = G[a][b] X.
+// Only the exact kernel's closing factors read the spin of an elemental view.
+inline LcView kron_right(LcView v, const Spin& G) {
+  if (v.kind != LC_KRON) throw std::invalid_argument("kron_right of a dense view");
+  LcView r = v;
+  for (int b = 0; b < 4; b++) {
+    const int q = nz_row(G, b);
+    const cd ph = cd(v.ph_c[q][0], v.ph_c[q][1]) * G.m[q][b];
+    r.perm_c[b] = v.perm_c[q];
+    r.ph_c[b][0] = ph.real(); r.ph_c[b][1] = ph.imag();
+  }
+  return r;
+}
+
+// An elemental view read as X (x) 1: identity perms and unit phases.
+inline bool kron_plain(const LcView& v) {
+  if (v.kind != LC_KRON) return true;
+  for (int a = 0; a < 4; a++)
+    if (v.perm_r[a] != a || v.perm_c[a] != a || v.ph_r[a][0] != 1 || v.ph_r[a][1] != 0 || v.ph_c[a][0] != 1 ||
+        v.ph_c[a][1] != 0)
+      return false;
+  return true;
+}
+
 }  // namespace lc
