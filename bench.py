@@ -123,7 +123,7 @@ def _two_forms(label, call, ncorr, collect):
         key = "mixed" if mixed else "fp64"
         res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
                         gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
-        if mixed:
+        if mixed and tune.get("how") in ("tuned", "cached"):
             res[key]["tiles_per_wg"] = tune["tiles_per_wg"]
         log("%s %s: %.1f ms" % (label, key, dt * 1e3))
     res["mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
@@ -163,7 +163,7 @@ def d512_measurement(nmom, nv=128, nt=64, seed=5, mass=0.25):
         key = "mixed" if mixed else "fp64"
         res[key] = dict(value=round(ncorr / dt, 1), ms=round(dt * 1e3, 2),
                         gemm_tflops=round(p["gemm_flops"] / (p["gemm_ms"] * 1e-3) / 1e12, 2) if p["gemm_ms"] else None)
-        if mixed:
+        if mixed and tune.get("how") in ("tuned", "cached"):
             res[key]["tiles_per_wg"] = tune["tiles_per_wg"]
             res[key]["tile_candidates_ms"] = {k: round(v, 2) for k, v in tune["candidate_ms"].items()}
         log("d512 %s: %.1f ms" % (key, dt * 1e3))
@@ -259,11 +259,12 @@ def elementals_measurement(L=24, nv=64, nt=48, nmom=19, seed=6):
     return res
 
 
-def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4, bf16x3=False):
+def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4, bf16x3=False, only=None):
     """The batched sigma / sigma-pi / rho-pi set on its own workload, one timed pass per form after
     one warm-up pass: Nv = 64 (D = 256), Lt = 64, sources every 4th time slice (16), 27 momentum
     sets of 11 elemental types each, rho-pi polarisation idir = 2. Forms: FP64, the selected mixed
-    form ("mixed"), and with the exact form selected also BF16x3 for comparison."""
+    form ("mixed"), and with the exact form selected also BF16x3 for comparison; only = "fp64" or
+    "mixed": that form alone (profiling)."""
     import numpy as np
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd import inputs as I
@@ -279,6 +280,8 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4, bf16x3=Fals
     forms = [("fp64", False, sel), ("mixed", True, sel)]
     if not bf16x3:
         forms.append(("bf16x3", True, engine.MIXED_BF16X3))
+    if only:
+        forms = [f for f in forms if f[0] == only]
     for key, mixed, form in forms:
         engine.set_mixed_form(form)
         engine.sigma_set(tau, el, idir=2, src_step=src_step, mixed=mixed)  # warm-up (BF16x3: tunes the tile shape)
@@ -301,6 +304,8 @@ def sigma_set_measurement(nv=64, nt=64, nmom=27, src_step=4, seed=4, bf16x3=Fals
             res[key]["tile_candidates_ms"] = {k: round(v, 2) for k, v in tune["candidate_ms"].items()}
         log("sigma set %s: %.1f ms" % (key, dt * 1e3))
     engine.set_mixed_form(sel)
+    if only:
+        return res
     a = np.abs(outs["fp64"]).reshape(-1, 10)
 
     def err(k):
@@ -402,7 +407,7 @@ def main():
                          "19 momenta (reported under elementals)")
     ap.add_argument("--sigma-set", type=int, default=1,
                     help="1: also time the batched sigma / sigma-pi / rho-pi set at Nv=64, Lt=64, 16 sources, "
-                         "27 momentum sets (reported under sigma_set)")
+                         "27 momentum sets (reported under sigma_set; with --mode fp64 or mixed that form alone)")
     ap.add_argument("--bf16x3", type=int, default=0,
                     help="1: the mixed leg runs the BF16x3 form (FP32 storage, split BF16 products; it does not meet "
                          "the 1e-6 bar) instead of the exact int8-digit form")
@@ -476,6 +481,8 @@ def main():
             extra["elementals"] = elementals_measurement()
         if args.reader:
             extra["reader"] = reader_measurement()
+        if args.sigma_set:
+            extra["sigma_set"] = sigma_set_measurement(bf16x3=args.bf16x3, only=args.mode)
         if extra:
             log(json.dumps(extra))
         return

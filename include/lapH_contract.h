@@ -105,8 +105,8 @@ int lc_engine_rho_pipi(const void* d_tau, const void* d_phi, int nt, int nv, int
 /* Mixed-precision form of lc_engine_rho_pipi, same arguments and outputs. Two forms
  * (lc_set_mixed_form):
  *  LC_MIXED_EXACT (default): every real value of an operand is held as four balanced int8
- *    digits of base 255 under a power-of-two exponent per matrix line (30 bits below the line's
- *    largest magnitude); products run on the int8 matrix cores with exact int32 accumulation of
+ *    digits of base 255 under a power-of-two exponent per matrix line (about 31 bits below the
+ *    line's largest magnitude); products run on the int8 matrix cores with exact int32 accumulation of
  *    the digit pairs of total order <= 3, and every product, closing trace and source sum is
  *    FP64 (nv <= 128);
  *  LC_MIXED_BF16X3: tau and Phi rescaled per time slice to unit Frobenius norm (scales kept in
@@ -152,9 +152,11 @@ int lc_engine_pipi_sets_mixed(const void* d_tau, const void* d_elem, int nt, int
 int lc_engine_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, int idir, int src0,
                         int src_step, int nsrc, long long max_scratch_bytes, void* d_raw, void* d_loops);
 
-/* Mixed-precision form of lc_engine_sigma_set (same arguments and outputs): per-slice rescaled
- * FP32 values split into BF16 pieces on the BF16 matrix cores, FP64 traces and scale factors;
- * the per-time loops and hor_fish are FP64. */
+/* Mixed-precision form of lc_engine_sigma_set (same arguments and outputs), in the form
+ * lc_set_mixed_form selects: LC_MIXED_EXACT (default; int8 digits with exact integer
+ * accumulation, FP64 products and traces, nv <= 128) or LC_MIXED_BF16X3 (per-slice rescaled FP32
+ * values split into BF16 pieces on the BF16 matrix cores, FP64 traces and scale factors). The
+ * per-time loops and hor_fish are FP64 in both. */
 int lc_engine_sigma_set_mixed(const void* d_tau, const void* d_el, int nt, int nv, int np, int idir,
                               int src0, int src_step, int nsrc, long long max_scratch_bytes, void* d_raw,
                               void* d_loops);
