@@ -96,19 +96,26 @@ def test_mfma_bf16_layout(lc):
     np.testing.assert_allclose(d, ref, rtol=1e-6, atol=1e-5)
 
 
+@pytest.mark.parametrize("form", ["exact", "bf16x3"])
 @pytest.mark.parametrize("nt,nv,np_,src0,step", [(4, 16, 2, 0, 1), (3, 64, 1, 0, 1), (4, 64, 3, 0, 1),
                                                   (5, 64, 2, 1, 2), (4, 33, 1, 0, 1), (2, 128, 1, 0, 1)])
-def test_engine_mixed_vs_fp64(lc, nt, nv, np_, src0, step):
-    """Nv = 64 runs the fused-sandwich chain, other Nv the four-level chain."""
+def test_engine_mixed_vs_fp64(lc, form, nt, nv, np_, src0, step):
+    """Both mixed forms: the exact one within 1e-6 per correlator array, BF16x3 (at Nv = 64 the
+    fused-sandwich chain, other Nv the four-level chain) within 2e-6."""
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
+    bar = 1e-6 if form == "exact" else 2e-6
     tau, phi = _inputs(13, nt, nv, np_)
     t, f = to_device(tau), to_device(phi)
     _, c64, a64 = engine.rho_pipi(t, f, src0=src0, src_step=step)
-    _, cmx, amx = engine.rho_pipi(t, f, mixed=True, src0=src0, src_step=step)
+    engine.set_mixed_form(engine.MIXED_EXACT if form == "exact" else engine.MIXED_BF16X3)
+    try:
+        _, cmx, amx = engine.rho_pipi(t, f, mixed=True, src0=src0, src_step=step)
+    finally:
+        engine.set_mixed_form(engine.MIXED_EXACT)
     c64, cmx = c64.to_numpy(), cmx.to_numpy()
     errs = engine.rel_errors(amx.to_numpy(), a64.to_numpy())
-    print("mixed vs fp64:", errs)
+    print("mixed (%s) vs fp64:" % form, errs)
     for d in range(8):
-        assert relerr(cmx[..., d], c64[..., d]) < 2e-6, (d, relerr(cmx[..., d], c64[..., d]))
-    assert max(e["max_normwise"] for e in errs.values()) < 2e-6
+        assert relerr(cmx[..., d], c64[..., d]) < bar, (d, relerr(cmx[..., d], c64[..., d]))
+    assert max(e["max_normwise"] for e in errs.values()) < bar

@@ -8,7 +8,7 @@ of contractPiPi (lib/contract.cpp:748). Checks:
   brute force for every pair with tf != t0 (the brute force takes one elemental array for both
   ends, which the diagonal would need to hold both P1(t) and P4(t));
 * sets fed {Phi, Phi^dag, Phi^dag, Phi} against the back-to-back engine, all eight slots.
-FP64 bar 1e-12 relative per array; mixed 3e-6 normwise against FP64.
+FP64 bar 1e-12 relative per array; mixed against FP64: the exact form 1e-6 normwise, BF16x3 3e-6.
 """
 import numpy as np
 import pytest
@@ -93,23 +93,30 @@ def test_back_to_back_sets_equal_engine(lc, nt, nv):
     assert relerr(c_set.to_numpy(), c_b2b.to_numpy()) < 1e-13
 
 
+@pytest.mark.parametrize("form", ["exact", "bf16x3"])
 @pytest.mark.parametrize("nt,nv,nset", [(3, 16, 2), (2, 64, 2), (2, 128, 1)])
-def test_sets_mixed(lc, nt, nv, nset):
-    """Mixed form against FP64, with elementals whose (role, time) slices differ in scale by up to
-    10^3, so that every per-role scale factor is exercised."""
+def test_sets_mixed(lc, form, nt, nv, nset):
+    """Both mixed forms against FP64 (the exact one within 1e-6, BF16x3 within 3e-6), with
+    elementals whose (role, time) slices differ in scale by up to 10^3, so that every per-role
+    scale factor (BF16x3) and per-line exponent range (exact) is exercised."""
     from contractions_code_scattering_amd import engine
     from contractions_code_scattering_amd.devmem import to_device
+    bar = 1e-6 if form == "exact" else 3e-6
     g = I.rng(90 + nv)
     tau = I.perambulator(g, nt, nv, 0.3)
     el = _sets(91 + nv, nt, nv, nset, spread=1.5)
     t, e = to_device(tau), to_device(el)
-    print("pipi_sets mixed nt=%d nv=%d nset=%d" % (nt, nv, nset), flush=True)
+    print("pipi_sets mixed (%s) nt=%d nv=%d nset=%d" % (form, nt, nv, nset), flush=True)
     _, c64, a64 = engine.pipi_sets(t, e)
-    _, cmx, amx = engine.pipi_sets(t, e, mixed=True)
+    engine.set_mixed_form(engine.MIXED_EXACT if form == "exact" else engine.MIXED_BF16X3)
+    try:
+        _, cmx, amx = engine.pipi_sets(t, e, mixed=True)
+    finally:
+        engine.set_mixed_form(engine.MIXED_EXACT)
     c64, cmx = c64.to_numpy(), cmx.to_numpy()
     for d in range(8):
-        assert relerr(cmx[..., d], c64[..., d]) < 3e-6, (d, relerr(cmx[..., d], c64[..., d]))
-    assert relerr(amx.to_numpy(), a64.to_numpy()) < 3e-6
+        assert relerr(cmx[..., d], c64[..., d]) < bar, (d, relerr(cmx[..., d], c64[..., d]))
+    assert relerr(amx.to_numpy(), a64.to_numpy()) < bar
 
 
 def test_sets_rejects_bad_shapes(lc):
