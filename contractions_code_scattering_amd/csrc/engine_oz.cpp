@@ -140,8 +140,12 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     return a;
   };
   const long long pairs = (long long)pc * npair;
-  DArr Bd = darr(pairs), L1d = darr(pairs), Ld = darr(pairs), Rrd = darr(pairs), UTd = darr(pairs);
-  DArr B4d = sets ? darr(pairs) : DArr{nullptr, nullptr};
+  // B, L1 (and B4) are stored only when the GEMMs cannot chain them (P = 128): at P <= 64 each is
+  // formed on the chip as the A operand of its level-2 products
+  const DArr none{nullptr, nullptr};
+  DArr Bd = direct ? none : darr(pairs), L1d = direct ? none : darr(pairs);
+  DArr Ld = darr(pairs), Rrd = darr(pairs), UTd = darr(pairs);
+  DArr B4d = sets && !direct ? darr(pairs) : none;
   DArr Qat = darr((long long)pc * nt), Qd = darr((long long)pc * nt);
   DArr T1t = darr((long long)pc * ns), T1pt = darr((long long)pc * ns);
   DArr ZhTd = darr(pairs), RTd = darr((long long)pc * ns), RpTd = darr((long long)pc * ns);
@@ -254,10 +258,12 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     return v;
   };
 
-  // level 1
-  dgemm(role(3, true, false, false), tau_B, Bd, PR, nt, ns, pc);
-  dgemm(role(2, true, false, false), tau_B, L1d, PR, nt, ns, pc);
-  if (sets) dgemm(role(3, true, false, false, true), tau_B, B4d, PR, nt, ns, pc);
+  // level 1 (B, L1, B4: chained into level 2 at P <= 64)
+  if (!direct) {
+    dgemm(role(3, true, false, false), tau_B, Bd, PR, nt, ns, pc);
+    dgemm(role(2, true, false, false), tau_B, L1d, PR, nt, ns, pc);
+    if (sets) dgemm(role(3, true, false, false, true), tau_B, B4d, PR, nt, ns, pc);
+  }
   dgemm(role(3, true, false, true), Dtf_TB, Qat, SK, nt, 1, pc);
   dgemm(role(0, false, false, true), Dt0_TB, T1t, SR, 1, ns, pc);
   dgemm(role(1, false, false, true), Dt0_TB, T1pt, SR, 1, ns, pc);
@@ -270,16 +276,37 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     tl.push_back(slot(8, tau_dag_Y));
     const LcView P1B = role(0, false, true, false), P2B = role(1, false, true, false);
     const std::vector<FusedTrace> ty = {slot(7, tau_dag_Y)}, tx = {slot(6, tau_dag_Y)};
-    dgemm(dview(L1d, PR, 0), P1B, Ld, PR, nt, ns, pc, tl, false, &P2B, &ty);
     std::vector<FusedTrace> tr = {slot(9, tau_dag_Y)};
     if (!sets) {
       const std::vector<FusedTrace> r2 = rho_traces(true, {2});
       tr.push_back(r2[0]);
     }
-    dgemm(dview(Bd, PR, 0), P2B, Rrd, PR, nt, ns, pc, tr, false, &P1B, &tx);
-    if (sets)
-      pg.gemm_oz(dview(B4d, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
-                 rho_traces(false, {0, 1, 2}));
+    if (direct) {
+      // chained: L1 = P3 tau, B = P4 tau (and B4 = P4^dag tau) formed per strip on the chip
+      auto dout = [&](const DArr& a) {
+        DigitOut d;
+        d.d = a.d;
+        d.e = a.e;
+        d.cf = false;
+        for (int k = 0; k < 3; k++) {
+          d.ds[k] = PR[k] * bd;
+          d.es[k] = PR[k] * P4i;
+        }
+        return d;
+      };
+      const DigitOut dL = dout(Ld), dR = dout(Rrd);
+      pg.gemm_oz_chain(role(2, true, false, false), tau_B, P1B, &dL, nt, ns, pc, tl, &P2B, &ty);
+      pg.gemm_oz_chain(role(3, true, false, false), tau_B, P2B, &dR, nt, ns, pc, tr, &P1B, &tx);
+      if (sets)
+        pg.gemm_oz_chain(role(3, true, false, false, true), tau_B, role(0, false, true, false), nullptr, nt, ns, pc,
+                         rho_traces(false, {0, 1, 2}));
+    } else {
+      dgemm(dview(L1d, PR, 0), P1B, Ld, PR, nt, ns, pc, tl, false, &P2B, &ty);
+      dgemm(dview(Bd, PR, 0), P2B, Rrd, PR, nt, ns, pc, tr, false, &P1B, &tx);
+      if (sets)
+        pg.gemm_oz(dview(B4d, PR, 0), role(0, false, true, false), nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0,
+                   rho_traces(false, {0, 1, 2}));
+    }
     dgemm(role(2, true, false, false), dview(Qat, SK, LC_TV | LC_TS), Qd, SK, nt, 1, pc);
     dgemm(dview(T1t, SR, 0), role(1, false, true, true), RTd, SR, 1, ns, pc, {}, true);
     dgemm(dview(T1pt, SR, 0), role(0, false, true, true), RpTd, SR, 1, ns, pc, {}, true);
@@ -302,16 +329,18 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   pg.run(ctx().stream);
 }
 
-// Scratch of one momentum (pc = 1): digit pair buffers B, L1, L, Rr, Zh^T, U^T (and B4 for sets),
-// per sink Qa^T, Q, per source T1^T, T1'^T, R^T, R'^T; for P = 128 the FP64 staging of the largest
-// level (level 1: three pair products, one per sink, two per source).
+// Scratch of one momentum (pc = 1): digit pair buffers L, Rr, Zh^T, U^T (and, for P = 128, B, L1
+// and for sets B4), per sink Qa^T, Q, per source T1^T, T1'^T, R^T, R'^T; for P = 128 the FP64
+// staging of the largest level (level 1: three pair products, one per sink, two per source).
 size_t oz_bytes_per_momentum(const OzShape& E, bool sets, size_t* stage) {
   const long long npair = (long long)E.nt * E.nsrc;
+  const bool direct = 4 * E.P <= 256;
   const size_t dm = (size_t)E.bd * 8 + 4 * (size_t)E.P * 4 + 512;  // one digit matrix + exponents
   const size_t fm = (size_t)E.b * 16;
-  const size_t st = 4 * E.P <= 256 ? 0 : ((size_t)(sets ? 3 : 2) * npair + E.nt + 2 * E.nsrc) * fm + 16 * 256;
+  const size_t st = direct ? 0 : ((size_t)(sets ? 3 : 2) * npair + E.nt + 2 * E.nsrc) * fm + 16 * 256;
   if (stage) *stage = st;
-  return (size_t)(npair * (sets ? 7 : 6) + 2LL * E.nt + 4LL * E.nsrc) * dm + st + 16 * 256;
+  const long long npb = direct ? 4 : (sets ? 7 : 6);
+  return (size_t)(npair * npb + 2LL * E.nt + 4LL * E.nsrc) * dm + st + 16 * 256;
 }
 
 // ------------------------------------------------------------------------------------------

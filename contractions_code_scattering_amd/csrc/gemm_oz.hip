@@ -197,7 +197,8 @@ constexpr int lds_bytes() {
 // items sharing A (LcOzGroup::arun); without it the run logic is compiled out. XTR: the extended
 // trace path, up to LC_OZ_MAX_TRACES traces per product and elemental closing factors (X (x) S);
 // without it at most LC_MAX_FUSED_TRACES dense ones, with smaller tables and per-lane partials.
-template <int NW, bool K256, bool RUN, bool XTR>
+// CHAIN: the launch has chained groups (LcOzGroup::pre; not with RUN or XTR).
+template <int NW, bool K256, bool RUN, bool XTR, bool CHAIN>
 __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup* __restrict__ twins, int nv, int P,
                   long long total) {
@@ -222,6 +223,13 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   };
   __shared__ WaveTab wtab[2][NW];
   __shared__ long long aofft[NW][4];
+  // a chained group's first product (elemental x dense, LcOzGroup::pre): this wave's B block
+  // offset and A plane selectors
+  struct PreTab {
+    long long boff;
+    int flg, pad;
+  };
+  __shared__ PreTab ptab[NW];
 
   const int xcd = blockIdx.x & 7, nj = gridDim.x >> 3;
   const long long per = (total + 7) >> 3;
@@ -262,6 +270,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   int gi = find_group(groups, ng, wg);
   int kind = 0, nseg = 1, wpi_log = 0, rr = 0, a = 0, i0 = 0, c = 0, j0 = 0, nA = 1, sA = 0, nstep = 0;
   int twi = -1;                     // the group's twin in `twins`, or -1
+  int cpi = -1;                     // a chained group's first product in `twins`, or -1
+  bool conjB_p = false;             // (its B conjugated)
+  int ea_blk_p = 0, eb_blk_p = 0;   // (its exponent offsets, as ea_blk and eb_blk_s)
+  int ea_chain = 0;                 // the row exponent of the chained A (the first product's output)
   int arun = 1;                     // items per logical workgroup (a run sharing its staged A)
   int mi = 0;                       // item of the run in progress
   int ntr_s[2] = {0, 0};            // fused traces per sub-product
@@ -293,6 +305,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     kind = g.kind;
     nseg = g.nseg;
     twi = g.twin;
+    cpi = CHAIN ? g.pre : -1;
     arun = RUN ? g.arun : 1;
     wpi_log = 31 - __builtin_clz((unsigned)(nstrip * nseg));
     rr = (int)((wg - g.wg_begin) & ((1LL << wpi_log) - 1));
@@ -338,6 +351,25 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       }
       if (y < 4) aoff_l = 8 * stored_block(g.A, x, y) * P2;
       aofft[wave][idx] = aoff_l;
+    }
+    if (cpi >= 0) {
+      // the first product: elemental A1 (block (a, a)) times dense B1 (block (a, c)), K = P
+      const LcOzGroup& gp = twins[cpi];
+      conjB_p = (gp.B.flags & LC_CONJ) != 0;
+      ea_blk_p = (stored_block(gp.A, a, a) >> 2) * gp.A.ebs + i0;
+      eb_blk_p = (stored_block(gp.B, a, c) >> 2) * gp.B.ebs + j0;
+      if (lane == 0) {
+        double p1r, p1i, p2r, p2i;
+        block_phase(gp.A, a, a, p1r, p1i);
+        block_phase(gp.B, a, c, p2r, p2i);
+        const double phr = p1r * p2r - p1i * p2i, phi = p1r * p2i + p1i * p2r;
+        const int k = phi == 0.0 ? (phr > 0.0 ? 0 : 2) : (phi > 0.0 ? 1 : 3);
+        const int gai = (gp.A.flags & LC_CONJ) ? 3 : 1;
+        const int sre = k == 0 ? 0 : k == 1 ? (gai ^ 2) : k == 2 ? 2 : gai;
+        const int sim = k == 0 ? gai : k == 1 ? 0 : k == 2 ? (gai ^ 2) : 2;
+        ptab[wave].boff = 8 * stored_block(gp.B, a, c) * P2;
+        ptab[wave].flg = sre | (sim << 2);
+      }
     }
     for (int sub = 0; sub < (twi >= 0 ? 2 : 1); sub++) {
       const LcOzGroup& gs = sub ? twins[twi] : g;
@@ -420,6 +452,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
   struct ItemPos {
     int item, co[3];  // (items of a group count below 2^31)
     long long offA8, offB8;
+    long long offA8p, offB8p;  // (a chained group's first product)
   };
   auto vo = [](const long long* st, const int* co) {
     return st[0] * co[0] + st[1] * co[1] + st[2] * co[2];
@@ -435,6 +468,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     q.co[2] = (int)(q0 / n1);
     q.offA8 = 8 * vo(g.A.s, q.co);
     q.offB8 = 8 * vo(g.B.s, q.co);
+    q.offA8p = q.offB8p = 0;
+    if (cpi >= 0) {
+      const LcOzGroup& gp = twins[cpi];
+      q.offA8p = 8 * vo(gp.A.s, q.co);
+      q.offB8p = 8 * vo(gp.B.s, q.co);
+    }
     return q;
   };
 
@@ -456,10 +495,22 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       }
     }
   };
+  // A of a chained group's first product (elemental: one chunk per step, at block (a, a))
+  auto loadA_p = [&](int st, long long offA8) {
+    const LcOzGroup& gp = twins[cpi];
+#pragma unroll
+    for (int i = 0; i < LPW; i++) {
+      const int f = wave + i * NW;
+      if (i == 0 || f < 8) {  // (f < NW <= 8 for i = 0: an unconditional load, as in loadA)
+        const char* src = static_cast<const char*>(gp.A.base) + offA8 + (f & 7) * P2 + ((long long)st * P + i0) * 32;
+        ra[i] = gload<i32x4>(src + dma_off);
+      }
+    }
+  };
   // plane q of chunk s (q < 4: Ar digit q, else Ai digit q - 4) into plane group q / 4, and its
-  // negation into group q / 4 + 2
-  auto storeA = [&](int bf) {
-    const int nload = 8 * nA;
+  // negation into group q / 4 + 2; nc chunks (the product's nA, or 1 for a chained first product)
+  auto storeA = [&](int bf, int nc) {
+    const int nload = 8 * nc;
 #pragma unroll
     for (int i = 0; i < LPW; i++) {
       const int f = wave + i * NW;
@@ -478,9 +529,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
   };
-  // B of sub-product sub: this wave's fragments of step st come straight from memory (plane q at
-  // + q P^2)
+  // B of sub-product sub (2: a chained group's first product): this wave's fragments of step st come
+  // straight from memory (plane q at + q P^2)
   auto bsrc = [&](int st, long long offB8, int sub) {
+    if (sub == 2)
+      return static_cast<const char*>(twins[cpi].B.base) + offB8 + rfl64(ptab[wave].boff) +
+             ((long long)st * P + j0) * 32 + mem_off;
     const LcOzGroup& gs = sub ? twins[twi] : groups[gi];
     const int b = st / ksteps, kk = st - b * ksteps;
     return static_cast<const char*>(gs.B.base) + offB8 + rfl64(wtab[sub][wave].boff[b]) +
@@ -495,10 +549,15 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       br[4 + d] = gload<i32x4>(src + (4 + d) * P2);
     }
   };
-  // the first loads of an item: A of step 0 and B of step 0
+  // the first loads of an item: A of step 0 and B of step 0 (of the first product when chained)
   auto prologue = [&](const ItemPos& q) {
-    loadA(0, q.offA8);
-    loadB0(q.offB8, 0);
+    if (CHAIN && cpi >= 0) {
+      loadA_p(0, q.offA8p);
+      loadB0(q.offB8p, 2);
+    } else {
+      loadA(0, q.offA8);
+      loadB0(q.offB8, 0);
+    }
   };
 
   lane_consts();
@@ -515,13 +574,17 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     int ngi = gi;
     ItemPos nxt;  // (set when pre)
     // sub-product 0 is the group's product; 1, its twin, reuses the A operand staged for 0 (the
-    // twin needs every step of A in the stage buffers: nstep <= 2). One code path per sub-product.
+    // twin needs every step of A in the stage buffers: nstep <= 2); 2, a chained group's first
+    // product, runs before 0 and leaves its output in the stage buffers as 0's A (digits and row
+    // exponents as a digit output would have them). One code path per sub-product.
     auto run_sub = [&](auto SUB) {
       constexpr int sub = decltype(SUB)::value;
-      const LcOzGroup& gs = sub ? twins[twi] : g;
-      const int ntr = ntr_s[sub];
-      const bool conjB = conjB_s[sub];
-      const long long offB8 = sub ? 8 * vo(gs.B.s, cur.co) : cur.offB8;
+      constexpr bool FIRST = sub == 2;
+      const LcOzGroup& gs = FIRST ? twins[cpi] : sub ? twins[twi] : g;
+      const int ntr = FIRST ? 0 : ntr_s[sub & 1];
+      const bool conjB = FIRST ? conjB_p : conjB_s[sub & 1];
+      const long long offB8 = FIRST ? cur.offB8p : sub ? 8 * vo(gs.B.s, cur.co) : cur.offB8;
+      const int nst = FIRST ? ksteps : nstep;  // (the first product: elemental x dense, K = P)
       i32x16 re[NORD], im[NORD];
 #pragma unroll
       for (int o = 0; o < NORD; o++) {
@@ -532,11 +595,11 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       // next step right after their last use here, so every reload has a whole step to land and one
       // register set suffices (the compiler waits for each plane before its first use).
       auto compute = [&](int st, int bf, bool more) {
-        const int fl = rfl(wtab[sub][wave].flg[st / ksteps]);
+        const int fl = FIRST ? rfl(ptab[wave].flg) : rfl(wtab[sub & 1][wave].flg[st / ksteps]);
         const int ore = (fl & 3) * 4096, oim = ((fl >> 2) & 3) * 4096;  // plane groups of re(w), im(w)
         const int onim = oim ^ 8192;                                     // and of -im(w)
         const char* nsrc = more ? bsrc(st + 1, offB8, sub) : nullptr;
-        const unsigned char* asrc = &lds[((bf * (RUN ? 1 : MAXA) + sA) * 16) * 1024 + frag_off];
+        const unsigned char* asrc = &lds[((bf * (RUN ? 1 : MAXA) + (FIRST ? 0 : sA)) * 16) * 1024 + frag_off];
         auto rd = [&](int off) { return *reinterpret_cast<const i32x4*>(asrc + off); };
         // one digit pair: re += Br re(w) - (sB Bi) im(w), im += Br im(w) + (sB Bi) re(w); re is reused
         // two MFMAs later
@@ -589,27 +652,39 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       // of loads in flight is exact on every path and each wait covers only what it needs. Stage
       // buffer of step st: st in a run (every step kept for the run's later items), else st % 2.
       auto sbuf = [&](int st) { return RUN ? st : (st & 1); };
-      if (sub == 0 && (!RUN || mi == 0)) {
-        storeA(0);
+      if (FIRST) {
+        storeA(0, 1);
+        lds_barrier();
+        for (int st = 0; st + 1 < nst; st++) {
+          loadA_p(st + 1, cur.offA8p);
+          compute(st, st & 1, true);
+          storeA((st + 1) & 1, 1);
+          lds_barrier();
+        }
+      } else if (sub == 0 && (!RUN || mi == 0) && cpi < 0) {
+        storeA(0, nA);
         lds_barrier();
         for (int st = 0; st + 1 < nstep; st++) {
           // A of step st + 1 is loaded during step st and written to the other stage buffer after it;
           // one barrier per step suffices (that buffer was last read in step st - 1)
           loadA(st + 1, cur.offA8);
           compute(st, sbuf(st), true);
-          storeA(sbuf(st + 1));
+          storeA(sbuf(st + 1), nA);
           lds_barrier();
         }
       } else {
-        // the twin, or a later item of a run: every step of A is still in the stage buffers
+        // the twin, a later item of a run, or a chained product: every step of A is in the stage
+        // buffers
         for (int st = 0; st + 1 < nstep; st++) compute(st, sbuf(st), true);
       }
       // exponents of the epilogue, loaded before the last step: row i0 + lane % 32 of A; columns
       // j0 + 8 g + 4 (lane / 32) + r of B
-      const int ea = gload<int>(g.A.ex + vo(g.A.es, cur.co) + ea_blk + w_);
+      const int ea = FIRST ? gload<int>(gs.A.ex + vo(gs.A.es, cur.co) + ea_blk_p + w_)
+                     : cpi >= 0 ? ea_chain
+                                : gload<int>(g.A.ex + vo(g.A.es, cur.co) + ea_blk + w_);
       int eb[16];
       {
-        const int* pe = gs.B.ex + vo(gs.B.es, cur.co) + eb_blk_s[sub] + 4 * h_;
+        const int* pe = gs.B.ex + vo(gs.B.es, cur.co) + (FIRST ? eb_blk_p : eb_blk_s[sub & 1]) + 4 * h_;
 #pragma unroll
         for (int gg = 0; gg < 4; gg++) {
           const i32x4 v = gload<i32x4>(pe + 8 * gg);
@@ -617,7 +692,9 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           for (int r = 0; r < 4; r++) eb[4 * gg + r] = v[r];
         }
       }
-      compute(nstep - 1, sbuf(nstep - 1), false);
+      compute(nst - 1, FIRST ? ((nst - 1) & 1) : sbuf(nstep - 1), false);
+      // a chained group: the product's first B fragments, while the first product's epilogue runs
+      if constexpr (FIRST) loadB0(cur.offB8, 0);
 
       // ---- epilogue
       // S = ((acc0 255 + acc1) 255 + acc2) 255 + acc3, exact in FP64 (|acc_o| < 2^27, |S| < 2^51);
@@ -643,9 +720,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           vi[q] = alr * xi + ali * xr;
         }
       }
-      // digit output: exponent of the D-row = the largest |value| over every wave of the strip (the
-      // whole row), exchanged through LDS
-      if (gs.Cd) {
+      // digit output (and a chained group's first product): exponent of the D-row = the largest
+      // |value| over every wave of the strip (the whole row), exchanged through LDS
+      const bool dig = FIRST || gs.Cd;
+      if (dig) {
         double mx = 0.0;
 #pragma unroll
         for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
@@ -656,12 +734,12 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       }
       // The FP64 tile in LDS serves only FP64 outputs and FP64 closing factors; traces in the
       // closing-factor form, closing-factor outputs and digit outputs work on the lane's registers.
-      const bool use_tile = gs.C || (ntr > 0 && !gs.Y[0].ex);
+      const bool use_tile = !FIRST && (gs.C || (ntr > 0 && !gs.Y[0].ex));
       // every wave is past the K loop (the stage buffers are free for the epilogue tiles) and the row
       // exponents are visible
-      if (gs.Cd || use_tile) lds_barrier();
+      if (dig || use_tile) lds_barrier();
       int E = 0;
-      if (gs.Cd) {
+      if (dig) {
         E = INT_MIN;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
@@ -669,7 +747,8 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           E = x > E ? x : E;
         }
         if (E == INT_MIN) E = 0;
-        if (wave == 0 && lane < 32) gstore(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, E);
+        if (FIRST) ea_chain = E;
+        else if (wave == 0 && lane < 32) gstore(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, E);
       }
       const double sE = __builtin_ldexp(1.0, 30 - E);
       // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
@@ -683,7 +762,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           *reinterpret_cast<d2v*>(tile + w_ * TSTR + col * 16) = d2v{vr[q], vi[q]};
         }
       }
-      if (gs.C) {
+      if (!FIRST && gs.C) {
         // whole rows: lanes 0..31 one row, 32..63 the next
         Cplx<double>* C = reinterpret_cast<Cplx<double>*>(gs.C) + vo(gs.cs, cur.co) + (long long)(a * 4 + c) * nv2;
 #pragma unroll 4
@@ -697,7 +776,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         }
       }
 
-      if (ntr > 0) {
+      if constexpr (!FIRST) if (ntr > 0) {
         const int w = w_, h = h_;
         // the trace value of this lane into the partial sums: summed over its lane quad first, with
         // the lane swaps of v_permlane32_swap (lanes l, l ^ 32) and v_permlane16_swap (l, l ^ 16)
@@ -847,8 +926,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         }
       }
 
-      const bool last_sub = sub == 1 || twi < 0;  // (uniform)
-      if (RUN && last_sub && mi + 1 < arun) {
+      const bool last_sub = !FIRST && (sub == 1 || twi < 0);  // (uniform)
+      if (FIRST) {
+        // (the product's first B fragments went out after the last K step)
+      } else if (RUN && last_sub && mi + 1 < arun) {
         // the next item of the run: its first B fragments (A is on the chip)
         const LcOzGroup& g0 = groups[gi];
         loadB0(cur.offB8 + 8 * g0.B.s[0], 0);
@@ -869,7 +950,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         loadB0(8 * vo(twins[twi].B.s, cur.co), 1);
       }
 
-      if (gs.Cd && gs.cf) {
+      if (!FIRST && gs.Cd && gs.cf) {
         // closing-factor form: load k of the chunk holds the lane's own entries q = 2 k, 2 k + 1 (see
         // traces_cf); one coalesced kilobyte per k
         unsigned char* base = static_cast<unsigned char*>(gs.Cd) + 8 * (vo(gs.cds, cur.co) + (long long)(a * 4 + c) * P2) +
@@ -880,7 +961,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
                            scaled_int(vi[2 * k + 1], sE)};
           gstore(base + 1024 * k, v);
         }
-      } else if (gs.Cd) {
+      } else if (dig) {
         // digits of the lane's 16 values: word W[d][gg] packs digit plane d (re digits 0..3, then im)
         // of columns 8 gg + 4 (l / 32) .. + 3 of row l % 32. A stored plane row of a lane is 16
         // adjacent columns (16 (l / 32) ..): lane l keeps W[.][0..1] (l < 32) or W[.][2..3] and takes
@@ -904,18 +985,40 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
             wd[4 + d][gg] = pack4p(di[0][d], di[1][d], di[2][d], di[3][d]);
           }
         }
-        unsigned char* base = static_cast<unsigned char*>(gs.Cd) + 8 * (vo(gs.cds, cur.co) + (long long)(a * 4 + c) * P2) +
-                              ((long long)(j0 >> 5) * P + i0) * 32;
+        if constexpr (FIRST) {
+          // as the chained product's A: chunk (K half j0 / 32, block column c) of the stage buffers,
+          // whose lines are this strip's rows; the lane's 16 bytes of plane q (row l % 32, columns
+          // 16 (l / 32) ..) go where storeA would put them (the fragment offset), with their negation
+          unsigned char* base = &lds[(((j0 >> 5) * MAXA + c) * 16) * 1024 + frag_off];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const auto s0 = __builtin_amdgcn_permlane32_swap(wd[q][0], wd[q][2], false, false);
-          const auto s1 = __builtin_amdgcn_permlane32_swap(wd[q][1], wd[q][3], false, false);
-          const i32x4 v = {(int)s0[0], (int)s0[1], (int)s1[0], (int)s1[1]};
-          gstore(base + q * P2 + mem_off, v);
+          for (int q = 0; q < 8; q++) {
+            const auto s0 = __builtin_amdgcn_permlane32_swap(wd[q][0], wd[q][2], false, false);
+            const auto s1 = __builtin_amdgcn_permlane32_swap(wd[q][1], wd[q][3], false, false);
+            i32x4 v = {(int)s0[0], (int)s0[1], (int)s1[0], (int)s1[1]};
+            *reinterpret_cast<i32x4*>(base + q * 1024) = v;
+            neg16(v);
+            *reinterpret_cast<i32x4*>(base + (8 + q) * 1024) = v;
+          }
+          // every chunk written before the chained product reads them
+          lds_barrier();
+        } else {
+          unsigned char* base = static_cast<unsigned char*>(gs.Cd) + 8 * (vo(gs.cds, cur.co) + (long long)(a * 4 + c) * P2) +
+                                ((long long)(j0 >> 5) * P + i0) * 32;
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const auto s0 = __builtin_amdgcn_permlane32_swap(wd[q][0], wd[q][2], false, false);
+            const auto s1 = __builtin_amdgcn_permlane32_swap(wd[q][1], wd[q][3], false, false);
+            const i32x4 v = {(int)s0[0], (int)s0[1], (int)s1[0], (int)s1[1]};
+            gstore(base + q * P2 + mem_off, v);
+          }
         }
       }
 
     };
+    if constexpr (CHAIN) {
+      // a chained group: its first product, whose output stays in the stage buffers
+      if (cpi >= 0) run_sub(std::integral_constant<int, 2>{});
+    }
     run_sub(std::integral_constant<int, 0>{});
     if (twi >= 0) {
       // the twin writes the trace sums that slower waves may still be reading
@@ -1087,31 +1190,34 @@ __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* 
 
 // NW = 4 for P = 32 (strips of 128 columns), 8 otherwise (256 columns); K256 for P <= 64.
 // run: every group of the launch is a run (LcOzGroup::arun > 1; P <= 64). xtr: a group of the
-// launch has more than LC_MAX_FUSED_TRACES traces or elemental closing factors.
-template <bool XTR>
+// launch has more than LC_MAX_FUSED_TRACES traces or elemental closing factors. chain: a group of
+// the launch is chained (LcOzGroup::pre; P <= 64, neither run nor xtr).
+template <bool XTR, bool CHAIN>
 static void launch_oz(dim3 grid, const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins, long long total_wg,
                       int nv, int P, int run, hipStream_t s) {
   if (P == 32 && run)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, true, XTR>), grid, dim3(256), 0, s, d_groups, ng, d_twins, nv,
-                       P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, true, XTR, false>), grid, dim3(256), 0, s, d_groups, ng,
+                       d_twins, nv, P, total_wg);
   else if (P == 32)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, false, XTR>), grid, dim3(256), 0, s, d_groups, ng, d_twins,
-                       nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<4, true, false, XTR, CHAIN>), grid, dim3(256), 0, s, d_groups, ng,
+                       d_twins, nv, P, total_wg);
   else if (P == 64 && run)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, true, XTR>), grid, dim3(512), 0, s, d_groups, ng, d_twins, nv,
-                       P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, true, XTR, false>), grid, dim3(512), 0, s, d_groups, ng,
+                       d_twins, nv, P, total_wg);
   else if (P == 64)
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, false, XTR>), grid, dim3(512), 0, s, d_groups, ng, d_twins,
-                       nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, true, false, XTR, CHAIN>), grid, dim3(512), 0, s, d_groups, ng,
+                       d_twins, nv, P, total_wg);
   else
-    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false, false, XTR>), grid, dim3(512), 0, s, d_groups, ng, d_twins,
-                       nv, P, total_wg);
+    hipLaunchKernelGGL((lc::lc_gemm_oz_kernel<8, false, false, XTR, false>), grid, dim3(512), 0, s, d_groups, ng,
+                       d_twins, nv, P, total_wg);
 }
 
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
-                                        long long total_wg, int nv, int P, int run, int xtr, hipStream_t s) {
+                                        long long total_wg, int nv, int P, int run, int xtr, int chain,
+                                        hipStream_t s) {
   if (ng <= 0 || total_wg <= 0) return hipSuccess;
   if (run && P > 64) return hipErrorInvalidValue;
+  if (chain && (run || xtr || P > 64)) return hipErrorInvalidValue;
   // persistent: as many workgroups per XCD as are resident at once (one per CU), at most one per
   // logical workgroup
   static int ncu = 0;
@@ -1124,8 +1230,9 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
   const long long slots = ncu / 8;  // (the stage buffers take 128 KB of LDS: one workgroup per CU)
   const long long per = (total_wg + 7) / 8;
   const dim3 grid((unsigned)(8 * (per < slots ? per : slots)));
-  if (xtr) launch_oz<true>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
-  else launch_oz<false>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
+  if (xtr) launch_oz<true, false>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
+  else if (chain) launch_oz<false, true>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
+  else launch_oz<false, false>(grid, d_groups, ng, d_twins, total_wg, nv, P, run, s);
   return hipGetLastError();
 }
 

@@ -146,6 +146,11 @@ struct LcOzGroup {
   // its strip of A once (every K step in its own stage buffer, K <= 8 * 32) and forms the products
   // of the n[0] items along axis 0 with it
   int arun;
+  // index in the launch's side array (the one holding twins) of a first product whose output is
+  // this product's A, or -1. Chained: per item and strip the kernel forms the first product
+  // (elemental x dense), writes its digits straight into the stage buffers as this product's A
+  // (dense x elemental; 4 P <= 256), and never stores it; this group's A view is unused
+  int pre;
   long long wg_begin;
   LcView Y[LC_OZ_MAX_TRACES];
   double tr_alpha[LC_OZ_MAX_TRACES][2];

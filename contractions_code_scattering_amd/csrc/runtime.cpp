@@ -419,9 +419,52 @@ void Program::gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, 
            twin_B ? &tr2 : nullptr);
 }
 
+void Program::gemm_oz_chain(const LcView& A1, const LcView& B1, const LcView& B2, const DigitOut* dout, int n0, int n1,
+                            int n2, const std::vector<FusedTrace>& traces, const LcView* twin_B,
+                            const std::vector<FusedTrace>* twin_traces) {
+  if (mode_ != OZ) fail(LC_ERR_ARG, "exact-form product outside the exact mode");
+  // the chained product's A: the first product, never stored (a placeholder view)
+  const LcView chained = make_view(nullptr, 0, 0, 0, LC_DENSE, 0);
+  const long long cs[3] = {0, 0, 0};
+  if (!last_axis_fastest_ || n2 <= 1) {
+    oz_items(chained, B2, nullptr, cs, dout, n0, n1, n2, 1.0, 0.0, traces, twin_B, twin_traces, &A1, &B1);
+    return;
+  }
+  // the axes relabelled as in gemm_oz
+  const int o1 = second_axis_ == 1 ? 1 : 0, o2 = 1 - o1;
+  auto rot3 = [&](const long long* x, long long* y) {
+    const long long t[3] = {x[0], x[1], x[2]};
+    y[0] = t[2]; y[1] = t[o1]; y[2] = t[o2];
+  };
+  auto rot = [&](LcView v) {
+    rot3(v.s, v.s);
+    rot3(v.es, v.es);
+    return v;
+  };
+  std::vector<FusedTrace> tr = traces, tr2 = twin_traces ? *twin_traces : std::vector<FusedTrace>{};
+  for (std::vector<FusedTrace>* v : {&tr, &tr2})
+    for (FusedTrace& f : *v) {
+      f.Y = rot(f.Y);
+      rot3(f.os, f.os);
+    }
+  LcView tb;
+  if (twin_B) tb = rot(*twin_B);
+  DigitOut d;
+  if (dout) {
+    d = *dout;
+    rot3(dout->ds, d.ds);
+    rot3(dout->es, d.es);
+  }
+  const LcView a1 = rot(A1), b1 = rot(B1);
+  const int nn[3] = {n0, n1, n2};
+  oz_items(chained, rot(B2), nullptr, cs, dout ? &d : nullptr, nn[2], nn[o1], nn[o2], 1.0, 0.0, tr,
+           twin_B ? &tb : nullptr, twin_B ? &tr2 : nullptr, &a1, &b1);
+}
+
 void Program::oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
                        int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
-                       const LcView* twin_B, const std::vector<FusedTrace>* twin_traces) {
+                       const LcView* twin_B, const std::vector<FusedTrace>* twin_traces, const LcView* pre_A,
+                       const LcView* pre_B) {
   if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
   const int P = oz_pad(nv_);
   if (nv_ > 128) fail(LC_ERR_ARG, "exact form supports nv <= 128 (got %d)", nv_);
@@ -430,9 +473,25 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   if (!C && !dout && traces.empty()) fail(LC_ERR_ARG, "product with neither output nor trace");
   if (!kron_plain(A) || !kron_plain(B) || (twin_B && !kron_plain(*twin_B)))
     fail(LC_ERR_ARG, "an elemental operand with a spin matrix");
-  if ((A.flags & LC_TV) || !(B.flags & LC_TV))
+  // a chained product (A = pre_A pre_B, formed on the chip): elemental pre_A times dense pre_B, then
+  // this product dense x elemental, with the whole D-row in one workgroup strip
+  const bool chain = pre_A != nullptr;
+  if (chain) {
+    if (!pre_B || pre_A->kind != LC_KRON || pre_B->kind != LC_DENSE || B.kind != LC_KRON || C || 4 * P > 256)
+      fail(LC_ERR_ARG, "a chained product is (elemental x dense) x elemental with 4 P <= 256 and no FP64 output");
+    if (!kron_plain(*pre_A) || (pre_A->flags & LC_TV) || !(pre_B->flags & LC_TV) || !pre_A->ex || !pre_B->ex)
+      fail(LC_ERR_ARG, "a chained first product's operands must be read along contiguous K, with exponents");
+    if (pre_B->ebs && !(pre_B->flags & LC_TS))
+      fail(LC_ERR_ARG, "exact-form dense operand lines cross stored block rows");
+    for (const LcView* v : {pre_A, pre_B})
+      for (int a = 0; a < 4; a++)
+        for (const double* ph : {v->ph_r[a], v->ph_c[a]})
+          if (!((ph[1] == 0 && (ph[0] == 1 || ph[0] == -1)) || (ph[0] == 0 && (ph[1] == 1 || ph[1] == -1))))
+            fail(LC_ERR_ARG, "exact-form operand phase (%g, %g) is not a power of i", ph[0], ph[1]);
+  }
+  if ((!chain && (A.flags & LC_TV)) || !(B.flags & LC_TV))
     fail(LC_ERR_ARG, "exact-form operands must be read along contiguous K (A by rows, B by columns)");
-  if (!A.ex || !B.ex) fail(LC_ERR_ARG, "exact-form operand without exponents");
+  if ((!chain && !A.ex) || !B.ex) fail(LC_ERR_ARG, "exact-form operand without exponents");
   for (const LcView* v : {&A, &B})
     for (int a = 0; a < 4; a++)
       for (const double* ph : {v->ph_r[a], v->ph_c[a]})
@@ -514,6 +573,7 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   g.kind = kind;
   g.nseg = nseg;
   g.twin = -1;
+  g.pre = -1;
   g.arun = run_ok ? n0 : 1;
   g.wg_begin = l->total_wg;
   const long long items = (long long)n0 * n1 * n2;
@@ -553,6 +613,24 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
     }
     g.twin = (int)oztw_.size();
     oztw_.push_back(t);
+    gemm_flops_ += (double)items * 16 * 8.0 * nv_ * (double)nv_ * nv_;
+  }
+  if (chain) {
+    // the first product in the side array: elemental x dense, no outputs of its own
+    LcOzGroup f;
+    std::memset(&f, 0, sizeof f);
+    f.A = *pre_A;
+    f.B = *pre_B;
+    f.alpha[0] = 1.0;
+    f.n[0] = n0; f.n[1] = n1; f.n[2] = n2;
+    f.kind = 1;
+    f.nseg = 1;
+    f.twin = -1;
+    f.pre = -1;
+    f.arun = 1;
+    g.pre = (int)oztw_.size();
+    oztw_.push_back(f);
+    l->chain = true;
     gemm_flops_ += (double)items * 16 * 8.0 * nv_ * (double)nv_ * nv_;
   }
   ozs_.push_back(g);
@@ -680,8 +758,9 @@ void Program::run(hipStream_t s) {
     } else if (variant == kOzGemm || variant == kOzGemmRun) {
       const LcOzGroup* og = dozs + l.first;
       const int P = oz_pad(nv);
-      const int run = variant == kOzGemmRun ? 1 : 0, xtr = l.xtr ? 1 : 0;
-      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, run, xtr, st); },
+      const int run = variant == kOzGemmRun ? 1 : 0, xtr = l.xtr ? 1 : 0, chain = l.chain ? 1 : 0;
+      if (chain && xtr) fail(LC_ERR_ARG, "a launch with chained groups and the extended trace path");
+      launch([=](hipStream_t st) { return lc_launch_gemm_oz(og, count, dtw, total_wg, nv, P, run, xtr, chain, st); },
              "exact gemm launch");
     } else {
       const LcTraceGroup* tg = dt + l.first;

@@ -30,7 +30,7 @@ extern "C" hipError_t lc_launch_mfma_selftest(int is_double, const void* a, cons
                                               void* d, hipStream_t s);
 extern "C" int lc_gemm_tile(int nv);
 extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const LcOzGroup* d_twins,
-                                        long long total_wg, int nv, int P, int run, int xtr,
+                                        long long total_wg, int nv, int P, int run, int xtr, int chain,
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
@@ -262,6 +262,13 @@ class Program {
                const DigitOut* dout, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
                const std::vector<FusedTrace>& traces = {}, const LcView* twin_B = nullptr,
                const std::vector<FusedTrace>* twin_traces = nullptr);
+  // OZ mode, chained: (A1 B1) B2 with A1 an elemental view, B1 a D-space view and B2 an elemental
+  // view (4 P <= 256), outputs and traces (and twin) of the second product as in gemm_oz; the
+  // intermediate A1 B1 stays on the chip (it becomes the second product's A operand in the stage
+  // buffers, with its digits and row exponents exactly as a digit output would have them).
+  void gemm_oz_chain(const LcView& A1, const LcView& B1, const LcView& B2, const DigitOut* dout, int n0, int n1,
+                     int n2, const std::vector<FusedTrace>& traces = {}, const LcView* twin_B = nullptr,
+                     const std::vector<FusedTrace>* twin_traces = nullptr);
   // OZ mode: FP64 D-space matrices (nmat of them, `stride` complex elements apart) to digits with
   // per-D-row exponents (the form gemm_oz writes), as a level of its own.
   void to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf = false);
@@ -289,6 +296,7 @@ class Program {
     bool closed;
     int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzGemmRun, kOzConvert
     bool xtr = false;  // OZ: a group with more than LC_MAX_FUSED_TRACES traces or elemental closing factors
+    bool chain = false;  // OZ: a chained group (LcOzGroup::pre)
     size_t first;
     int count;
     long long total_wg;
@@ -306,7 +314,8 @@ class Program {
   std::vector<Convert> converts_;
   void oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
                 int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
-                const LcView* twin_B, const std::vector<FusedTrace>* twin_traces);
+                const LcView* twin_B, const std::vector<FusedTrace>* twin_traces,
+                const LcView* pre_A = nullptr, const LcView* pre_B = nullptr);
   int nv_;
   bool dbl_;
   Mode mode_;
