@@ -1,6 +1,7 @@
 // Exact form of the batched engines: the rho + pi-pi chains of engine.cpp and the sigma / sigma-pi
 // / rho-pi chains of engine_sigma.cpp, with every product on the int8 matrix cores through digit
 // storage and exact integer accumulation (gemm_oz.hip), FP64 outputs, closing factors and traces.
+#include <climits>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -133,10 +134,13 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   const long long P4i = 4LL * P;  // exponents per D-space matrix
   const bool direct = 4 * P <= 256;  // digit outputs written by the GEMMs themselves
   char* cur = scratch;
+  Program pg(E.nv, Program::OZ);
+  // (P = 128: every stored product's exponents are folded in by the product itself from INT_MIN)
   auto darr = [&](long long nmat) {
     DArr a;
     a.d = bump(cur, (size_t)nmat * bd * 8);
     a.e = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 4));
+    if (!direct) pg.fill_int(a.e, nmat * P4i, INT_MIN);
     return a;
   };
   const long long pairs = (long long)pc * npair;
@@ -162,7 +166,6 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   auto cs_of = [&](const long long* st, long long* cs) {
     for (int d = 0; d < 3; d++) cs[d] = st[d] * b;
   };
-  Program pg(E.nv, Program::OZ);
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   // digit output of a product into `a` (items laid out with strides st); with twin_B, the twin
   // product A twin_B traced by twin_tr (formed by the same kernel from A on the chip when the
@@ -186,14 +189,17 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     char* x = bump(stg.cur, (size_t)nmat * b * 16);
     long long cs[3];
     cs_of(st, cs);
-    pg.gemm_oz(A, Bv, x, cs[0], cs[1], cs[2], nullptr, n0, n1, n2, 1.0, 0.0, tr);
+    DigitOut ex;  // the row exponents only
+    ex.e = a.e;
+    for (int k = 0; k < 3; k++) ex.es[k] = st[k] * P4i;
+    pg.gemm_oz(A, Bv, x, cs[0], cs[1], cs[2], &ex, n0, n1, n2, 1.0, 0.0, tr);
     stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
     if (twin_B) pg.gemm_oz(A, *twin_B, nullptr, 0, 0, 0, nullptr, n0, n1, n2, 1.0, 0.0, *twin_tr);
   };
   // close a level: a barrier, then the digit conversions of its FP64-staged products
   auto close_level = [&] {
     pg.barrier();
-    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf);
+    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf, true);
     stg.pending.clear();
     stg.cur = stg.base;
   };
@@ -376,10 +382,12 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
   const long long P4i = 4LL * P;
   const bool direct = 4 * P <= 256;
   char* cur = scratch;
+  Program pg(E.nv, Program::OZ);
   auto darr = [&](long long nmat) {
     DArr a;
     a.d = bump(cur, (size_t)nmat * bd * 8);
     a.e = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 4));
+    if (!direct) pg.fill_int(a.e, nmat * P4i, INT_MIN);  // (exponents folded in by the products)
     return a;
   };
   const long long pairs = (long long)pc * npair;
@@ -394,7 +402,6 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
     LcView v = make_view(a.d, st[0] * bd, st[1] * bd, st[2] * bd, LC_DENSE, flags);
     return with_ex(v, a.e, st[0] * P4i, st[1] * P4i, st[2] * P4i, P);
   };
-  Program pg(E.nv, Program::OZ);
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
                    bool cf) {
@@ -412,7 +419,10 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
     }
     const long long nmat = (long long)n0 * n1 * n2;
     char* x = bump(stg.cur, (size_t)nmat * b * 16);
-    pg.gemm_oz(A, Bv, x, st[0] * b, st[1] * b, st[2] * b, nullptr, n0, n1, n2);
+    DigitOut ex;  // the row exponents only
+    ex.e = a.e;
+    for (int k = 0; k < 3; k++) ex.es[k] = st[k] * P4i;
+    pg.gemm_oz(A, Bv, x, st[0] * b, st[1] * b, st[2] * b, &ex, n0, n1, n2);
     stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
   };
   // trace-only pair products, with a twin on the same A where the kernel forms it (P <= 64)
@@ -427,7 +437,7 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
   };
   auto close_level = [&] {
     pg.barrier();
-    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf);
+    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf, true);
     stg.pending.clear();
     stg.cur = stg.base;
   };
@@ -571,12 +581,12 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
   auto call = [&] {
     const long long b = E.b, m = E.m;
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, tau_cf, nullptr,
+             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, tau_cf, nullptr, 0,
                                          st);
            },
            "tau to digits");
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, nullptr, nullptr, st);
+             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, nullptr, nullptr, 0, st);
            },
            "elementals to digits");
     OzRole R[4];
@@ -631,12 +641,12 @@ void oz_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, i
   char* el_cft = el_t + h.el_d + 2 * h.el_e;
   const long long b = E.b, m = E.m, nel = (long long)np * nt * SG_NEL;
   launch([=](hipStream_t st) {
-           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, nullptr, nullptr,
+           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, nullptr, nullptr, 0,
                                        st);
          },
          "tau to digits");
   launch([=](hipStream_t st) {
-           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, 0, eex_r, eex_t, el_r, el_t, nullptr, el_cft, st);
+           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, 0, eex_r, eex_t, el_r, el_t, nullptr, el_cft, 0, st);
          },
          "elementals to digits");
   const OzSigmaIn in{tau_r, tau_t, el_r, el_t, el_cft, tex_r, tex_t, eex_r, eex_t};
@@ -669,9 +679,9 @@ int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4,
     hip_check(hipMemcpyAsync(xb.p, h_b, nbb * nv2 * 16, hipMemcpyHostToDevice, s), "copy b");
     int* exa = static_cast<int*>(ea.p);
     int* exb = static_cast<int*>(eb.p);
-    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, nullptr, s),
+    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, nullptr, 0, s),
               "convert a");
-    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, nullptr, s),
+    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, nullptr, 0, s),
               "convert b");
     // A is read by rows: its row-line planes, or the column-line ones for an eigenvector transpose;
     // B is read by columns: the column-line planes, or the row-line ones for a transpose

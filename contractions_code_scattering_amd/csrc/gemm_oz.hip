@@ -723,7 +723,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       // digit output (and a chained group's first product): exponent of the D-row = the largest
       // |value| over every wave of the strip (the whole row), exchanged through LDS
       const bool dig = FIRST || gs.Cd;
-      if (dig) {
+      // an FP64 output whose D-row exponents are wanted (P = 128: the strip is half a D-row, whose
+      // maximum goes into the row's exponent by an atomic maximum)
+      const bool xmax = !K256 && !FIRST && gs.C && gs.Ce;
+      if (dig || xmax) {
         double mx = 0.0;
 #pragma unroll
         for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
@@ -739,6 +742,16 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       // exponents are visible
       if (dig || use_tile) lds_barrier();
       int E = 0;
+      if (xmax) {
+        int Eh = INT_MIN;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          const int x = rowe[w][w_];
+          Eh = x > Eh ? x : Eh;
+        }
+        if (wave == 0 && lane < 32 && Eh != INT_MIN)
+          atomicMax(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, Eh);
+      }
       if (dig) {
         E = INT_MIN;
 #pragma unroll
@@ -1094,7 +1107,7 @@ __global__ void __launch_bounds__(256) lc_oz_exps_kernel(const double2* __restri
 // Optionally the closing-factor form under the row exponents (ycf), and in mode 0 that of the
 // transposed block under the column exponents (ycft: element (j, i) of X^T at its place).
 __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __restrict__ x, long long stride, int nv,
-                                                           int P, int nblk, int mode, const int* __restrict__ ex_r,
+                                                           int P, int nblk, int mode, int* __restrict__ ex_r,
                                                            const int* __restrict__ ex_t, unsigned char* __restrict__ yr,
                                                            unsigned char* __restrict__ yt, int2* __restrict__ ycf,
                                                            int2* __restrict__ ycft) {
@@ -1115,7 +1128,14 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
     double2 v = make_double2(0.0, 0.0);
     if (row < nv && col < nv) v = p[(long long)row * nv + col];
     int dr[4], di[4];
-    const int er = ex_r[m * nr + (mode ? (blk >> 2) * P : 0) + row];
+    int* erp = ex_r + m * nr + (mode ? (blk >> 2) * P : 0) + row;
+    int er = *erp;
+    if (er == INT_MIN) {
+      // an all-zero row whose exponent came from a product's maxima: 0, as the exponent pass gives
+      // it, also for the products that read these digits (every tile of the row writes the same)
+      er = 0;
+      *erp = 0;
+    }
     if (ycf)
       ycf[(bi * 8 * P2 + cf_off(row, col, P)) / 8] =
           make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - er)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - er)));
@@ -1159,6 +1179,11 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
       *reinterpret_cast<uint4*>(yt + bi * 8 * P2 + q * P2 + ((long long)tr * P + tc * 32 + ln) * 32 + 16 * half) = w;
     }
   }
+}
+
+// n ints set to v (grid-stride)
+__global__ void __launch_bounds__(256) lc_fill_int_kernel(int* __restrict__ p, long long n, int v) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
 }
 
 // Digits back to FP64 (tests and diagnostics): D-space or elemental digit matrices with
@@ -1241,18 +1266,28 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
 // mode 1: y_rows and ex_r (nmat * 4 P ints) only. y_cf (optional, the size of y_rows): the
 // closing-factor form under the row exponents; y_rows may be null when only that form is wanted.
 // y_cft (optional, mode 0): the closing-factor form of every transposed block under the column
-// exponents.
+// exponents. have_exps (mode 1): ex_r already holds the row exponents (the maxima a product folded
+// in; INT_MIN for an all-zero row), so only the digits are made.
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           void* y_cf, void* y_cft, hipStream_t s) {
+                                           void* y_cf, void* y_cft, int have_exps, hipStream_t s) {
   if (nmat <= 0) return hipSuccess;
   if (y_cft && mode != 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(lc::lc_oz_exps_kernel, dim3((unsigned)nmat), dim3(256), 0, s, (const double2*)x, stride, nv, P,
-                     nblk, mode, ex_r, ex_t);
+  if (have_exps && mode != 1) return hipErrorInvalidValue;
+  if (!have_exps)
+    hipLaunchKernelGGL(lc::lc_oz_exps_kernel, dim3((unsigned)nmat), dim3(256), 0, s, (const double2*)x, stride, nv, P,
+                       nblk, mode, ex_r, ex_t);
   const long long tiles = nmat * nblk * (long long)(P / 32) * (P / 32);
   hipLaunchKernelGGL(lc::lc_oz_digits_kernel, dim3((unsigned)tiles), dim3(256), 0, s, (const double2*)x, stride, nv,
-                     P, nblk, mode, (const int*)ex_r, (const int*)ex_t, (unsigned char*)y_rows,
+                     P, nblk, mode, ex_r, (const int*)ex_t, (unsigned char*)y_rows,
                      (unsigned char*)y_trans, (int2*)y_cf, (int2*)y_cft);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t lc_launch_fill_int(int* p, long long n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(lc::lc_fill_int_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 

@@ -34,7 +34,8 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
                                            int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           void* y_cf, void* y_cft, hipStream_t s);
+                                           void* y_cf, void* y_cft, int have_exps, hipStream_t s);
+extern "C" hipError_t lc_launch_fill_int(int* p, long long n, int v, hipStream_t s);
 extern "C" hipError_t lc_launch_oz_decode(const void* y, const int* ex, int nv, int P, int nblk, int mode,
                                           long long nmat, void* out, hipStream_t s);
 
@@ -208,6 +209,9 @@ inline int oz_pad(int nv) { return nv <= 32 ? 32 : nv <= 64 ? 64 : 128; }
 
 // Digit output of an exact-form product: digit planes (strides in complex digit elements, P^2 per
 // block) and the per-D-row exponents ([4][P] ints per matrix, strides in ints).
+// Digits of a product with per-D-row exponents. With d == nullptr (and an FP64 output C): the
+// exponents alone, each workgroup strip's row maximum folded into e by an atomic maximum (e filled
+// with INT_MIN beforehand, fill_int), for a later to_digits(..., have_e = true).
 struct DigitOut {
   void* d = nullptr;
   int* e = nullptr;
@@ -271,7 +275,11 @@ class Program {
                      const std::vector<FusedTrace>* twin_traces = nullptr);
   // OZ mode: FP64 D-space matrices (nmat of them, `stride` complex elements apart) to digits with
   // per-D-row exponents (the form gemm_oz writes), as a level of its own.
-  void to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf = false);
+  // have_e: the exponents are already in e (an exponents-only DigitOut of the product).
+  void to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf = false,
+                 bool have_e = false);
+  // n ints at p set to v, as a level of its own (a fill kernel: no memset nodes in graphs).
+  void fill_int(int* p, long long n, int v);
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -290,7 +298,14 @@ class Program {
     void* y;
     int* e;
     bool cf;
+    bool have_e;
   };
+  struct Fill {
+    int* p;
+    long long n;
+    int v;
+  };
+  std::vector<Fill> fills_;
   struct Level {
     bool is_trace;
     bool closed;
@@ -309,7 +324,7 @@ class Program {
     bool oz = false;
     bool twin = false;  // (oz) the group is in the twin array
   };
-  static constexpr int kOzGemm = 10, kOzConvert = 11, kOzGemmRun = 12;
+  static constexpr int kOzGemm = 10, kOzConvert = 11, kOzGemmRun = 12, kFill = 13;
   std::vector<LcOzGroup> ozs_, oztw_;
   std::vector<Convert> converts_;
   void oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
