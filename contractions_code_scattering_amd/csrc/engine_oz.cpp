@@ -60,10 +60,12 @@ struct OzShape {
 };
 
 // A digit matrix array: planes and exponents (exponents per D-row: 4 P ints per D-space matrix,
-// or per line index: P ints per matrix).
+// or per line index: P ints per matrix); at P = 128, h receives a product's half-row exponents
+// (8 P ints per matrix) until the level's fix-up makes e.
 struct DArr {
   char* d;
   int* e;
+  int* h = nullptr;
 };
 
 inline char* bump(char*& cur, size_t bytes) {
@@ -91,20 +93,28 @@ struct OzRole {
   long long stride;
 };
 
-// Where the traced and stored products of one level go. A product stored in digits is written by
-// the GEMM itself when the workgroup strip covers whole D-rows (4 P <= 256); otherwise (P = 128)
-// it is written in FP64 into a staging area and converted after the level.
-struct Staging {
-  char* base = nullptr;
-  char* cur = nullptr;
-  struct Pending {
-    const void* x;
-    long long nmat;
-    DArr y;
-    bool cf;
-  };
-  std::vector<Pending> pending;
+// The stored products of one level whose half-row exponents are made whole after it (P = 128: a
+// workgroup strip covers half a D-row, lc_oz_fix_kernel).
+struct PendingFix {
+  DArr y;
+  long long nmat;
+  bool cf;
 };
+
+// Digit output of a product into `a` (items laid out with strides st, in matrices): the D-row
+// exponents directly (4 P <= 256), or the half-row ones, with a fix-up queued for the level's end.
+inline DigitOut digit_out(const DArr& a, const long long* st, long long bd, int P, bool cf) {
+  const bool direct = 4 * P <= 256;
+  DigitOut d;
+  d.d = a.d;
+  d.e = direct ? a.e : a.h;
+  d.cf = cf;
+  for (int k = 0; k < 3; k++) {
+    d.ds[k] = st[k] * bd;
+    d.es[k] = st[k] * (direct ? 4LL : 8LL) * P;
+  }
+  return d;
+}
 
 // rho + pi-pi chain of one momentum chunk in the exact form. Per (tf, t0, p), with P1 = P1(t0),
 // P2 = P2(t0), P3 = P3(tf), P4 = P4(tf) and tau = tau(tf, t0):
@@ -127,20 +137,19 @@ struct Staging {
 // closing factor of the short diagrams is the FP64 perambulator itself.
 void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* tau_r, const char* tau_t,
                   const int* tex_r, const int* tex_t, const OzRole* R, int p0, int pc, char* scratch,
-                  size_t stage_bytes, void* raw) {
+                  void* raw) {
   const int nt = E.nt, ns = E.nsrc, P = E.P;
-  const long long b = E.b, bd = E.bd, md = E.md;
+  const long long bd = E.bd, md = E.md;
   const long long npair = (long long)nt * ns;
   const long long P4i = 4LL * P;  // exponents per D-space matrix
   const bool direct = 4 * P <= 256;  // digit outputs written by the GEMMs themselves
   char* cur = scratch;
   Program pg(E.nv, Program::OZ);
-  // (P = 128: every stored product's exponents are folded in by the product itself from INT_MIN)
   auto darr = [&](long long nmat) {
     DArr a;
     a.d = bump(cur, (size_t)nmat * bd * 8);
     a.e = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 4));
-    if (!direct) pg.fill_int(a.e, nmat * P4i, INT_MIN);
+    if (!direct) a.h = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 8));
     return a;
   };
   const long long pairs = (long long)pc * npair;
@@ -153,8 +162,7 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   DArr Qat = darr((long long)pc * nt), Qd = darr((long long)pc * nt);
   DArr T1t = darr((long long)pc * ns), T1pt = darr((long long)pc * ns);
   DArr ZhTd = darr(pairs), RTd = darr((long long)pc * ns), RpTd = darr((long long)pc * ns);
-  Staging stg;
-  stg.base = stg.cur = direct ? nullptr : bump(cur, stage_bytes * pc);
+  std::vector<PendingFix> pending;
 
   // batch strides of the item (axis 0 sink tf, axis 1 source k, axis 2 momentum p) for the pair,
   // per-sink and per-source buffers, in matrices
@@ -163,45 +171,27 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     LcView v = make_view(a.d, st[0] * bd, st[1] * bd, st[2] * bd, LC_DENSE, flags);
     return with_ex(v, a.e, st[0] * P4i, st[1] * P4i, st[2] * P4i, P);
   };
-  auto cs_of = [&](const long long* st, long long* cs) {
-    for (int d = 0; d < 3; d++) cs[d] = st[d] * b;
-  };
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   // digit output of a product into `a` (items laid out with strides st); with twin_B, the twin
-  // product A twin_B traced by twin_tr (formed by the same kernel from A on the chip when the
-  // digits are written directly, else as a product of its own)
+  // product A twin_B traced by twin_tr (formed by the same kernel from A on the chip at P <= 64,
+  // else as a product of its own)
   auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
                    const std::vector<FusedTrace>& tr = {}, bool cf = false, const LcView* twin_B = nullptr,
                    const std::vector<FusedTrace>* twin_tr = nullptr) {
+    const DigitOut d = digit_out(a, st, bd, P, cf);
     if (direct) {
-      DigitOut d;
-      d.d = a.d;
-      d.e = a.e;
-      d.cf = cf;
-      for (int k = 0; k < 3; k++) {
-        d.ds[k] = st[k] * bd;
-        d.es[k] = st[k] * P4i;
-      }
       pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr, twin_B, twin_tr);
       return;
     }
-    const long long nmat = (long long)n0 * n1 * n2;
-    char* x = bump(stg.cur, (size_t)nmat * b * 16);
-    long long cs[3];
-    cs_of(st, cs);
-    DigitOut ex;  // the row exponents only
-    ex.e = a.e;
-    for (int k = 0; k < 3; k++) ex.es[k] = st[k] * P4i;
-    pg.gemm_oz(A, Bv, x, cs[0], cs[1], cs[2], &ex, n0, n1, n2, 1.0, 0.0, tr);
-    stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
+    pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr);
+    pending.push_back(PendingFix{a, (long long)n0 * n1 * n2, cf});
     if (twin_B) pg.gemm_oz(A, *twin_B, nullptr, 0, 0, 0, nullptr, n0, n1, n2, 1.0, 0.0, *twin_tr);
   };
-  // close a level: a barrier, then the digit conversions of its FP64-staged products
+  // close a level: a barrier, then (P = 128) the fix-ups of its stored products
   auto close_level = [&] {
     pg.barrier();
-    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf, true);
-    stg.pending.clear();
-    stg.cur = stg.base;
+    for (const PendingFix& q : pending) pg.fix_digits(q.y.d, q.y.h, q.y.e, q.nmat, q.cf);
+    pending.clear();
   };
 
   // perambulator views: tau(tf, t0_k) from the row-line (A role) or column-line (B role) planes
@@ -289,18 +279,7 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
     }
     if (direct) {
       // chained: L1 = P3 tau, B = P4 tau (and B4 = P4^dag tau) formed per strip on the chip
-      auto dout = [&](const DArr& a) {
-        DigitOut d;
-        d.d = a.d;
-        d.e = a.e;
-        d.cf = false;
-        for (int k = 0; k < 3; k++) {
-          d.ds[k] = PR[k] * bd;
-          d.es[k] = PR[k] * P4i;
-        }
-        return d;
-      };
-      const DigitOut dL = dout(Ld), dR = dout(Rrd);
+      const DigitOut dL = digit_out(Ld, PR, bd, P, false), dR = digit_out(Rrd, PR, bd, P, false);
       pg.gemm_oz_chain(role(2, true, false, false), tau_B, P1B, &dL, nt, ns, pc, tl, &P2B, &ty);
       pg.gemm_oz_chain(role(3, true, false, false), tau_B, P2B, &dR, nt, ns, pc, tr, &P1B, &tx);
       if (sets)
@@ -335,18 +314,19 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   pg.run(ctx().stream);
 }
 
+// One digit matrix with its exponents (and, at P = 128, its half-row exponents), with the
+// alignment of each piece.
+size_t oz_digit_matrix_bytes(const OzShape& E) {
+  return (size_t)E.bd * 8 + (4 * E.P <= 256 ? 4 : 12) * (size_t)E.P * 4 + 768;
+}
+
 // Scratch of one momentum (pc = 1): digit pair buffers L, Rr, Zh^T, U^T (and, for P = 128, B, L1
-// and for sets B4), per sink Qa^T, Q, per source T1^T, T1'^T, R^T, R'^T; for P = 128 the FP64
-// staging of the largest level (level 1: three pair products, one per sink, two per source).
-size_t oz_bytes_per_momentum(const OzShape& E, bool sets, size_t* stage) {
+// and for sets B4), per sink Qa^T, Q, per source T1^T, T1'^T, R^T, R'^T.
+size_t oz_bytes_per_momentum(const OzShape& E, bool sets) {
   const long long npair = (long long)E.nt * E.nsrc;
   const bool direct = 4 * E.P <= 256;
-  const size_t dm = (size_t)E.bd * 8 + 4 * (size_t)E.P * 4 + 512;  // one digit matrix + exponents
-  const size_t fm = (size_t)E.b * 16;
-  const size_t st = direct ? 0 : ((size_t)(sets ? 3 : 2) * npair + E.nt + 2 * E.nsrc) * fm + 16 * 256;
-  if (stage) *stage = st;
   const long long npb = direct ? 4 : (sets ? 7 : 6);
-  return (size_t)(npair * npb + 2LL * E.nt + 4LL * E.nsrc) * dm + st + 16 * 256;
+  return (size_t)(npair * npb + 2LL * E.nt + 4LL * E.nsrc) * oz_digit_matrix_bytes(E) + 16 * 256;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -375,9 +355,9 @@ struct OzSigmaIn {
 };
 
 void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0, int pc, char* scratch,
-                        size_t stage_bytes, void* raw) {
+                        void* raw) {
   const int nt = E.nt, ns = E.nsrc, P = E.P;
-  const long long b = E.b, bd = E.bd, md = E.md;
+  const long long bd = E.bd, md = E.md;
   const long long npair = (long long)nt * ns;
   const long long P4i = 4LL * P;
   const bool direct = 4 * P <= 256;
@@ -387,15 +367,14 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
     DArr a;
     a.d = bump(cur, (size_t)nmat * bd * 8);
     a.e = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 4));
-    if (!direct) pg.fill_int(a.e, nmat * P4i, INT_MIN);  // (exponents folded in by the products)
+    if (!direct) a.h = reinterpret_cast<int*>(bump(cur, (size_t)nmat * P4i * 8));
     return a;
   };
   const long long pairs = (long long)pc * npair;
   const DArr Bl[2] = {darr(pairs), darr(pairs)};
   const DArr YTd = darr(pairs);
   const DArr T1t = darr((long long)pc * ns), Rd = darr((long long)pc * ns);
-  Staging stg;
-  stg.base = stg.cur = direct ? nullptr : bump(cur, stage_bytes * pc);
+  std::vector<PendingFix> pending;
 
   const long long PR[3] = {1, nt, npair}, SR[3] = {0, 1, ns};
   auto dview = [&](const DArr& a, const long long* st, int flags) {
@@ -405,25 +384,9 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
                    bool cf) {
-    if (direct) {
-      DigitOut d;
-      d.d = a.d;
-      d.e = a.e;
-      d.cf = cf;
-      for (int k = 0; k < 3; k++) {
-        d.ds[k] = st[k] * bd;
-        d.es[k] = st[k] * P4i;
-      }
-      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2);
-      return;
-    }
-    const long long nmat = (long long)n0 * n1 * n2;
-    char* x = bump(stg.cur, (size_t)nmat * b * 16);
-    DigitOut ex;  // the row exponents only
-    ex.e = a.e;
-    for (int k = 0; k < 3; k++) ex.es[k] = st[k] * P4i;
-    pg.gemm_oz(A, Bv, x, st[0] * b, st[1] * b, st[2] * b, &ex, n0, n1, n2);
-    stg.pending.push_back(Staging::Pending{x, nmat, a, cf});
+    const DigitOut d = digit_out(a, st, bd, P, cf);
+    pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2);
+    if (!direct) pending.push_back(PendingFix{a, (long long)n0 * n1 * n2, cf});
   };
   // trace-only pair products, with a twin on the same A where the kernel forms it (P <= 64)
   auto tgemm = [&](const LcView& A, const LcView& Bv, const std::vector<FusedTrace>& tr, const LcView* twin_B = nullptr,
@@ -437,9 +400,8 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
   };
   auto close_level = [&] {
     pg.barrier();
-    for (const Staging::Pending& q : stg.pending) pg.to_digits(q.x, b, q.nmat, q.y.d, q.y.e, q.cf, true);
-    stg.pending.clear();
-    stg.cur = stg.base;
+    for (const PendingFix& q : pending) pg.fix_digits(q.y.d, q.y.h, q.y.e, q.nmat, q.cf);
+    pending.clear();
   };
 
   // perambulator tau(tf, t0_k): row-line planes (A role) or column-line planes (B role)
@@ -507,15 +469,10 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
   pg.run(ctx().stream);
 }
 
-// Scratch of one momentum set: pair buffers B_0, B_1, Y^T, per source T1^T, R; for P = 128 the
-// FP64 staging of the largest level (level 1: two pair products and one per source).
-size_t oz_sigma_bytes_per_momentum_(const OzShape& E, size_t* stage) {
+// Scratch of one momentum set: pair buffers B_0, B_1, Y^T, per source T1^T, R.
+size_t oz_sigma_bytes_per_momentum_(const OzShape& E) {
   const long long npair = (long long)E.nt * E.nsrc;
-  const size_t dm = (size_t)E.bd * 8 + 4 * (size_t)E.P * 4 + 512;
-  const size_t fm = (size_t)E.b * 16;
-  const size_t st = 4 * E.P <= 256 ? 0 : ((size_t)2 * npair + E.nsrc) * fm + 8 * 256;
-  if (stage) *stage = st;
-  return (size_t)(3 * npair + 2LL * E.nsrc) * dm + st + 8 * 256;
+  return (size_t)(3 * npair + 2LL * E.nsrc) * oz_digit_matrix_bytes(E) + 8 * 256;
 }
 
 OzShape oz_shape(int nt, int nv, int src0, int src_step, int nsrc) {
@@ -558,8 +515,7 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
   const size_t phi_d = ((size_t)nphi * E.md * 8 + 255) & ~size_t(255);
   const size_t phi_e = ((size_t)nphi * P * 4 + 255) & ~size_t(255);
   const size_t head = 3 * tau_d + 2 * (tau_e + phi_d + phi_e);  // tau: row, column and closing-factor form
-  size_t stage = 0;
-  const size_t per_p = oz_bytes_per_momentum(E, sets, &stage);
+  const size_t per_p = oz_bytes_per_momentum(E, sets);
   if (max_scratch_bytes <= 0) max_scratch_bytes = default_scratch_budget();
   const long long avail = max_scratch_bytes - (long long)head;
   int pc = avail > 0 ? (int)(avail / (long long)per_p) : 0;
@@ -577,16 +533,15 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
   int* pex_t = reinterpret_cast<int*>(phi_t + phi_d + phi_e);
   char* tau_cf = phi_t + phi_d + 2 * phi_e;
   char* scratch = base + head;
-  const size_t stage_b = stage;
   auto call = [&] {
     const long long b = E.b, m = E.m;
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, tau_cf, nullptr, 0,
+             return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, tex_r, tex_t, tau_r, tau_t, tau_cf, nullptr,
                                          st);
            },
            "tau to digits");
     launch([=](hipStream_t st) {
-             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, 0, pex_r, pex_t, phi_r, phi_t, nullptr, nullptr, 0, st);
+             return lc_launch_oz_convert(d_phi, m, nphi, nv, P, 1, pex_r, pex_t, phi_r, phi_t, nullptr, nullptr, st);
            },
            "elementals to digits");
     OzRole R[4];
@@ -597,7 +552,7 @@ void oz_engine_rho_pipi(bool sets, const void* d_tau, const void* d_phi, int nt,
     }
     for (int p0 = 0; p0 < np; p0 += pc) {
       const int n = (np - p0) < pc ? (np - p0) : pc;
-      run_chunk_oz(E, sets, tau_cf, tau_r, tau_t, tex_r, tex_t, R, p0, n, scratch, stage_b, d_raw);
+      run_chunk_oz(E, sets, tau_cf, tau_r, tau_t, tex_r, tex_t, R, p0, n, scratch, d_raw);
     }
     const double* sp[4] = {nullptr, nullptr, nullptr, nullptr};
     const long long ps = (long long)nel * nt;
@@ -620,15 +575,13 @@ size_t oz_sigma_head_bytes(int nt, int nv, int np) {
 }
 
 size_t oz_sigma_bytes_per_momentum(int nt, int nv, int nsrc) {
-  return oz_sigma_bytes_per_momentum_(oz_shape(nt, nv, 0, 1, nsrc), nullptr);
+  return oz_sigma_bytes_per_momentum_(oz_shape(nt, nv, 0, 1, nsrc));
 }
 
 void oz_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, int idir, int src0, int src_step,
                   int nsrc, char* head, char* scratch, int pc, void* d_raw) {
   const OzShape E = oz_shape(nt, nv, src0, src_step, nsrc);
   const OzSigmaHead h = oz_sigma_head(E, np);
-  size_t stage = 0;
-  (void)oz_sigma_bytes_per_momentum_(E, &stage);
   const int P = E.P;
   char* tau_r = head;
   char* tau_t = tau_r + h.tau_d;
@@ -641,18 +594,18 @@ void oz_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, i
   char* el_cft = el_t + h.el_d + 2 * h.el_e;
   const long long b = E.b, m = E.m, nel = (long long)np * nt * SG_NEL;
   launch([=](hipStream_t st) {
-           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, 0, tex_r, tex_t, tau_r, tau_t, nullptr, nullptr, 0,
+           return lc_launch_oz_convert(d_tau, b, (long long)nt * nt, nv, P, 16, tex_r, tex_t, tau_r, tau_t, nullptr, nullptr,
                                        st);
          },
          "tau to digits");
   launch([=](hipStream_t st) {
-           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, 0, eex_r, eex_t, el_r, el_t, nullptr, el_cft, 0, st);
+           return lc_launch_oz_convert(d_el, m, nel, nv, P, 1, eex_r, eex_t, el_r, el_t, nullptr, el_cft, st);
          },
          "elementals to digits");
   const OzSigmaIn in{tau_r, tau_t, el_r, el_t, el_cft, tex_r, tex_t, eex_r, eex_t};
   for (int p0 = 0; p0 < np; p0 += pc) {
     const int n = (np - p0) < pc ? (np - p0) : pc;
-    run_chunk_oz_sigma(E, idir, in, p0, n, scratch, stage, d_raw);
+    run_chunk_oz_sigma(E, idir, in, p0, n, scratch, d_raw);
   }
 }
 
@@ -674,14 +627,14 @@ int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4,
     hipStream_t s = ctx().stream;
     Scoped xa(nba * nv2 * 16), xb(nbb * nv2 * 16);
     Scoped ya(2 * nba * 8 * P2), yb(2 * nbb * 8 * P2), ea(2 * P * 4), eb(2 * P * 4);
-    Scoped c(16 * nv2 * 16), cd(16 * 8 * P2), ce(4 * P * 4), dec(16 * nv2 * 16);
+    Scoped c(16 * nv2 * 16), cd(16 * 8 * P2), ce(12 * P * 4), dec(16 * nv2 * 16);
     hip_check(hipMemcpyAsync(xa.p, h_a, nba * nv2 * 16, hipMemcpyHostToDevice, s), "copy a");
     hip_check(hipMemcpyAsync(xb.p, h_b, nbb * nv2 * 16, hipMemcpyHostToDevice, s), "copy b");
     int* exa = static_cast<int*>(ea.p);
     int* exb = static_cast<int*>(eb.p);
-    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, 0, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, nullptr, 0, s),
+    hip_check(lc_launch_oz_convert(xa.p, 0, 1, nv, P, nba, exa, exa + P, ya.p, ya.c() + nba * 8 * P2, nullptr, nullptr, s),
               "convert a");
-    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, 0, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, nullptr, 0, s),
+    hip_check(lc_launch_oz_convert(xb.p, 0, 1, nv, P, nbb, exb, exb + P, yb.p, yb.c() + nbb * 8 * P2, nullptr, nullptr, s),
               "convert b");
     // A is read by rows: its row-line planes, or the column-line ones for an eigenvector transpose;
     // B is read by columns: the column-line planes, or the row-line ones for a transpose
@@ -698,16 +651,24 @@ int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4,
       if (gam4[2]) B = left(B, gamma(gam4[2]));
       if (gam4[3]) B = right(B, gamma(gam4[3]));
     }
-    const bool digits = h_cd && 4 * P <= 256;
-    DigitOut d;
-    d.d = cd.p;
-    d.e = static_cast<int*>(ce.p);
+    // the digit output (P = 128: with half-row exponents, then the fix-up)
+    const bool digits = h_cd != nullptr;
+    DArr da;
+    da.d = cd.c();
+    da.e = static_cast<int*>(ce.p);
+    da.h = static_cast<int*>(ce.p) + 4 * P;
+    const long long one[3] = {0, 0, 0};
+    const DigitOut d = digit_out(da, one, 16 * P2, P, false);
     Program pg(nv, Program::OZ);
     pg.gemm_oz(A, B, c.p, 0, 0, 0, digits ? &d : nullptr, 1, 1, 1);
+    if (digits && 4 * P > 256) {
+      pg.barrier();
+      pg.fix_digits(da.d, da.h, da.e, 1, false);
+    }
     pg.run(s);
     hip_check(hipMemcpyAsync(h_c, c.p, 16 * nv2 * 16, hipMemcpyDeviceToHost, s), "copy c");
     if (digits) {
-      hip_check(lc_launch_oz_decode(cd.p, d.e, nv, P, 16, 1, 1, dec.p, s), "decode");
+      hip_check(lc_launch_oz_decode(cd.p, da.e, nv, P, 16, 1, 1, dec.p, s), "decode");
       hip_check(hipMemcpyAsync(h_cd, dec.p, 16 * nv2 * 16, hipMemcpyDeviceToHost, s), "copy cd");
     }
     hip_check(hipStreamSynchronize(s), "sync");

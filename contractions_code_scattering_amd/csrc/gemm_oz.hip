@@ -721,12 +721,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         }
       }
       // digit output (and a chained group's first product): exponent of the D-row = the largest
-      // |value| over every wave of the strip (the whole row), exchanged through LDS
+      // |value| over every wave of the strip (the whole row at P <= 64, half of it at P = 128),
+      // exchanged through LDS
       const bool dig = FIRST || gs.Cd;
-      // an FP64 output whose D-row exponents are wanted (P = 128: the strip is half a D-row, whose
-      // maximum goes into the row's exponent by an atomic maximum)
-      const bool xmax = !K256 && !FIRST && gs.C && gs.Ce;
-      if (dig || xmax) {
+      if (dig) {
         double mx = 0.0;
 #pragma unroll
         for (int q = 0; q < 16; q++) mx = __builtin_fmax(mx, __builtin_fmax(__builtin_fabs(vr[q]), __builtin_fabs(vi[q])));
@@ -742,16 +740,6 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       // exponents are visible
       if (dig || use_tile) lds_barrier();
       int E = 0;
-      if (xmax) {
-        int Eh = INT_MIN;
-#pragma unroll
-        for (int w = 0; w < NW; w++) {
-          const int x = rowe[w][w_];
-          Eh = x > Eh ? x : Eh;
-        }
-        if (wave == 0 && lane < 32 && Eh != INT_MIN)
-          atomicMax(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, Eh);
-      }
       if (dig) {
         E = INT_MIN;
 #pragma unroll
@@ -759,9 +747,15 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           const int x = rowe[w][w_];
           E = x > E ? x : E;
         }
+        if constexpr (!K256) {
+          // half a D-row (P = 128, the strip's columns 256 h ..): its exponent as it is (INT_MIN when
+          // the half is zero) at [D-row][h], the digits under it; lc_oz_fix_kernel makes the row's
+          if (wave == 0 && lane < 32)
+            gstore(gs.Ce + vo(gs.ces, cur.co) + 2 * ((long long)a * P + i0 + lane) + ((c * P + j0) >> 8), E);
+        }
         if (E == INT_MIN) E = 0;
         if (FIRST) ea_chain = E;
-        else if (wave == 0 && lane < 32) gstore(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, E);
+        else if (K256 && wave == 0 && lane < 32) gstore(gs.Ce + vo(gs.ces, cur.co) + (long long)a * P + i0 + lane, E);
       }
       const double sE = __builtin_ldexp(1.0, 30 - E);
       // this wave's tile in LDS as FP64 complex, row-major with a padded row stride; only this wave
@@ -1066,48 +1060,48 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
 }
 
 // ------------------------------------------------------------------------------------------
-// Conversion FP64 -> digits.
+// Conversion FP64 -> digits (inputs).
 //
-// Exponents. mode 0 (inputs): per matrix, one exponent per row index (over every block and
-// column) into ex_r and one per column index (over every block and row) into ex_t, P ints each.
-// mode 1 (products written in FP64): one exponent per D-row (block row R, row i over the four
-// blocks of R) into ex_r[R * P + i]. nblk = 16 (D-space) or 1 (elemental); matrices of nblk
-// blocks of nv x nv complex doubles, `stride` complex elements apart. One workgroup per matrix.
+// Exponents: per matrix, one exponent per row index (over every block and column) into ex_r and one
+// per column index (over every block and row) into ex_t, P ints each. nblk = 16 (D-space) or 1
+// (elemental); matrices of nblk blocks of nv x nv complex doubles, `stride` complex elements apart.
+// One workgroup per matrix.
 __global__ void __launch_bounds__(256) lc_oz_exps_kernel(const double2* __restrict__ x, long long stride, int nv,
-                                                         int P, int nblk, int mode, int* __restrict__ ex_r,
+                                                         int P, int nblk, int* __restrict__ ex_r,
                                                          int* __restrict__ ex_t) {
-  __shared__ int er[512], et[128];
+  __shared__ int er[128], et[128];
   const long long m = blockIdx.x;
-  const int nr = mode ? 4 * P : P;
-  for (int i = threadIdx.x; i < nr; i += 256) er[i] = INT_MIN;
-  for (int i = threadIdx.x; i < P; i += 256) et[i] = INT_MIN;
+  for (int i = threadIdx.x; i < P; i += 256) {
+    er[i] = INT_MIN;
+    et[i] = INT_MIN;
+  }
   __syncthreads();
   const double2* p = x + m * stride;
   const long long nv2 = (long long)nv * nv;
   const long long n = nblk * nv2;
   for (long long e = threadIdx.x; e < n; e += 256) {
-    const int blk = (int)(e / nv2);
     const int r = (int)((e / nv) % nv), cc = (int)(e % nv);
     const double2 v = p[e];
     const int e1 = exp_of(v.x), e2 = exp_of(v.y);
     const int ev = e1 > e2 ? e1 : e2;
     if (ev == INT_MIN) continue;
-    atomicMax(&er[mode ? (blk >> 2) * P + r : r], ev);
-    if (!mode) atomicMax(&et[cc], ev);
+    atomicMax(&er[r], ev);
+    atomicMax(&et[cc], ev);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nr; i += 256) ex_r[m * nr + i] = er[i] == INT_MIN ? 0 : er[i];
-  if (!mode)
-    for (int i = threadIdx.x; i < P; i += 256) ex_t[m * P + i] = et[i] == INT_MIN ? 0 : et[i];
+  for (int i = threadIdx.x; i < P; i += 256) {
+    ex_r[m * P + i] = er[i] == INT_MIN ? 0 : er[i];
+    ex_t[m * P + i] = et[i] == INT_MIN ? 0 : et[i];
+  }
 }
 
 // Digits. One workgroup per 32 x 32 tile of a P x P block (zero outside nv x nv): the row-line
-// planes (line = row, K = column) always, the column-line planes (line = column, K = row) in mode
-// 0. Both are one contiguous kilobyte per plane and tile; the bytes are gathered in LDS first.
-// Optionally the closing-factor form under the row exponents (ycf), and in mode 0 that of the
-// transposed block under the column exponents (ycft: element (j, i) of X^T at its place).
+// planes (line = row, K = column) and the column-line planes (line = column, K = row), one
+// contiguous kilobyte per plane and tile; the bytes are gathered in LDS first. Optionally the
+// closing-factor form under the row exponents (ycf), and that of the transposed block under the
+// column exponents (ycft: element (j, i) of X^T at its place).
 __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __restrict__ x, long long stride, int nv,
-                                                           int P, int nblk, int mode, int* __restrict__ ex_r,
+                                                           int P, int nblk, const int* __restrict__ ex_r,
                                                            const int* __restrict__ ex_t, unsigned char* __restrict__ yr,
                                                            unsigned char* __restrict__ yt, int2* __restrict__ ycf,
                                                            int2* __restrict__ ycft) {
@@ -1120,7 +1114,6 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
   const int tr = (int)((t / tpb) % tpb), tc = (int)(t % tpb);
   const double2* p = x + m * stride + (long long)blk * nv * nv;
   const long long P2 = (long long)P * P;
-  const int nr = mode ? 4 * P : P;
 #pragma unroll
   for (int pass = 0; pass < 4; pass++) {
     const int r = (threadIdx.x >> 5) + 8 * pass, cc = threadIdx.x & 31;
@@ -1128,23 +1121,13 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
     double2 v = make_double2(0.0, 0.0);
     if (row < nv && col < nv) v = p[(long long)row * nv + col];
     int dr[4], di[4];
-    int* erp = ex_r + m * nr + (mode ? (blk >> 2) * P : 0) + row;
-    int er = *erp;
-    if (er == INT_MIN) {
-      // an all-zero row whose exponent came from a product's maxima: 0, as the exponent pass gives
-      // it, also for the products that read these digits (every tile of the row writes the same)
-      er = 0;
-      *erp = 0;
-    }
+    const int er = ex_r[m * P + row], et = ex_t[m * P + col];
     if (ycf)
       ycf[(bi * 8 * P2 + cf_off(row, col, P)) / 8] =
           make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - er)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - er)));
-    if (ycft) {
-      const int ec = ex_t[m * P + col];
+    if (ycft)
       ycft[(bi * 8 * P2 + cf_off(col, row, P)) / 8] =
-          make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - ec)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - ec)));
-    }
-    if (!yr) continue;
+          make_int2((int)__builtin_rint(__builtin_ldexp(v.x, 30 - et)), (int)__builtin_rint(__builtin_ldexp(v.y, 30 - et)));
     digits_of(v.x, er, dr);
     digits_of(v.y, er, di);
 #pragma unroll
@@ -1152,21 +1135,18 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
       tr_[d][r][cc] = (unsigned char)dr[d];
       tr_[4 + d][r][cc] = (unsigned char)di[d];
     }
-    if (!mode) {
-      const int et = ex_t[m * P + col];
-      digits_of(v.x, et, dr);
-      digits_of(v.y, et, di);
+    digits_of(v.x, et, dr);
+    digits_of(v.y, et, di);
 #pragma unroll
-      for (int d = 0; d < 4; d++) {
-        tt_[d][cc][r] = (unsigned char)dr[d];
-        tt_[4 + d][cc][r] = (unsigned char)di[d];
-      }
+    for (int d = 0; d < 4; d++) {
+      tt_[d][cc][r] = (unsigned char)dr[d];
+      tt_[4 + d][cc][r] = (unsigned char)di[d];
     }
   }
-  if (!yr) return;  // (uniform: no barrier is skipped by part of the workgroup)
   __syncthreads();
   // 8 planes x 1 KB each: 512 16-byte pieces, two per thread
   unsigned char* ob = yr + bi * 8 * P2;
+  unsigned char* obt = yt + bi * 8 * P2;
 #pragma unroll
   for (int pass = 0; pass < 2; pass++) {
     const int k = threadIdx.x + 256 * pass;
@@ -1174,16 +1154,97 @@ __global__ void __launch_bounds__(256) lc_oz_digits_kernel(const double2* __rest
     const int ln = piece >> 1, half = piece & 1;
     const uint4 v = *reinterpret_cast<const uint4*>(&tr_[q][ln][16 * half]);
     *reinterpret_cast<uint4*>(ob + q * P2 + ((long long)tc * P + tr * 32 + ln) * 32 + 16 * half) = v;
-    if (!mode) {
-      const uint4 w = *reinterpret_cast<const uint4*>(&tt_[q][ln][16 * half]);
-      *reinterpret_cast<uint4*>(yt + bi * 8 * P2 + q * P2 + ((long long)tr * P + tc * 32 + ln) * 32 + 16 * half) = w;
-    }
+    const uint4 w = *reinterpret_cast<const uint4*>(&tt_[q][ln][16 * half]);
+    *reinterpret_cast<uint4*>(obt + q * P2 + ((long long)tr * P + tc * 32 + ln) * 32 + 16 * half) = w;
   }
 }
 
-// n ints set to v (grid-stride)
-__global__ void __launch_bounds__(256) lc_fill_int_kernel(int* __restrict__ p, long long n, int v) {
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
+// ------------------------------------------------------------------------------------------
+// Row exponents of a P = 128 digit output. A product's workgroup strip there covers half a D-row
+// (block columns 2 h, 2 h + 1), so the GEMM writes each half's digits (or closing-factor values)
+// under the half's own exponent, eh[row][h] (INT_MIN for an all-zero half). This kernel makes the
+// D-row's: E = max(eh[row][0], eh[row][1]) into ex[row] (0 for an all-zero row, as the input
+// conversion gives it), and the half with the smaller exponent, k = E - eh[row][h] > 0, is
+// rescaled in place: its integer value v (of the digits, or the closing-factor int32) becomes
+// rint(v 2^-k), ties to even, the value under E (re-split into balanced digits). v itself is
+// rint(x s 2^k) for the scale s of E, so the result is within one unit of rint(x s).
+// One workgroup per (matrix, 32-row strip, half): 32 rows x 256 columns, 8 planes (digits, 64
+// pieces of 1 KB: block, 32-column chunk, plane) or 64 KB of closing-factor values; a lane keeps
+// one row throughout (digits: lane l, row l / 2, bytes 16 (l % 2) of each plane's row; closing
+// factors: row l % 32, the 16 bytes at (32 (l / 32) + row) 16 of each kilobyte).
+__global__ void __launch_bounds__(256) lc_oz_fix_kernel(unsigned char* __restrict__ y, const int* __restrict__ eh,
+                                                        int* __restrict__ ex, int P, int cf) {
+  const int nst = P >> 5;
+  const long long t = blockIdx.x;
+  const int h = (int)(t & 1);
+  const long long u = t >> 1;
+  const int strip = (int)(u % (4 * nst));
+  const long long m = u / (4 * nst);
+  const int a = strip / nst, i0 = (strip - a * nst) * 32;
+  const long long P2 = (long long)P * P;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int* ehm = eh + m * 8 * P;
+  if (h == 0 && tid < 32) {
+    const int r = a * P + i0 + tid;
+    const int e0 = ehm[2 * r], e1 = ehm[2 * r + 1];
+    const int E = e0 > e1 ? e0 : e1;
+    ex[m * 4 * P + r] = E == INT_MIN ? 0 : E;
+  }
+  const int rr = cf ? (lane & 31) : (lane >> 1);
+  const int r = a * P + i0 + rr;
+  const int e0 = ehm[2 * r], e1 = ehm[2 * r + 1];
+  const int eo = h ? e1 : e0, E = e0 > e1 ? e0 : e1;
+  if (eo == INT_MIN || eo >= E) return;  // (no barrier in this kernel)
+  const double sc = __builtin_ldexp(1.0, eo - E);
+  unsigned char* yb = y + (m * 16 + a * 4 + 2 * h) * 8 * P2;  // block (a, 2 h)
+  if (cf) {
+    // 2 blocks x nst x nst chunks of 8 KB, those of the strip's rows: (cb, jc) at (jc nst + i0 / 32) 8 KB
+    const int npc = 2 * nst * 8;
+    for (int pc = wv; pc < npc; pc += 4) {
+      const int cb = pc / (nst * 8), rest = pc - cb * nst * 8, jc = rest >> 3, k = rest & 7;
+      i32x4* q = reinterpret_cast<i32x4*>(yb + cb * 8 * P2 + ((long long)jc * nst + (i0 >> 5)) * 8192 + k * 1024 +
+                                          (32 * (lane >> 5) + rr) * 16);
+      i32x4 v = *q;
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[e] = lo32(__builtin_fma((double)v[e], sc, RND));
+      *q = v;
+    }
+    return;
+  }
+  // digits: 2 blocks x nst chunks of 32 columns; per chunk 8 planes, P^2 bytes apart
+  for (int pc = wv; pc < 2 * nst; pc += 4) {
+    const int cb = pc / nst, jc = pc - cb * nst;
+    unsigned char* q = yb + cb * 8 * P2 + ((long long)jc * P + i0 + rr) * 32 + 16 * (lane & 1);
+    unsigned w[8][4];
+#pragma unroll
+    for (int pl = 0; pl < 8; pl++) {
+      const i32x4 v = *reinterpret_cast<const i32x4*>(q + pl * P2);
+#pragma unroll
+      for (int g = 0; g < 4; g++) w[pl][g] = (unsigned)v[g];
+    }
+    unsigned o[8][4];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      int dg[2][4][4];  // [re / im][column g 4 + b][digit]
+#pragma unroll
+      for (int part = 0; part < 2; part++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          int v = 0;
+#pragma unroll
+          for (int d = 0; d < 4; d++) v = v * 255 + (int)(signed char)(w[4 * part + d][g] >> (8 * b));
+          digits_fast((double)v, sc, dg[part][b]);
+        }
+#pragma unroll
+      for (int part = 0; part < 2; part++)
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+          o[4 * part + d][g] = pack4p(dg[part][0][d], dg[part][1][d], dg[part][2][d], dg[part][3][d]);
+    }
+#pragma unroll
+    for (int pl = 0; pl < 8; pl++)
+      *reinterpret_cast<i32x4*>(q + pl * P2) = i32x4{(int)o[pl][0], (int)o[pl][1], (int)o[pl][2], (int)o[pl][3]};
+  }
 }
 
 // Digits back to FP64 (tests and diagnostics): D-space or elemental digit matrices with
@@ -1261,33 +1322,33 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
   return hipGetLastError();
 }
 
-// x: nmat matrices of nblk blocks (nv x nv complex doubles), `stride` complex elements apart.
-// mode 0: y_rows / y_trans (nmat * nblk * 8 P^2 bytes each), ex_r / ex_t (nmat * P ints each).
-// mode 1: y_rows and ex_r (nmat * 4 P ints) only. y_cf (optional, the size of y_rows): the
-// closing-factor form under the row exponents; y_rows may be null when only that form is wanted.
-// y_cft (optional, mode 0): the closing-factor form of every transposed block under the column
-// exponents. have_exps (mode 1): ex_r already holds the row exponents (the maxima a product folded
-// in; INT_MIN for an all-zero row), so only the digits are made.
+// x: nmat matrices of nblk blocks (nv x nv complex doubles), `stride` complex elements apart, to
+// digits: y_rows / y_trans (nmat * nblk * 8 P^2 bytes each), ex_r / ex_t (nmat * P ints each).
+// y_cf (optional, the size of y_rows): the closing-factor form under the row exponents; y_cft
+// (optional): the closing-factor form of every transposed block under the column exponents.
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
-                                           int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           void* y_cf, void* y_cft, int have_exps, hipStream_t s) {
+                                           int* ex_r, int* ex_t, void* y_rows, void* y_trans, void* y_cf, void* y_cft,
+                                           hipStream_t s) {
   if (nmat <= 0) return hipSuccess;
-  if (y_cft && mode != 0) return hipErrorInvalidValue;
-  if (have_exps && mode != 1) return hipErrorInvalidValue;
-  if (!have_exps)
-    hipLaunchKernelGGL(lc::lc_oz_exps_kernel, dim3((unsigned)nmat), dim3(256), 0, s, (const double2*)x, stride, nv, P,
-                       nblk, mode, ex_r, ex_t);
+  if (!y_rows || !y_trans || !ex_r || !ex_t || P > 128) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lc::lc_oz_exps_kernel, dim3((unsigned)nmat), dim3(256), 0, s, (const double2*)x, stride, nv, P,
+                     nblk, ex_r, ex_t);
   const long long tiles = nmat * nblk * (long long)(P / 32) * (P / 32);
   hipLaunchKernelGGL(lc::lc_oz_digits_kernel, dim3((unsigned)tiles), dim3(256), 0, s, (const double2*)x, stride, nv,
-                     P, nblk, mode, ex_r, (const int*)ex_t, (unsigned char*)y_rows,
-                     (unsigned char*)y_trans, (int2*)y_cf, (int2*)y_cft);
+                     P, nblk, (const int*)ex_r, (const int*)ex_t, (unsigned char*)y_rows, (unsigned char*)y_trans,
+                     (int2*)y_cf, (int2*)y_cft);
   return hipGetLastError();
 }
 
-extern "C" hipError_t lc_launch_fill_int(int* p, long long n, int v, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const long long blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(lc::lc_fill_int_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, p, n, v);
+// nmat consecutive D-space digit matrices (8 * 16 P^2 bytes each; closing-factor form with cf)
+// written by a P = 128 product with half-row exponents eh (8 P ints per matrix): the D-row
+// exponents into ex (4 P ints per matrix) and each row's halves under them (lc_oz_fix_kernel).
+extern "C" hipError_t lc_launch_oz_fix(void* y, const int* eh, int* ex, long long nmat, int P, int cf, hipStream_t s) {
+  if (nmat <= 0) return hipSuccess;
+  if (P != 128) return hipErrorInvalidValue;  // (only there is a strip half a D-row)
+  const long long wgs = nmat * 4 * (P / 32) * 2;
+  if (wgs > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(lc::lc_oz_fix_kernel, dim3((unsigned)wgs), dim3(256), 0, s, (unsigned char*)y, eh, ex, P, cf);
   return hipGetLastError();
 }
 

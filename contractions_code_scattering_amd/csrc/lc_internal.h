@@ -119,8 +119,9 @@ struct LcTraceGroup {
 // per order), and the FP64 epilogue combines them, so the only rounding is the digit truncation
 // of the operands and the dropped orders 4..6 (about 2^-32 of the largest products). Outputs:
 // FP64 complex C (blocks of nv x nv), and/or digits Cd with one exponent per D-row written to Ce
-// ([4][P] ints per matrix; needs the workgroup strip to cover whole rows, i.e. 4 P <= 256), and
-// fused closing traces as in the other kernels. A closing factor in the closing-factor form may be
+// ([4][P] ints per matrix; at P = 128, where a workgroup strip is half a D-row, one per half,
+// [4 P][2] ints, each half under its own, for lc_oz_fix_kernel), and fused closing traces as in
+// the other kernels. A closing factor in the closing-factor form may be
 // an elemental view with a spin matrix S (X (x) S, block (a, b) = ph_r[a] ph_c[b] X where
 // perm_r[a] == perm_c[b], else zero); every other reader of an elemental view takes it as X (x) 1
 // and requires the identity perms and phases.

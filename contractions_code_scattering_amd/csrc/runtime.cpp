@@ -518,9 +518,7 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   check_traces(traces);
   const int NW = P == 32 ? 4 : 8;
   const int nseg = (4 * P) / (NW * 32);
-  if (dout && dout->d && nseg != 1) fail(LC_ERR_ARG, "exact-form digit output needs 4 P <= 256 (P = %d)", P);
-  if (dout && !dout->d && (!C || !dout->e || dout->cf || 4 * P <= 256))
-    fail(LC_ERR_ARG, "an exponents-only output needs an FP64 output and exponents, at 4 P > 256");
+  if (dout && (!dout->d || !dout->e)) fail(LC_ERR_ARG, "exact-form digit output without planes or exponents");
   // a twin reuses the A operand the kernel staged for this product: every K step of it must still
   // be in the two stage buffers (one block of K <= 64), and nothing of the product may use the
   // FP64 tile that overlaps them (no FP64 output, closing factors in the closing-factor form)
@@ -638,22 +636,13 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   ozs_.push_back(g);
 }
 
-void Program::to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf, bool have_e) {
-  if (mode_ != OZ) fail(LC_ERR_ARG, "digit conversion outside the exact mode");
+void Program::fix_digits(void* y, const int* eh, int* e, long long nmat, bool cf) {
+  if (mode_ != OZ || oz_pad(nv_) != 128) fail(LC_ERR_ARG, "half-row digit fix-up outside the exact mode at P = 128");
   if (nmat <= 0) return;
-  Level* l = open_level(false, kOzConvert);
-  l->first = converts_.size();
+  Level* l = open_level(false, kOzFix);
+  l->first = fixes_.size();
   l->count = 1;
-  converts_.push_back(Convert{x, stride, nmat, y, e, cf, have_e});
-  l->closed = true;
-}
-
-void Program::fill_int(int* p, long long n, int v) {
-  if (n <= 0) return;
-  Level* l = open_level(false, kFill);
-  l->first = fills_.size();
-  l->count = 1;
-  fills_.push_back(Fill{p, n, v});
+  fixes_.push_back(Fix{y, eh, e, nmat, cf});
   l->closed = true;
 }
 
@@ -758,17 +747,10 @@ void Program::run(hipStream_t s) {
     const bool dbl = dbl_, is_trace = l.is_trace, mx = mode_ == MX;
     const int variant = l.variant, count = l.count, nv = nv_;
     const long long total_wg = l.total_wg;
-    if (variant == kFill) {
-      const Fill f = fills_[l.first];
-      launch([=](hipStream_t st) { return lc_launch_fill_int(f.p, f.n, f.v, st); }, "fill");
-    } else if (variant == kOzConvert) {
-      const Convert cv = converts_[l.first];
-      const int P = oz_pad(nv);
-      launch([=](hipStream_t st) {
-               return lc_launch_oz_convert(cv.x, cv.stride, cv.nmat, nv, P, 16, 1, cv.e, nullptr, cv.cf ? nullptr : cv.y,
-                                           nullptr, cv.cf ? cv.y : nullptr, nullptr, cv.have_e ? 1 : 0, st);
-             },
-             "digit conversion");
+    if (variant == kOzFix) {
+      const Fix f = fixes_[l.first];
+      launch([=](hipStream_t st) { return lc_launch_oz_fix(f.y, f.eh, f.e, f.nmat, 128, f.cf ? 1 : 0, st); },
+             "digit fix-up");
     } else if (variant == kOzGemm || variant == kOzGemmRun) {
       const LcOzGroup* og = dozs + l.first;
       const int P = oz_pad(nv);

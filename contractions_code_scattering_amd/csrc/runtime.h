@@ -33,9 +33,9 @@ extern "C" hipError_t lc_launch_gemm_oz(const LcOzGroup* d_groups, int ng, const
                                         long long total_wg, int nv, int P, int run, int xtr, int chain,
                                         hipStream_t s);
 extern "C" hipError_t lc_launch_oz_convert(const void* x, long long stride, long long nmat, int nv, int P, int nblk,
-                                           int mode, int* ex_r, int* ex_t, void* y_rows, void* y_trans,
-                                           void* y_cf, void* y_cft, int have_exps, hipStream_t s);
-extern "C" hipError_t lc_launch_fill_int(int* p, long long n, int v, hipStream_t s);
+                                           int* ex_r, int* ex_t, void* y_rows, void* y_trans, void* y_cf, void* y_cft,
+                                           hipStream_t s);
+extern "C" hipError_t lc_launch_oz_fix(void* y, const int* eh, int* ex, long long nmat, int P, int cf, hipStream_t s);
 extern "C" hipError_t lc_launch_oz_decode(const void* y, const int* ex, int nv, int P, int nblk, int mode,
                                           long long nmat, void* out, hipStream_t s);
 
@@ -208,10 +208,10 @@ inline FusedTrace fused(const LcView& Y, void* out, long long os0, long long os1
 inline int oz_pad(int nv) { return nv <= 32 ? 32 : nv <= 64 ? 64 : 128; }
 
 // Digit output of an exact-form product: digit planes (strides in complex digit elements, P^2 per
-// block) and the per-D-row exponents ([4][P] ints per matrix, strides in ints).
-// Digits of a product with per-D-row exponents. With d == nullptr (and an FP64 output C): the
-// exponents alone, each workgroup strip's row maximum folded into e by an atomic maximum (e filled
-// with INT_MIN beforehand, fill_int), for a later to_digits(..., have_e = true).
+// block) and the per-D-row exponents ([4][P] ints per matrix, strides in ints). At P = 128 a
+// workgroup strip is half a D-row: e then receives the half-row exponents ([4 P][2] ints per
+// matrix, INT_MIN for an all-zero half) with each half under its own, and Program::fix_digits
+// makes the D-row form.
 struct DigitOut {
   void* d = nullptr;
   int* e = nullptr;
@@ -273,13 +273,10 @@ class Program {
   void gemm_oz_chain(const LcView& A1, const LcView& B1, const LcView& B2, const DigitOut* dout, int n0, int n1,
                      int n2, const std::vector<FusedTrace>& traces = {}, const LcView* twin_B = nullptr,
                      const std::vector<FusedTrace>* twin_traces = nullptr);
-  // OZ mode: FP64 D-space matrices (nmat of them, `stride` complex elements apart) to digits with
-  // per-D-row exponents (the form gemm_oz writes), as a level of its own.
-  // have_e: the exponents are already in e (an exponents-only DigitOut of the product).
-  void to_digits(const void* x, long long stride, long long nmat, void* y, int* e, bool cf = false,
-                 bool have_e = false);
-  // n ints at p set to v, as a level of its own (a fill kernel: no memset nodes in graphs).
-  void fill_int(int* p, long long n, int v);
+  // OZ mode, P = 128: nmat consecutive digit matrices y (closing-factor form with cf) written with
+  // half-row exponents eh (a DigitOut of gemm_oz) to the D-row exponents e, rescaling the half
+  // of each row with the smaller exponent (one unit of rounding), as a level of its own.
+  void fix_digits(void* y, const int* eh, int* e, long long nmat, bool cf);
   void trace(const LcView& X, const LcView& Y, void* out, long long os0, long long os1,
              long long os2, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
              bool accumulate = false);
@@ -292,24 +289,17 @@ class Program {
   double trace_flops() const { return trace_flops_; }
 
  private:
-  struct Convert {
-    const void* x;
-    long long stride, nmat;
+  struct Fix {
     void* y;
+    const int* eh;
     int* e;
+    long long nmat;
     bool cf;
-    bool have_e;
   };
-  struct Fill {
-    int* p;
-    long long n;
-    int v;
-  };
-  std::vector<Fill> fills_;
   struct Level {
     bool is_trace;
     bool closed;
-    int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzGemmRun, kOzConvert
+    int variant;  // MX kernel: 1 full-tile, 0 general; OZ: kOzGemm, kOzGemmRun, kOzFix
     bool xtr = false;  // OZ: a group with more than LC_MAX_FUSED_TRACES traces or elemental closing factors
     bool chain = false;  // OZ: a chained group (LcOzGroup::pre)
     size_t first;
@@ -324,9 +314,9 @@ class Program {
     bool oz = false;
     bool twin = false;  // (oz) the group is in the twin array
   };
-  static constexpr int kOzGemm = 10, kOzConvert = 11, kOzGemmRun = 12, kFill = 13;
+  static constexpr int kOzGemm = 10, kOzFix = 11, kOzGemmRun = 12;
   std::vector<LcOzGroup> ozs_, oztw_;
-  std::vector<Convert> converts_;
+  std::vector<Fix> fixes_;
   void oz_items(const LcView& A, const LcView& B, void* C, const long long* cs, const DigitOut* dout, int n0,
                 int n1, int n2, double ar, double ai, const std::vector<FusedTrace>& traces,
                 const LcView* twin_B, const std::vector<FusedTrace>* twin_traces,

@@ -289,8 +289,9 @@ int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d);
  * A' = G(gam4[0]) f_fa(A) G(gam4[1]), B' = G(gam4[2]) f_fb(B) G(gam4[3]) with f a combination of
  * LC_VIEW_CONJ | LC_VIEW_TV | LC_VIEW_TS (element conjugate, eigenvector transpose, spin-block
  * transpose) and G(mu) = gamma_mu (0: none; D-space operands only). h_c receives C
- * [16][nv][nv] in FP64; h_cd (optional, nv <= 64) C after its round trip through the product's
- * own digit output. nv <= 128. No reference counterpart. */
+ * [16][nv][nv] in FP64; h_cd (optional) C after its round trip through the product's own digit
+ * output (at nv > 64 with half-row exponents and the fix-up that makes them whole rows). nv <= 128.
+ * No reference counterpart. */
 #define LC_VIEW_CONJ 1
 #define LC_VIEW_TV 2
 #define LC_VIEW_TS 4

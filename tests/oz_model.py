@@ -81,6 +81,28 @@ def product(A, B, EA, EB, maxo=MAXO):
     return np.ldexp(comb(re), e) + 1j * np.ldexp(comb(im), e)
 
 
+def half_rows_round_trip(C):
+    """C written back as a P = 128 product writes its digit output, decoded: each half of a row
+    (columns [0, n/2) and [n/2, n)) digitised under its own exponent, then the half with the
+    smaller one brought to the row's exponent E, v -> rint(v 2^-k) with k = E - E_half (ties to
+    even, exact in FP64 for |v| < 2^31)."""
+    n = C.shape[1] // 2
+    E = line_exps(C, 1)[:, None]
+    out = np.empty_like(C)
+    for h in range(2):
+        X = C[:, h * n:(h + 1) * n]
+        Eh = line_exps(X, 1)[:, None]  # (0 for an all-zero half, whose digits are zero)
+        k = E - Eh
+        parts = []
+        for x in (X.real, X.imag):
+            d = digits(x, Eh)
+            v = ((d[0] * BASE + d[1]) * BASE + d[2]) * BASE + d[3]
+            v2 = np.rint(np.ldexp(v.astype(np.float64), -k)).astype(np.int64)
+            parts.append(np.ldexp(v2.astype(np.float64), E - FRAC) / SIGMA)
+        out[:, h * n:(h + 1) * n] = parts[0] + 1j * parts[1]
+    return out
+
+
 def round_trip(C, axis=1):
     """C written back as digits with one exponent per line (rows: axis=1) and decoded."""
     E = line_exps(C, axis)

@@ -66,7 +66,7 @@ def _operand(x, kron, flags, gl, gr, nv):
     return m
 
 
-def _run(nv, ka, kb, fa, fb, gam, seed, spread=0):
+def _run(nv, ka, kb, fa, fb, gam, seed, spread=0, unscale=True):
     from contractions_code_scattering_amd import rows
     rows.init(0)
     g = np.random.default_rng(seed)
@@ -90,7 +90,7 @@ def _run(nv, ka, kb, fa, fb, gam, seed, spread=0):
     ref = _operand(a, ka, fa, gam[0], gam[1], nv) @ _operand(b, kb, fb, gam[2], gam[3], nv)
     got = _dense(c.reshape(4, 4, nv, nv))
     dig = _dense(cd.reshape(4, 4, nv, nv))
-    if spread:
+    if spread and unscale:
         un = 1.0 / (sa[:, None] * sb[None, :])
         return got * un, dig, ref * un
     return got, dig, ref
@@ -127,6 +127,24 @@ def test_dense_product(nv):
     if nv <= 64:
         # the digit output: one exponent per D-row, bit for bit as the model writes it
         assert np.array_equal(dig, M.round_trip(got))
+    else:
+        # P = 128: each half D-row under its own exponent, then brought to the row's
+        assert np.array_equal(dig, M.half_rows_round_trip(got))
+
+
+@pytest.mark.parametrize("nv,spread", [(100, 6), (128, 6), (128, 40)])
+def test_half_row_digit_output(nv, spread):
+    """P = 128 digit output over lines of many scales: bit for bit the model's halves (the GEMM's
+    half-row exponents and the fix-up's rescaling of the smaller half), within 3/4 of a unit of the
+    row's scale from the value, and rows of both kinds (halves of equal and of unequal exponent)."""
+    got, dig, _ = _run(nv, 0, 0, 0, 0, (0, 0, 0, 0), 6, spread=spread, unscale=False)
+    assert np.array_equal(dig, M.half_rows_round_trip(got))
+    E = M.line_exps(got, 1)
+    Eh = np.stack([M.line_exps(got[:, :2 * nv], 1), M.line_exps(got[:, 2 * nv:], 1)])
+    assert (Eh.min(0) < E).any() and (Eh.min(0) == E).any()
+    unit = np.ldexp(1.0, E - 30)[:, None] / M.SIGMA
+    for part in ("real", "imag"):
+        assert (np.abs(getattr(dig, part) - getattr(got, part)) <= 0.75 * unit * (1 + 2.0 ** -40)).all()
 
 
 @pytest.mark.parametrize("ka,kb", [(1, 0), (0, 1)])

@@ -80,3 +80,36 @@ def test_round_trip_bound():
     bound = 2.0 ** -30 / M.SIGMA * rmax * (1 + 2.0 ** -40)
     assert (np.abs(R.real - C.real) <= bound).all()
     assert (np.abs(R.imag - C.imag) <= bound).all()
+
+
+def test_half_rows_round_trip():
+    # rows whose halves have unequal exponents (the right half scaled down by 2^0 .. 2^9 per row):
+    # the smaller half under the row's exponent after one more rounding, v -> rint(v 2^-k)
+    from fractions import Fraction
+    g = np.random.default_rng(4)
+    n = 256
+    C = g.standard_normal((64, 2 * n)) + 1j * g.standard_normal((64, 2 * n))
+    C[:, n:] *= np.exp2(-g.integers(0, 10, (64, 1)))
+    C[5, n:] = 0  # an all-zero half
+    R = M.half_rows_round_trip(C)
+    E = M.line_exps(C, 1)
+    unit = np.ldexp(1.0, E - 30)[:, None] / M.SIGMA
+    Eh = M.line_exps(C[:, n:], 1)
+    same = Eh == E
+    assert same.any() and (~same).any()
+    # a row whose halves share the exponent is the plain round trip
+    assert np.array_equal(R[same], M.round_trip(C)[same])
+    # otherwise within 1/2 + 1/4 of a unit of the row's scale (the second rounding at k >= 1)
+    for part in ("real", "imag"):
+        err = np.abs(getattr(R, part) - getattr(C, part))
+        assert (err <= 0.75 * unit * (1 + 2.0 ** -40)).all()
+    # against exact rationals: v = rint(x SIGMA 2^(30 - E_h)), then rint(v 2^(E_h - E)), ties to even
+    def rne(q):
+        fl, rem = divmod(q, 1)
+        return fl + (1 if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2 == 1) else 0)
+    for r in np.nonzero(~same)[0][:8]:
+        for j in range(n, 2 * n, 37):
+            x = float(C[r, j].real)
+            v = rne(Fraction(x) * Fraction(63, 32) * Fraction(2) ** (30 - int(Eh[r])))
+            v2 = rne(v * Fraction(2) ** (int(Eh[r]) - int(E[r])))
+            assert R[r, j].real == float(v2 * Fraction(2) ** (int(E[r]) - 30) / Fraction(63, 32)), (r, j)
