@@ -18,10 +18,13 @@
 
 namespace lc {
 
+// K chunks of 8 keep the LDS stages at 34 KB (FP64, TM = 64) and the registers within three waves
+// per SIMD, so that three workgroups share a CU: one workgroup's first loads and epilogue then
+// overlap the others' matrix-core work (chunks of 16 allowed two: 13% slower).
 template <typename T, int TM>
-__global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restrict__ groups,
-                                                      int ng, int nv, long long total) {
-  constexpr int KT = 16;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+lc_gemm_kernel(const LcGemmGroup* __restrict__ groups, int ng, int nv, long long total) {
+  constexpr int KT = 8;
   constexpr int E = TM * KT / 256;  // complex elements staged per thread per operand
   constexpr int WM = TM / 2;        // per-wave output tile (2 x 2 waves)
   constexpr int MI = WM / 16;       // 16x16 MFMA tiles per wave side
@@ -73,7 +76,9 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
   BlockRef<T> A, B;
   int cur_term = -1;
 
-  // Fetch chunk `ch` of both operands into registers (phase of the term applied to A).
+  // Fetch chunk `ch` of both operands into registers as stored; the view's conjugation and the
+  // term's phase are applied when the chunk goes to LDS, so that the loads are waited for only
+  // after the matrix cores have worked through the current chunk.
   auto load = [&](int ch) {
     const int term = ch / nk;
     const int k0 = (ch - term * nk) * KT;
@@ -94,20 +99,14 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int i = row0 + i0 + e, kk = k0 + k;
-        Cplx<T> z = {T(0), T(0)};
-        if (i < nv && kk < nv) z = elem(A, i, kk, nv);
-        ra[e].x = A.sr * z.x - A.si * z.y;
-        ra[e].y = A.sr * z.y + A.si * z.x;
+        ra[e] = (i < nv && kk < nv) ? gload_c(A.p + (long long)kk * nv + i) : Cplx<T>{T(0), T(0)};
       }
     } else {
       const int i = tid / (KT / E), kq = (tid % (KT / E)) * E;
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int ii = row0 + i, kk = k0 + kq + e;
-        Cplx<T> z = {T(0), T(0)};
-        if (ii < nv && kk < nv) z = elem(A, ii, kk, nv);
-        ra[e].x = A.sr * z.x - A.si * z.y;
-        ra[e].y = A.sr * z.y + A.si * z.x;
+        ra[e] = (ii < nv && kk < nv) ? gload_c(A.p + (long long)ii * nv + kk) : Cplx<T>{T(0), T(0)};
       }
     }
     if (B.flags & LC_TV) {
@@ -115,23 +114,28 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int jj = col0 + j, kk = k0 + kq + e;
-        Cplx<T> z = {T(0), T(0)};
-        if (jj < nv && kk < nv) z = elem(B, kk, jj, nv);
-        rb[e] = z;
+        rb[e] = (jj < nv && kk < nv) ? gload_c(B.p + (long long)jj * nv + kk) : Cplx<T>{T(0), T(0)};
       }
     } else {
       const int k = tid / (TM / E), j0 = (tid % (TM / E)) * E;
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int jj = col0 + j0 + e, kk = k0 + k;
-        Cplx<T> z = {T(0), T(0)};
-        if (jj < nv && kk < nv) z = elem(B, kk, jj, nv);
-        rb[e] = z;
+        rb[e] = (jj < nv && kk < nv) ? gload_c(B.p + (long long)kk * nv + jj) : Cplx<T>{T(0), T(0)};
       }
     }
   };
 
   auto store = [&](int buf) {
+    // A: conjugation, then the term's phase; B: conjugation
+    const T ca = (A.flags & LC_CONJ) ? T(-1) : T(1), cb = (B.flags & LC_CONJ) ? T(-1) : T(1);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const T zx = ra[e].x, zy = ca * ra[e].y;
+      ra[e].x = A.sr * zx - A.si * zy;
+      ra[e].y = A.sr * zy + A.si * zx;
+      rb[e].y = cb * rb[e].y;
+    }
     if (A.flags & LC_TV) {
       const int k = tid / (TM / E), i0 = (tid % (TM / E)) * E;
 #pragma unroll
@@ -187,21 +191,12 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
     __syncthreads();
   }
 
-  // Epilogue: scale by alpha, optionally store block (a,c), fused trace pieces.
+  // Epilogue: optionally block (a,c) of alpha C, and the fused trace pieces, which are linear in C:
+  // alpha Tr[C Y] is formed from the accumulators as they are and scaled once (no second copy of
+  // the tile in registers, which would keep the occupancy at two waves per SIMD).
   const T alr = T(g.alpha[0]), ali = T(g.alpha[1]);
   Cplx<T>* C = g.C ? reinterpret_cast<Cplx<T>*>(g.C) + voff(g.cs, co) + (long long)(a * 4 + c) * nv2
                    : nullptr;
-  T zr[MI][MI][4], zi[MI][MI][4];
-#pragma unroll
-  for (int m = 0; m < MI; m++)
-#pragma unroll
-    for (int n = 0; n < MI; n++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const T xr = cre[m][n][q], xi = cim[m][n][q];
-        zr[m][n][q] = alr * xr - ali * xi;
-        zi[m][n][q] = alr * xi + ali * xr;
-      }
   if (C) {
 #pragma unroll
     for (int m = 0; m < MI; m++)
@@ -212,9 +207,10 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
           const int row = row0 + wm * WM + m * 16 + acc_row<T>(lane, q);
           const int col = col0 + wn * WM + n * 16 + (lane & 15);
           if (row < nv && col < nv) {
+            const T xr = cre[m][n][q], xi = cim[m][n][q];
             Cplx<T> z;
-            z.x = zr[m][n][q];
-            z.y = zi[m][n][q];
+            z.x = alr * xr - ali * xi;
+            z.y = alr * xi + ali * xr;
             gstore_c(C + (long long)row * nv + col, z);
           }
         }
@@ -251,13 +247,14 @@ __global__ void __launch_bounds__(256) lc_gemm_kernel(const LcGemmGroup* __restr
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const double yr = yv[m][n][q].x, yi = cj * (double)yv[m][n][q].y;
-          const double xr = zr[m][n][q], xi = zi[m][n][q];
+          const double xr = cre[m][n][q], xi = cim[m][n][q];
           sr += xr * yr - xi * yi;
           si += xr * yi + xi * yr;
         }
-    // Y block phase, then the trace coefficient
+    // alpha, the Y block phase, then the trace coefficient
+    const double ur = alr * sr - ali * si, ui = alr * si + ali * sr;
     const double yr = Y.sr, yi = Y.si;
-    const double pr = sr * yr - si * yi, pi = sr * yi + si * yr;
+    const double pr = ur * yr - ui * yi, pi = ur * yi + ui * yr;
     const double ar = g.tr_alpha[t][0], ai = g.tr_alpha[t][1];
     tr_re[t] = ar * pr - ai * pi;
     tr_im[t] = ar * pi + ai * pr;
