@@ -173,19 +173,13 @@ void run_chunk_oz(const OzShape& E, bool sets, const char* tau_cf, const char* t
   };
   pg.set_last_axis_fastest(true);  // momenta of one (tf, t0) pair next to each other
   // digit output of a product into `a` (items laid out with strides st); with twin_B, the twin
-  // product A twin_B traced by twin_tr (formed by the same kernel from A on the chip at P <= 64,
-  // else as a product of its own)
+  // product A twin_B traced by twin_tr, formed by the same kernel from A on the chip
   auto dgemm = [&](const LcView& A, const LcView& Bv, const DArr& a, const long long* st, int n0, int n1, int n2,
                    const std::vector<FusedTrace>& tr = {}, bool cf = false, const LcView* twin_B = nullptr,
                    const std::vector<FusedTrace>* twin_tr = nullptr) {
     const DigitOut d = digit_out(a, st, bd, P, cf);
-    if (direct) {
-      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr, twin_B, twin_tr);
-      return;
-    }
-    pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr);
-    pending.push_back(PendingFix{a, (long long)n0 * n1 * n2, cf});
-    if (twin_B) pg.gemm_oz(A, *twin_B, nullptr, 0, 0, 0, nullptr, n0, n1, n2, 1.0, 0.0, *twin_tr);
+    pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2, 1.0, 0.0, tr, twin_B, twin_tr);
+    if (!direct) pending.push_back(PendingFix{a, (long long)n0 * n1 * n2, cf});
   };
   // close a level: a barrier, then (P = 128) the fix-ups of its stored products
   auto close_level = [&] {
@@ -388,14 +382,9 @@ void run_chunk_oz_sigma(const OzShape& E, int idir, const OzSigmaIn& in, int p0,
     pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, &d, n0, n1, n2);
     if (!direct) pending.push_back(PendingFix{a, (long long)n0 * n1 * n2, cf});
   };
-  // trace-only pair products, with a twin on the same A where the kernel forms it (P <= 64)
+  // trace-only pair products, with a twin on the same A
   auto tgemm = [&](const LcView& A, const LcView& Bv, const std::vector<FusedTrace>& tr, const LcView* twin_B = nullptr,
                    const std::vector<FusedTrace>* twin_tr = nullptr) {
-    if (twin_B && !direct) {
-      pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0, tr);
-      pg.gemm_oz(A, *twin_B, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0, *twin_tr);
-      return;
-    }
     pg.gemm_oz(A, Bv, nullptr, 0, 0, 0, nullptr, nt, ns, pc, 1.0, 0.0, tr, twin_B, twin_tr);
   };
   auto close_level = [&] {

@@ -203,8 +203,8 @@ __global__ void __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2,
 lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup* __restrict__ twins, int nv, int P,
                   long long total) {
   // stage buffers during the K loop, [2][MAXA] chunks of 16 planes x 1 KB (the 8 digit planes of
-  // an A chunk and their negations), or [nstep <= 8][1] for a run of dense products; the per-wave
-  // FP64 epilogue tiles overlap them
+  // an A chunk and their negations), [nstep <= 8][1] for a run of dense products, or [4][2] for a
+  // dense x elemental product at P = 128; the per-wave FP64 epilogue tiles overlap them
   __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<NW>()];
   __shared__ int rowe[NW][32];
   // partial sums of the fused traces (re, im), reduced by one wave per value: one per lane, or with
@@ -507,6 +507,10 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
       }
     }
   };
+  // chunks between stage buffers: one in a run; nA for a dense x elemental product at P = 128 (its
+  // 4 K steps of nA = 2 chunks fill the 8 slots); MAXA otherwise (at P <= 64 a dense x elemental
+  // product has nA = MAXA, so its two steps fill them too)
+  auto sstride = [&]() { return RUN ? 1 : (!K256 && kind == 2) ? nA : MAXA; };
   // plane q of chunk s (q < 4: Ar digit q, else Ai digit q - 4) into plane group q / 4, and its
   // negation into group q / 4 + 2; nc chunks (the product's nA, or 1 for a chained first product)
   auto storeA = [&](int bf, int nc) {
@@ -515,7 +519,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     for (int i = 0; i < LPW; i++) {
       const int f = wave + i * NW;
       if (f < nload) {
-        unsigned char* d = &lds[((bf * (RUN ? 1 : MAXA) + (f >> 3)) * 16 + (f & 7)) * 1024 + 16 * ln];
+        unsigned char* d = &lds[((bf * sstride() + (f >> 3)) * 16 + (f & 7)) * 1024 + 16 * ln];
         *reinterpret_cast<i32x4*>(d) = ra[i];
         i32x4 n = ra[i];
         neg16(n);
@@ -574,7 +578,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
     int ngi = gi;
     ItemPos nxt;  // (set when pre)
     // sub-product 0 is the group's product; 1, its twin, reuses the A operand staged for 0 (the
-    // twin needs every step of A in the stage buffers: nstep <= 2); 2, a chained group's first
+    // twin needs every step of A in the stage buffers: a dense x elemental product); 2, a chained group's first
     // product, runs before 0 and leaves its output in the stage buffers as 0's A (digits and row
     // exponents as a digit output would have them). One code path per sub-product.
     auto run_sub = [&](auto SUB) {
@@ -599,7 +603,7 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
         const int ore = (fl & 3) * 4096, oim = ((fl >> 2) & 3) * 4096;  // plane groups of re(w), im(w)
         const int onim = oim ^ 8192;                                     // and of -im(w)
         const char* nsrc = more ? bsrc(st + 1, offB8, sub) : nullptr;
-        const unsigned char* asrc = &lds[((bf * (RUN ? 1 : MAXA) + (FIRST ? 0 : sA)) * 16) * 1024 + frag_off];
+        const unsigned char* asrc = &lds[((bf * sstride() + (FIRST ? 0 : sA)) * 16) * 1024 + frag_off];
         auto rd = [&](int off) { return *reinterpret_cast<const i32x4*>(asrc + off); };
         // one digit pair: re += Br re(w) - (sB Bi) im(w), im += Br im(w) + (sB Bi) re(w); re is reused
         // two MFMAs later
@@ -650,8 +654,9 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
 
       // The loop runs every step but the last with unconditional loads, so that the compiler's count
       // of loads in flight is exact on every path and each wait covers only what it needs. Stage
-      // buffer of step st: st in a run (every step kept for the run's later items), else st % 2.
-      auto sbuf = [&](int st) { return RUN ? st : (st & 1); };
+      // buffer of step st: st in a run (every step kept for the run's later items) and for a dense x
+      // elemental product at P = 128 (4 steps of 2 chunks: every step kept for a twin), else st % 2.
+      auto sbuf = [&](int st) { return (RUN || (!K256 && kind == 2)) ? st : (st & 1); };
       if (FIRST) {
         storeA(0, 1);
         lds_barrier();
@@ -1247,8 +1252,9 @@ __global__ void __launch_bounds__(256) lc_oz_fix_kernel(unsigned char* __restric
   }
 }
 
-// Digits back to FP64 (tests and diagnostics): D-space or elemental digit matrices with
-// exponents in the mode-1 layout (mode 1) or per line index (mode 0), row-line planes.
+// Digits back to FP64 (tests and diagnostics): D-space or elemental digit matrices, row-line
+// planes, with one exponent per D-row (mode 1: block row R, row i at R * P + i, as a product's
+// digit output has them) or per line index (mode 0, as the input conversion makes them).
 __global__ void __launch_bounds__(256) lc_oz_decode_kernel(const unsigned char* __restrict__ y, const int* __restrict__ ex,
                                                            int nv, int P, int nblk, int mode, long long nmat,
                                                            double2* __restrict__ out) {

@@ -141,7 +141,8 @@ struct LcOzGroup {
   int nseg;                 // column segments of the workgroup strip per D-row
   // index of the group's twin in the launch's twin array, or -1. A twin is a second product with
   // the same A operand, kind and items and its own B and fused traces (no C or Cd), formed per item
-  // right after this one from the A operand already staged on the chip (K <= 2 * 32 per block)
+  // right after this one from the A operand already staged on the chip (every K step of it: an
+  // elemental factor, K <= 2 * 32 per block, or at P = 128 an elemental B, 4 steps of 2 chunks)
   int twin;
   // items per logical workgroup (1, or n[0] when A does not depend on axis 0): a workgroup stages
   // its strip of A once (every K step in its own stage buffer, K <= 8 * 32) and forms the products

@@ -525,7 +525,10 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   if (twin_B) {
     if (!twin_traces || twin_traces->empty() || (int)twin_traces->size() > LC_MAX_FUSED_TRACES)
       fail(LC_ERR_ARG, "a twin product needs 1..%d fused traces", LC_MAX_FUSED_TRACES);  // (the kernel's twin path)
-    if (kind == 0 || P > 64 || nseg != 1) fail(LC_ERR_ARG, "twin products need an elemental factor and P <= 64");
+    // (every K step of A in the stage buffers: an elemental factor, and at P = 128 an elemental B,
+    // whose 4 steps of 2 chunks fill them)
+    if (kind == 0 || (P > 64 && kind != 2))
+      fail(LC_ERR_ARG, "twin products need an elemental factor (at P = 128, an elemental B)");
     if ((twin_B->kind == LC_KRON) != (B.kind == LC_KRON) || !(twin_B->flags & LC_TV) || !twin_B->ex)
       fail(LC_ERR_ARG, "a twin's B must be read like B (same kind, by columns, with exponents)");
     for (int a = 0; a < 4; a++)

@@ -260,8 +260,9 @@ class Program {
   // OZ mode: C = alpha * A * B with A, B digit views (A read by rows, B by columns: B carries
   // LC_TV), phases powers of i. Outputs: FP64 C (optional, strides cs in complex elements of nv^2
   // blocks), digits with per-D-row exponents (optional; needs 4 P <= 256), fused traces (FP64 Y).
-  // With twin_B (P <= 64, no C): the twin product A twin_B over the same batch, formed per item
-  // right after A B from the A operand already on the chip, delivering only twin_traces.
+  // With twin_B (an elemental factor, at P = 128 an elemental B; no C): the twin product A twin_B
+  // over the same batch, formed per item right after A B from the A operand already on the chip,
+  // delivering only twin_traces.
   void gemm_oz(const LcView& A, const LcView& B, void* C, long long cs0, long long cs1, long long cs2,
                const DigitOut* dout, int n0, int n1, int n2, double ar = 1.0, double ai = 0.0,
                const std::vector<FusedTrace>& traces = {}, const LcView* twin_B = nullptr,
