@@ -520,13 +520,12 @@ void Program::oz_items(const LcView& A, const LcView& B, void* C, const long lon
   const int nseg = (4 * P) / (NW * 32);
   if (dout && (!dout->d || !dout->e)) fail(LC_ERR_ARG, "exact-form digit output without planes or exponents");
   // a twin reuses the A operand the kernel staged for this product: every K step of it must still
-  // be in the two stage buffers (one block of K <= 64), and nothing of the product may use the
-  // FP64 tile that overlaps them (no FP64 output, closing factors in the closing-factor form)
+  // be in the stage buffers (an elemental factor, one block of K <= 64; at P = 128 an elemental B,
+  // whose 4 steps of 2 chunks fill them), and nothing of the product may use the FP64 tile that
+  // overlaps them (no FP64 output, closing factors in the closing-factor form)
   if (twin_B) {
     if (!twin_traces || twin_traces->empty() || (int)twin_traces->size() > LC_MAX_FUSED_TRACES)
       fail(LC_ERR_ARG, "a twin product needs 1..%d fused traces", LC_MAX_FUSED_TRACES);  // (the kernel's twin path)
-    // (every K step of A in the stage buffers: an elemental factor, and at P = 128 an elemental B,
-    // whose 4 steps of 2 chunks fill them)
     if (kind == 0 || (P > 64 && kind != 2))
       fail(LC_ERR_ARG, "twin products need an elemental factor (at P = 128, an elemental B)");
     if ((twin_B->kind == LC_KRON) != (B.kind == LC_KRON) || !(twin_B->flags & LC_TV) || !twin_B->ex)
