@@ -35,7 +35,7 @@ def _same(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("nv,mixed", [(16, False), (64, True), (40, True)])
+@pytest.mark.parametrize("nv,mixed", [(16, False), (64, True), (40, True), (100, True)])
 def test_rho_pipi_replays(eng, nv, mixed):
     from contractions_code_scattering_amd.devmem import copy, to_device
     nt, np_ = 3, 2
@@ -142,7 +142,7 @@ def relerr_ok(a, b):
     return all(np.abs(x - y).max() <= 1e-13 * np.abs(y).max() for x, y in zip(a, b))
 
 
-@pytest.mark.parametrize("nv,mixed", [(16, False), (64, True)])
+@pytest.mark.parametrize("nv,mixed", [(16, False), (64, True), (100, True)])
 def test_pipi_sets_replays(eng, nv, mixed):
     """The momentum-set engine: recorded once, replayed three times with eager engine calls in
     between, each replay bit-identical to the eager result; new elementals in the same buffer are
