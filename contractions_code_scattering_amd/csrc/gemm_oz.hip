@@ -177,7 +177,6 @@ __device__ __forceinline__ long long readlane64(long long v, int l) {
 }
 
 constexpr int TSTR = 32 * 16 + 16;  // LDS row stride of an FP64 epilogue tile (bytes)
-constexpr int DSTR = 48;            // LDS row stride of a staged digit plane (bytes)
 constexpr int MAXA = 4;             // A chunks per step at most (dense x elemental: one per block column)
 
 template <int NW>
@@ -634,16 +633,20 @@ lc_gemm_oz_kernel(const LcOzGroup* __restrict__ groups, int ng, const LcOzGroup*
           pair(db, xr, xi, z0, z1, z2);
 #pragma unroll
           for (int da = 1; da + db < NORD; da++) {
+            // (compile-time after unrolling) the fragments of the next pair, if any
+            const bool nxt = da + 1 + db < NORD;
             i32x4 n0, n1, n2;
-            if (da + 1 + db < NORD) {
+            if (nxt) {
               n0 = rd(ore + (da + 1) * 1024);
               n1 = rd(oim + (da + 1) * 1024);
               n2 = rd(onim + (da + 1) * 1024);
             }
             pair(da + db, xr, xi, c0, c1, c2);
-            c0 = n0;
-            c1 = n1;
-            c2 = n2;
+            if (nxt) {
+              c0 = n0;
+              c1 = n1;
+              c2 = n2;
+            }
           }
           if (more) {
             br[db] = gload<i32x4>(nsrc + db * P2);
