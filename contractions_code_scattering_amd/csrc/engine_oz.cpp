@@ -600,15 +600,18 @@ void oz_sigma_set(const void* d_tau, const void* d_el, int nt, int nv, int np, i
 
 }  // namespace lc
 
-extern "C" {
+namespace {
 
-int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a, const void* h_b,
-                        void* h_c, void* h_cd) {
+// One exact-form product of host operands (the test entry points below): C in FP64 into h_c and,
+// with h_out, its digit output decoded (cf false) or its closing-factor output as stored (cf true,
+// with the D-row exponents into h_e).
+int selftest_oz(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a, const void* h_b, void* h_c,
+                void* h_out, bool cf, int* h_e) {
   return guarded([&] {
     require_init();
     if (nv < 1 || nv > 128) fail(LC_ERR_ARG, "nv=%d out of range", nv);
     if (ka && kb) fail(LC_ERR_ARG, "elemental x elemental product");
-    if (!h_a || !h_b || !h_c || !gam4) fail(LC_ERR_ARG, "null pointer");
+    if (!h_a || !h_b || !h_c || !gam4 || (cf && (!h_out || !h_e))) fail(LC_ERR_ARG, "null pointer");
     if ((fa | fb) & ~(LC_CONJ | LC_TV | LC_TS)) fail(LC_ERR_ARG, "bad view flags");
     const int P = oz_pad(nv);
     const long long nv2 = (long long)nv * nv, P2 = (long long)P * P;
@@ -640,28 +643,45 @@ int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4,
       if (gam4[2]) B = left(B, gamma(gam4[2]));
       if (gam4[3]) B = right(B, gamma(gam4[3]));
     }
-    // the digit output (P = 128: with half-row exponents, then the fix-up)
-    const bool digits = h_cd != nullptr;
+    // the digit or closing-factor output (P = 128: with half-row exponents, then the fix-up)
+    const bool out = h_out != nullptr;
     DArr da;
     da.d = cd.c();
     da.e = static_cast<int*>(ce.p);
     da.h = static_cast<int*>(ce.p) + 4 * P;
     const long long one[3] = {0, 0, 0};
-    const DigitOut d = digit_out(da, one, 16 * P2, P, false);
+    const DigitOut d = digit_out(da, one, 16 * P2, P, cf);
     Program pg(nv, Program::OZ);
-    pg.gemm_oz(A, B, c.p, 0, 0, 0, digits ? &d : nullptr, 1, 1, 1);
-    if (digits && 4 * P > 256) {
+    pg.gemm_oz(A, B, c.p, 0, 0, 0, out ? &d : nullptr, 1, 1, 1);
+    if (out && 4 * P > 256) {
       pg.barrier();
-      pg.fix_digits(da.d, da.h, da.e, 1, false);
+      pg.fix_digits(da.d, da.h, da.e, 1, cf);
     }
     pg.run(s);
     hip_check(hipMemcpyAsync(h_c, c.p, 16 * nv2 * 16, hipMemcpyDeviceToHost, s), "copy c");
-    if (digits) {
+    if (out && cf) {
+      hip_check(hipMemcpyAsync(h_out, cd.p, 16 * 8 * P2, hipMemcpyDeviceToHost, s), "copy cf");
+      hip_check(hipMemcpyAsync(h_e, da.e, 4 * P * 4, hipMemcpyDeviceToHost, s), "copy exponents");
+    } else if (out) {
       hip_check(lc_launch_oz_decode(cd.p, da.e, nv, P, 16, 1, 1, dec.p, s), "decode");
-      hip_check(hipMemcpyAsync(h_cd, dec.p, 16 * nv2 * 16, hipMemcpyDeviceToHost, s), "copy cd");
+      hip_check(hipMemcpyAsync(h_out, dec.p, 16 * nv2 * 16, hipMemcpyDeviceToHost, s), "copy cd");
     }
     hip_check(hipStreamSynchronize(s), "sync");
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a, const void* h_b,
+                        void* h_c, void* h_cd) {
+  return selftest_oz(nv, ka, kb, fa, fb, gam4, h_a, h_b, h_c, h_cd, false, nullptr);
+}
+
+int lc_selftest_oz_gemm_cf(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a, const void* h_b,
+                           void* h_c, void* h_cf, int* h_e) {
+  return selftest_oz(nv, ka, kb, fa, fb, gam4, h_a, h_b, h_c, h_cf, true, h_e);
 }
 
 }  // extern "C"

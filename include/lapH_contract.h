@@ -297,6 +297,14 @@ int lc_selftest_mfma(int prec, const void* h_a, const void* h_b, void* h_d);
 #define LC_VIEW_TS 4
 int lc_selftest_oz_gemm(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a,
                         const void* h_b, void* h_c, void* h_cd);
+/* The same product with a closing-factor output (one int32 per real, v = rint(x 2^(30 - E)) under
+ * the D-row exponent E; at nv > 64 written under half-row exponents and brought to E by the
+ * fix-up) returned as stored: h_cf receives 16 blocks of 8 P^2 bytes (P = 32, 64 or 128 for
+ * nv <= 32, 64, 128), element (r, j) of block b at 8 b P^2 + ((j/32) (P/32) + r/32) 8192 +
+ * (2 ((j%32)/8) + (j/2)%2) 1024 + (32 ((j/4)%2) + r%32) 16 + 8 (j%2) bytes (re, im); h_e the 4 P
+ * D-row exponents (block row R, row i at R P + i). No reference counterpart. */
+int lc_selftest_oz_gemm_cf(int nv, int ka, int kb, int fa, int fb, const int* gam4, const void* h_a,
+                           const void* h_b, void* h_c, void* h_cf, int* h_e);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,31 @@ def half_rows_round_trip(C):
     return out
 
 
+def cf_values(x, E):
+    """Closing-factor form of x under exponent E (broadcast): rint(x 2^(30-E)), ties to even (the
+    scaling is exact)."""
+    return np.rint(np.ldexp(np.asarray(x, dtype=np.float64), FRAC - np.asarray(E))).astype(np.int64)
+
+
+def cf_rows(C, halves=False):
+    """C in the closing-factor form as a product writes it: one exponent per row (E, returned), and
+    with halves (P = 128) each half of a row under its own exponent first, then brought to E by
+    v -> rint(v 2^-k), k = E - E_half, ties to even. Returns (re ints, im ints, E)."""
+    E = line_exps(C, 1)[:, None]
+    if not halves:
+        return cf_values(C.real, E), cf_values(C.imag, E), E[:, 0]
+    n = C.shape[1] // 2
+    re = np.empty(C.shape, np.int64)
+    im = np.empty(C.shape, np.int64)
+    for h in range(2):
+        X = C[:, h * n:(h + 1) * n]
+        Eh = line_exps(X, 1)[:, None]
+        for out, x in ((re, X.real), (im, X.imag)):
+            v = cf_values(x, Eh)
+            out[:, h * n:(h + 1) * n] = np.rint(np.ldexp(v.astype(np.float64), Eh - E)).astype(np.int64)
+    return re, im, E[:, 0]
+
+
 def round_trip(C, axis=1):
     """C written back as digits with one exponent per line (rows: axis=1) and decoded."""
     E = line_exps(C, axis)

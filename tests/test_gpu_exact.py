@@ -22,6 +22,7 @@ CONJ, TV, TS = 1, 2, 4
 TOL = 2e-8
 
 _capi.register("lc_selftest_oz_gemm", [ctypes.c_int] * 5 + [ctypes.c_void_p] * 5)
+_capi.register("lc_selftest_oz_gemm_cf", [ctypes.c_int] * 5 + [ctypes.c_void_p] * 6)
 
 I = 1j
 GAMMA = {
@@ -174,3 +175,50 @@ def test_lines_over_many_decades():
     # accuracy of every element's own scale
     got, _, ref = _run(64, 0, 0, 0, 0, (0, 0, 0, 0), 5, spread=40)
     _check(got, ref, TOL)
+
+
+def _cf_offset(r, j, P):
+    """Byte offset of element (r, j) in a P x P block of the closing-factor form (the layout the
+    entry point documents)."""
+    jj, rr = j % 32, r % 32
+    return (((j // 32) * (P // 32) + r // 32) * 8192 + (2 * (jj // 8) + (jj // 2) % 2) * 1024
+            + (32 * ((jj // 4) % 2) + rr) * 16 + 8 * (jj % 2))
+
+
+@pytest.mark.parametrize("nv,spread", [(32, 0), (64, 6), (100, 6), (128, 0), (128, 40)])
+def test_closing_factor_output(nv, spread):
+    """The closing-factor output (one int32 per real under the D-row exponent; at P = 128 under
+    half-row exponents, then the fix-up's rescaling of the smaller half) bit for bit as the model
+    writes it from the product's FP64 output, with the D-row exponents."""
+    from contractions_code_scattering_amd import rows
+    rows.init(0)
+    g = np.random.default_rng(7)
+    P = 32 if nv <= 32 else 64 if nv <= 64 else 128
+    a = g.standard_normal(16 * nv * nv) + 1j * g.standard_normal(16 * nv * nv)
+    b = g.standard_normal(16 * nv * nv) + 1j * g.standard_normal(16 * nv * nv)
+    if spread:
+        sa = np.tile(np.exp2(g.integers(-spread, spread, nv)).astype(float), 4)
+        a = _blocks(sa[:, None] * _dense(a.reshape(4, 4, nv, nv)), nv).ravel().copy()
+    c = np.zeros(16 * nv * nv, np.complex128)
+    cf = np.zeros(16 * 8 * P * P, np.uint8)
+    e = np.zeros(4 * P, np.int32)
+    g4 = (ctypes.c_int * 4)(0, 0, 0, 0)
+    _capi.check(_capi.lib().lc_selftest_oz_gemm_cf(nv, 0, 0, 0, 0, g4, a.ctypes.data, b.ctypes.data, c.ctypes.data,
+                                                  cf.ctypes.data, e.ctypes.data))
+    got = _dense(c.reshape(4, 4, nv, nv))
+    vre, vim, E = M.cf_rows(got, halves=P == 128)
+    if P == 128:
+        # rows of both kinds: halves of equal exponent, and a smaller half the fix-up rescales
+        Eh = np.stack([M.line_exps(got[:, :2 * nv], 1), M.line_exps(got[:, 2 * nv:], 1)])
+        assert (Eh.min(0) < E).any() and (Eh.min(0) == E).any()
+    words = cf.view(np.int32)
+    r = np.arange(nv)[:, None]
+    j = np.arange(nv)[None, :]
+    off = _cf_offset(r, j, P) // 4
+    for ba in range(4):
+        assert np.array_equal(e[ba * P:ba * P + nv], E[ba * nv:(ba + 1) * nv]), ba
+        for bc in range(4):
+            base = (ba * 4 + bc) * 8 * P * P // 4
+            rows_, cols = slice(ba * nv, (ba + 1) * nv), slice(bc * nv, (bc + 1) * nv)
+            assert np.array_equal(words[base + off], vre[rows_, cols]), (ba, bc)
+            assert np.array_equal(words[base + off + 1], vim[rows_, cols]), (ba, bc)
