@@ -113,3 +113,35 @@ def test_half_rows_round_trip():
             v = rne(Fraction(x) * Fraction(63, 32) * Fraction(2) ** (30 - int(Eh[r])))
             v2 = rne(v * Fraction(2) ** (int(Eh[r]) - int(E[r])))
             assert R[r, j].real == float(v2 * Fraction(2) ** (int(E[r]) - 30) / Fraction(63, 32)), (r, j)
+
+
+def test_cf_rows_against_rationals():
+    # the closing-factor form: rint(x 2^(30 - E)) ties to even under the row exponent, and with
+    # halves the smaller half through rint(v 2^(E_h - E)); checked with exact rationals, and the
+    # ties of the second rounding go to even (a planted truncation would fail both)
+    from fractions import Fraction
+    g = np.random.default_rng(5)
+    n = 64
+    C = g.standard_normal((16, 2 * n)) + 1j * g.standard_normal((16, 2 * n))
+    C[:, n:] *= np.exp2(-g.integers(0, 6, (16, 1)))
+    def rne(q):
+        fl, rem = divmod(q, 1)
+        return fl + (1 if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2 == 1) else 0)
+    re, im, E = M.cf_rows(C)
+    for r in range(16):
+        for j in range(0, 2 * n, 7):
+            assert re[r, j] == rne(Fraction(float(C[r, j].real)) * Fraction(2) ** (30 - int(E[r])))
+    re2, im2, E2 = M.cf_rows(C, halves=True)
+    assert np.array_equal(E, E2)
+    Eh = M.line_exps(C[:, n:], 1)
+    for r in range(16):
+        for j in range(n, 2 * n, 5):
+            v = rne(Fraction(float(C[r, j].imag)) * Fraction(2) ** (30 - int(Eh[r])))
+            assert im2[r, j] == rne(v * Fraction(2) ** (int(Eh[r]) - int(E[r]))), (r, j)
+    # rows whose halves share the exponent are the plain form
+    same = Eh == E
+    assert same.any() and (~same).any()
+    assert np.array_equal(re2[same], re[same]) and np.array_equal(im2[same], im[same])
+    # halves of an exact tie: 2^k v + 2^(k-1) goes to the even neighbour
+    v = np.array([[1.0, 3.0, 5.0, -3.0]])
+    assert np.array_equal(np.rint(np.ldexp(v, -1)), [[0.0, 2.0, 2.0, -2.0]])
